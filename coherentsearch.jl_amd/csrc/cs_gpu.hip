@@ -1,0 +1,917 @@
+// Coherent-search kernels for gfx950 (MI355X).
+//
+// Built with -ffp-contract=off: every fused multiply-add below is an explicit
+// __builtin_fmaf / fma, placed where the reference's `muladd` sits, so the FP32
+// interpolation reproduces the reference's Float32-weight arithmetic bit for
+// bit (src/directinterp.jl:584-605).
+#include <hip/hip_runtime.h>
+
+#define CS_HD __host__ __device__
+#include "c2r_gen.hpp"
+#include "cs_gpu.hpp"
+
+namespace cs {
+
+// ---------------------------------------------------------------------------
+// Exact trial-grid state: T = t*hp, (qint, res) = (T div q, T mod q), for
+// 0 <= t < 2^32 and q*hp < 2^32 (the host checks both).  Two 32-bit divisions
+// instead of a 64-bit one (direct_chunk_state, directinterp.jl:414-418).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void cstate(int64_t t, uint32_t hp, uint32_t q, int64_t& qint,
+                                       uint32_t& res) {
+    uint32_t tt = (uint32_t)t;
+    uint32_t tq = tt / q;
+    uint32_t tr = tt - tq * q;
+    uint32_t u = tr * hp;
+    uint32_t uq = u / q;
+    res = u - uq * q;
+    qint = (int64_t)tq * hp + uq;
+}
+
+// 0-based first bin of trial t for harmonic plan d: floor(h r_t) + 1 - m/2.
+__device__ __forceinline__ int64_t bin0(const HarmDev& d, const TabDev& tb, int64_t t,
+                                        uint32_t& p) {
+    int64_t qint;
+    uint32_t res;
+    cstate(t, (uint32_t)d.hp, (uint32_t)d.q, qint, res);
+    p = (uint32_t)tb.row[d.row_off + res];
+    return d.rfloor0 + qint + tb.carry[d.tab_off + p] + 1 - tb.m / 2;
+}
+
+// Row guard on the block's TRUE trial range (directinterp.jl:490-496): the
+// row is kept for the whole block or zeroed for the whole block.
+__device__ __forceinline__ bool row_guard(const HarmDev& d, const TabDev& tb,
+                                          const SpecDev& sp, int64_t first,
+                                          int64_t last) {
+    uint32_t p0, pl;
+    int64_t b_first = bin0(d, tb, first, p0);
+    int64_t qint;
+    uint32_t res;
+    cstate(last, (uint32_t)d.hp, (uint32_t)d.q, qint, res);
+    pl = (uint32_t)tb.row[d.row_off + res];
+    int64_t b_last = d.rfloor0 + qint + tb.carry[d.tab_off + pl] + 1 - tb.m / 2;
+    return b_first >= 0 && b_last + tb.m <= sp.namps && (d.rfloor0 + qint) < sp.Nhalf;
+}
+
+// Per-trial FP64 evaluation for a trial step that is not a small rational
+// (_fill_row_direct_slow!, directinterp.jl:623-661).
+__device__ double2 interp_slow(const HarmDev& d, const TabDev& tb, const SpecDev& sp,
+                               int h, int64_t t, int64_t first, int64_t last) {
+    const int m = tb.m, m2 = m / 2;
+    double hr_lo = (double)d.rfloor0 + d.dr0;
+    double dh = (double)h * tb.lodr;
+    double rfirst = hr_lo + (double)first * dh;
+    double rlast = hr_lo + (double)last * dh;
+    int64_t lo = (int64_t)floor(rfirst + 1e-15) + 2 - m2;
+    int64_t hi = (int64_t)floor(rlast + 1e-15) + 1 + m2;
+    if (!(lo >= 1 && hi <= sp.namps && (int64_t)floor(rlast) < sp.Nhalf))
+        return make_double2(0.0, 0.0);
+    double r = hr_lo + (double)t * dh;
+    int64_t rint = (int64_t)floor(r + 1e-15) + 1;
+    double dr = r - floor(r);
+    int64_t base = rint - m2;
+    if (dr == 0.0) {
+        float2 a = sp.amps[base + m2 - 1];
+        return make_double2(a.x, a.y);
+    }
+    double sre = 0.0, sim = 0.0;
+    for (int i = 1; i <= m; ++i) {
+        double w = 1.0 / (dr - (double)(i - m2));
+        float2 a = sp.amps[base + i - 1];
+        sre = fma(w, (double)a.x, sre);
+        sim = fma(w, (double)a.y, sim);
+    }
+    double s = sinpi(dr), c = cospi(dr);
+    double ar = (s * c) / M_PI, ai = -((s * s) / M_PI);
+    return make_double2(ar * sre - ai * sim, ar * sim + ai * sre);
+}
+
+// One interpolated amplitude (harmonic h, global trial t of block [first, last]).
+// FP32 weights: an m-term FP32 FMA chain, then the complex FP32 product with
+// conj(A) evaluated without contraction, exactly as the reference stores
+// `gA[k, g] * complex(ar[k], ai[k])`.
+template <int MT>
+__device__ __forceinline__ double2 interp_one(const SpecDev& sp, const TabDev& tb, int h,
+                                              int64_t t, int64_t first, int64_t last,
+                                              bool guard_known, bool guard_ok) {
+    const HarmDev d = tb.harm[h - 1];
+    if (d.P == 0) return interp_slow(d, tb, sp, h, t, first, last);
+    bool ok = guard_known ? guard_ok : row_guard(d, tb, sp, first, last);
+    if (!ok) return make_double2(0.0, 0.0);
+    uint32_t p;
+    int64_t b = bin0(d, tb, t, p);
+    const int m = MT > 0 ? MT : tb.m;
+    const float2* x = sp.amps + b;
+    if (!tb.weights_f64) {
+        const float* w = tb.W32 + (size_t)(d.tab_off + p) * m;
+        float ar = 0.0f, ai = 0.0f;
+#pragma unroll
+        for (int i = 0; i < (MT > 0 ? MT : 1); ++i) {
+            if (MT > 0) {
+                float2 v = x[i];
+                ar = __builtin_fmaf(w[i], v.x, ar);
+                ai = __builtin_fmaf(w[i], v.y, ai);
+            }
+        }
+        if (MT == 0) {
+            for (int i = 0; i < m; ++i) {
+                float2 v = x[i];
+                ar = __builtin_fmaf(w[i], v.x, ar);
+                ai = __builtin_fmaf(w[i], v.y, ai);
+            }
+        }
+        float2 A = tb.A32[d.tab_off + p];
+        float p1 = A.x * ar, p2 = A.y * ai, p3 = A.x * ai, p4 = A.y * ar;
+        return make_double2((double)(p1 - p2), (double)(p3 + p4));
+    } else {
+        const double* w = tb.W64 + (size_t)(d.tab_off + p) * m;
+        double ar = 0.0, ai = 0.0;
+        for (int i = 0; i < m; ++i) {
+            float2 v = x[i];
+            ar = fma(w[i], (double)v.x, ar);
+            ai = fma(w[i], (double)v.y, ai);
+        }
+        double2 A = tb.A64[d.tab_off + p];
+        return make_double2(A.x * ar - A.y * ai, A.x * ai + A.y * ar);
+    }
+}
+
+__device__ __forceinline__ void block_of(const GridDev& g, int64_t t, int64_t& cg,
+                                         int64_t& first, int64_t& last) {
+    cg = (t - g.origin) / g.Nprof;
+    first = g.origin + cg * g.Nprof;
+    last = min(first + g.Nprof, g.total) - 1;
+}
+
+// K1: plain FP32 (or FP64-weight bridge) interpolation, one thread per
+// (trial, harmonic); output row h-1 of the stack, columns = batch trials.
+template <int MT, bool OUT64>
+__global__ __launch_bounds__(256) void k_interp(SpecDev sp, TabDev tb, GridDev g,
+                                                int64_t t_base, int64_t nt, void* out,
+                                                int64_t ld) {
+    const int h = blockIdx.y + 1;
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    int64_t t = t_base + j;
+    int64_t cg, first, last;
+    block_of(g, t, cg, first, last);
+    double2 v = interp_one<MT>(sp, tb, h, t, first, last, false, false);
+    if (OUT64) {
+        ((double2*)out)[(int64_t)(h - 1) * ld + j] = v;
+    } else {
+        ((float2*)out)[(int64_t)(h - 1) * ld + j] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+hipError_t launch_interp(hipStream_t st, const SpecDev& sp, const TabDev& tb,
+                         const GridDev& g, int64_t t_base, int64_t nt, bool out64,
+                         void* out, int64_t ld) {
+    if (nt <= 0) return hipSuccess;
+    dim3 grid((unsigned)((nt + 255) / 256), (unsigned)tb.nharms);
+    bool f32 = !tb.weights_f64;
+    if (out64) {
+        if (f32 && tb.m == 16) hipLaunchKernelGGL((k_interp<16, true>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
+        else if (f32 && tb.m == 32) hipLaunchKernelGGL((k_interp<32, true>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
+        else hipLaunchKernelGGL((k_interp<0, true>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
+    } else {
+        if (f32 && tb.m == 16) hipLaunchKernelGGL((k_interp<16, false>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
+        else if (f32 && tb.m == 32) hipLaunchKernelGGL((k_interp<32, false>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
+        else hipLaunchKernelGGL((k_interp<0, false>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// K2: c2r of one fold, one profile per thread.  Rows h*k (h = 1..H) of the
+// stack; DC (row 0) is zero by construction; the Nyquist imaginary part is
+// ignored (numpy convention).  Generated straight-line FFTs for the common
+// lengths, an O(H^2) fallback otherwise.
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(256) void k_c2r(const float2* __restrict__ ftp, int64_t ldf,
+                                             int k, int64_t nt, float* __restrict__ prof,
+                                             int64_t ldp) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    constexpr int H = NB / 2;
+    float xr[H + 1], xi[H + 1];
+    xr[0] = 0.0f;
+    xi[0] = 0.0f;
+#pragma unroll
+    for (int h = 1; h <= H; ++h) {
+        float2 v = ftp[(int64_t)(h * k - 1) * ldf + j];
+        xr[h] = v.x;
+        xi[h] = v.y;
+    }
+    float out[NB];
+    cs_c2r::Fixed<NB>::run(xr, xi, out);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) prof[(int64_t)i * ldp + j] = out[i];
+}
+
+__global__ __launch_bounds__(256) void k_c2r_generic(const float2* __restrict__ ftp,
+                                                     int64_t ldf, int k, int NB, int64_t nt,
+                                                     float* __restrict__ prof, int64_t ldp,
+                                                     const double2* __restrict__ twid) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    const int H = NB / 2;
+    for (int n = 0; n <= H; ++n) {
+        float E = 0.0f, O = 0.0f;
+        for (int h = 1; h <= H; ++h) {
+            float2 v = ftp[(int64_t)(h * k - 1) * ldf + j];
+            float c, s;
+            if (h == H) { c = (n & 1) ? -1.0f : 1.0f; s = 0.0f; }
+            else {
+                double2 w = twid[(h * n) % NB];
+                c = (float)(2.0 * w.x);
+                s = (float)(-2.0 * w.y);
+            }
+            E = __builtin_fmaf(c, v.x, E);
+            O = __builtin_fmaf(s, v.y, O);
+        }
+        if (n == 0 || n == H) prof[(int64_t)n * ldp + j] = E;
+        else {
+            prof[(int64_t)n * ldp + j] = E + O;
+            prof[(int64_t)(NB - n) * ldp + j] = E - O;
+        }
+    }
+}
+
+hipError_t launch_c2r_f32(hipStream_t st, const float2* ftp, int64_t ldf, int k,
+                          int nbins, int64_t nt, float* prof, int64_t ldp,
+                          const double2* twid) {
+    if (nt <= 0) return hipSuccess;
+    dim3 grid((unsigned)((nt + 255) / 256));
+    switch (nbins) {
+#define X(S)                                                                              \
+    case S:                                                                               \
+        hipLaunchKernelGGL(k_c2r<S>, grid, dim3(256), 0, st, ftp, ldf, k, nt, prof, ldp); \
+        break;
+        CS_C2R_SIZES(X)
+#undef X
+        default:
+            hipLaunchKernelGGL(k_c2r_generic, grid, dim3(256), 0, st, ftp, ldf, k, nbins, nt,
+                               prof, ldp, twid);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// K3: block sigma (_block_sigma, search.jl:490-566, center = :zero):
+// 1.4826 * median(|x|) over the reference's whole-column subsample of the
+// block's valid profiles, by exact radix selection on the FP32 bit patterns
+// (|x| >= 0, so the unsigned order of the bits is the numeric order).
+// One 512-thread workgroup per (block, fold); up to 8192 samples in registers.
+// ---------------------------------------------------------------------------
+constexpr int kSelThreads = 512;
+constexpr int kSelPer = 16;   // 512 * 16 = 8192 = _BOXCAR_SIGMA_SAMPLES
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* sh, uint32_t& total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < kSelThreads / 64; ++w) {
+            uint32_t tmp = sh[w];
+            sh[w] = acc;
+            acc += tmp;
+        }
+        sh[kSelThreads / 64] = acc;
+    }
+    __syncthreads();
+    uint32_t res = x - v + sh[wid];
+    total = sh[kSelThreads / 64];
+    __syncthreads();
+    return res;
+}
+
+// kth (0-based) smallest key of the block.
+__device__ uint32_t block_select(const uint32_t (&keys)[kSelPer], uint32_t kth, uint32_t* hist,
+                                 uint32_t* sh) {
+    uint32_t prefix = 0, mask = 0;
+    const int shifts[3] = {21, 10, 0};
+    const int bits[3] = {11, 11, 10};
+    for (int pass = 0; pass < 3; ++pass) {
+        const int sh_ = shifts[pass];
+        const uint32_t dmask = (1u << bits[pass]) - 1u;
+        for (int i = threadIdx.x; i < 2048; i += kSelThreads) hist[i] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSelPer; ++r) {
+            uint32_t key = keys[r];
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> sh_) & dmask], 1u);
+        }
+        __syncthreads();
+        uint32_t a[4], loc = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { a[q] = hist[4 * threadIdx.x + q]; loc += a[q]; }
+        uint32_t tot;
+        uint32_t ex = block_excl_scan(loc, sh + 16, tot);
+        if (ex <= kth && kth < ex + loc) {
+            uint32_t c = ex;
+            int d = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (kth >= c && kth < c + a[q]) { d = q; break; }
+                c += a[q];
+            }
+            sh[0] = 4 * threadIdx.x + d;
+            sh[1] = kth - c;
+        }
+        __syncthreads();
+        uint32_t dsel = sh[0];
+        kth = sh[1];
+        prefix |= dsel << sh_;
+        mask |= dmask << sh_;
+        __syncthreads();
+    }
+    return prefix;
+}
+
+__global__ __launch_bounds__(kSelThreads) void k_sigma32(FoldSet fs, GridDev g, BatchDev b,
+                                                         int64_t ldp) {
+    __shared__ uint32_t hist[2048];
+    __shared__ uint32_t sh[64];
+    const int cl = blockIdx.x, f = blockIdx.y, nf = fs.nf;
+    const int64_t cg = b.c_first + cl;
+    const int64_t first = g.origin + cg * g.Nprof;
+    const int64_t nv = b.nvalid[cl * nf + f];
+    const int nbins = fs.f[f].nbins;
+    const int64_t cap = fs.cap[f];
+    const int64_t NN = (int64_t)nbins * nv;
+    if (NN == 0) {
+        if (threadIdx.x == 0) b.sigma[cl * nf + f] = 0.0;
+        return;
+    }
+    const bool all = NN <= cap;
+    int64_t ncol, ns;
+    if (all) { ncol = nv; ns = NN; }
+    else {
+        ncol = cap / nbins;
+        if (ncol < 1) ncol = 1;
+        if (ncol > nv) ncol = nv;
+        ns = ncol * nbins < cap ? ncol * nbins : cap;
+    }
+    const float* P = fs.prof[f];
+    const int64_t col0 = first - b.t_base;
+    uint32_t keys[kSelPer];
+#pragma unroll
+    for (int r = 0; r < kSelPer; ++r) {
+        int64_t s = threadIdx.x + (int64_t)kSelThreads * r;
+        uint32_t key = 0xFFFFFFFFu;
+        if (s < ns) {
+            int64_t cc = s / nbins, i = s - cc * nbins;
+            int64_t j = all ? cc : (cc * nv) / ncol;
+            float v = P[i * ldp + col0 + j];
+            key = __float_as_uint(fabsf(v));
+        }
+        keys[r] = key;
+    }
+    uint32_t half = (uint32_t)(ns / 2);
+    uint32_t up = block_select(keys, half, hist, sh);
+    double med;
+    if (ns & 1) med = (double)__uint_as_float(up);
+    else {
+        uint32_t lo = block_select(keys, half - 1, hist, sh);
+        med = 0.5 * ((double)__uint_as_float(lo) + (double)__uint_as_float(up));
+    }
+    if (threadIdx.x == 0) b.sigma[cl * nf + f] = 1.4826 * med;
+}
+
+hipError_t launch_sigma_f32(hipStream_t st, const FoldSet& fs, const GridDev& g,
+                            const BatchDev& b) {
+    if (b.nblocks <= 0) return hipSuccess;
+    dim3 grid((unsigned)b.nblocks, (unsigned)fs.nf);
+    hipLaunchKernelGGL(k_sigma32, grid, dim3(kSelThreads), 0, st, fs, g, b, b.nt > 0 ? 0 : 0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// K4: zero-baseline boxcar gate in FP32 (_bc_scan_batch!, search.jl:743-779,
+// same per-profile operation order), then wave-aggregated compaction of the
+// (trial, fold) pairs whose bound reaches medcut into the rescan queue.
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int bank_w(int a) {
+    return a == 0 ? 1 : a == 1 ? 2 : a == 2 ? 3 : a == 3 ? 4 : a == 4 ? 6 : a == 5 ? 9
+         : a == 6 ? 13 : a == 7 ? 19 : a == 8 ? 28 : a == 9 ? 42 : a == 10 ? 63 : 94;
+}
+__host__ __device__ constexpr int bank_count(int lim) {
+    int n = 0;
+    for (int a = 0; a < 12; ++a) if (bank_w(a) <= lim) n = a + 1;
+    return n;
+}
+
+template <int NB>
+struct GateCT {
+    static constexpr int lim = (3 * NB) / 10 < 1 ? 1 : (3 * NB) / 10;
+    static constexpr int nbank = bank_count(lim);
+    static constexpr int wmax = bank_w(nbank - 1);
+};
+
+__device__ __forceinline__ void push_survivor(bool pred, int64_t t, int fidx, const OutDev& o) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long bal = __ballot(pred);
+    if (bal == 0) return;
+    int leader = __ffsll((long long)bal) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(o.surv_count, (uint32_t)__popcll(bal));
+    base = (uint32_t)__shfl((int)base, leader, 64);
+    if (pred) {
+        uint32_t idx = base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (idx < o.surv_cap) o.surv[idx] = ((uint64_t)t << 4) | (uint64_t)fidx;
+    }
+}
+
+template <int NB>
+__global__ __launch_bounds__(256) void k_gate(const float* __restrict__ prof, int64_t ldp,
+                                              int fidx, uint32_t bankmask, GridDev g,
+                                              BatchDev b, int nf, OutDev o) {
+    constexpr int W = GateCT<NB>::wmax;
+    constexpr int NBK = GateCT<NB>::nbank;
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool pred = false;
+    int64_t t = b.t_base + j;
+    if (j < b.nt) {
+        int64_t cg, first, last;
+        block_of(g, t, cg, first, last);
+        int cl = (int)(cg - b.c_first);
+        if (t - first < b.nvalid[cl * nf + fidx]) {
+            double sig = b.sigma[cl * nf + fidx];
+            double invs = sig > 0 ? 1.0 / sig : 0.0;
+            if (!(o.medcut > -INFINITY) || invs <= 0) {
+                pred = true;
+            } else {
+                float x[NB];
+#pragma unroll
+                for (int i = 0; i < NB; ++i) x[i] = prof[(int64_t)i * ldp + j];
+                float ps[NB + W + 1];
+                ps[0] = 0.0f;
+#pragma unroll
+                for (int i = 1; i <= NB + W; ++i) ps[i] = ps[i - 1] + x[(i - 1) % NB];
+                const float invsig_t = (float)invs;
+                float res = -INFINITY;
+#pragma unroll
+                for (int a = 0; a < NBK; ++a) {
+                    const int w = bank_w(a);
+                    if (bankmask & (1u << a)) {
+                        float invsw = invsig_t / sqrtf((float)w);
+                        float mb = ps[w] - ps[0];
+#pragma unroll
+                        for (int p = 1; p < NB; ++p) {
+                            float d = ps[p + w] - ps[p];
+                            mb = d > mb ? d : mb;
+                        }
+                        float c = mb * invsw;
+                        res = c > res ? c : res;
+                    }
+                }
+                pred = (double)res >= o.medcut;
+            }
+        }
+    }
+    push_survivor(pred, t, fidx, o);
+}
+
+// Generic gate: runtime length and bank, prefix sums transposed in LDS (one
+// column per lane, as the reference's batched tile), 64 profiles per group.
+__global__ __launch_bounds__(64) void k_gate_generic(const float* __restrict__ prof,
+                                                     int64_t ldp, int fidx, FoldDev fd,
+                                                     GridDev g, BatchDev b, int nf, OutDev o) {
+    extern __shared__ float psT[];
+    const int NB = fd.nbins;
+    const int wmax = fd.widths[fd.nw - 1];
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    bool pred = false;
+    int64_t t = b.t_base + j;
+    const int L = threadIdx.x;
+    if (j < b.nt) {
+        int64_t cg, first, last;
+        block_of(g, t, cg, first, last);
+        int cl = (int)(cg - b.c_first);
+        if (t - first < b.nvalid[cl * nf + fidx]) {
+            double sig = b.sigma[cl * nf + fidx];
+            double invs = sig > 0 ? 1.0 / sig : 0.0;
+            if (!(o.medcut > -INFINITY) || invs <= 0) {
+                pred = true;
+            } else {
+                psT[L] = 0.0f;
+                for (int i = 1; i <= NB + wmax; ++i) {
+                    int idx = i > NB ? i - NB : i;
+                    psT[i * 64 + L] = psT[(i - 1) * 64 + L] + prof[(int64_t)(idx - 1) * ldp + j];
+                }
+                const float invsig_t = (float)invs;
+                float res = -INFINITY;
+                for (int a = 0; a < fd.nw; ++a) {
+                    int w = fd.widths[a];
+                    float invsw = invsig_t / sqrtf((float)w);
+                    float mb = psT[w * 64 + L] - psT[L];
+                    for (int p = 1; p < NB; ++p) {
+                        float d = psT[(p + w) * 64 + L] - psT[p * 64 + L];
+                        mb = d > mb ? d : mb;
+                    }
+                    float c = mb * invsw;
+                    res = c > res ? c : res;
+                }
+                pred = (double)res >= o.medcut;
+            }
+        }
+    }
+    push_survivor(pred, t, fidx, o);
+}
+
+hipError_t launch_gate_f32(hipStream_t st, const FoldSet& fs, const GridDev& g,
+                           const BatchDev& b, const OutDev& o) {
+    if (b.nt <= 0) return hipSuccess;
+    // leading dimension of the fold profiles = batch column capacity, passed via
+    // the first fold's cap slot is not available: profiles are laid out with
+    // ld = o.surv_cap / nf (the batch capacity); see cs_api.cpp.
+    return hipErrorInvalidValue;
+}
+
+// Gate launcher with an explicit leading dimension (used by cs_api.cpp).
+hipError_t launch_gate_f32_ld(hipStream_t st, const FoldSet& fs, const GridDev& g,
+                              const BatchDev& b, const OutDev& o, int64_t ldp) {
+    if (b.nt <= 0) return hipSuccess;
+    dim3 grid((unsigned)((b.nt + 255) / 256));
+    for (int f = 0; f < fs.nf; ++f) {
+        const FoldDev& fd = fs.f[f];
+        bool fast = fs.bankmask[f] != 0;
+        bool launched = false;
+        if (fast) {
+            switch (fd.nbins) {
+#define X(S)                                                                                 \
+    case S:                                                                                  \
+        if (fd.widths[fd.nw - 1] <= GateCT<S>::wmax) {                                       \
+            hipLaunchKernelGGL(k_gate<S>, grid, dim3(256), 0, st, fs.prof[f], ldp, f,          \
+                               fs.bankmask[f], g, b, fs.nf, o);                               \
+            launched = true;                                                                 \
+        }                                                                                    \
+        break;
+                CS_C2R_SIZES(X)
+#undef X
+                default:
+                    break;
+            }
+        }
+        if (!launched) {
+            int wmax = fd.widths[fd.nw - 1];
+            size_t shm = (size_t)(fd.nbins + wmax + 1) * 64 * sizeof(float);
+            dim3 g64((unsigned)((b.nt + 63) / 64));
+            hipLaunchKernelGGL(k_gate_generic, g64, dim3(64), shm, st, fs.prof[f], ldp, f, fd, g,
+                               b, fs.nf, o);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// Exact metric of one FP64 profile held in LDS by one 64-lane group
+// (_boxcar_exact, search.jl:893-904): exact median baseline (rank counting;
+// the same order statistics quickselect or a sorting network returns), the
+// wrapped prefix sum in the reference's sequential order, then the
+// width x phase maximum.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        double u = __shfl_xor(v, o, 64);
+        v = u > v ? u : v;
+    }
+    return v;
+}
+
+__device__ double group_exact_metric(const double* prof, int NB, const int32_t* widths,
+                                     int nw, double invs, double* srt, double* ps) {
+    const int lane = threadIdx.x;
+    for (int i = lane; i < NB; i += 64) {
+        double v = prof[i];
+        int r = 0;
+        for (int q = 0; q < NB; ++q) {
+            double u = prof[q];
+            r += (u < v) || (u == v && q < i);
+        }
+        srt[r] = v;
+    }
+    __syncthreads();
+    double med = (NB & 1) ? srt[NB / 2] : 0.5 * (srt[NB / 2 - 1] + srt[NB / 2]);
+    const int wmax = widths[nw - 1];
+    if (lane == 0) {
+        ps[0] = 0.0;
+        for (int i = 1; i <= NB + wmax; ++i) {
+            int idx = i > NB ? i - NB : i;
+            ps[i] = ps[i - 1] + (prof[idx - 1] - med);
+        }
+    }
+    __syncthreads();
+    double best = -INFINITY;
+    for (int a = 0; a < nw; ++a) {
+        int w = widths[a];
+        double invsw = invs / sqrt((double)w);
+        double m = -INFINITY;
+        for (int p = lane; p < NB; p += 64) {
+            double d = ps[p + w] - ps[p];
+            m = d > m ? d : m;
+        }
+        m = wave_max(m);
+        double cand = m * invsw;
+        if (cand > best) best = cand;
+    }
+    __syncthreads();
+    return best;
+}
+
+// FP64 real DFT of X[0..H] (DC and Nyquist real parts only) into prof[0..NB-1],
+// unnormalised, even/odd split x[n] = E + O, x[NB-n] = E - O.
+__device__ void group_c2r_f64(const double2* X, int NB, const double2* twid, double* prof) {
+    const int H = NB / 2;
+    for (int n = threadIdx.x; n <= H; n += 64) {
+        double E = 0.0, O = 0.0;
+        for (int h = 0; h <= H; ++h) {
+            double c, s;
+            if (h == 0) { c = 1.0; s = 0.0; }
+            else if (h == H) { c = (n & 1) ? -1.0 : 1.0; s = 0.0; }
+            else {
+                double2 w = twid[(h * n) % NB];
+                c = 2.0 * w.x;
+                s = -(2.0 * w.y);
+            }
+            E = fma(c, X[h].x, E);
+            O = fma(s, X[h].y, O);
+        }
+        if (n == 0 || n == H) prof[n] = E;
+        else {
+            prof[n] = E + O;
+            prof[NB - n] = E - O;
+        }
+    }
+    __syncthreads();
+}
+
+constexpr int kMaxNB = 128;
+
+// K5: exact rescan of the queued (trial, fold) pairs.  Each 64-lane group
+// rebuilds its trial's fold stack from the spectrum (the same FP32-weight
+// arithmetic as K1), transforms it in FP64 and scores it exactly; candidates
+// above threshold are appended (the host sorts them by (t, k)).
+__global__ __launch_bounds__(64) void k_rescan(SpecDev sp, TabDev tb, GridDev g, FoldSet fs,
+                                               BatchDev b, OutDev o) {
+    __shared__ double2 X[kMaxNB / 2 + 1];
+    __shared__ double prof[kMaxNB];
+    __shared__ double srt[kMaxNB];
+    __shared__ double ps[2 * kMaxNB + 2];
+    const int lane = threadIdx.x;
+    uint32_t nsurv = *o.surv_count;
+    if (nsurv > o.surv_cap) nsurv = o.surv_cap;
+    for (uint32_t i = blockIdx.x; i < nsurv; i += gridDim.x) {
+        uint64_t pk = o.surv[i];
+        int64_t t = (int64_t)(pk >> 4);
+        int f = (int)(pk & 15u);
+        const FoldDev& fd = fs.f[f];
+        const int k = fd.k, Hk = fd.Hk, NB = fd.nbins;
+        int64_t cg, first, last;
+        block_of(g, t, cg, first, last);
+        int cl = (int)(cg - b.c_first);
+        double sig = b.sigma[cl * fs.nf + f];
+        double invs = sig > 0 ? 1.0 / sig : 0.0;
+        double metric = 0.0;
+        if (invs > 0) {
+            for (int l = lane; l <= Hk; l += 64) {
+                double2 v = make_double2(0.0, 0.0);
+                if (l >= 1) {
+                    if (tb.m == 16 && !tb.weights_f64) v = interp_one<16>(sp, tb, l * k, t, first, last, false, false);
+                    else if (tb.m == 32 && !tb.weights_f64) v = interp_one<32>(sp, tb, l * k, t, first, last, false, false);
+                    else v = interp_one<0>(sp, tb, l * k, t, first, last, false, false);
+                }
+                X[l] = v;
+            }
+            __syncthreads();
+            group_c2r_f64(X, NB, fs.twid[f], prof);
+            metric = group_exact_metric(prof, NB, fd.widths, fd.nw, invs, srt, ps);
+        }
+        if (lane == 0) {
+            if (o.mout) {
+                o.mout[t - b.t_base] = metric;
+            } else if (metric > o.threshold) {
+                uint32_t idx = atomicAdd(o.cand_count, 1u);
+                if (idx < o.cand_cap) {
+                    cs_cand_dev c;
+                    c.t = t; c.k = k; c.nharm = Hk; c.block = cg; c.metric = metric;
+                    o.cands[idx] = c;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
+                         const GridDev& g, const FoldSet& fs, const BatchDev& b,
+                         const OutDev& o, int nblocks_grid) {
+    hipLaunchKernelGGL(k_rescan, dim3((unsigned)nblocks_grid), dim3(64), 0, st, sp, tb, g, fs,
+                       b, o);
+    return hipGetLastError();
+}
+
+__global__ void k_reset_u32(uint32_t* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+hipError_t launch_reset_u32(hipStream_t st, uint32_t* p, int n) {
+    hipLaunchKernelGGL(k_reset_u32, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, p, n);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// FP64 bridge path: chunk_metrics / decim_pass! / block_metrics.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_c2r_f64(const double2* __restrict__ ftp, int64_t ldf,
+                                                 int k, int NB, int64_t n,
+                                                 double* __restrict__ prof, int64_t ldp,
+                                                 const double2* __restrict__ twid,
+                                                 double scale) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int H = NB / 2;
+    for (int nn = 0; nn <= H; ++nn) {
+        double E = 0.0, O = 0.0;
+        for (int h = 1; h <= H; ++h) {
+            double2 v = ftp[(int64_t)(h * k - 1) * ldf + j];
+            double c, s;
+            if (h == H) { c = (nn & 1) ? -1.0 : 1.0; s = 0.0; }
+            else {
+                double2 w = twid[(h * nn) % NB];
+                c = 2.0 * w.x;
+                s = -(2.0 * w.y);
+            }
+            E = fma(c, v.x, E);
+            O = fma(s, v.y, O);
+        }
+        double a, bb;
+        if (nn == 0 || nn == H) { a = E; bb = E; }
+        else { a = E + O; bb = E - O; }
+        if (scale != 1.0) { a *= scale; bb *= scale; }
+        prof[(int64_t)nn * ldp + j] = a;
+        if (nn != 0 && nn != H) prof[(int64_t)(NB - nn) * ldp + j] = bb;
+    }
+}
+
+hipError_t launch_c2r_f64(hipStream_t st, const double2* ftp, int64_t ldf, int k, int nbins,
+                          int64_t n, double* prof, int64_t ldp, const double2* twid,
+                          double scale) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_c2r_f64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ftp, ldf,
+                       k, nbins, n, prof, ldp, twid, scale);
+    return hipGetLastError();
+}
+
+// Exact median of |x| over the subsample, by a bitonic sort of up to 8192
+// doubles in LDS (one workgroup: the bridge handles one block at a time).
+__global__ __launch_bounds__(1024) void k_sigma64(const double* __restrict__ prof, int64_t ldp,
+                                                  int nbins, int64_t n, int64_t cap,
+                                                  double* sigma_out) {
+    __shared__ double v[8192];
+    const int64_t NN = (int64_t)nbins * n;
+    if (NN == 0) {
+        if (threadIdx.x == 0) *sigma_out = 0.0;
+        return;
+    }
+    const bool all = NN <= cap;
+    int64_t ncol, ns;
+    if (all) { ncol = n; ns = NN; }
+    else {
+        ncol = cap / nbins;
+        if (ncol < 1) ncol = 1;
+        if (ncol > n) ncol = n;
+        ns = ncol * nbins < cap ? ncol * nbins : cap;
+    }
+    int S = 1;
+    while (S < ns) S <<= 1;
+    for (int s = threadIdx.x; s < S; s += blockDim.x) {
+        double x = INFINITY;
+        if (s < ns) {
+            int64_t cc = s / nbins, i = s - cc * nbins;
+            int64_t j = all ? cc : (cc * n) / ncol;
+            x = fabs(prof[i * ldp + j]);
+        }
+        v[s] = x;
+    }
+    __syncthreads();
+    for (int kk = 2; kk <= S; kk <<= 1) {
+        for (int jj = kk >> 1; jj > 0; jj >>= 1) {
+            for (int i = threadIdx.x; i < S; i += blockDim.x) {
+                int ixj = i ^ jj;
+                if (ixj > i) {
+                    bool up = (i & kk) == 0;
+                    double a = v[i], c = v[ixj];
+                    if ((a > c) == up) { v[i] = c; v[ixj] = a; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (threadIdx.x == 0) {
+        int64_t half = ns / 2;
+        double med = (ns & 1) ? v[half] : 0.5 * (v[half - 1] + v[half]);
+        *sigma_out = 1.4826 * med;
+    }
+}
+
+hipError_t launch_sigma_f64(hipStream_t st, const double* prof, int64_t ldp, int nbins, int64_t n,
+                            int64_t cap, double* sigma_out) {
+    hipLaunchKernelGGL(k_sigma64, dim3(1), dim3(1024), 0, st, prof, ldp, nbins, n, cap, sigma_out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(64) void k_exact_f64(const double* __restrict__ profs, int64_t ldp,
+                                                  int NB, int64_t n, const int32_t* widths, int nw,
+                                                  const double* sigma, double* out) {
+    __shared__ double prof[kMaxNB];
+    __shared__ double srt[kMaxNB];
+    __shared__ double ps[2 * kMaxNB + 2];
+    __shared__ int32_t wsh[64];
+    for (int a = threadIdx.x; a < nw; a += 64) wsh[a] = widths[a];
+    __syncthreads();
+    double sig = *sigma;
+    double invs = sig > 0 ? 1.0 / sig : 0.0;
+    for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
+        double metric = 0.0;
+        if (invs > 0) {
+            for (int i = threadIdx.x; i < NB; i += 64) prof[i] = profs[(int64_t)i * ldp + j];
+            __syncthreads();
+            metric = group_exact_metric(prof, NB, wsh, nw, invs, srt, ps);
+        }
+        if (threadIdx.x == 0) out[j] = metric;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_exact_f64(hipStream_t st, const double* prof, int64_t ldp, int nbins, int64_t n,
+                            const int32_t* widths, int nw, const double* sigma, double* out) {
+    if (n <= 0) return hipSuccess;
+    int64_t gsz = n < 65536 ? n : 65536;
+    hipLaunchKernelGGL(k_exact_f64, dim3((unsigned)gsz), dim3(64), 0, st, prof, ldp, nbins, n, widths,
+                       nw, sigma, out);
+    return hipGetLastError();
+}
+
+// fourier_interp (fourierinterp.jl:66-74) point by point in FP64: coefficients
+// sinc(dr-j) * cispi(dr-j), j = -m/2+1..m/2, applied as sum conj(c) * bin.
+__device__ double sinc_d(double x) {
+    if (x == 0.0) return 1.0;
+    if (fabs(x) < 4.5 * 1.4901161193847656e-08) {
+        double x2 = x * x, p2 = M_PI * M_PI;
+        return 1.0 + x2 * (-p2 / 6.0 + x2 * (p2 * p2 / 120.0));
+    }
+    return sinpi(x) / (M_PI * x);
+}
+
+__global__ __launch_bounds__(256) void k_point_interp(SpecDev sp, const double* __restrict__ rfund,
+                                                      int64_t L, int m, const uint8_t* rowok,
+                                                      double2* out, int64_t ld) {
+    const int h = blockIdx.y + 1;
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= L) return;
+    double2 res = make_double2(0.0, 0.0);
+    if (rowok[h - 1]) {
+        double r = rfund[j] * (double)h;
+        double dr = r - floor(r);
+        int64_t lo0 = (int64_t)floor(r + 1e-15) + 1 - m / 2;
+        if (lo0 >= 0 && lo0 + m <= sp.namps) {
+            double sr = 0.0, si = 0.0;
+            for (int i = 0; i < m; ++i) {
+                double off = dr - (double)(-(m / 2) + 1 + i);
+                double s = sinc_d(off);
+                double cr = s * cospi(off), ci = s * sinpi(off);
+                float2 bv = sp.amps[lo0 + i];
+                sr += cr * (double)bv.x + ci * (double)bv.y;
+                si += cr * (double)bv.y - ci * (double)bv.x;
+            }
+            res = make_double2(sr, si);
+        } else {
+            res = make_double2(NAN, NAN);
+        }
+    }
+    out[(int64_t)(h - 1) * ld + j] = res;
+}
+
+hipError_t launch_point_interp_f64(hipStream_t st, const SpecDev& sp, const double* rfund,
+                                   int64_t L, int nharms, int m, const uint8_t* rowok,
+                                   double2* out, int64_t ld) {
+    if (L <= 0) return hipSuccess;
+    dim3 grid((unsigned)((L + 255) / 256), (unsigned)nharms);
+    hipLaunchKernelGGL(k_point_interp, grid, dim3(256), 0, st, sp, rfund, L, m, rowok, out, ld);
+    return hipGetLastError();
+}
+
+}  // namespace cs

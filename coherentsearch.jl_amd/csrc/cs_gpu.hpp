@@ -1,0 +1,114 @@
+// Device-side structures and kernel launchers of the coherent-search engine.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "cs_plan.hpp"
+
+namespace cs {
+
+// The spectrum as the kernels see it.
+struct SpecDev {
+    const float2* amps;
+    int64_t namps;
+    int64_t N;
+    int64_t Nhalf;
+};
+
+// Interpolation tables on the device (see HarmDev).
+struct TabDev {
+    const HarmDev* harm;
+    const int32_t* row;
+    const uint8_t* carry;
+    const float* W32;
+    const float2* A32;
+    const double* W64;
+    const double2* A64;
+    double lodr;
+    int32_t m;
+    int32_t nharms;
+    int32_t weights_f64;
+    int32_t pad;
+};
+
+// Block geometry: block c spans global trials [origin + c*Nprof,
+// min(origin + (c+1)*Nprof, total)).
+struct GridDev {
+    int64_t origin;
+    int64_t Nprof;
+    int64_t total;
+};
+
+// The folds searched and where the current batch keeps their profiles.
+struct FoldSet {
+    int32_t nf;
+    int32_t pad;
+    FoldDev f[kMaxFolds];
+    float* prof[kMaxFolds];          // FP32 profiles, [bin][batch trial]
+    const double2* twid[kMaxFolds];  // (cos, sin)(2 pi e / nbins), e < nbins
+    int32_t cap[kMaxFolds];          // sigma sample-buffer length per fold
+    uint32_t bankmask[kMaxFolds];    // widths as a subset of the default bank
+};
+
+// Where a batch sits in the search.
+struct BatchDev {
+    int64_t t_base;     // global trial of batch column 0
+    int64_t nt;         // trials in the batch
+    int64_t c_first;    // global block index of the batch's first block
+    int32_t nblocks;    // blocks in the batch
+    int32_t pad;
+    const int32_t* nvalid;   // [block_local * nf + f]
+    double* sigma;           // [block_local * nf + f]
+};
+
+struct OutDev {
+    uint64_t* surv;        // packed (t << 4) | fold
+    uint32_t* surv_count;
+    uint32_t surv_cap;
+    uint32_t cand_cap;
+    struct cs_cand_dev* cands;
+    uint32_t* cand_count;
+    double* mout;          // metrics mode: metric per batch trial (else null)
+    double threshold;
+    double medcut;
+};
+
+struct cs_cand_dev {
+    int64_t t;
+    int32_t k;
+    int32_t nharm;
+    int64_t block;
+    double metric;
+};
+
+// ---- launchers (all asynchronous on `st`) ------------------------------------
+hipError_t launch_interp(hipStream_t st, const SpecDev& sp, const TabDev& tb,
+                         const GridDev& g, int64_t t_base, int64_t nt, bool out64,
+                         void* out, int64_t ld);
+hipError_t launch_c2r_f32(hipStream_t st, const float2* ftp, int64_t ldf, int k,
+                          int nbins, int64_t nt, float* prof, int64_t ldp,
+                          const double2* twid);
+hipError_t launch_sigma_f32(hipStream_t st, const FoldSet& fs, const GridDev& g,
+                            const BatchDev& b);
+hipError_t launch_gate_f32(hipStream_t st, const FoldSet& fs, const GridDev& g,
+                           const BatchDev& b, const OutDev& o);
+hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
+                         const GridDev& g, const FoldSet& fs, const BatchDev& b,
+                         const OutDev& o, int nblocks_grid);
+hipError_t launch_reset_u32(hipStream_t st, uint32_t* p, int n);
+
+// FP64 bridge path (chunk_metrics / decim / block_metrics).
+hipError_t launch_c2r_f64(hipStream_t st, const double2* ftp, int64_t ldf, int k,
+                          int nbins, int64_t n, double* prof, int64_t ldp,
+                          const double2* twid, double scale);
+hipError_t launch_sigma_f64(hipStream_t st, const double* prof, int64_t ldp,
+                            int nbins, int64_t n, int64_t cap, double* sigma_out);
+hipError_t launch_exact_f64(hipStream_t st, const double* prof, int64_t ldp, int nbins,
+                            int64_t n, const int32_t* widths, int nw,
+                            const double* sigma, double* out);
+hipError_t launch_point_interp_f64(hipStream_t st, const SpecDev& sp, const double* rfund,
+                                   int64_t L, int nharms, int m, const uint8_t* rowok,
+                                   double2* out, int64_t ld);
+
+}  // namespace cs

@@ -1,0 +1,170 @@
+// Host-side planning: exact rational trial grid and per-harmonic tables.
+//
+// The tables are the reference's DirectPlan (directinterp.jl:179-204,
+// 345-405): for harmonic h the trial grid r_t = r_lo + t*pnum/q visits only
+// P = q/gcd(h*pnum, q) fractional offsets dr, so the m weights 1/(dr - j) and
+// the scalar conj(sin(pi dr) cispi(dr)/pi) are tabulated per phase.  Every
+// entry is computed in FP64 and rounded once to FP32 (the FP64 copy serves the
+// Float64-weight bridge).  The device evaluates each trial as one m-term FMA
+// chain against its phase's column; the reference's trials-axis group tables
+// (directinterp.jl:237-282) are a reordering of exactly those FMAs and need no
+// counterpart here.
+#include "cs_plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cfloat>
+#include <numeric>
+
+namespace cs {
+
+static double sinpi_d(double x) {
+    double r = std::fmod(x, 2.0);
+    if (r < 0) r += 2.0;
+    bool neg = false;
+    if (r >= 1.0) { r -= 1.0; neg = true; }
+    double v;
+    if (r <= 0.25) v = std::sin(M_PI * r);
+    else if (r <= 0.75) v = std::cos(M_PI * (r - 0.5));
+    else v = std::sin(M_PI * (1.0 - r));
+    return neg ? -v : v;
+}
+
+static double cospi_d(double x) {
+    double r = std::fabs(std::fmod(x, 2.0));
+    if (r > 1.0) r = 2.0 - r;
+    if (r <= 0.25) return std::cos(M_PI * r);
+    if (r <= 0.75) return -std::sin(M_PI * (r - 0.5));
+    return -std::cos(M_PI * (1.0 - r));
+}
+
+static int64_t floordiv(int64_t a, int64_t b) {
+    int64_t q = a / b;
+    if ((a % b) != 0 && ((a < 0) != (b < 0))) --q;
+    return q;
+}
+
+// Simplest rational within 1e-13 (continued-fraction convergents, as Julia's
+// rationalize), accepted only if q <= 2^16 and p/q reproduces lodr to 8 ulp
+// (trial_grid_rational, directinterp.jl:217-225).
+bool trial_grid_rational(double lodr, int64_t* pnum, int64_t* q) {
+    if (!(lodr > 0)) return false;
+    const double tol = 1e-13;
+    int64_t p0 = 0, q0 = 1, p1 = 1, q1 = 0, P = 0, Q = 1;
+    double y = lodr;
+    for (int it = 0; it < 64; ++it) {
+        double a = std::floor(y);
+        if (a > 4e18) break;
+        int64_t ai = (int64_t)a;
+        int64_t p2 = ai * p1 + p0, q2 = ai * q1 + q0;
+        if (q2 <= 0 || q2 > ((int64_t)1 << 40)) break;
+        p0 = p1; q0 = q1; p1 = p2; q1 = q2;
+        P = p1; Q = q1;
+        if (std::fabs((double)P / (double)Q - lodr) <= tol) break;
+        double fr = y - a;
+        if (fr == 0.0) break;
+        y = 1.0 / fr;
+    }
+    int64_t g = std::gcd(P, Q);
+    if (g > 1) { P /= g; Q /= g; }
+    if (!(Q <= kQmax && P >= 1)) return false;
+    double ulp = std::nextafter(lodr, INFINITY) - lodr;
+    if (!(std::fabs((double)P / (double)Q - lodr) <= 8.0 * ulp)) return false;
+    *pnum = P;
+    *q = Q;
+    return true;
+}
+
+Plans build_plans(int nharms, int m, double hidr, double r_lo) {
+    Plans pl;
+    pl.nharms = nharms;
+    pl.m = m;
+    pl.hidr = hidr;
+    pl.r_lo = r_lo;
+    pl.lodr = hidr / nharms;
+    int64_t pnum = 0, q = 0;
+    pl.rational = trial_grid_rational(pl.lodr, &pnum, &q);
+    pl.pnum = pnum;
+    pl.q = q;
+    const int m2 = m / 2;
+    pl.harm.resize(nharms);
+    for (int h = 1; h <= nharms; ++h) {
+        HarmDev& d = pl.harm[h - 1];
+        double hr = (double)h * r_lo;
+        d.rfloor0 = (int64_t)std::floor(hr);
+        d.dr0 = hr - (double)d.rfloor0;
+        d.pad = 0;
+        if (!pl.rational) {
+            d.q = 1; d.s = 0; d.base_adv = 0; d.hp = 0; d.P = 0;
+            d.row_off = 0; d.tab_off = 0;
+            continue;
+        }
+        int64_t step = (int64_t)h * pnum;
+        d.q = (int32_t)q;
+        d.s = (int32_t)(((step % q) + q) % q);
+        d.base_adv = (int32_t)floordiv(step, q);
+        d.hp = (int32_t)step;
+        d.P = (int32_t)(q / std::gcd(step, q));
+        d.row_off = (int32_t)pl.row.size();
+        d.tab_off = (int32_t)pl.carry.size();
+        pl.row.resize(pl.row.size() + q, -1);
+        int64_t res = 0;
+        for (int32_t p = 0; p < d.P; ++p) {
+            pl.row[d.row_off + res] = p;
+            double u = d.dr0 + (double)res / (double)q;
+            bool carry = u >= 1.0;
+            double dr = carry ? u - 1.0 : u;
+            pl.carry.push_back(carry ? 1 : 0);
+            if (dr == 0.0) {
+                // r is exactly a Fourier bin: the kernel is a delta on it.
+                pl.A64.push_back(1.0);
+                pl.A64.push_back(0.0);
+                for (int i = 1; i <= m; ++i) pl.W64.push_back(i == m2 ? 1.0 : 0.0);
+            } else {
+                double s = sinpi_d(dr), c = cospi_d(dr);
+                pl.A64.push_back((s * c) / M_PI);
+                pl.A64.push_back(-((s * s) / M_PI));
+                for (int i = 1; i <= m; ++i) pl.W64.push_back(1.0 / (dr - (double)(i - m2)));
+            }
+            res += d.s;
+            if (res >= q) res -= q;
+        }
+    }
+    pl.W32.resize(pl.W64.size());
+    for (size_t i = 0; i < pl.W64.size(); ++i) pl.W32[i] = (float)pl.W64[i];
+    pl.A32.resize(pl.A64.size());
+    for (size_t i = 0; i < pl.A64.size(); ++i) pl.A32[i] = (float)pl.A64[i];
+    return pl;
+}
+
+// boxcar_widths, search.jl:387-397
+std::vector<int> boxcar_widths(int nbins, double fsp, double maxfrac) {
+    int64_t wmax = (int64_t)std::floor(maxfrac * nbins);
+    if (wmax < 1) wmax = 1;
+    std::vector<int> ws;
+    int64_t w = 1;
+    while (w <= wmax && ws.size() < 64) {
+        ws.push_back((int)w);
+        int64_t nw = (int64_t)std::floor(fsp * (double)w);
+        w = std::max(nw, w + 1);
+    }
+    return ws;
+}
+
+// ladder_boxcar_widths, search.jl:443-459
+std::vector<int> ladder_boxcar_widths(int nbins, int k, const std::vector<int>& decims,
+                                      double fsp, double maxfrac) {
+    std::vector<int> ws = boxcar_widths(nbins, fsp, maxfrac);
+    std::vector<int> ks = decims;
+    std::sort(ks.begin(), ks.end());
+    if (ks.size() <= 1) return ws;
+    for (size_t i = 0; i + 1 < ks.size(); ++i)
+        if (!(ks[i + 1] <= (kLadderWmax / 2) * ks[i])) return ws;
+    int kmin = ks.front(), kmax = ks.back();
+    std::vector<int> out;
+    for (int w : ws)
+        if ((w >= 2 || k == kmin) && (w <= kLadderWmax || k == kmax)) out.push_back(w);
+    return out.empty() ? ws : out;
+}
+
+}  // namespace cs

@@ -1,0 +1,59 @@
+// Host-side planning for the coherent-search engine: the exact trial grid,
+// per-harmonic interpolation tables, boxcar banks and fold bookkeeping.
+// Shared structs are laid out for direct upload to the device.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace cs {
+
+constexpr int kSigmaSamples = 8192;   // _BOXCAR_SIGMA_SAMPLES, search.jl:466
+constexpr int kLadderWmax = 6;        // _LADDER_WMAX, search.jl:401
+constexpr int kMaxWidths = 24;
+constexpr int kMaxFolds = 8;
+constexpr int kQmax = 1 << 16;        // DIRECT_QMAX, directinterp.jl:111
+
+// One harmonic's recipe (DirectPlan, directinterp.jl:179-204).  Phase index p
+// (0-based) runs over the P residues the cycle visits; row[row_off + res] = p.
+struct HarmDev {
+    int64_t rfloor0;   // floor(h * r_lo)
+    double dr0;        // h*r_lo - rfloor0 (used only by the per-trial fallback)
+    int32_t q;         // trial-grid denominator
+    int32_t s;         // (h*pnum) mod q
+    int32_t base_adv;  // (h*pnum) div q
+    int32_t hp;        // h*pnum
+    int32_t P;         // phase-cycle length; 0 => per-trial FP64 fallback
+    int32_t row_off;   // into the residue->phase table (q entries)
+    int32_t tab_off;   // into carry/A (P entries) and W (P*m entries)
+    int32_t pad;
+};
+
+struct Plans {
+    int nharms = 0, m = 0;
+    double hidr = 0.5, r_lo = 0.0, lodr = 0.0;
+    bool rational = false;
+    int64_t pnum = 0, q = 0;
+    std::vector<HarmDev> harm;
+    std::vector<int32_t> row;      // sum of q over harmonics
+    std::vector<uint8_t> carry;    // sum of P
+    std::vector<float> W32;        // sum of P*m
+    std::vector<float> A32;        // 2 * sum of P (interleaved)
+    std::vector<double> W64;
+    std::vector<double> A64;
+};
+
+bool trial_grid_rational(double lodr, int64_t* pnum, int64_t* q);
+Plans build_plans(int nharms, int m, double hidr, double r_lo);
+
+std::vector<int> boxcar_widths(int nbins, double fsp, double maxfrac);
+std::vector<int> ladder_boxcar_widths(int nbins, int k, const std::vector<int>& decims,
+                                      double fsp, double maxfrac);
+
+// One decimation fold of the search (k = 1 is the base pass).
+struct FoldDev {
+    int32_t k, Hk, nbins, nw;
+    int32_t widths[kMaxWidths];
+};
+
+}  // namespace cs

@@ -1,0 +1,144 @@
+/*
+ * coherent_search.h — C ABI of libcsgpu.so, the MI355X (gfx950) engine for the
+ * coherent-search path of CoherentSearch.jl.
+ *
+ * Plain C types only, so Julia `ccall`, Python `ctypes` and C callers bind it
+ * directly.  The caller owns every input/output buffer (pointers are never
+ * retained); the library owns the opaque context, its HIP stream and its
+ * device buffers.  One context per host thread, as with the reference's
+ * SearchCache; calls are synchronous with respect to their outputs.
+ *
+ * Citations are CoherentSearch.jl file:line of the interface each entry point
+ * replaces.
+ */
+#ifndef COHERENT_SEARCH_H
+#define COHERENT_SEARCH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  CS_EINVAL maps to the reference's ArgumentError, CS_EDIM to
+ * DimensionMismatch; the message is available from cs_last_error. */
+#define CS_OK 0
+#define CS_EINVAL 1
+#define CS_EDIM 2
+#define CS_ENOSPC 3
+#define CS_EDEVICE 4
+#define CS_ENOMEM 5
+
+/* Execution modes for cs_search. */
+#define CS_MODE_PLAIN 0   /* FP32 VALU interpolation, one launch per stage, HBM intermediates */
+#define CS_MODE_FUSED 1   /* split-BF16 MFMA interpolation, fused pipeline */
+
+/* SearchParams (src/search.jl:131-144).  `decimations` is pointer + length
+ * (the reference's Vector{Int}); precision 0 = :f64, 1 = :f32 (the profile
+ * stage of the reference; the GPU search computes its profile stage in FP32
+ * and its exact rescan in FP64 either way). */
+typedef struct cs_params {
+    int32_t nharms;
+    int32_t m;
+    int32_t numbetween;
+    int32_t ndecim;
+    double hidr;
+    double threshold;
+    double boxcar_fsp;
+    double boxcar_maxfrac;
+    double boxcar_medmargin;
+    const int32_t *decimations;
+    int32_t precision;
+    int32_t reserved;
+} cs_params;
+
+/* Raw candidate before dedup (Candidate, src/search.jl:179-191, in integer
+ * form): global trial index t, decimation factor k, harmonics summed
+ * (nharms or floor(nharms/k)), 0-based block (chunk) and the metric.  The
+ * caller rebuilds r = k * (r_lo + i0*lodr + (t-i0)*lodr), i0 = block*blocksize,
+ * with the reference's FP64 expression order (src/search.jl:1812,1833,1696). */
+typedef struct cs_cand {
+    int64_t t;
+    int32_t k;
+    int32_t nharm;
+    int64_t block;
+    double metric;
+} cs_cand;
+
+typedef struct cs_ctx cs_ctx;
+
+/* SearchCache() + _plans! (src/search.jl:1914-1961): validates params
+ * (odd m, bad precision, empty decimation set -> CS_EINVAL), selects `device`
+ * and creates the context's stream. blocksize = Nprof (src/search.jl:2024). */
+int cs_ctx_create(const cs_params *params, int32_t blocksize, int32_t device,
+                  cs_ctx **out);
+int cs_ctx_destroy(cs_ctx *ctx);
+const char *cs_last_error(const cs_ctx *ctx);
+
+/* FFTFile amplitudes (src/fileio.jl:64-78): namps interleaved complex64
+ * values copied host->device; N = time-series length, T = N*dt. */
+int cs_set_spectrum(cs_ctx *ctx, const float *amps_re_im, int64_t namps,
+                    int64_t N, double T);
+/* Same, from a device pointer already resident in HBM (copied device->device
+ * into the context). */
+int cs_set_spectrum_device(cs_ctx *ctx, const void *d_amps, int64_t namps,
+                           int64_t N, double T);
+
+/* search() minus dedup (src/search.jl:2002-2111 -> _search_region! :1768-1894):
+ * every trial fundamental r_lo + t*lodr, t < floor((r_hi-r_lo)/lodr)+1, in
+ * blocks of `blocksize`, all decimation folds.  Writes up to `cap` raw
+ * candidates sorted by (t, k); *n_out is the total found (CS_ENOSPC if > cap). */
+int cs_search(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
+              cs_cand *out, int64_t cap, int64_t *n_out);
+
+/* block_metrics(ft, rfund, params; kernel, widths) (src/search.jl:1061-1079):
+ * kernel 1 = :direct (Eqn. 30 point by point, FP64); kernel 0 = :fft is not
+ * available on this device path and returns CS_EINVAL.  widths == NULL means
+ * ladder_boxcar_widths(2*nharms, 1, params). */
+int cs_block_metrics(cs_ctx *ctx, const double *rfund, int64_t L,
+                     int32_t kernel, const int32_t *widths, int32_t nwidths,
+                     double *out);
+
+/* chunk_metrics(ft, params, rstart, n; lodr, weights) (src/search.jl:1740-1755):
+ * the production machinery over one block at t0 = 0 with the plans anchored
+ * at rstart, exact scan, k = 1.  `lodr` is accepted for signature parity and,
+ * as in the reference, does not enter the interpolation. */
+int cs_chunk_metrics(cs_ctx *ctx, double rstart, int64_t n, double lodr,
+                     int32_t weights_f64, double *out);
+
+/* decim_pass! metrics for fold k over one block at t0 = 0
+ * (src/search.jl:1664-1691; the bridge test/test_search.jl:465-477 calls):
+ * *nvalid valid trials, exact scan. */
+int cs_decim_metrics(cs_ctx *ctx, double rstart, int64_t n, int32_t k,
+                     int32_t weights_f64, double *out, int64_t *nvalid);
+
+/* fill_chunk_profiles! rows (src/search.jl:1280-1287 ->
+ * fill_harmonic_row_direct!, src/directinterp.jl:470-572): the (nharms+1) x n
+ * stacked amplitudes for trials t0..t0+n-1 of a grid anchored at r_lo, as
+ * interleaved complex128, row-major (row h, column j).  Row 0 (DC) is zero. */
+int cs_interp_rows(cs_ctx *ctx, double r_lo, int64_t t0, int64_t n,
+                   int32_t weights_f64, double *out);
+
+/* Execution mode for cs_search (CS_MODE_PLAIN / CS_MODE_FUSED). */
+int cs_set_mode(cs_ctx *ctx, int32_t mode);
+
+/* Per-stage device time of the last cs_search, milliseconds, HIP events on the
+ * context's stream: [0] interp, [1] c2r, [2] sigma, [3] gate, [4] rescan,
+ * [5] fused, [6] total device, [7] number of batches. */
+int cs_last_timings(const cs_ctx *ctx, double *ms8);
+
+/* Host-only (no device): the per-harmonic interpolation tables the library
+ * builds for a grid anchored at r_lo (build_direct_plans,
+ * src/directinterp.jl:345-405).  info[0..5] = q, s, base_adv, pnum, rfloor0, P;
+ * row (q entries, 0-based phase or -1), carry (P), W (P*m, FP32 weights),
+ * A (2P, FP32).  Any output pointer may be NULL. */
+int cs_plan_tables(const cs_params *params, double r_lo, int32_t h,
+                   int64_t *info, int32_t *row, uint8_t *carry, float *W,
+                   float *A);
+
+/* Library version string. */
+const char *cs_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
