@@ -1,0 +1,18 @@
+"""MI355X coherent-search engine: the coherent harmonic-summing search path of
+CoherentSearch.jl (Eqn-30 interpolation, harmonic stack with decimation
+folding, batched c2r, boxcar matched-filter S/N) on gfx950, behind the C ABI of
+include/coherent_search.h."""
+from .dedup import harmonically_related, remove_duplicates, remove_harmonics
+from .engine import (Engine, SearchCache, block_metrics, chunk_metrics, raw_to_candidates,
+                     search, search_block)
+from .fileio import FFTFile, SimpleInf, freqs, write_fft
+from .params import (Candidate, SearchParams, boxcar_widths, candidate_r, decimation_set,
+                     ladder_boxcar_widths, proftype, trial_grid)
+
+__all__ = [
+    "Candidate", "Engine", "FFTFile", "SearchCache", "SearchParams", "SimpleInf",
+    "block_metrics", "boxcar_widths", "candidate_r", "chunk_metrics", "decimation_set",
+    "freqs", "harmonically_related", "ladder_boxcar_widths", "proftype", "raw_to_candidates",
+    "remove_duplicates", "remove_harmonics", "search", "search_block", "trial_grid",
+    "write_fft",
+]

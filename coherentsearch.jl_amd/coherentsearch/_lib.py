@@ -1,0 +1,133 @@
+"""ctypes binding of libcsgpu.so (include/coherent_search.h).
+
+The product path has no CPU fallback: if the library is missing or no GPU is
+visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcsgpu.so")
+HEADER = os.path.join(_HERE, "..", "..", "include", "coherent_search.h")
+
+CS_OK, CS_EINVAL, CS_EDIM, CS_ENOSPC, CS_EDEVICE, CS_ENOMEM = range(6)
+CS_MODE_PLAIN, CS_MODE_FUSED = 0, 1
+
+
+class DimensionMismatch(ValueError):
+    """The reference's DimensionMismatch."""
+
+
+class CsParams(C.Structure):
+    _fields_ = [
+        ("nharms", C.c_int32),
+        ("m", C.c_int32),
+        ("numbetween", C.c_int32),
+        ("ndecim", C.c_int32),
+        ("hidr", C.c_double),
+        ("threshold", C.c_double),
+        ("boxcar_fsp", C.c_double),
+        ("boxcar_maxfrac", C.c_double),
+        ("boxcar_medmargin", C.c_double),
+        ("decimations", C.POINTER(C.c_int32)),
+        ("precision", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+class CsCand(C.Structure):
+    _fields_ = [
+        ("t", C.c_int64),
+        ("k", C.c_int32),
+        ("nharm", C.c_int32),
+        ("block", C.c_int64),
+        ("metric", C.c_double),
+    ]
+
+
+CAND_DTYPE = np.dtype([("t", "<i8"), ("k", "<i4"), ("nharm", "<i4"),
+                       ("block", "<i8"), ("metric", "<f8")])
+
+_LIB = None
+
+
+def lib():
+    """Load libcsgpu.so, raising loudly if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `make -C coherentsearch.jl_amd` "
+                           "(or __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    fp = C.POINTER(C.c_float)
+    i32p = C.POINTER(C.c_int32)
+    i64p = C.POINTER(C.c_int64)
+    vp = C.c_void_p
+    L.cs_version.restype = C.c_char_p
+    L.cs_last_error.argtypes = [vp]
+    L.cs_last_error.restype = C.c_char_p
+    L.cs_ctx_create.argtypes = [C.POINTER(CsParams), C.c_int32, C.c_int32, C.POINTER(vp)]
+    L.cs_ctx_destroy.argtypes = [vp]
+    L.cs_set_spectrum.argtypes = [vp, fp, C.c_int64, C.c_int64, C.c_double]
+    L.cs_set_spectrum_device.argtypes = [vp, vp, C.c_int64, C.c_int64, C.c_double]
+    L.cs_search.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.POINTER(CsCand),
+                            C.c_int64, i64p]
+    L.cs_block_metrics.argtypes = [vp, dp, C.c_int64, C.c_int32, i32p, C.c_int32, dp]
+    L.cs_chunk_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_double, C.c_int32, dp]
+    L.cs_decim_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, C.c_int32, dp, i64p]
+    L.cs_interp_rows.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, C.c_int32, dp]
+    L.cs_set_mode.argtypes = [vp, C.c_int32]
+    L.cs_last_timings.argtypes = [vp, dp]
+    L.cs_plan_tables.argtypes = [C.POINTER(CsParams), C.c_double, C.c_int32, i64p, i32p,
+                                 C.POINTER(C.c_uint8), fp, fp]
+    _LIB = L
+    return L
+
+
+def declared_symbols(header: str = HEADER):
+    """Every function the public header declares."""
+    with open(header) as fh:
+        txt = fh.read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(cs_\w+)\s*\(", txt, re.M)))
+
+
+def raise_for(status: int, ctx=None, what: str = ""):
+    if status == CS_OK:
+        return
+    msg = ""
+    if ctx is not None:
+        try:
+            m = lib().cs_last_error(ctx)
+            msg = m.decode() if m else ""
+        except Exception:  # pragma: no cover
+            msg = ""
+    text = f"{what}: {msg}" if msg else what
+    if status == CS_EINVAL:
+        raise ValueError(text)
+    if status == CS_EDIM:
+        raise DimensionMismatch(text)
+    if status == CS_ENOMEM:
+        raise MemoryError(text)
+    if status == CS_ENOSPC:
+        raise OverflowError(text)
+    raise RuntimeError(f"{text} (status {status})")
+
+
+def make_cparams(params):
+    dec = np.ascontiguousarray(np.asarray(params.decimations, dtype=np.int32))
+    prec = {"f64": 0, "f32": 1}.get(params.precision)
+    if prec is None:
+        raise ValueError(f"precision must be :f64 or :f32, got :{params.precision}")
+    cp = CsParams(int(params.nharms), int(params.m), int(params.numbetween), len(dec),
+                  float(params.hidr), float(params.threshold), float(params.boxcar_fsp),
+                  float(params.boxcar_maxfrac), float(params.boxcar_medmargin),
+                  dec.ctypes.data_as(C.POINTER(C.c_int32)), prec, 0)
+    cp._keep = dec
+    return cp

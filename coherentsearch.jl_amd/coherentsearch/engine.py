@@ -1,0 +1,239 @@
+"""The device engine and the reference's public search API on top of it.
+
+``search``, ``search_block``, ``block_metrics``, ``chunk_metrics`` and
+``SearchCache`` mirror CoherentSearch.jl's functions of the same names
+(src/search.jl); the arithmetic runs in libcsgpu.so on the GPU.  Candidate
+frequencies are rebuilt on the host from the integer (trial, fold) pairs the
+device reports, with the reference's FP64 expression order.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from .dedup import remove_duplicates, remove_harmonics
+from .fileio import FFTFile
+from .params import (Candidate, SearchParams, candidate_r, ladder_boxcar_widths,
+                     proftype, trial_grid)
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Engine:
+    """One device context: the reference's SearchCache plus the spectrum held in
+    HBM.  Not thread-safe (one per host thread)."""
+
+    def __init__(self, params: SearchParams, blocksize: int = 2048, device: int = 0):
+        proftype(params)
+        if params.m % 2 != 0:
+            raise ValueError("m must be even")
+        self.params = params
+        self.blocksize = int(blocksize)
+        self._cp = _lib.make_cparams(params)
+        h = C.c_void_p()
+        st = _lib.lib().cs_ctx_create(C.byref(self._cp), self.blocksize, int(device), C.byref(h))
+        if st != 0:
+            _lib.raise_for(st, None, "cs_ctx_create (invalid parameters or no GPU)")
+        self._h = h
+        self._spec_key = None
+        self.N = None
+        self.T = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().cs_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, st, what):
+        _lib.raise_for(st, self._h, what)
+
+    # -- spectrum --------------------------------------------------------------
+    def set_spectrum(self, ft: FFTFile):
+        key = (id(ft), ft.N, ft.T, len(ft.amps))
+        if key == self._spec_key:
+            return
+        a = np.ascontiguousarray(np.asarray(ft.amps, dtype=np.complex64))
+        self._chk(_lib.lib().cs_set_spectrum(self._h, a.view(np.float32).ctypes.data_as(
+            C.POINTER(C.c_float)), len(a), int(ft.N), float(ft.T)), "cs_set_spectrum")
+        self._spec_key = key
+        self.N, self.T = ft.N, ft.T
+
+    def set_spectrum_device(self, ptr: int, namps: int, N: int, T: float):
+        """Amplitudes already resident in HBM (e.g. a torch tensor's data_ptr())."""
+        self._chk(_lib.lib().cs_set_spectrum_device(self._h, C.c_void_p(ptr), int(namps), int(N),
+                                                    float(T)), "cs_set_spectrum_device")
+        self._spec_key = ("device", ptr, namps, N, T)
+        self.N, self.T = N, T
+
+    def set_mode(self, mode: str):
+        m = {"plain": _lib.CS_MODE_PLAIN, "fused": _lib.CS_MODE_FUSED}[mode]
+        self._chk(_lib.lib().cs_set_mode(self._h, m), "cs_set_mode")
+
+    # -- entry points ------------------------------------------------------------
+    def search_raw(self, r_lo: float, r_hi: float, threshold: float) -> np.ndarray:
+        cap = 1 << 16
+        while True:
+            out = (_lib.CsCand * cap)()
+            n = C.c_int64()
+            st = _lib.lib().cs_search(self._h, float(r_lo), float(r_hi), float(threshold), out,
+                                      cap, C.byref(n))
+            if st == _lib.CS_ENOSPC and n.value > cap:
+                cap = n.value
+                continue
+            self._chk(st, "cs_search")
+            return np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
+
+    def timings(self) -> dict:
+        a = np.zeros(8)
+        self._chk(_lib.lib().cs_last_timings(self._h, _dp(a)), "cs_last_timings")
+        keys = ("interp", "c2r", "sigma", "gate", "rescan", "fused", "total", "batches")
+        return dict(zip(keys, a.tolist()))
+
+    def chunk_metrics(self, rstart: float, n: int, lodr: Optional[float] = None,
+                      weights_f64: bool = False) -> np.ndarray:
+        out = np.zeros(max(n, 0))
+        lodr = self.params.hidr / self.params.nharms if lodr is None else lodr
+        self._chk(_lib.lib().cs_chunk_metrics(self._h, float(rstart), int(n), float(lodr),
+                                              int(bool(weights_f64)), _dp(out)), "chunk_metrics")
+        return out
+
+    def decim_metrics(self, rstart: float, n: int, k: int, weights_f64: bool = False):
+        out = np.zeros(max(n, 0))
+        nv = C.c_int64()
+        self._chk(_lib.lib().cs_decim_metrics(self._h, float(rstart), int(n), int(k),
+                                              int(bool(weights_f64)), _dp(out), C.byref(nv)),
+                  "decim_metrics")
+        return out[:nv.value]
+
+    def block_metrics(self, rfund: Sequence[float], kernel: str = "direct",
+                      widths: Optional[Sequence[int]] = None) -> np.ndarray:
+        if kernel not in ("fft", "direct"):
+            raise ValueError(f"kernel must be :fft or :direct, got :{kernel}")
+        r = np.ascontiguousarray(np.asarray(rfund, dtype=np.float64))
+        out = np.zeros(len(r))
+        if widths is None:
+            wp, nw = None, 0
+        else:
+            w = np.ascontiguousarray(np.asarray(widths, dtype=np.int32))
+            wp, nw = w.ctypes.data_as(C.POINTER(C.c_int32)), len(w)
+        self._chk(_lib.lib().cs_block_metrics(self._h, _dp(r), len(r),
+                                              1 if kernel == "direct" else 0, wp, nw, _dp(out)),
+                  "block_metrics")
+        return out
+
+    def interp_rows(self, r_lo: float, t0: int, n: int, weights_f64: bool = False) -> np.ndarray:
+        nh = self.params.nharms
+        out = np.zeros(2 * (nh + 1) * max(n, 0))
+        self._chk(_lib.lib().cs_interp_rows(self._h, float(r_lo), int(t0), int(n),
+                                            int(bool(weights_f64)), _dp(out)), "interp_rows")
+        return out.view(np.complex128).reshape(nh + 1, max(n, 0))
+
+
+class SearchCache:
+    """Reusable per-SearchParams device context (src/search.jl:1914-1961).
+    Reuse is keyed on the params object's identity and the block size."""
+
+    def __init__(self):
+        self.params = None
+        self.Nprof = 0
+        self.engine: Optional[Engine] = None
+
+    def get(self, params: SearchParams, Nprof: int, device: int = 0) -> Engine:
+        if self.engine is not None and self.params is params and self.Nprof == Nprof:
+            return self.engine
+        if self.engine is not None:
+            self.engine.close()
+        self.engine = Engine(params, Nprof, device)
+        self.params = params
+        self.Nprof = Nprof
+        return self.engine
+
+
+def raw_to_candidates(raw: np.ndarray, ft: FFTFile, r_lo: float, lodr: float,
+                      Nprof: int) -> List[Candidate]:
+    """Candidate(freq, metric, r, nharm) from the device's integer records."""
+    out = []
+    for rec in raw:
+        r = candidate_r(r_lo, lodr, Nprof, int(rec["t"]), int(rec["k"]))
+        out.append(Candidate(r / ft.T, float(rec["metric"]), r, int(rec["nharm"])))
+    return out
+
+
+def search(ft: FFTFile, params: Optional[SearchParams] = None, lofreq: float = 0.1,
+           hifreq: float = 100.0, lobin: int = 100, blocksize: int = 2048,
+           threshold: Optional[float] = None, remove: bool = True, dr_tol: float = 1.0,
+           harm_remove: bool = True, numharm: int = 16, harm_tol: float = 1.0,
+           progress: str = "none", metricstats=None, normalize: bool = False,
+           verbose: bool = False, wisdom: bool = True, wisdom_file=None,
+           cache: Optional[SearchCache] = None, device: int = 0,
+           mode: str = "plain") -> List[Candidate]:
+    """search(ft, params; ...) (src/search.jl:2002-2111) on the GPU.
+
+    ``wisdom``/``wisdom_file`` are FFTW plan caches with no device counterpart
+    and are accepted for signature parity.  ``metricstats`` and ``normalize``
+    (the opt-in diagnostics) are not provided by the device path."""
+    params = SearchParams() if params is None else params
+    if progress not in ("none", "text", "bar"):
+        raise ValueError(f"progress must be :none, :text or :bar, got :{progress}")
+    if metricstats is not None or normalize:
+        raise NotImplementedError("metricstats / normalize are not provided by the device path")
+    thr = params.threshold if threshold is None else float(threshold)
+    r_lo, r_hi, lodr, total, Nprof, nchunks = trial_grid(ft.T, params, lofreq, hifreq, lobin,
+                                                         blocksize)
+    if total == 0:
+        return []
+    eng = (cache or SearchCache()).get(params, Nprof, device) if cache is not None else \
+        Engine(params, Nprof, device)
+    eng.set_mode(mode)
+    eng.set_spectrum(ft)
+    raw = eng.search_raw(r_lo, r_hi, thr)
+    cands = raw_to_candidates(raw, ft, r_lo, lodr, Nprof)
+    if verbose:
+        import sys
+        print(f"Search: {total} trials in {nchunks} chunks of {Nprof} over r = {r_lo:.1f} … "
+              f"{r_hi:.1f} bins, {len(cands)} above threshold", file=sys.stderr)
+    if remove:
+        cands = remove_duplicates(cands, dr_tol=dr_tol)
+    else:
+        cands = sorted(cands, key=lambda c: c.freq)
+    if harm_remove:
+        cands = remove_harmonics(cands, numharm=numharm, tol=harm_tol)
+    return cands
+
+
+def block_metrics(ft: FFTFile, rfund, params: SearchParams, kernel: str = "direct",
+                  widths=None, engine: Optional[Engine] = None) -> np.ndarray:
+    """block_metrics (src/search.jl:1061-1079); the device provides :direct."""
+    eng = engine or Engine(params, 2048)
+    eng.set_spectrum(ft)
+    return eng.block_metrics(rfund, kernel=kernel, widths=widths)
+
+
+def search_block(ft: FFTFile, rfund, params: SearchParams, threshold: Optional[float] = None,
+                 kernel: str = "direct", engine: Optional[Engine] = None) -> List[Candidate]:
+    """search_block (src/search.jl:1087-1097)."""
+    thr = params.threshold if threshold is None else threshold
+    m = block_metrics(ft, rfund, params, kernel=kernel, engine=engine)
+    return [Candidate(float(r) / ft.T, float(v), float(r), params.nharms)
+            for r, v in zip(rfund, m) if v > thr]
+
+
+def chunk_metrics(ft: FFTFile, params: SearchParams, rstart: float, n: int,
+                  lodr: Optional[float] = None, weights: str = "f32",
+                  engine: Optional[Engine] = None) -> np.ndarray:
+    """chunk_metrics (src/search.jl:1740-1755)."""
+    eng = engine or Engine(params, max(1, n))
+    eng.set_spectrum(ft)
+    return eng.chunk_metrics(rstart, n, lodr, weights_f64=(weights == "f64"))

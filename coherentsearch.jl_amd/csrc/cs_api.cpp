@@ -1,0 +1,606 @@
+// C ABI of the coherent-search engine: context, buffers, batching and the
+// entry points declared in include/coherent_search.h.
+#include "coherent_search.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "cs_gpu.hpp"
+#include "cs_plan.hpp"
+
+using namespace cs;
+
+namespace {
+
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+constexpr int kBankW[12] = {1, 2, 3, 4, 6, 9, 13, 19, 28, 42, 63, 94};
+constexpr int64_t kBatchTrials = (int64_t)1 << 21;   // trials per batch (HBM scratch bound)
+constexpr int kMaxHarm = 64;                         // device path: profiles of <= 128 bins
+
+}  // namespace
+
+struct cs_ctx {
+    // parameters
+    int32_t nharms = 0, m = 0, numbetween = 16, precision = 0;
+    double hidr = 0.5, threshold = 8.0, fsp = 1.5, maxfrac = 0.3, medmargin = 2.0;
+    std::vector<int> decims;
+    int64_t Nprof = 2048;
+    int device = 0;
+    int mode = CS_MODE_PLAIN;
+    hipStream_t st = nullptr;
+    std::string err;
+    // spectrum
+    DBuf amps;
+    int64_t namps = 0, N = 0;
+    double T = 0.0;
+    // tables
+    DBuf harm, row, carry, W32, A32, W64, A64;
+    std::map<int, DBuf> twid;
+    // scratch
+    DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout;
+    std::vector<hipEvent_t> ev;
+    double timings[8] = {0};
+
+    int fail(int code, const std::string& msg) {
+        err = msg;
+        return code;
+    }
+};
+
+#define HIPCHK(ctx, expr)                                                                  \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess)                                                              \
+            return (ctx)->fail(e_ == hipErrorOutOfMemory ? CS_ENOMEM : CS_EDEVICE,          \
+                               std::string(#expr) + ": " + hipGetErrorString(e_));         \
+    } while (0)
+
+namespace {
+
+int ensure(cs_ctx* c, DBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 16;
+    if (b.bytes >= bytes) return CS_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        b.p = nullptr;
+        b.bytes = 0;
+        if (e != hipSuccess) return c->fail(CS_EDEVICE, std::string("hipFree: ") + hipGetErrorString(e));
+    }
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        return c->fail(CS_ENOMEM, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    b.bytes = bytes;
+    return CS_OK;
+}
+
+void release(DBuf& b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+template <typename T>
+int upload(cs_ctx* c, DBuf& b, const std::vector<T>& v) {
+    int s = ensure(c, b, v.size() * sizeof(T));
+    if (s) return s;
+    if (!v.empty())
+        HIPCHK(c, hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
+}
+
+// Upload the interpolation tables for a grid anchored at r_lo and fill `tb`.
+int set_tables(cs_ctx* c, double r_lo, bool weights_f64, TabDev& tb, int64_t last_trial) {
+    Plans pl = build_plans(c->nharms, c->m, c->hidr, r_lo);
+    if (pl.rational) {
+        for (const HarmDev& d : pl.harm)
+            if ((uint64_t)d.q * (uint64_t)(d.hp > 0 ? d.hp : 1) >= ((uint64_t)1 << 32))
+                return c->fail(CS_EINVAL, "trial grid q*h*pnum exceeds 2^32");
+    }
+    if (last_trial >= ((int64_t)1 << 32))
+        return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
+    int s;
+    if ((s = upload(c, c->harm, pl.harm))) return s;
+    if ((s = upload(c, c->row, pl.row))) return s;
+    if ((s = upload(c, c->carry, pl.carry))) return s;
+    if ((s = upload(c, c->W32, pl.W32))) return s;
+    if ((s = upload(c, c->A32, pl.A32))) return s;
+    if ((s = upload(c, c->W64, pl.W64))) return s;
+    if ((s = upload(c, c->A64, pl.A64))) return s;
+    tb.harm = (const HarmDev*)c->harm.p;
+    tb.row = (const int32_t*)c->row.p;
+    tb.carry = (const uint8_t*)c->carry.p;
+    tb.W32 = (const float*)c->W32.p;
+    tb.A32 = (const float2*)c->A32.p;
+    tb.W64 = (const double*)c->W64.p;
+    tb.A64 = (const double2*)c->A64.p;
+    tb.lodr = pl.lodr;
+    tb.m = c->m;
+    tb.nharms = c->nharms;
+    tb.weights_f64 = weights_f64 ? 1 : 0;
+    tb.pad = 0;
+    return CS_OK;
+}
+
+int get_twid(cs_ctx* c, int nbins, const double2** out) {
+    auto it = c->twid.find(nbins);
+    if (it == c->twid.end()) {
+        std::vector<double> v(2 * nbins);
+        for (int e = 0; e < nbins; ++e) {
+            double a = 2.0 * (double)e / (double)nbins;
+            v[2 * e] = cospi_d(a);
+            v[2 * e + 1] = sinpi_d(a);
+        }
+        DBuf b;
+        int s = upload(c, b, v);
+        if (s) return s;
+        it = c->twid.emplace(nbins, b).first;
+    }
+    *out = (const double2*)it->second.p;
+    return CS_OK;
+}
+
+SpecDev spec_of(cs_ctx* c) {
+    SpecDev sp;
+    sp.amps = (const float2*)c->amps.p;
+    sp.namps = c->namps;
+    sp.N = c->N;
+    sp.Nhalf = c->N / 2;
+    return sp;
+}
+
+std::vector<int> ladder(cs_ctx* c, int nbins, int k) {
+    return ladder_boxcar_widths(nbins, k, c->decims, c->fsp, c->maxfrac);
+}
+
+uint32_t bank_mask(const std::vector<int>& ws) {
+    uint32_t mask = 0;
+    for (int w : ws) {
+        int a = -1;
+        for (int i = 0; i < 12; ++i) if (kBankW[i] == w) a = i;
+        if (a < 0) return 0;
+        mask |= 1u << a;
+    }
+    return mask;
+}
+
+// nvalid: the prefix j of block trials with k*(rstart + j*lodr) < N/2
+// (decim_pass!, search.jl:1679-1691); the predicate is monotone in j.
+int64_t nvalid_of(int k, double rstart, double lodr, int64_t n, double nyq) {
+    if (k == 1) return n;
+    auto ok = [&](int64_t j) { return (double)k * (rstart + (double)j * lodr) < nyq; };
+    if (n == 0 || ok(n - 1)) return n;
+    int64_t lo = 0, hi = n - 1;   // ok(hi) false
+    if (!ok(0)) return 0;
+    while (hi - lo > 1) {
+        int64_t mid = lo + (hi - lo) / 2;
+        if (ok(mid)) lo = mid;
+        else hi = mid;
+    }
+    return hi;
+}
+
+int check_spectrum(cs_ctx* c) {
+    if (!c->amps.p || c->namps <= 0) return c->fail(CS_EINVAL, "no spectrum set (cs_set_spectrum)");
+    return CS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cs_version(void) { return "coherentsearch-mi355x 0.1 (gfx950)"; }
+
+const char* cs_last_error(const cs_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int cs_ctx_create(const cs_params* p, int32_t blocksize, int32_t device, cs_ctx** out) {
+    if (!out || !p) return CS_EINVAL;
+    *out = nullptr;
+    if (p->m % 2 != 0 || p->m <= 0) return CS_EINVAL;
+    if (p->nharms < 1 || p->nharms > kMaxHarm) return CS_EINVAL;
+    if (p->precision != 0 && p->precision != 1) return CS_EINVAL;
+    if (p->ndecim < 0 || p->ndecim > kMaxFolds - 1 + 1) return CS_EINVAL;
+    cs_ctx* c = new cs_ctx();
+    c->nharms = p->nharms;
+    c->m = p->m;
+    c->numbetween = p->numbetween;
+    c->precision = p->precision;
+    c->hidr = p->hidr;
+    c->threshold = p->threshold;
+    c->fsp = p->boxcar_fsp;
+    c->maxfrac = p->boxcar_maxfrac;
+    c->medmargin = p->boxcar_medmargin;
+    for (int i = 0; i < p->ndecim; ++i) c->decims.push_back(p->decimations[i]);
+    if (c->decims.empty()) c->decims.push_back(1);
+    int nf = 1;
+    for (int k : c->decims) {
+        if (k < 1 || c->nharms / k < 1) { delete c; return CS_EINVAL; }
+        if (k > 1) ++nf;
+    }
+    if (nf > kMaxFolds) { delete c; return CS_EINVAL; }
+    c->Nprof = std::max<int64_t>(1, blocksize);
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return CS_EDEVICE;
+    }
+    *out = c;
+    return CS_OK;
+}
+
+int cs_ctx_destroy(cs_ctx* c) {
+    if (!c) return CS_OK;
+    (void)hipSetDevice(c->device);
+    if (c->st) (void)hipStreamSynchronize(c->st);
+    DBuf* bufs[] = {&c->amps, &c->harm, &c->row, &c->carry, &c->W32, &c->A32, &c->W64, &c->A64,
+                    &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
+                    &c->widths, &c->rfund, &c->rowok, &c->mout};
+    for (DBuf* b : bufs) release(*b);
+    for (auto& kv : c->twid) release(kv.second);
+    for (hipEvent_t ev : c->ev) (void)hipEventDestroy(ev);
+    if (c->st) (void)hipStreamDestroy(c->st);
+    delete c;
+    return CS_OK;
+}
+
+int cs_set_mode(cs_ctx* c, int32_t mode) {
+    if (!c) return CS_EINVAL;
+    if (mode != CS_MODE_PLAIN && mode != CS_MODE_FUSED) return c->fail(CS_EINVAL, "unknown mode");
+    if (mode == CS_MODE_FUSED) return c->fail(CS_EINVAL, "fused mode is not available in this build");
+    c->mode = mode;
+    return CS_OK;
+}
+
+int cs_last_timings(const cs_ctx* c, double* ms8) {
+    if (!c || !ms8) return CS_EINVAL;
+    for (int i = 0; i < 8; ++i) ms8[i] = c->timings[i];
+    return CS_OK;
+}
+
+static int set_spectrum_impl(cs_ctx* c, const void* src, hipMemcpyKind kind, int64_t namps,
+                             int64_t N, double T) {
+    if (!c) return CS_EINVAL;
+    if (namps <= 0 || N <= 0 || !(T > 0)) return c->fail(CS_EINVAL, "invalid spectrum dimensions");
+    HIPCHK(c, hipSetDevice(c->device));
+    int s = ensure(c, c->amps, (size_t)namps * sizeof(float2));
+    if (s) return s;
+    HIPCHK(c, hipMemcpyAsync(c->amps.p, src, (size_t)namps * sizeof(float2), kind, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->namps = namps;
+    c->N = N;
+    c->T = T;
+    return CS_OK;
+}
+
+int cs_set_spectrum(cs_ctx* c, const float* amps, int64_t namps, int64_t N, double T) {
+    return set_spectrum_impl(c, amps, hipMemcpyHostToDevice, namps, N, T);
+}
+
+int cs_set_spectrum_device(cs_ctx* c, const void* d_amps, int64_t namps, int64_t N, double T) {
+    return set_spectrum_impl(c, d_amps, hipMemcpyDeviceToDevice, namps, N, T);
+}
+
+int cs_interp_rows(cs_ctx* c, double r_lo, int64_t t0, int64_t n, int32_t weights_f64,
+                   double* out) {
+    if (!c || !out) return CS_EINVAL;
+    if (n < 0 || t0 < 0) return c->fail(CS_EINVAL, "n and t0 must be non-negative");
+    int s = check_spectrum(c);
+    if (s) return s;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nh = c->nharms;
+    std::memset(out, 0, sizeof(double) * 2 * (size_t)(nh + 1) * (size_t)n);
+    if (n == 0) return CS_OK;
+    TabDev tb;
+    if ((s = set_tables(c, r_lo, weights_f64 != 0, tb, t0 + n))) return s;
+    if ((s = ensure(c, c->ftp, (size_t)nh * n * sizeof(double2)))) return s;
+    GridDev g{t0, n, t0 + n};
+    HIPCHK(c, launch_interp(c->st, spec_of(c), tb, g, t0, n, true, c->ftp.p, n));
+    HIPCHK(c, hipMemcpyAsync(out + 2 * n, c->ftp.p, (size_t)nh * n * sizeof(double2),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
+}
+
+// Shared FP64 bridge: stack (already in c->ftp as double2, ld = n) -> fold k
+// profiles -> sigma over nv columns (sample buffer `cap`) -> exact metrics.
+static int bridge_metrics(cs_ctx* c, int k, int64_t n, int64_t nv, int64_t cap,
+                          const std::vector<int>& ws, double scale, double* out) {
+    const int nbins = 2 * (c->nharms / k);
+    if (ws.empty() || ws.size() > 64) return c->fail(CS_EINVAL, "boxcar bank must hold 1..64 widths");
+    for (int w : ws)
+        if (w < 1 || w > nbins) return c->fail(CS_EINVAL, "boxcar widths must lie in 1..nbins");
+    const double2* tw;
+    int s;
+    if ((s = get_twid(c, nbins, &tw))) return s;
+    if ((s = ensure(c, c->prof, (size_t)nbins * n * sizeof(double)))) return s;
+    if ((s = ensure(c, c->sigma, sizeof(double)))) return s;
+    if ((s = ensure(c, c->mout, (size_t)std::max<int64_t>(nv, 1) * sizeof(double)))) return s;
+    if ((s = upload(c, c->widths, std::vector<int32_t>(ws.begin(), ws.end())))) return s;
+    double* prof = (double*)c->prof.p;
+    HIPCHK(c, launch_c2r_f64(c->st, (const double2*)c->ftp.p, n, k, nbins, n, prof, n, tw, scale));
+    HIPCHK(c, launch_sigma_f64(c->st, prof, n, nbins, nv, cap, (double*)c->sigma.p));
+    HIPCHK(c, launch_exact_f64(c->st, prof, n, nbins, nv, (const int32_t*)c->widths.p, (int)ws.size(),
+                               (const double*)c->sigma.p, (double*)c->mout.p));
+    if (nv > 0)
+        HIPCHK(c, hipMemcpyAsync(out, c->mout.p, (size_t)nv * sizeof(double), hipMemcpyDeviceToHost,
+                                 c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
+}
+
+int cs_chunk_metrics(cs_ctx* c, double rstart, int64_t n, double lodr, int32_t weights_f64,
+                     double* out) {
+    (void)lodr;
+    if (!c) return CS_EINVAL;
+    if (n < 1) return CS_OK;
+    int s = check_spectrum(c);
+    if (s) return s;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nh = c->nharms, nbins = 2 * nh;
+    TabDev tb;
+    if ((s = set_tables(c, rstart, weights_f64 != 0, tb, n))) return s;
+    if ((s = ensure(c, c->ftp, (size_t)nh * n * sizeof(double2)))) return s;
+    GridDev g{0, n, n};
+    HIPCHK(c, launch_interp(c->st, spec_of(c), tb, g, 0, n, true, c->ftp.p, n));
+    int64_t cap = std::min<int64_t>((int64_t)nbins * n, kSigmaSamples);
+    return bridge_metrics(c, 1, n, n, cap, ladder(c, nbins, 1), 1.0, out);
+}
+
+int cs_decim_metrics(cs_ctx* c, double rstart, int64_t n, int32_t k, int32_t weights_f64,
+                     double* out, int64_t* nvalid) {
+    if (!c || !nvalid) return CS_EINVAL;
+    if (k < 1 || c->nharms / k < 1) return c->fail(CS_EINVAL, "invalid decimation factor");
+    *nvalid = 0;
+    if (n < 1) return CS_OK;
+    int s = check_spectrum(c);
+    if (s) return s;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nh = c->nharms, nbins = 2 * (nh / k);
+    TabDev tb;
+    if ((s = set_tables(c, rstart, weights_f64 != 0, tb, n))) return s;
+    if ((s = ensure(c, c->ftp, (size_t)nh * n * sizeof(double2)))) return s;
+    GridDev g{0, n, n};
+    HIPCHK(c, launch_interp(c->st, spec_of(c), tb, g, 0, n, true, c->ftp.p, n));
+    const double lodr = c->hidr / nh;
+    int64_t nv = nvalid_of(k, rstart, lodr, n, (double)c->N / 2.0);
+    *nvalid = nv;
+    int64_t cap = std::min<int64_t>((int64_t)nbins * n, kSigmaSamples);
+    return bridge_metrics(c, k, n, nv, cap, ladder(c, nbins, k), 1.0, out);
+}
+
+int cs_block_metrics(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel,
+                     const int32_t* widths, int32_t nwidths, double* out) {
+    if (!c) return CS_EINVAL;
+    if (kernel != 0 && kernel != 1) return c->fail(CS_EINVAL, "kernel must be :fft or :direct");
+    if (kernel == 0)
+        return c->fail(CS_EINVAL, "kernel :fft (FFT-correlation) is not available on the device path");
+    if (L < 1) return CS_OK;
+    if (widths && nwidths < 1) return c->fail(CS_EINVAL, "empty width bank");
+    int s = check_spectrum(c);
+    if (s) return s;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nh = c->nharms, nbins = 2 * nh, m2 = c->m / 2;
+    std::vector<uint8_t> ok(nh, 0);
+    const int64_t Nhalf = c->N / 2;
+    for (int h = 1; h <= nh; ++h) {
+        double rmin = rfund[0] * h, rmax = rfund[L - 1] * h;
+        int64_t lobin = (int64_t)std::floor(rmin);
+        int64_t hibin = (int64_t)std::ceil(rmax) + 1;
+        int64_t numbins = hibin - lobin;
+        ok[h - 1] = (lobin >= m2 && (lobin + numbins + m2) <= c->namps && hibin < Nhalf) ? 1 : 0;
+    }
+    if ((s = upload(c, c->rowok, ok))) return s;
+    std::vector<double> rv(rfund, rfund + L);
+    if ((s = upload(c, c->rfund, rv))) return s;
+    if ((s = ensure(c, c->ftp, (size_t)nh * L * sizeof(double2)))) return s;
+    HIPCHK(c, launch_point_interp_f64(c->st, spec_of(c), (const double*)c->rfund.p, L, nh, c->m,
+                                      (const uint8_t*)c->rowok.p, (double2*)c->ftp.p, L));
+    std::vector<int> ws;
+    if (widths) ws.assign(widths, widths + nwidths);
+    else ws = ladder(c, nbins, 1);
+    int64_t cap = std::min<int64_t>((int64_t)nbins * L, kSigmaSamples);
+    s = bridge_metrics(c, 1, L, L, cap, ws, 1.0 / (double)nbins, out);
+    if (s) return s;
+    for (int64_t j = 0; j < L; ++j)
+        if (std::isnan(out[j])) return c->fail(CS_EDIM, "an interpolation window falls outside the amplitudes");
+    return CS_OK;
+}
+
+int cs_plan_tables(const cs_params* p, double r_lo, int32_t h, int64_t* info, int32_t* row,
+                   uint8_t* carry, float* W, float* A) {
+    if (!p || p->m % 2 != 0 || p->nharms < 1 || h < 1 || h > p->nharms) return CS_EINVAL;
+    Plans pl = build_plans(p->nharms, p->m, p->hidr, r_lo);
+    if (!pl.rational) return CS_EINVAL;
+    const HarmDev& d = pl.harm[h - 1];
+    if (info) {
+        info[0] = d.q; info[1] = d.s; info[2] = d.base_adv; info[3] = pl.pnum;
+        info[4] = d.rfloor0; info[5] = d.P;
+    }
+    if (row) for (int i = 0; i < d.q; ++i) row[i] = pl.row[d.row_off + i];
+    if (carry) for (int i = 0; i < d.P; ++i) carry[i] = pl.carry[d.tab_off + i];
+    if (W) for (int i = 0; i < d.P * p->m; ++i) W[i] = pl.W32[(size_t)d.tab_off * p->m + i];
+    if (A) for (int i = 0; i < 2 * d.P; ++i) A[i] = pl.A32[2 * (size_t)d.tab_off + i];
+    return CS_OK;
+}
+
+int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* out, int64_t cap,
+              int64_t* n_out) {
+    if (!c || !n_out) return CS_EINVAL;
+    *n_out = 0;
+    for (double& v : c->timings) v = 0.0;
+    int s = check_spectrum(c);
+    if (s) return s;
+    if (c->mode != CS_MODE_PLAIN) return c->fail(CS_EINVAL, "fused mode is not available in this build");
+    HIPCHK(c, hipSetDevice(c->device));
+    const int nh = c->nharms;
+    const double lodr = c->hidr / nh;
+    double span = std::floor((r_hi - r_lo) / lodr);
+    int64_t total = span >= 0 ? (int64_t)span + 1 : 0;
+    if (total <= 0) return CS_OK;
+    const int64_t Nprof = c->Nprof;
+    const int64_t nchunks = (total + Nprof - 1) / Nprof;
+    TabDev tb;
+    if ((s = set_tables(c, r_lo, false, tb, total))) return s;
+
+    // folds: base pass, then every k > 1 in the order the parameters list them
+    FoldSet fs;
+    std::memset(&fs, 0, sizeof(fs));
+    std::vector<int> ks{1};
+    for (int k : c->decims) if (k > 1) ks.push_back(k);
+    fs.nf = (int)ks.size();
+    const int nf = fs.nf;
+    int64_t rows_total = 0;
+    for (int f = 0; f < nf; ++f) {
+        FoldDev& fd = fs.f[f];
+        fd.k = ks[f];
+        fd.Hk = nh / ks[f];
+        fd.nbins = 2 * fd.Hk;
+        std::vector<int> ws = ladder(c, fd.nbins, fd.k);
+        if ((int)ws.size() > kMaxWidths) return c->fail(CS_EINVAL, "boxcar bank too long");
+        if (ws.back() > fd.nbins) return c->fail(CS_EINVAL, "boxcar width exceeds the profile length");
+        fd.nw = (int)ws.size();
+        for (int a = 0; a < fd.nw; ++a) fd.widths[a] = ws[a];
+        fs.bankmask[f] = bank_mask(ws);
+        fs.cap[f] = (int32_t)std::min<int64_t>((int64_t)fd.nbins * Nprof, kSigmaSamples);
+        if ((s = get_twid(c, fd.nbins, &fs.twid[f]))) return s;
+        rows_total += fd.nbins;
+    }
+    const int64_t Bc = std::max<int64_t>(1, std::min<int64_t>(nchunks, kBatchTrials / Nprof));
+    const int64_t BT = Bc * Nprof;
+    fs.ldp = BT;
+
+    // per-block valid counts for every fold (host, exact FP64 predicate)
+    std::vector<int32_t> nv((size_t)nchunks * nf);
+    const double nyq = (double)c->N / 2.0;
+    for (int64_t cc = 0; cc < nchunks; ++cc) {
+        int64_t i0 = cc * Nprof;
+        int64_t n = std::min(Nprof, total - i0);
+        double rstart = r_lo + (double)i0 * lodr;
+        for (int f = 0; f < nf; ++f) nv[cc * nf + f] = (int32_t)nvalid_of(ks[f], rstart, lodr, n, nyq);
+    }
+    if ((s = upload(c, c->nvalid, nv))) return s;
+    if ((s = ensure(c, c->sigma, (size_t)nchunks * nf * sizeof(double)))) return s;
+    if ((s = ensure(c, c->ftp, (size_t)nh * BT * sizeof(float2)))) return s;
+    if ((s = ensure(c, c->prof, (size_t)rows_total * BT * sizeof(float)))) return s;
+    if ((s = ensure(c, c->surv, (size_t)BT * nf * sizeof(uint64_t)))) return s;
+    if ((s = ensure(c, c->counters, 4 * sizeof(uint32_t)))) return s;
+    int64_t rowo = 0;
+    for (int f = 0; f < nf; ++f) {
+        fs.prof[f] = (float*)c->prof.p + rowo * BT;
+        rowo += fs.f[f].nbins;
+    }
+    uint32_t cand_cap = (uint32_t)std::max<int64_t>(1 << 16, std::min<int64_t>(cap, (int64_t)1 << 26));
+    const int64_t nbatch = (nchunks + Bc - 1) / Bc;
+    const int nev = 6;
+    while ((int64_t)c->ev.size() < nbatch * nev) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->ev.push_back(e);
+    }
+    GridDev g{0, Nprof, total};
+    SpecDev sp = spec_of(c);
+    uint32_t* surv_count = (uint32_t*)c->counters.p;
+    uint32_t* cand_count = surv_count + 1;
+    double medcut = (c->medmargin == INFINITY) ? -INFINITY : threshold - c->medmargin;
+
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if ((s = ensure(c, c->cands, (size_t)cand_cap * sizeof(cs_cand_dev)))) return s;
+        HIPCHK(c, launch_reset_u32(c->st, surv_count, 2));
+        for (int64_t bi = 0; bi < nbatch; ++bi) {
+            hipEvent_t* E = &c->ev[bi * nev];
+            const int64_t c0 = bi * Bc;
+            const int64_t nb = std::min(Bc, nchunks - c0);
+            const int64_t t_base = c0 * Nprof;
+            const int64_t nt = std::min(nb * Nprof, total - t_base);
+            BatchDev b;
+            b.t_base = t_base;
+            b.nt = nt;
+            b.c_first = c0;
+            b.nblocks = (int32_t)nb;
+            b.pad = 0;
+            b.nvalid = (const int32_t*)c->nvalid.p + c0 * nf;
+            b.sigma = (double*)c->sigma.p + c0 * nf;
+            OutDev o;
+            o.surv = (uint64_t*)c->surv.p;
+            o.surv_count = surv_count;
+            o.surv_cap = (uint32_t)(BT * nf);
+            o.cand_cap = cand_cap;
+            o.cands = (cs_cand_dev*)c->cands.p;
+            o.cand_count = cand_count;
+            o.mout = nullptr;
+            o.threshold = threshold;
+            o.medcut = medcut;
+            HIPCHK(c, hipEventRecord(E[0], c->st));
+            HIPCHK(c, launch_interp(c->st, sp, tb, g, t_base, nt, false, c->ftp.p, BT));
+            HIPCHK(c, hipEventRecord(E[1], c->st));
+            for (int f = 0; f < nf; ++f)
+                HIPCHK(c, launch_c2r_f32(c->st, (const float2*)c->ftp.p, BT, fs.f[f].k, fs.f[f].nbins,
+                                         nt, fs.prof[f], BT, fs.twid[f]));
+            HIPCHK(c, hipEventRecord(E[2], c->st));
+            HIPCHK(c, launch_sigma_f32(c->st, fs, g, b));
+            HIPCHK(c, hipEventRecord(E[3], c->st));
+            HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
+            HIPCHK(c, launch_gate_f32(c->st, fs, g, b, o));
+            HIPCHK(c, hipEventRecord(E[4], c->st));
+            HIPCHK(c, launch_rescan(c->st, sp, tb, g, fs, b, o, 8192));
+            HIPCHK(c, hipEventRecord(E[5], c->st));
+        }
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        uint32_t counts[2];
+        HIPCHK(c, hipMemcpy(counts, c->counters.p, sizeof(counts), hipMemcpyDeviceToHost));
+        if (counts[1] > cand_cap && attempt == 0) {
+            cand_cap = counts[1] + 1024;
+            continue;
+        }
+        int64_t nc = std::min<int64_t>(counts[1], cand_cap);
+        std::vector<cs_cand_dev> hc((size_t)nc);
+        if (nc)
+            HIPCHK(c, hipMemcpy(hc.data(), c->cands.p, (size_t)nc * sizeof(cs_cand_dev),
+                                hipMemcpyDeviceToHost));
+        std::sort(hc.begin(), hc.end(), [](const cs_cand_dev& a, const cs_cand_dev& b) {
+            return a.t != b.t ? a.t < b.t : a.k < b.k;
+        });
+        for (int64_t bi = 0; bi < nbatch; ++bi) {
+            hipEvent_t* E = &c->ev[bi * nev];
+            for (int q = 0; q < 5; ++q) {
+                float ms = 0.f;
+                HIPCHK(c, hipEventElapsedTime(&ms, E[q], E[q + 1]));
+                c->timings[q] += ms;
+            }
+            float tot = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&tot, E[0], E[5]));
+            c->timings[6] += tot;
+        }
+        c->timings[7] = (double)nbatch;
+        *n_out = nc;
+        if (out) {
+            int64_t w = std::min<int64_t>(nc, cap);
+            for (int64_t i = 0; i < w; ++i) {
+                out[i].t = hc[i].t;
+                out[i].k = hc[i].k;
+                out[i].nharm = hc[i].nharm;
+                out[i].block = hc[i].block;
+                out[i].metric = hc[i].metric;
+            }
+        }
+        if (nc > cap) return c->fail(CS_ENOSPC, "candidate buffer too small");
+        return CS_OK;
+    }
+    return c->fail(CS_ENOSPC, "candidate buffer could not be sized");
+}
+
+}  // extern "C"

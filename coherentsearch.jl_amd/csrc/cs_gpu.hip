@@ -336,8 +336,8 @@ __device__ uint32_t block_select(const uint32_t (&keys)[kSelPer], uint32_t kth, 
     return prefix;
 }
 
-__global__ __launch_bounds__(kSelThreads) void k_sigma32(FoldSet fs, GridDev g, BatchDev b,
-                                                         int64_t ldp) {
+__global__ __launch_bounds__(kSelThreads) void k_sigma32(FoldSet fs, GridDev g, BatchDev b) {
+    const int64_t ldp = fs.ldp;
     __shared__ uint32_t hist[2048];
     __shared__ uint32_t sh[64];
     const int cl = blockIdx.x, f = blockIdx.y, nf = fs.nf;
@@ -390,7 +390,7 @@ hipError_t launch_sigma_f32(hipStream_t st, const FoldSet& fs, const GridDev& g,
                             const BatchDev& b) {
     if (b.nblocks <= 0) return hipSuccess;
     dim3 grid((unsigned)b.nblocks, (unsigned)fs.nf);
-    hipLaunchKernelGGL(k_sigma32, grid, dim3(kSelThreads), 0, st, fs, g, b, b.nt > 0 ? 0 : 0);
+    hipLaunchKernelGGL(k_sigma32, grid, dim3(kSelThreads), 0, st, fs, g, b);
     return hipGetLastError();
 }
 
@@ -430,12 +430,34 @@ __device__ __forceinline__ void push_survivor(bool pred, int64_t t, int fidx, co
     }
 }
 
-template <int NB>
+// One bank width (compile-time w = bank_w(A)) of the zero-baseline scan, then
+// the next; every prefix-sum index is a constant, so ps stays in registers.
+template <int NB, int WM, int A>
+__device__ __forceinline__ void gate_widths(const float* ps, uint32_t bankmask, float invsig_t,
+                                            float& res) {
+    if constexpr (A < bank_count(WM)) {
+        constexpr int w = bank_w(A);
+        if (bankmask & (1u << A)) {
+            float invsw = invsig_t / sqrtf((float)w);
+            float mb = ps[w] - ps[0];
+#pragma unroll
+            for (int p = 1; p < NB; ++p) {
+                float d = ps[p + w] - ps[p];
+                mb = d > mb ? d : mb;
+            }
+            float c = mb * invsw;
+            res = c > res ? c : res;
+        }
+        gate_widths<NB, WM, A + 1>(ps, bankmask, invsig_t, res);
+    }
+}
+
+// WM: compile-time bound on the fold's widest boxcar (6, the ladder cap, or the
+// full default bank of this length), so the prefix sums stay in registers.
+template <int NB, int WM>
 __global__ __launch_bounds__(256) void k_gate(const float* __restrict__ prof, int64_t ldp,
                                               int fidx, uint32_t bankmask, GridDev g,
                                               BatchDev b, int nf, OutDev o) {
-    constexpr int W = GateCT<NB>::wmax;
-    constexpr int NBK = GateCT<NB>::nbank;
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool pred = false;
     int64_t t = b.t_base + j;
@@ -449,30 +471,15 @@ __global__ __launch_bounds__(256) void k_gate(const float* __restrict__ prof, in
             if (!(o.medcut > -INFINITY) || invs <= 0) {
                 pred = true;
             } else {
-                float x[NB];
-#pragma unroll
-                for (int i = 0; i < NB; ++i) x[i] = prof[(int64_t)i * ldp + j];
-                float ps[NB + W + 1];
+                float ps[NB + WM + 1];
                 ps[0] = 0.0f;
 #pragma unroll
-                for (int i = 1; i <= NB + W; ++i) ps[i] = ps[i - 1] + x[(i - 1) % NB];
-                const float invsig_t = (float)invs;
+                for (int i = 1; i <= NB; ++i) ps[i] = ps[i - 1] + prof[(int64_t)(i - 1) * ldp + j];
+#pragma unroll
+                for (int i = NB + 1; i <= NB + WM; ++i)
+                    ps[i] = ps[i - 1] + prof[(int64_t)(i - NB - 1) * ldp + j];
                 float res = -INFINITY;
-#pragma unroll
-                for (int a = 0; a < NBK; ++a) {
-                    const int w = bank_w(a);
-                    if (bankmask & (1u << a)) {
-                        float invsw = invsig_t / sqrtf((float)w);
-                        float mb = ps[w] - ps[0];
-#pragma unroll
-                        for (int p = 1; p < NB; ++p) {
-                            float d = ps[p + w] - ps[p];
-                            mb = d > mb ? d : mb;
-                        }
-                        float c = mb * invsw;
-                        res = c > res ? c : res;
-                    }
-                }
+                gate_widths<NB, WM, 0>(ps, bankmask, (float)invs, res);
                 pred = (double)res >= o.medcut;
             }
         }
@@ -530,16 +537,7 @@ __global__ __launch_bounds__(64) void k_gate_generic(const float* __restrict__ p
 hipError_t launch_gate_f32(hipStream_t st, const FoldSet& fs, const GridDev& g,
                            const BatchDev& b, const OutDev& o) {
     if (b.nt <= 0) return hipSuccess;
-    // leading dimension of the fold profiles = batch column capacity, passed via
-    // the first fold's cap slot is not available: profiles are laid out with
-    // ld = o.surv_cap / nf (the batch capacity); see cs_api.cpp.
-    return hipErrorInvalidValue;
-}
-
-// Gate launcher with an explicit leading dimension (used by cs_api.cpp).
-hipError_t launch_gate_f32_ld(hipStream_t st, const FoldSet& fs, const GridDev& g,
-                              const BatchDev& b, const OutDev& o, int64_t ldp) {
-    if (b.nt <= 0) return hipSuccess;
+    const int64_t ldp = fs.ldp;
     dim3 grid((unsigned)((b.nt + 255) / 256));
     for (int f = 0; f < fs.nf; ++f) {
         const FoldDev& fd = fs.f[f];
@@ -549,9 +547,13 @@ hipError_t launch_gate_f32_ld(hipStream_t st, const FoldSet& fs, const GridDev& 
             switch (fd.nbins) {
 #define X(S)                                                                                 \
     case S:                                                                                  \
-        if (fd.widths[fd.nw - 1] <= GateCT<S>::wmax) {                                       \
-            hipLaunchKernelGGL(k_gate<S>, grid, dim3(256), 0, st, fs.prof[f], ldp, f,          \
+        if (fd.widths[fd.nw - 1] <= 6 && GateCT<S>::wmax >= 6) {                             \
+            hipLaunchKernelGGL((k_gate<S, 6>), grid, dim3(256), 0, st, fs.prof[f], ldp, f,     \
                                fs.bankmask[f], g, b, fs.nf, o);                               \
+            launched = true;                                                                 \
+        } else if (fd.widths[fd.nw - 1] <= GateCT<S>::wmax) {                                \
+            hipLaunchKernelGGL((k_gate<S, GateCT<S>::wmax>), grid, dim3(256), 0, st,          \
+                               fs.prof[f], ldp, f, fs.bankmask[f], g, b, fs.nf, o);          \
             launched = true;                                                                 \
         }                                                                                    \
         break;

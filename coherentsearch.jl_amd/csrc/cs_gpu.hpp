@@ -44,6 +44,7 @@ struct GridDev {
 struct FoldSet {
     int32_t nf;
     int32_t pad;
+    int64_t ldp;                     // leading dimension of every fold's profiles
     FoldDev f[kMaxFolds];
     float* prof[kMaxFolds];          // FP32 profiles, [bin][batch trial]
     const double2* twid[kMaxFolds];  // (cos, sin)(2 pi e / nbins), e < nbins

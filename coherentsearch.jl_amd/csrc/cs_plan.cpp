@@ -18,7 +18,7 @@
 
 namespace cs {
 
-static double sinpi_d(double x) {
+double sinpi_d(double x) {
     double r = std::fmod(x, 2.0);
     if (r < 0) r += 2.0;
     bool neg = false;
@@ -30,7 +30,7 @@ static double sinpi_d(double x) {
     return neg ? -v : v;
 }
 
-static double cospi_d(double x) {
+double cospi_d(double x) {
     double r = std::fabs(std::fmod(x, 2.0));
     if (r > 1.0) r = 2.0 - r;
     if (r <= 0.25) return std::cos(M_PI * r);

@@ -43,6 +43,10 @@ struct Plans {
     std::vector<double> A64;
 };
 
+// sin(pi x), cos(pi x) with exact argument reduction.
+double sinpi_d(double x);
+double cospi_d(double x);
+
 bool trial_grid_rational(double lodr, int64_t* pnum, int64_t* q);
 Plans build_plans(int nharms, int m, double hidr, double r_lo);
 

@@ -1,0 +1,205 @@
+"""GPU parity: libcsgpu.so (through its C ABI) against the CPU oracle on the
+same seeded inputs.
+
+Bars: integer work (trial indices, folds, blocks, row-zeroing) bit-exact; the
+FP32-weight interpolation bit-exact (same FMA chain); FP64 bridge metrics
+(chunk_metrics / decim / block_metrics :direct) within 1e-12 relative;
+search candidates: identical (t, k) sets away from threshold and S/N within
+1e-4 relative (the FP32 profile stage moves the block sigma by ~1e-7).
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import noise_amps
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cs():
+    import coherentsearch
+    return coherentsearch
+
+
+@pytest.fixture(scope="module")
+def O():
+    import oracle
+    return oracle
+
+
+def _engine(cs, **kw):
+    bs = kw.pop("blocksize", 2048)
+    p = cs.SearchParams(**kw)
+    return cs.Engine(p, bs), p
+
+
+def _ft(cs, amps, N, dt=64e-6):
+    return cs.FFTFile.from_arrays(amps, N, dt)
+
+
+@pytest.mark.parametrize("nharms,m,r_lo,t0,n", [
+    (32, 32, 10010.0, 0, 256),
+    (32, 32, 10010.0, 37, 100),      # misaligned with the 32-trial groups
+    (60, 16, 5000.0, 0, 300),
+    (60, 16, 5000.25, 1000, 77),
+    (8, 16, 67.108864, 0, 2048),      # C1-like grid
+    (60, 16, 4000.0, 0, 129),         # integer r_lo: dr == 0 delta phases
+    (32, 32, 30000.0, 0, 200),        # top harmonics run off the amplitudes (rows zeroed)
+])
+@pytest.mark.parametrize("f64", [False, True])
+def test_interp_rows_bit_exact(cs, O, nharms, m, r_lo, t0, n, f64):
+    _say(f"interp nharms={nharms} m={m} r_lo={r_lo} t0={t0} n={n} f64={f64}")
+    N = 1 << 17
+    amps = noise_amps(N // 2, seed=nharms + m + n)
+    eng, p = _engine(cs, nharms=nharms, m=m)
+    eng.set_spectrum(_ft(cs, amps, N))
+    got = eng.interp_rows(r_lo, t0, n, weights_f64=f64)
+    want = O.Plans(nharms, m, 0.5, r_lo, f64).fill_chunk(amps, N, t0, n)
+    assert got.shape == want.shape
+    zero_rows_g = [h for h in range(nharms + 1) if not np.any(got[h])]
+    zero_rows_w = [h for h in range(nharms + 1) if not np.any(want[h])]
+    assert zero_rows_g == zero_rows_w
+    assert np.array_equal(got, want), np.max(np.abs(got - want))
+
+
+@pytest.mark.parametrize("f64", [False, True])
+@pytest.mark.parametrize("nharms,m,rstart,n", [(32, 32, 10010.0, 256), (60, 16, 5000.0, 512),
+                                               (8, 16, 1000.5, 100)])
+def test_chunk_metrics_vs_oracle(cs, O, nharms, m, rstart, n, f64):
+    _say(f"chunk_metrics nharms={nharms} m={m} rstart={rstart} n={n} f64={f64}")
+    N = 1 << 18
+    amps = noise_amps(N // 2, seed=5)
+    eng, p = _engine(cs, nharms=nharms, m=m, blocksize=n)
+    eng.set_spectrum(_ft(cs, amps, N))
+    got = eng.chunk_metrics(rstart, n, weights_f64=f64)
+    want = O.chunk_metrics(O.make_params(nharms=nharms, m=m), amps, N, rstart, n, f64)
+    rel = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    _say(f"  rel={rel:.3e}")
+    assert rel < 1e-12
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 6])
+def test_decim_metrics_vs_oracle(cs, O, k):
+    _say(f"decim k={k}")
+    N = 1 << 18
+    amps = noise_amps(N // 2, seed=9)
+    eng, p = _engine(cs, nharms=60, m=32, decimations=[1, k], blocksize=64)
+    eng.set_spectrum(_ft(cs, amps, N))
+    got = eng.decim_metrics(5000.0, 64, k, weights_f64=True)
+    want = O.decim_metrics(O.make_params(nharms=60, m=32, decimations=[1, k]), amps, N, 5000.0,
+                           64, k, True)
+    assert len(got) == len(want) == 64
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-12
+
+
+def test_decim_nvalid_near_nyquist(cs, O):
+    """Decimated trials past Nyquist are cut at the reference's exact FP64 prefix."""
+    N = 1 << 16
+    amps = noise_amps(N // 2, seed=3)
+    # k * r crosses N/2 = 32768 inside the block
+    rstart = 32768 / 3 - 0.7
+    eng, p = _engine(cs, nharms=6, m=16, decimations=[1, 3], blocksize=128)
+    eng.set_spectrum(_ft(cs, amps, N))
+    got = eng.decim_metrics(rstart, 128, 3)
+    want = O.decim_metrics(O.make_params(nharms=6, m=16, decimations=[1, 3]), amps, N, rstart, 128, 3)
+    assert len(got) == len(want) and 0 < len(want) < 128
+    assert np.allclose(got, want, rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("nharms,m,widths", [(32, 32, None), (16, 16, [1, 2, 3, 4, 6, 9])])
+def test_block_metrics_direct_vs_oracle(cs, O, nharms, m, widths):
+    N = 1 << 18
+    amps = noise_amps(N // 2, seed=21)
+    eng, p = _engine(cs, nharms=nharms, m=m)
+    eng.set_spectrum(_ft(cs, amps, N))
+    rf = 10010.0 + np.arange(256) * (0.5 / nharms)
+    got = eng.block_metrics(rf, "direct", widths)
+    want = O.block_metrics(O.make_params(nharms=nharms, m=m), amps, N, rf, "direct", widths)
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-10
+
+
+def test_block_metrics_fft_kernel_rejected(cs):
+    N = 1 << 16
+    eng, p = _engine(cs, nharms=8, m=16)
+    eng.set_spectrum(_ft(cs, noise_amps(N // 2, 1), N))
+    with pytest.raises(ValueError):
+        eng.block_metrics(1000.0 + np.arange(16) / 16.0, "fft")
+
+
+def _compare_search(cands, want, threshold, tol=1e-4):
+    """(t, k) sets equal apart from trials within tol of threshold; metrics agree."""
+    g = {(int(c["t"]), int(c["k"])): c for c in cands}
+    w = {(int(c["t"]), int(c["k"])): c for c in want}
+    only_g = [key for key in g if key not in w]
+    only_w = [key for key in w if key not in g]
+    for key in only_g:
+        assert abs(g[key]["metric"] - threshold) <= tol * threshold, ("extra", key, g[key])
+    for key in only_w:
+        assert abs(w[key]["metric"] - threshold) <= tol * threshold, ("missing", key, w[key])
+    worst = 0.0
+    for key in set(g) & set(w):
+        assert g[key]["nharm"] == w[key]["nharm"] and g[key]["block"] == w[key]["block"]
+        worst = max(worst, abs(g[key]["metric"] - w[key]["metric"]) / abs(w[key]["metric"]))
+    assert worst <= tol, worst
+    return worst, len(set(g) & set(w))
+
+
+@pytest.mark.parametrize("nharms,maxdecim,blocksize,thr", [(8, 1, 2048, 6.0), (32, 4, 512, 6.0),
+                                                           (60, 6, 2048, 6.5), (60, 6, 1000, 6.5)])
+def test_search_vs_oracle(cs, O, small_spectrum, nharms, maxdecim, blocksize, thr):
+    amps, N, dt = small_spectrum
+    T = N * dt
+    dec = cs.decimation_set(nharms, maxdecim)
+    _say(f"search nharms={nharms} decims={dec} blocksize={blocksize}")
+    eng, p = _engine(cs, nharms=nharms, m=16, decimations=dec, blocksize=blocksize)
+    eng.set_spectrum(_ft(cs, amps, N, dt))
+    r_lo, r_hi = 1.0 * T, 60.0 * T
+    lodr = 0.5 / nharms
+    total = math.floor((r_hi - r_lo) / lodr) + 1
+    got = eng.search_raw(r_lo, r_hi, thr)
+    want = O.search_region(O.make_params(nharms=nharms, m=16, decimations=dec), amps, N, r_lo,
+                           total, blocksize, threshold=thr)
+    assert len(want) > 0
+    worst, ncommon = _compare_search(got, want, thr)
+    _say(f"  {len(got)} vs {len(want)} candidates, common {ncommon}, worst rel {worst:.2e}")
+
+
+def test_search_gate_equals_exact(cs, small_spectrum):
+    """boxcar_medmargin = Inf (exact scan everywhere) gives the identical list
+    (test/test_search.jl:287-309)."""
+    amps, N, dt = small_spectrum
+    T = N * dt
+    dec = cs.decimation_set(60, 4)
+    e1, _ = _engine(cs, nharms=60, m=32, decimations=dec, blocksize=512)
+    e2, _ = _engine(cs, nharms=60, m=32, decimations=dec, blocksize=512, boxcar_medmargin=math.inf)
+    for e in (e1, e2):
+        e.set_spectrum(_ft(cs, amps, N, dt))
+    a = e1.search_raw(12.0 * T, 15.0 * T, 6.0)
+    b = e2.search_raw(12.0 * T, 15.0 * T, 6.0)
+    assert len(a) > 0
+    assert np.array_equal(a, b)
+
+
+def test_search_detects_injection(cs, small_spectrum):
+    amps, N, dt = small_spectrum
+    ft = _ft(cs, amps, N, dt)
+    p = cs.SearchParams(nharms=32, m=16, decimations=cs.decimation_set(32, 4))
+    cands = cs.search(ft, p, lofreq=5.0, hifreq=30.0, threshold=8.0)
+    assert cands
+    best = max(cands, key=lambda c: c.metric)
+    assert abs(best.freq - 13.7) < 1.0 / (N * dt)
+
+
+def test_invalid_params_raise(cs):
+    with pytest.raises(ValueError):
+        cs.Engine(cs.SearchParams(m=15))
+    with pytest.raises(ValueError):
+        cs.Engine(cs.SearchParams(precision="f16"))
