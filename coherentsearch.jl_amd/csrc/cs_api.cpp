@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "cs_fused.hpp"
 #include "cs_gpu.hpp"
 #include "cs_plan.hpp"
 
@@ -24,7 +25,8 @@ struct DBuf {
 };
 
 constexpr int kBankW[12] = {1, 2, 3, 4, 6, 9, 13, 19, 28, 42, 63, 94};
-constexpr int64_t kBatchTrials = (int64_t)1 << 21;   // trials per batch (HBM scratch bound)
+constexpr int64_t kBatchTrials = (int64_t)1 << 21;   // plain path: trials per batch (HBM scratch bound)
+constexpr int64_t kBatchTrialsFused = (int64_t)1 << 25;   // fused path: ~1 GB of bounds + samples per batch
 constexpr int kMaxHarm = 64;                         // device path: profiles of <= 128 bins
 
 }  // namespace
@@ -47,7 +49,7 @@ struct cs_ctx {
     DBuf harm, row, carry, W32, A32, W64, A64;
     std::map<int, DBuf> twid;
     // scratch
-    DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout;
+    DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
     std::vector<hipEvent_t> ev;
     double timings[8] = {0};
 
@@ -248,7 +250,7 @@ int cs_ctx_destroy(cs_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     DBuf* bufs[] = {&c->amps, &c->harm, &c->row, &c->carry, &c->W32, &c->A32, &c->W64, &c->A64,
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
-                    &c->widths, &c->rfund, &c->rowok, &c->mout};
+                    &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples};
     for (DBuf* b : bufs) release(*b);
     for (auto& kv : c->twid) release(kv.second);
     for (hipEvent_t ev : c->ev) (void)hipEventDestroy(ev);
@@ -260,7 +262,6 @@ int cs_ctx_destroy(cs_ctx* c) {
 int cs_set_mode(cs_ctx* c, int32_t mode) {
     if (!c) return CS_EINVAL;
     if (mode != CS_MODE_PLAIN && mode != CS_MODE_FUSED) return c->fail(CS_EINVAL, "unknown mode");
-    if (mode == CS_MODE_FUSED) return c->fail(CS_EINVAL, "fused mode is not available in this build");
     c->mode = mode;
     return CS_OK;
 }
@@ -444,7 +445,6 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
     for (double& v : c->timings) v = 0.0;
     int s = check_spectrum(c);
     if (s) return s;
-    if (c->mode != CS_MODE_PLAIN) return c->fail(CS_EINVAL, "fused mode is not available in this build");
     HIPCHK(c, hipSetDevice(c->device));
     const int nh = c->nharms;
     const double lodr = c->hidr / nh;
@@ -479,7 +479,12 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         if ((s = get_twid(c, fd.nbins, &fs.twid[f]))) return s;
         rows_total += fd.nbins;
     }
-    const int64_t Bc = std::max<int64_t>(1, std::min<int64_t>(nchunks, kBatchTrials / Nprof));
+    const bool fused = c->mode == CS_MODE_FUSED;
+    if (fused && !fused_supported(fs, nh))
+        return c->fail(CS_EINVAL, "fused mode supports profile lengths 16/20/24/30/32/40/60/64/120/128 "
+                                  "with the default boxcar bank; use the plain mode");
+    const int64_t Bc = std::max<int64_t>(
+        1, std::min<int64_t>(nchunks, (fused ? kBatchTrialsFused : kBatchTrials) / Nprof));
     const int64_t BT = Bc * Nprof;
     fs.ldp = BT;
 
@@ -494,12 +499,23 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
     }
     if ((s = upload(c, c->nvalid, nv))) return s;
     if ((s = ensure(c, c->sigma, (size_t)nchunks * nf * sizeof(double)))) return s;
-    if ((s = ensure(c, c->ftp, (size_t)nh * BT * sizeof(float2)))) return s;
-    if ((s = ensure(c, c->prof, (size_t)rows_total * BT * sizeof(float)))) return s;
+    FusedDev fu{};
+    if (fused) {
+        if ((s = ensure(c, c->mraw, (size_t)nf * BT * sizeof(float)))) return s;
+        if ((s = ensure(c, c->samples, (size_t)Bc * nf * kSigmaSamples * sizeof(float)))) return s;
+        if ((s = ensure(c, c->rowok, (size_t)Bc * nh))) return s;
+        fu.mraw = (float*)c->mraw.p;
+        fu.ldm = BT;
+        fu.samples = (float*)c->samples.p;
+        fu.rowok = (const uint8_t*)c->rowok.p;
+    } else {
+        if ((s = ensure(c, c->ftp, (size_t)nh * BT * sizeof(float2)))) return s;
+        if ((s = ensure(c, c->prof, (size_t)rows_total * BT * sizeof(float)))) return s;
+    }
     if ((s = ensure(c, c->surv, (size_t)BT * nf * sizeof(uint64_t)))) return s;
     if ((s = ensure(c, c->counters, 4 * sizeof(uint32_t)))) return s;
     int64_t rowo = 0;
-    for (int f = 0; f < nf; ++f) {
+    for (int f = 0; f < nf && !fused; ++f) {
         fs.prof[f] = (float*)c->prof.p + rowo * BT;
         rowo += fs.f[f].nbins;
     }
@@ -545,17 +561,28 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             o.threshold = threshold;
             o.medcut = medcut;
             HIPCHK(c, hipEventRecord(E[0], c->st));
-            HIPCHK(c, launch_interp(c->st, sp, tb, g, t_base, nt, false, c->ftp.p, BT));
-            HIPCHK(c, hipEventRecord(E[1], c->st));
-            for (int f = 0; f < nf; ++f)
-                HIPCHK(c, launch_c2r_f32(c->st, (const float2*)c->ftp.p, BT, fs.f[f].k, fs.f[f].nbins,
-                                         nt, fs.prof[f], BT, fs.twid[f]));
-            HIPCHK(c, hipEventRecord(E[2], c->st));
-            HIPCHK(c, launch_sigma_f32(c->st, fs, g, b));
-            HIPCHK(c, hipEventRecord(E[3], c->st));
-            HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
-            HIPCHK(c, launch_gate_f32(c->st, fs, g, b, o));
-            HIPCHK(c, hipEventRecord(E[4], c->st));
+            if (fused) {
+                HIPCHK(c, launch_guard(c->st, sp, tb, g, b, (uint8_t*)c->rowok.p));
+                HIPCHK(c, launch_fused(c->st, sp, tb, g, fs, b, fu));
+                HIPCHK(c, hipEventRecord(E[1], c->st));
+                HIPCHK(c, hipEventRecord(E[2], c->st));
+                HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
+                HIPCHK(c, launch_select(c->st, fs, g, b, fu, o));
+                HIPCHK(c, hipEventRecord(E[3], c->st));
+                HIPCHK(c, hipEventRecord(E[4], c->st));
+            } else {
+                HIPCHK(c, launch_interp(c->st, sp, tb, g, t_base, nt, false, c->ftp.p, BT));
+                HIPCHK(c, hipEventRecord(E[1], c->st));
+                for (int f = 0; f < nf; ++f)
+                    HIPCHK(c, launch_c2r_f32(c->st, (const float2*)c->ftp.p, BT, fs.f[f].k,
+                                             fs.f[f].nbins, nt, fs.prof[f], BT, fs.twid[f]));
+                HIPCHK(c, hipEventRecord(E[2], c->st));
+                HIPCHK(c, launch_sigma_f32(c->st, fs, g, b));
+                HIPCHK(c, hipEventRecord(E[3], c->st));
+                HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
+                HIPCHK(c, launch_gate_f32(c->st, fs, g, b, o));
+                HIPCHK(c, hipEventRecord(E[4], c->st));
+            }
             HIPCHK(c, launch_rescan(c->st, sp, tb, g, fs, b, o, 8192));
             HIPCHK(c, hipEventRecord(E[5], c->st));
         }
@@ -579,7 +606,8 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             for (int q = 0; q < 5; ++q) {
                 float ms = 0.f;
                 HIPCHK(c, hipEventElapsedTime(&ms, E[q], E[q + 1]));
-                c->timings[q] += ms;
+                // fused mode: interval 0 is the fused interpolation/c2r/bound kernel
+                c->timings[(fused && q == 0) ? 5 : q] += ms;
             }
             float tot = 0.f;
             HIPCHK(c, hipEventElapsedTime(&tot, E[0], E[5]));

@@ -1,0 +1,361 @@
+// Fused search pipeline for gfx950.
+//
+// One workgroup owns a tile of 64 consecutive trials.  It interpolates every
+// harmonic of the tile into LDS, transforms each decimation fold in registers
+// (one profile per lane), and leaves only two things in HBM: the zero-baseline
+// boxcar bound of every (trial, fold) before sigma scaling, and the sigma
+// samples of the block's sampled columns.  A second kernel then selects each
+// (block, fold)'s exact sigma, thresholds the bounds and queues survivors for
+// the exact FP64 rescan (k_rescan, shared with the plain path).
+#include <hip/hip_runtime.h>
+
+#define CS_HD __host__ __device__
+#include "c2r_gen.hpp"
+#include "cs_device.hpp"
+#include "cs_fused.hpp"
+
+namespace cs {
+
+constexpr int kTile = 64;
+
+// Row guard of every (block, harmonic) of the batch (directinterp.jl:490-496).
+__global__ void k_guard(SpecDev sp, TabDev tb, GridDev g, int64_t c_first, int nblocks,
+                        uint8_t* rowok) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nblocks * tb.nharms) return;
+    int cl = i / tb.nharms, h = i - cl * tb.nharms + 1;
+    int64_t cg = c_first + cl;
+    int64_t first = g.origin + cg * g.Nprof;
+    int64_t last = min(first + g.Nprof, g.total) - 1;
+    const HarmDev d = tb.harm[h - 1];
+    bool ok;
+    if (d.P == 0) ok = true;   // the per-trial fallback guards itself
+    else ok = row_guard(d, tb, sp, first, last);
+    rowok[i] = ok ? 1 : 0;
+}
+
+// Raw zero-baseline bound: max over the fold's bank of max_p(boxsum)/sqrt(w),
+// prefix sums in the reference's sequential FP32 order.
+template <int NB, int WM, int A>
+__device__ __forceinline__ void raw_widths(const float* ps, uint32_t bankmask, float& res) {
+    if constexpr (A < bank_count(WM)) {
+        constexpr int w = bank_w(A);
+        if (bankmask & (1u << A)) {
+            float mb = ps[w] - ps[0];
+#pragma unroll
+            for (int p = 1; p < NB; ++p) {
+                float d = ps[p + w] - ps[p];
+                mb = d > mb ? d : mb;
+            }
+            float c = mb * (1.0f / sqrtf((float)w));
+            res = c > res ? c : res;
+        }
+        raw_widths<NB, WM, A + 1>(ps, bankmask, res);
+    }
+}
+
+template <int NB, int WM>
+__device__ __forceinline__ void fold_tile(const float2 (*ftp)[kTile], int lane, int k,
+                                          uint32_t bankmask, bool valid, int64_t j, int64_t nv,
+                                          int64_t cap, float* samp, float* mraw_out) {
+    constexpr int H = NB / 2;
+    if (!valid) return;
+    if (j >= nv) {
+        *mraw_out = -INFINITY;
+        return;
+    }
+    float xr[H + 1], xi[H + 1];
+    xr[0] = 0.0f;
+    xi[0] = 0.0f;
+#pragma unroll
+    for (int h = 1; h <= H; ++h) {
+        float2 v = ftp[h * k - 1][lane];
+        xr[h] = v.x;
+        xi[h] = v.y;
+    }
+    float out[NB];
+    cs_c2r::Fixed<NB>::run(xr, xi, out);
+    // sigma samples (_block_sigma, search.jl:504-538): whole columns
+    const int64_t NN = (int64_t)NB * nv;
+    int64_t slot = -1, take = NB;
+    if (NN <= cap) {
+        slot = j * NB;
+    } else {
+        int64_t ncol = cap / NB;
+        if (ncol < 1) ncol = 1;
+        if (ncol > nv) ncol = nv;
+        int64_t c = (j * ncol + nv - 1) / nv;
+        if (c < ncol && (c * nv) / ncol == j) {
+            slot = c * NB;
+            take = cap - slot < NB ? cap - slot : NB;
+        }
+    }
+    if (slot >= 0) {
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+            if (i < take) samp[slot + i] = fabsf(out[i]);
+    }
+    float ps[NB + WM + 1];
+    ps[0] = 0.0f;
+#pragma unroll
+    for (int i = 1; i <= NB + WM; ++i) ps[i] = ps[i - 1] + out[(i - 1) % NB];
+    float res = -INFINITY;
+    raw_widths<NB, WM, 0>(ps, bankmask, res);
+    *mraw_out = res;
+}
+
+#define CS_FUSED_SIZES(X) X(16) X(20) X(24) X(30) X(32) X(40) X(60) X(64) X(120) X(128)
+
+__host__ __device__ constexpr bool fused_size(int nb) {
+    return nb == 16 || nb == 20 || nb == 24 || nb == 30 || nb == 32 || nb == 40 || nb == 60 ||
+           nb == 64 || nb == 120 || nb == 128;
+}
+
+template <int MT>
+__global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g, FoldSet fs,
+                                               BatchDev b, FusedDev fu) {
+    __shared__ float2 ftp[64][kTile];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int64_t jl = (int64_t)blockIdx.x * kTile + lane;     // batch column
+    const bool valid = jl < b.nt;
+    const int64_t t = b.t_base + jl;
+    int64_t cg = 0, first = 0, last = 0;
+    if (valid) block_of(g, t, cg, first, last);
+    const int cl = (int)(cg - b.c_first);
+    const int nh = tb.nharms;
+    for (int h = wv + 1; h <= nh; h += 4) {
+        double2 a = make_double2(0.0, 0.0);
+        if (valid && fu.rowok[cl * nh + h - 1]) a = interp_one<MT>(sp, tb, h, t, first, last, true, true);
+        ftp[h - 1][lane] = make_float2((float)a.x, (float)a.y);
+    }
+    __syncthreads();
+    const int64_t j = t - first;
+    for (int f = wv; f < fs.nf; f += 4) {
+        const FoldDev& fd = fs.f[f];
+        const int64_t nv = valid ? b.nvalid[cl * fs.nf + f] : 0;
+        float* samp = fu.samples + ((int64_t)cl * fs.nf + f) * kSigmaSamples;
+        float* mo = fu.mraw + (int64_t)f * fu.ldm + jl;
+        const bool small = fd.widths[fd.nw - 1] <= 6;
+        switch (fd.nbins) {
+#define X(S)                                                                                  \
+    case S:                                                                                   \
+        if (small && GateCT<S>::wmax >= 6)                                                    \
+            fold_tile<S, 6>(ftp, lane, fd.k, fs.bankmask[f], valid, j, nv, fs.cap[f], samp, mo); \
+        else                                                                                  \
+            fold_tile<S, GateCT<S>::wmax>(ftp, lane, fd.k, fs.bankmask[f], valid, j, nv,       \
+                                          fs.cap[f], samp, mo);                               \
+        break;
+            CS_FUSED_SIZES(X)
+#undef X
+            default:
+                break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Exact selection of the median |x| of one (block, fold)'s samples: 3-pass
+// radix select on the FP32 bit patterns with wave-aggregated LDS histogram
+// updates (lanes holding the same digit add once), then the lower central
+// order statistic from a count and a max.
+// ---------------------------------------------------------------------------
+constexpr int kST = 512;
+constexpr int kSPer = 16;
+
+__device__ __forceinline__ void hist_add(uint32_t* hist, uint32_t digit, bool active, int nbits) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long peers = __ballot(active);
+    if (peers == 0) return;
+    for (int bt = 0; bt < nbits; ++bt) {
+        unsigned long long v = __ballot(active && ((digit >> bt) & 1u));
+        peers &= ((digit >> bt) & 1u) ? v : ~v;
+    }
+    if (active) {
+        int leader = __ffsll((long long)peers) - 1;
+        if (lane == leader) atomicAdd(&hist[digit], (uint32_t)__popcll(peers));
+    }
+}
+
+__device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* sh) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sh[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < kST / 64; ++w) {
+            uint32_t tmp = sh[w];
+            sh[w] = acc;
+            acc += tmp;
+        }
+    }
+    __syncthreads();
+    uint32_t r = x - v + sh[wid];
+    __syncthreads();
+    return r;
+}
+
+__device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t kth, uint32_t* hist,
+                               uint32_t* sh) {
+    uint32_t prefix = 0, mask = 0;
+    const int shifts[3] = {21, 10, 0};
+    const int bits[3] = {11, 11, 10};
+    for (int pass = 0; pass < 3; ++pass) {
+        const int s_ = shifts[pass];
+        const uint32_t dmask = (1u << bits[pass]) - 1u;
+        for (int i = threadIdx.x; i < 2048; i += kST) hist[i] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kSPer; ++r) {
+            uint32_t key = keys[r];
+            bool act = (key & mask) == prefix;
+            hist_add(hist, (key >> s_) & dmask, act, bits[pass]);
+        }
+        __syncthreads();
+        uint32_t a[4], loc = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { a[q] = hist[4 * threadIdx.x + q]; loc += a[q]; }
+        uint32_t ex = blk_excl_scan(loc, sh + 16);
+        if (ex <= kth && kth < ex + loc) {
+            uint32_t c = ex;
+            int d = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (kth >= c && kth < c + a[q]) { d = q; break; }
+                c += a[q];
+            }
+            sh[0] = 4 * threadIdx.x + d;
+            sh[1] = kth - c;
+        }
+        __syncthreads();
+        uint32_t dsel = sh[0];
+        kth = sh[1];
+        prefix |= dsel << s_;
+        mask |= dmask << s_;
+        __syncthreads();
+    }
+    return prefix;
+}
+
+__global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev b, FusedDev fu,
+                                                OutDev o) {
+    __shared__ uint32_t hist[2048];
+    __shared__ uint32_t sh[64];
+    const int cl = blockIdx.x, f = blockIdx.y, nf = fs.nf;
+    const int64_t cg = b.c_first + cl;
+    const int64_t first = g.origin + cg * g.Nprof;
+    const int64_t nv = b.nvalid[cl * nf + f];
+    const int nbins = fs.f[f].nbins;
+    const int64_t cap = fs.cap[f];
+    const int64_t NN = (int64_t)nbins * nv;
+    double sigma = 0.0;
+    if (NN > 0) {
+        int64_t ns;
+        if (NN <= cap) ns = NN;
+        else {
+            int64_t ncol = cap / nbins;
+            if (ncol < 1) ncol = 1;
+            if (ncol > nv) ncol = nv;
+            ns = ncol * nbins < cap ? ncol * nbins : cap;
+        }
+        const float* S = fu.samples + ((int64_t)cl * nf + f) * kSigmaSamples;
+        uint32_t keys[kSPer];
+#pragma unroll
+        for (int r = 0; r < kSPer; ++r) {
+            int64_t s = threadIdx.x + (int64_t)kST * r;
+            keys[r] = s < ns ? __float_as_uint(S[s]) : 0xFFFFFFFFu;
+        }
+        const uint32_t half = (uint32_t)(ns / 2);
+        const uint32_t up = select_kth(keys, half, hist, sh);
+        double med;
+        if (ns & 1) {
+            med = (double)__uint_as_float(up);
+        } else {
+            // lower central statistic: `up` itself if at least half keys are
+            // below-or-equal... i.e. if fewer than `half` keys are strictly below it
+            uint32_t below = 0, mx = 0;
+#pragma unroll
+            for (int r = 0; r < kSPer; ++r) {
+                uint32_t key = keys[r];
+                if (key < up) { below++; mx = key > mx ? key : mx; }
+            }
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                below += __shfl_xor(below, off, 64);
+                uint32_t om = __shfl_xor(mx, off, 64);
+                mx = om > mx ? om : mx;
+            }
+            const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+            if (lane == 0) { hist[wid] = below; hist[16 + wid] = mx; }
+            __syncthreads();
+            uint32_t tb_ = 0, tm = 0;
+            for (int w = 0; w < kST / 64; ++w) {
+                tb_ += hist[w];
+                tm = hist[16 + w] > tm ? hist[16 + w] : tm;
+            }
+            uint32_t lo = tb_ >= half ? tm : up;
+            med = 0.5 * ((double)__uint_as_float(lo) + (double)__uint_as_float(up));
+        }
+        sigma = 1.4826 * med;
+    }
+    if (threadIdx.x == 0) b.sigma[cl * nf + f] = sigma;
+    // threshold the block's bounds: m0 = bound * invsigma against medcut
+    const double invs = sigma > 0 ? 1.0 / sigma : 0.0;
+    const bool all = !(o.medcut > -INFINITY) || invs <= 0;
+    const int64_t col0 = first - b.t_base;
+    for (int64_t j0 = 0; j0 < nv; j0 += kST) {
+        int64_t j = j0 + threadIdx.x;
+        bool pred = false;
+        if (j < nv) {
+            if (all) pred = true;
+            else {
+                float m = fu.mraw[(int64_t)f * fu.ldm + col0 + j];
+                pred = (double)m * invs >= o.medcut;
+            }
+        }
+        push_survivor(pred, first + j, f, o);
+    }
+}
+
+hipError_t launch_guard(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
+                        const BatchDev& b, uint8_t* rowok) {
+    int n = b.nblocks * tb.nharms;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_guard, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, sp, tb, g,
+                       b.c_first, b.nblocks, rowok);
+    return hipGetLastError();
+}
+
+bool fused_supported(const FoldSet& fs, int nharms) {
+    if (nharms > 64) return false;
+    for (int f = 0; f < fs.nf; ++f) {
+        if (!fused_size(fs.f[f].nbins) || fs.bankmask[f] == 0) return false;
+    }
+    return true;
+}
+
+hipError_t launch_fused(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
+                        const FoldSet& fs, const BatchDev& b, const FusedDev& fu) {
+    if (b.nt <= 0) return hipSuccess;
+    dim3 grid((unsigned)((b.nt + kTile - 1) / kTile));
+    if (tb.m == 16 && !tb.weights_f64)
+        hipLaunchKernelGGL(k_fused<16>, grid, dim3(256), 0, st, sp, tb, g, fs, b, fu);
+    else
+        hipLaunchKernelGGL(k_fused<0>, grid, dim3(256), 0, st, sp, tb, g, fs, b, fu);
+    return hipGetLastError();
+}
+
+hipError_t launch_select(hipStream_t st, const FoldSet& fs, const GridDev& g, const BatchDev& b,
+                         const FusedDev& fu, const OutDev& o) {
+    if (b.nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_select, dim3((unsigned)b.nblocks, (unsigned)fs.nf), dim3(kST), 0, st, fs,
+                       g, b, fu, o);
+    return hipGetLastError();
+}
+
+}  // namespace cs

@@ -1,0 +1,22 @@
+// Fused search pipeline: launchers and the buffers it leaves in HBM.
+#pragma once
+#include "cs_gpu.hpp"
+
+namespace cs {
+
+struct FusedDev {
+    float* mraw;            // [fold][batch column]: zero-baseline bound before sigma
+    int64_t ldm;            // leading dimension of mraw
+    float* samples;         // [(block_local * nf + fold) * 8192 + slot]: sigma samples |x|
+    const uint8_t* rowok;   // [block_local * nharms + h - 1]: harmonic row kept for the block
+};
+
+bool fused_supported(const FoldSet& fs, int nharms);
+hipError_t launch_guard(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
+                        const BatchDev& b, uint8_t* rowok);
+hipError_t launch_fused(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
+                        const FoldSet& fs, const BatchDev& b, const FusedDev& fu);
+hipError_t launch_select(hipStream_t st, const FoldSet& fs, const GridDev& g, const BatchDev& b,
+                         const FusedDev& fu, const OutDev& o);
+
+}  // namespace cs

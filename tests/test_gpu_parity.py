@@ -152,14 +152,16 @@ def _compare_search(cands, want, threshold, tol=1e-4):
     return worst, len(set(g) & set(w))
 
 
+@pytest.mark.parametrize("mode", ["plain", "fused"])
 @pytest.mark.parametrize("nharms,maxdecim,blocksize,thr", [(8, 1, 2048, 6.0), (32, 4, 512, 6.0),
                                                            (60, 6, 2048, 6.5), (60, 6, 1000, 6.5)])
-def test_search_vs_oracle(cs, O, small_spectrum, nharms, maxdecim, blocksize, thr):
+def test_search_vs_oracle(cs, O, small_spectrum, nharms, maxdecim, blocksize, thr, mode):
     amps, N, dt = small_spectrum
     T = N * dt
     dec = cs.decimation_set(nharms, maxdecim)
-    _say(f"search nharms={nharms} decims={dec} blocksize={blocksize}")
+    _say(f"search mode={mode} nharms={nharms} decims={dec} blocksize={blocksize}")
     eng, p = _engine(cs, nharms=nharms, m=16, decimations=dec, blocksize=blocksize)
+    eng.set_mode(mode)
     eng.set_spectrum(_ft(cs, amps, N, dt))
     r_lo, r_hi = 1.0 * T, 60.0 * T
     lodr = 0.5 / nharms
@@ -172,7 +174,8 @@ def test_search_vs_oracle(cs, O, small_spectrum, nharms, maxdecim, blocksize, th
     _say(f"  {len(got)} vs {len(want)} candidates, common {ncommon}, worst rel {worst:.2e}")
 
 
-def test_search_gate_equals_exact(cs, small_spectrum):
+@pytest.mark.parametrize("mode", ["plain", "fused"])
+def test_search_gate_equals_exact(cs, small_spectrum, mode):
     """boxcar_medmargin = Inf (exact scan everywhere) gives the identical list
     (test/test_search.jl:287-309)."""
     amps, N, dt = small_spectrum
@@ -181,6 +184,7 @@ def test_search_gate_equals_exact(cs, small_spectrum):
     e1, _ = _engine(cs, nharms=60, m=32, decimations=dec, blocksize=512)
     e2, _ = _engine(cs, nharms=60, m=32, decimations=dec, blocksize=512, boxcar_medmargin=math.inf)
     for e in (e1, e2):
+        e.set_mode(mode)
         e.set_spectrum(_ft(cs, amps, N, dt))
     a = e1.search_raw(12.0 * T, 15.0 * T, 6.0)
     b = e2.search_raw(12.0 * T, 15.0 * T, 6.0)
@@ -203,3 +207,21 @@ def test_invalid_params_raise(cs):
         cs.Engine(cs.SearchParams(m=15))
     with pytest.raises(ValueError):
         cs.Engine(cs.SearchParams(precision="f16"))
+
+
+@pytest.mark.parametrize("nharms,maxdecim,blocksize,m", [(60, 6, 2048, 16), (32, 4, 777, 32),
+                                                         (8, 1, 2048, 16)])
+def test_fused_equals_plain(cs, small_spectrum, nharms, maxdecim, blocksize, m):
+    """Both modes feed the same exact rescan from the same FP32 profiles, so the
+    raw candidate lists are identical."""
+    amps, N, dt = small_spectrum
+    T = N * dt
+    dec = cs.decimation_set(nharms, maxdecim)
+    out = {}
+    for mode in ("plain", "fused"):
+        e, _ = _engine(cs, nharms=nharms, m=m, decimations=dec, blocksize=blocksize)
+        e.set_mode(mode)
+        e.set_spectrum(_ft(cs, amps, N, dt))
+        out[mode] = e.search_raw(2.0 * T, 40.0 * T, 6.0)
+    assert len(out["plain"]) > 0
+    assert np.array_equal(out["plain"], out["fused"])
