@@ -73,7 +73,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="plain", choices=["plain", "fused"])
+    ap.add_argument("--mode", default="plain", choices=["plain", "fused", "fused_valu"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
@@ -134,7 +134,7 @@ def main():
 
     # roofline of the dominant stage (per launch: one launch per batch of blocks)
     nb = max(1.0, stage.get("batches", 1.0))
-    dom = max(("interp", "c2r", "sigma", "gate", "rescan"), key=lambda k: stage.get(k, 0.0))
+    dom = max(("interp", "c2r", "sigma", "gate", "rescan", "fused"), key=lambda k: stage.get(k, 0.0))
     trials_per_launch = total / nb
     if dom == "interp":
         # algorithmic HBM bytes per trial: 4*(nharms+1)*hidr gathered + the FP32

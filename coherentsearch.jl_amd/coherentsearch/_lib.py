@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libcsgpu.so")
 HEADER = os.path.join(_HERE, "..", "..", "include", "coherent_search.h")
 
 CS_OK, CS_EINVAL, CS_EDIM, CS_ENOSPC, CS_EDEVICE, CS_ENOMEM = range(6)
-CS_MODE_PLAIN, CS_MODE_FUSED = 0, 1
+CS_MODE_PLAIN, CS_MODE_FUSED, CS_MODE_FUSED_VALU = 0, 1, 2
 
 
 class DimensionMismatch(ValueError):

@@ -78,7 +78,8 @@ class Engine:
         self.N, self.T = N, T
 
     def set_mode(self, mode: str):
-        m = {"plain": _lib.CS_MODE_PLAIN, "fused": _lib.CS_MODE_FUSED}[mode]
+        m = {"plain": _lib.CS_MODE_PLAIN, "fused": _lib.CS_MODE_FUSED,
+             "fused_valu": _lib.CS_MODE_FUSED_VALU}[mode]
         self._chk(_lib.lib().cs_set_mode(self._h, m), "cs_set_mode")
 
     # -- entry points ------------------------------------------------------------

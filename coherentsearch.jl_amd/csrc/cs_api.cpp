@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "cs_fused.hpp"
+#include "cs_mfma.hpp"
 #include "cs_gpu.hpp"
 #include "cs_plan.hpp"
 
@@ -50,6 +51,7 @@ struct cs_ctx {
     std::map<int, DBuf> twid;
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
+    DBuf m_whi, m_wlo, m_gA, m_cls, m_clst, m_ks, m_chi, m_clo;
     std::vector<hipEvent_t> ev;
     double timings[8] = {0};
 
@@ -250,7 +252,8 @@ int cs_ctx_destroy(cs_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     DBuf* bufs[] = {&c->amps, &c->harm, &c->row, &c->carry, &c->W32, &c->A32, &c->W64, &c->A64,
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
-                    &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples};
+                    &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples,
+                    &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_cls, &c->m_clst, &c->m_ks, &c->m_chi, &c->m_clo};
     for (DBuf* b : bufs) release(*b);
     for (auto& kv : c->twid) release(kv.second);
     for (hipEvent_t ev : c->ev) (void)hipEventDestroy(ev);
@@ -261,7 +264,8 @@ int cs_ctx_destroy(cs_ctx* c) {
 
 int cs_set_mode(cs_ctx* c, int32_t mode) {
     if (!c) return CS_EINVAL;
-    if (mode != CS_MODE_PLAIN && mode != CS_MODE_FUSED) return c->fail(CS_EINVAL, "unknown mode");
+    if (mode != CS_MODE_PLAIN && mode != CS_MODE_FUSED && mode != CS_MODE_FUSED_VALU)
+        return c->fail(CS_EINVAL, "unknown mode");
     c->mode = mode;
     return CS_OK;
 }
@@ -479,12 +483,49 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         if ((s = get_twid(c, fd.nbins, &fs.twid[f]))) return s;
         rows_total += fd.nbins;
     }
-    const bool fused = c->mode == CS_MODE_FUSED;
-    if (fused && !fused_supported(fs, nh))
-        return c->fail(CS_EINVAL, "fused mode supports profile lengths 16/20/24/30/32/40/60/64/120/128 "
+    const bool mfma = c->mode == CS_MODE_FUSED;
+    const bool fused = c->mode == CS_MODE_FUSED_VALU || mfma;
+    if (fused && !(mfma ? mfma_supported(fs, nh) : fused_supported(fs, nh)))
+        return c->fail(CS_EINVAL, "fused modes support profile lengths 16/20/24/30/32/40/60/64/120/128 "
                                   "with the default boxcar bank; use the plain mode");
-    const int64_t Bc = std::max<int64_t>(
-        1, std::min<int64_t>(nchunks, (fused ? kBatchTrialsFused : kBatchTrials) / Nprof));
+    MfmaDev md{};
+    int64_t mfma_nwg = 0;
+    if (mfma) {
+        std::vector<int> nbs;
+        for (int f = 0; f < nf; ++f) nbs.push_back(fs.f[f].nbins);
+        MfmaTables mt = build_mfma_tables(build_plans(nh, c->m, c->hidr, r_lo), nbs);
+        if (!mt.ok) return c->fail(CS_EINVAL, "fused MFMA mode: " + mt.why);
+        if ((s = upload(c, c->m_whi, mt.wA_hi))) return s;
+        if ((s = upload(c, c->m_wlo, mt.wA_lo))) return s;
+        if ((s = upload(c, c->m_gA, mt.gA))) return s;
+        if ((s = upload(c, c->m_cls, mt.clsmap))) return s;
+        if ((s = upload(c, c->m_clst, mt.cls_tile))) return s;
+        if ((s = upload(c, c->m_ks, mt.ks))) return s;
+        if ((s = upload(c, c->m_chi, mt.cA_hi))) return s;
+        if ((s = upload(c, c->m_clo, mt.cA_lo))) return s;
+        md.wA_hi = (const uint4*)c->m_whi.p;
+        md.wA_lo = (const uint4*)c->m_wlo.p;
+        md.gA = (const float2*)c->m_gA.p;
+        md.clsmap = (const int32_t*)c->m_cls.p;
+        md.cls_tile = (const int32_t*)c->m_clst.p;
+        md.ks = (const int32_t*)c->m_ks.p;
+        md.cA_hi = (const uint4*)c->m_chi.p;
+        md.cA_lo = (const uint4*)c->m_clo.p;
+        for (int f = 0; f < kMaxFolds; ++f) {
+            md.c_base[f] = mt.c_base[f];
+            md.c_mt[f] = mt.c_mt[f];
+            md.c_ks[f] = mt.c_ks[f];
+        }
+        md.S = mt.S;
+        md.U = mt.U;
+        md.q = mt.q;
+        mfma_nwg = (total + 4 * mt.S - 1) / (4 * mt.S) * mt.U;
+        if (total + 4 * mt.S >= ((int64_t)1 << 32))
+            return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
+    }
+    const int64_t Bc = mfma ? nchunks
+                            : std::max<int64_t>(1, std::min<int64_t>(
+                                  nchunks, (fused ? kBatchTrialsFused : kBatchTrials) / Nprof));
     const int64_t BT = Bc * Nprof;
     fs.ldp = BT;
 
@@ -563,7 +604,8 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             HIPCHK(c, hipEventRecord(E[0], c->st));
             if (fused) {
                 HIPCHK(c, launch_guard(c->st, sp, tb, g, b, (uint8_t*)c->rowok.p));
-                HIPCHK(c, launch_fused(c->st, sp, tb, g, fs, b, fu));
+                if (mfma) HIPCHK(c, launch_fused_mfma(c->st, sp, tb, g, fs, b, fu, md, mfma_nwg));
+                else HIPCHK(c, launch_fused(c->st, sp, tb, g, fs, b, fu));
                 HIPCHK(c, hipEventRecord(E[1], c->st));
                 HIPCHK(c, hipEventRecord(E[2], c->st));
                 HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));

@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cfloat>
+#include <cstring>
 #include <numeric>
 
 namespace cs {
@@ -165,6 +166,124 @@ std::vector<int> ladder_boxcar_widths(int nbins, int k, const std::vector<int>& 
     for (int w : ws)
         if ((w >= 2 || k == kmin) && (w <= kLadderWmax || k == kmax)) out.push_back(w);
     return out.empty() ? ws : out;
+}
+
+static uint16_t f2bf(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);      // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+static float bf2f(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+static void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
+    hi = f2bf(x);
+    lo = f2bf(x - bf2f(hi));
+}
+
+MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins) {
+    MfmaTables t;
+    if (!pl.rational) { t.why = "trial step is not a small rational"; return t; }
+    const int m = pl.m, nh = pl.nharms;
+    const int64_t q = pl.q;
+    t.q = (int32_t)q;
+    t.S = 16 / std::gcd((int64_t)16, q) * q;
+    t.U = (int32_t)(t.S / 16);
+    t.clsmap.assign((size_t)nh * q, -1);
+    t.ks.resize(nh);
+    int ncls = 0, ntile = 0;
+    for (int h = 1; h <= nh; ++h) {
+        const HarmDev& d = pl.harm[h - 1];
+        // residues at which 16-trial groups (anchored at multiples of 16) start
+        std::vector<int64_t> starts;
+        int64_t g16 = (16 * (int64_t)d.s) % q, r = 0;
+        do {
+            starts.push_back(r);
+            r = (r + g16) % q;
+        } while (r != 0 && (int64_t)starts.size() < q);
+        // offsets and phases of the 16 trials of each class
+        int maxnj = 0;
+        std::vector<std::vector<int>> P(starts.size()), D(starts.size());
+        for (size_t c = 0; c < starts.size(); ++c) {
+            int64_t res = starts[c], B = 0, B0 = 0;
+            for (int i = 0; i < 16; ++i) {
+                int p = pl.row[d.row_off + res];
+                int64_t Bk = B + pl.carry[d.tab_off + p];
+                if (i == 0) B0 = Bk;
+                P[c].push_back(p);
+                D[c].push_back((int)(Bk - B0));
+                res += d.s;
+                B += d.base_adv;
+                if (res >= q) { res -= q; B += 1; }
+            }
+            maxnj = std::max(maxnj, m + D[c][15]);
+        }
+        int ks = (maxnj + 31) / 32;
+        if (ks > 2) { t.why = "interpolation window wider than 64 bins"; return t; }
+        t.ks[h - 1] = ks;
+        for (size_t c = 0; c < starts.size(); ++c) {
+            t.clsmap[(size_t)(h - 1) * q + starts[c]] = ncls;
+            t.cls_tile.push_back(ntile);
+            for (int i = 0; i < 16; ++i) {
+                t.gA.push_back(pl.A32[2 * (size_t)(d.tab_off + P[c][i])]);
+                t.gA.push_back(pl.A32[2 * (size_t)(d.tab_off + P[c][i]) + 1]);
+            }
+            for (int kk = 0; kk < ks; ++kk) {
+                for (int l = 0; l < 64; ++l) {
+                    int i = l % 16;
+                    for (int e = 0; e < 8; ++e) {
+                        int k = kk * 32 + (l / 16) * 8 + e;
+                        int tap = k - D[c][i];
+                        float w = (tap >= 0 && tap < m)
+                                      ? pl.W32[(size_t)(d.tab_off + P[c][i]) * m + tap] : 0.0f;
+                        uint16_t hi, lo;
+                        split_bf16(w, hi, lo);
+                        t.wA_hi.push_back(hi);
+                        t.wA_lo.push_back(lo);
+                    }
+                }
+                ++ntile;
+            }
+            ++ncls;
+        }
+    }
+    // c2r DFT tiles
+    int ctile = 0;
+    for (size_t f = 0; f < fold_nbins.size() && f < 8; ++f) {
+        const int NB = fold_nbins[f], H = NB / 2;
+        const int mt = (NB + 15) / 16, ks = (2 * H + 31) / 32;
+        t.c_base[f] = ctile;
+        t.c_mt[f] = mt;
+        t.c_ks[f] = ks;
+        for (int a = 0; a < mt; ++a)
+            for (int kk = 0; kk < ks; ++kk) {
+                for (int l = 0; l < 64; ++l) {
+                    int n = a * 16 + l % 16;
+                    for (int e = 0; e < 8; ++e) {
+                        int kap = kk * 32 + (l / 16) * 8 + e;
+                        int h = kap / 2 + 1, c = kap % 2;
+                        double v = 0.0;
+                        if (n < NB && h <= H) {
+                            int64_t ee = ((int64_t)h * n) % NB;
+                            double ang = 2.0 * (double)ee / (double)NB;
+                            double coef = h < H ? 2.0 : 1.0;
+                            v = c == 0 ? coef * cospi_d(ang) : (h < H ? -coef * sinpi_d(ang) : 0.0);
+                        }
+                        uint16_t hi, lo;
+                        split_bf16((float)v, hi, lo);
+                        t.cA_hi.push_back(hi);
+                        t.cA_lo.push_back(lo);
+                    }
+                }
+                ++ctile;
+            }
+    }
+    t.ok = true;
+    return t;
 }
 
 }  // namespace cs

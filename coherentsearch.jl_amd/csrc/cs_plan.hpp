@@ -54,10 +54,34 @@ std::vector<int> boxcar_widths(int nbins, double fsp, double maxfrac);
 std::vector<int> ladder_boxcar_widths(int nbins, int k, const std::vector<int>& decims,
                                       double fsp, double maxfrac);
 
+// Operand tables of the split-BF16 MFMA pipeline (16x16x32 bf16 tiles in the
+// hardware lane layout: lane l holds row l%16, k = 8*(l/16) .. +8).
+//  * interpolation: for each harmonic h and each residue class of 16-trial
+//    groups, the 16 x (m + delta) weight block Wx[i][k] = W[k - delta_i, p_i],
+//    split into bf16 hi + lo, plus the per-trial conj(A(dr)) factors;
+//  * c2r: for each fold, the real DFT matrix C[n][2(h-1)+c] (c = re, im) of the
+//    numpy-convention inverse transform, split likewise.
+struct MfmaTables {
+    bool ok = false;
+    std::string why;
+    int64_t S = 0;                  // occurrence stride lcm(16, q) (trials)
+    int32_t U = 0;                  // S / 16
+    int32_t q = 0;
+    std::vector<uint16_t> wA_hi, wA_lo;   // [tile][64 lanes][8]
+    std::vector<float> gA;                // [class][16][2]
+    std::vector<int32_t> clsmap;          // [(h-1)*q + residue] -> class, -1 if no group starts there
+    std::vector<int32_t> cls_tile;        // [class] -> first tile
+    std::vector<int32_t> ks;              // [h-1] -> K steps (1 or 2)
+    std::vector<uint16_t> cA_hi, cA_lo;   // c2r tiles
+    int32_t c_base[8] = {0}, c_mt[8] = {0}, c_ks[8] = {0};
+};
+
 // One decimation fold of the search (k = 1 is the base pass).
 struct FoldDev {
     int32_t k, Hk, nbins, nw;
     int32_t widths[kMaxWidths];
 };
+
+MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins);
 
 }  // namespace cs

@@ -29,8 +29,9 @@ extern "C" {
 #define CS_ENOMEM 5
 
 /* Execution modes for cs_search. */
-#define CS_MODE_PLAIN 0   /* FP32 VALU interpolation, one launch per stage, HBM intermediates */
-#define CS_MODE_FUSED 1   /* split-BF16 MFMA interpolation, fused pipeline */
+#define CS_MODE_PLAIN 0       /* FP32 VALU interpolation, one launch per stage, HBM intermediates */
+#define CS_MODE_FUSED 1       /* split-BF16 MFMA interpolation and c2r, fused pipeline */
+#define CS_MODE_FUSED_VALU 2  /* fused pipeline with FP32 VALU interpolation and c2r */
 
 /* SearchParams (src/search.jl:131-144).  `decimations` is pointer + length
  * (the reference's Vector{Int}); precision 0 = :f64, 1 = :f32 (the profile
@@ -118,7 +119,7 @@ int cs_decim_metrics(cs_ctx *ctx, double rstart, int64_t n, int32_t k,
 int cs_interp_rows(cs_ctx *ctx, double r_lo, int64_t t0, int64_t n,
                    int32_t weights_f64, double *out);
 
-/* Execution mode for cs_search (CS_MODE_PLAIN / CS_MODE_FUSED). */
+/* Execution mode for cs_search (CS_MODE_PLAIN / CS_MODE_FUSED / CS_MODE_FUSED_VALU). */
 int cs_set_mode(cs_ctx *ctx, int32_t mode);
 
 /* Per-stage device time of the last cs_search, milliseconds, HIP events on the
