@@ -51,7 +51,7 @@ struct cs_ctx {
     std::map<int, DBuf> twid;
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
-    DBuf m_whi, m_wlo, m_gA, m_cls, m_clst, m_ks, m_chi, m_clo;
+    DBuf m_whi, m_wlo, m_gA, m_cls, m_clst, m_ks, m_chi, m_clo, m_clsu, m_desc;
     std::vector<hipEvent_t> ev;
     double timings[8] = {0};
 
@@ -253,7 +253,8 @@ int cs_ctx_destroy(cs_ctx* c) {
     DBuf* bufs[] = {&c->amps, &c->harm, &c->row, &c->carry, &c->W32, &c->A32, &c->W64, &c->A64,
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
                     &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples,
-                    &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_cls, &c->m_clst, &c->m_ks, &c->m_chi, &c->m_clo};
+                    &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_cls, &c->m_clst, &c->m_ks, &c->m_chi, &c->m_clo,
+                    &c->m_clsu, &c->m_desc};
     for (DBuf* b : bufs) release(*b);
     for (auto& kv : c->twid) release(kv.second);
     for (hipEvent_t ev : c->ev) (void)hipEventDestroy(ev);
@@ -503,6 +504,7 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         if ((s = upload(c, c->m_ks, mt.ks))) return s;
         if ((s = upload(c, c->m_chi, mt.cA_hi))) return s;
         if ((s = upload(c, c->m_clo, mt.cA_lo))) return s;
+        if ((s = upload(c, c->m_clsu, mt.clsu))) return s;
         md.wA_hi = (const uint4*)c->m_whi.p;
         md.wA_lo = (const uint4*)c->m_wlo.p;
         md.gA = (const float2*)c->m_gA.p;
@@ -511,6 +513,10 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         md.ks = (const int32_t*)c->m_ks.p;
         md.cA_hi = (const uint4*)c->m_chi.p;
         md.cA_lo = (const uint4*)c->m_clo.p;
+        md.clsu = (const int32_t*)c->m_clsu.p;
+        int ksmax = 1;
+        for (int v : mt.ks) ksmax = std::max(ksmax, v);
+        md.kwin = 32 * ksmax;
         for (int f = 0; f < kMaxFolds; ++f) {
             md.c_base[f] = mt.c_base[f];
             md.c_mt[f] = mt.c_mt[f];
@@ -519,8 +525,13 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         md.S = mt.S;
         md.U = mt.U;
         md.q = mt.q;
-        mfma_nwg = (total + 4 * mt.S - 1) / (4 * mt.S) * mt.U;
-        if (total + 4 * mt.S >= ((int64_t)1 << 32))
+        mfma_nwg = (total + 8 * mt.S - 1) / (8 * mt.S) * mt.U;
+        {
+            size_t lds = mfma_lds_bytes(nh, fs, md.kwin, &md.nbp_max, &md.dft_tiles_max, md);
+            if (lds > 160 * 1024)
+                return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");
+        }
+        if (total + 8 * mt.S >= ((int64_t)1 << 32))
             return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
     }
     const int64_t Bc = mfma ? nchunks
@@ -604,7 +615,16 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             HIPCHK(c, hipEventRecord(E[0], c->st));
             if (fused) {
                 HIPCHK(c, launch_guard(c->st, sp, tb, g, b, (uint8_t*)c->rowok.p));
-                if (mfma) HIPCHK(c, launch_fused_mfma(c->st, sp, tb, g, fs, b, fu, md, mfma_nwg));
+                if (mfma) {
+                    std::vector<uint8_t> desc(sizeof(FoldSet) + sizeof(MfmaDev));
+                    std::memcpy(desc.data(), &fs, sizeof(FoldSet));
+                    std::memcpy(desc.data() + sizeof(FoldSet), &md, sizeof(MfmaDev));
+                    if ((s = upload(c, c->m_desc, desc))) return s;
+                    HIPCHK(c, launch_fused_mfma(c->st, sp, tb, g, fs, b, fu, md,
+                                                (const FoldSet*)c->m_desc.p,
+                                                (const MfmaDev*)((const uint8_t*)c->m_desc.p + sizeof(FoldSet)),
+                                                mfma_nwg));
+                }
                 else HIPCHK(c, launch_fused(c->st, sp, tb, g, fs, b, fu));
                 HIPCHK(c, hipEventRecord(E[1], c->st));
                 HIPCHK(c, hipEventRecord(E[2], c->st));

@@ -251,36 +251,50 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
             ++ncls;
         }
     }
-    // c2r DFT tiles
+    // residue of the group starting at trial 16u (S is a multiple of q, so this
+    // is also the residue of every occurrence 16u + o*S)
+    t.clsu.assign((size_t)nh * t.U * 2, -1);
+    for (int h = 1; h <= nh; ++h) {
+        const HarmDev& d = pl.harm[h - 1];
+        for (int u = 0; u < t.U; ++u) {
+            int64_t res = ((int64_t)16 * u % q) * (d.hp % q) % q;
+            int c = t.clsmap[(size_t)(h - 1) * q + res];
+            if (c < 0) { t.why = "group residue without a class"; return t; }
+            t.clsu[2 * ((size_t)(h - 1) * t.U + u)] = t.cls_tile[c];
+            t.clsu[2 * ((size_t)(h - 1) * t.U + u) + 1] = c;
+        }
+    }
+    // c2r: even/odd DFT tiles
     int ctile = 0;
     for (size_t f = 0; f < fold_nbins.size() && f < 8; ++f) {
         const int NB = fold_nbins[f], H = NB / 2;
-        const int mt = (NB + 15) / 16, ks = (2 * H + 31) / 32;
+        const int mt = (H + 1 + 15) / 16, ks = (H + 31) / 32;
         t.c_base[f] = ctile;
         t.c_mt[f] = mt;
         t.c_ks[f] = ks;
         for (int a = 0; a < mt; ++a)
-            for (int kk = 0; kk < ks; ++kk) {
-                for (int l = 0; l < 64; ++l) {
-                    int n = a * 16 + l % 16;
-                    for (int e = 0; e < 8; ++e) {
-                        int kap = kk * 32 + (l / 16) * 8 + e;
-                        int h = kap / 2 + 1, c = kap % 2;
-                        double v = 0.0;
-                        if (n < NB && h <= H) {
-                            int64_t ee = ((int64_t)h * n) % NB;
-                            double ang = 2.0 * (double)ee / (double)NB;
-                            double coef = h < H ? 2.0 : 1.0;
-                            v = c == 0 ? coef * cospi_d(ang) : (h < H ? -coef * sinpi_d(ang) : 0.0);
+            for (int kk = 0; kk < ks; ++kk)
+                for (int gsel = 0; gsel < 2; ++gsel) {
+                    for (int l = 0; l < 64; ++l) {
+                        int n = a * 16 + l % 16;
+                        for (int e = 0; e < 8; ++e) {
+                            int h = kk * 32 + (l / 16) * 8 + e + 1;
+                            double v = 0.0;
+                            if (n <= H && h <= H) {
+                                int64_t ee = ((int64_t)h * n) % NB;
+                                double ang = 2.0 * (double)ee / (double)NB;
+                                double coef = h < H ? 2.0 : 1.0;
+                                if (gsel == 0) v = coef * cospi_d(ang);
+                                else v = h < H ? -coef * sinpi_d(ang) : 0.0;
+                            }
+                            uint16_t hi, lo;
+                            split_bf16((float)v, hi, lo);
+                            t.cA_hi.push_back(hi);
+                            t.cA_lo.push_back(lo);
                         }
-                        uint16_t hi, lo;
-                        split_bf16((float)v, hi, lo);
-                        t.cA_hi.push_back(hi);
-                        t.cA_lo.push_back(lo);
                     }
+                    ++ctile;
                 }
-                ++ctile;
-            }
     }
     t.ok = true;
     return t;

@@ -59,8 +59,11 @@ std::vector<int> ladder_boxcar_widths(int nbins, int k, const std::vector<int>& 
 //  * interpolation: for each harmonic h and each residue class of 16-trial
 //    groups, the 16 x (m + delta) weight block Wx[i][k] = W[k - delta_i, p_i],
 //    split into bf16 hi + lo, plus the per-trial conj(A(dr)) factors;
-//  * c2r: for each fold, the real DFT matrix C[n][2(h-1)+c] (c = re, im) of the
-//    numpy-convention inverse transform, split likewise.
+//  * c2r: for each fold, the even/odd halves of the numpy-convention real
+//    inverse DFT, rows n = 0..H, columns h = 1..H:
+//      Ccos[n][h] = c_h cos(2 pi h n / NB),  Csin[n][h] = -c_h sin(2 pi h n / NB)
+//    (c_h = 2, c_H = 1, Csin[.][H] = 0), so x[n] = E + O and x[NB-n] = E - O
+//    with E = Ccos . Re X and O = Csin . Im X; split likewise.
 struct MfmaTables {
     bool ok = false;
     std::string why;
@@ -72,7 +75,8 @@ struct MfmaTables {
     std::vector<int32_t> clsmap;          // [(h-1)*q + residue] -> class, -1 if no group starts there
     std::vector<int32_t> cls_tile;        // [class] -> first tile
     std::vector<int32_t> ks;              // [h-1] -> K steps (1 or 2)
-    std::vector<uint16_t> cA_hi, cA_lo;   // c2r tiles
+    std::vector<int32_t> clsu;            // [(h-1)*U + u] -> {first tile, class} of the group at 16u
+    std::vector<uint16_t> cA_hi, cA_lo;   // c2r tiles: [(a*ks + kk)*2 + {cos, sin}]
     int32_t c_base[8] = {0}, c_mt[8] = {0}, c_ks[8] = {0};
 };
 
