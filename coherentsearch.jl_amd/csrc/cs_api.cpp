@@ -51,7 +51,10 @@ struct cs_ctx {
     std::map<int, DBuf> twid;
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
-    DBuf m_whi, m_wlo, m_gA, m_cls, m_clst, m_ks, m_chi, m_clo, m_clsu, m_desc;
+    DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask;
+    DBuf spl;                  // pre-split spectrum planes of the MFMA mode
+    bool spl_valid = false;    // spl matches the current spectrum
+    int ncu = 0;               // compute units of the device
     std::vector<hipEvent_t> ev;
     double timings[8] = {0};
 
@@ -237,6 +240,7 @@ int cs_ctx_create(const cs_params* p, int32_t blocksize, int32_t device, cs_ctx*
     c->Nprof = std::max<int64_t>(1, blocksize);
     c->device = device;
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
@@ -253,8 +257,8 @@ int cs_ctx_destroy(cs_ctx* c) {
     DBuf* bufs[] = {&c->amps, &c->harm, &c->row, &c->carry, &c->W32, &c->A32, &c->W64, &c->A64,
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
                     &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples,
-                    &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_cls, &c->m_clst, &c->m_ks, &c->m_chi, &c->m_clo,
-                    &c->m_clsu, &c->m_desc};
+                    &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
+                    &c->m_desc, &c->m_rowmask, &c->spl};
     for (DBuf* b : bufs) release(*b);
     for (auto& kv : c->twid) release(kv.second);
     for (hipEvent_t ev : c->ev) (void)hipEventDestroy(ev);
@@ -289,6 +293,7 @@ static int set_spectrum_impl(cs_ctx* c, const void* src, hipMemcpyKind kind, int
     c->namps = namps;
     c->N = N;
     c->T = T;
+    c->spl_valid = false;
     return CS_OK;
 }
 
@@ -490,49 +495,56 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         return c->fail(CS_EINVAL, "fused modes support profile lengths 16/20/24/30/32/40/60/64/120/128 "
                                   "with the default boxcar bank; use the plain mode");
     MfmaDev md{};
-    int64_t mfma_nwg = 0;
+    int mfma_nwg = 0;
     if (mfma) {
         std::vector<int> nbs;
         for (int f = 0; f < nf; ++f) nbs.push_back(fs.f[f].nbins);
         MfmaTables mt = build_mfma_tables(build_plans(nh, c->m, c->hidr, r_lo), nbs);
         if (!mt.ok) return c->fail(CS_EINVAL, "fused MFMA mode: " + mt.why);
+        if (total + 8 * mt.S >= ((int64_t)1 << 32))
+            return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
         if ((s = upload(c, c->m_whi, mt.wA_hi))) return s;
         if ((s = upload(c, c->m_wlo, mt.wA_lo))) return s;
         if ((s = upload(c, c->m_gA, mt.gA))) return s;
-        if ((s = upload(c, c->m_cls, mt.clsmap))) return s;
-        if ((s = upload(c, c->m_clst, mt.cls_tile))) return s;
-        if ((s = upload(c, c->m_ks, mt.ks))) return s;
+        if ((s = upload(c, c->m_hdesc, mt.hdesc))) return s;
         if ((s = upload(c, c->m_chi, mt.cA_hi))) return s;
         if ((s = upload(c, c->m_clo, mt.cA_lo))) return s;
-        if ((s = upload(c, c->m_clsu, mt.clsu))) return s;
+        if ((s = ensure(c, c->m_rowmask, (size_t)nchunks * sizeof(uint64_t)))) return s;
+        const int64_t spl_ld = c->namps + 2 * kSpecPad;
+        if (!c->spl_valid) {
+            if ((s = ensure(c, c->spl, (size_t)2 * spl_ld * sizeof(uint32_t)))) return s;
+            HIPCHK(c, launch_split_spectrum(c->st, spec_of(c), (uint32_t*)c->spl.p));
+            c->spl_valid = true;
+        }
         md.wA_hi = (const uint4*)c->m_whi.p;
         md.wA_lo = (const uint4*)c->m_wlo.p;
         md.gA = (const float2*)c->m_gA.p;
-        md.clsmap = (const int32_t*)c->m_cls.p;
-        md.cls_tile = (const int32_t*)c->m_clst.p;
-        md.ks = (const int32_t*)c->m_ks.p;
+        md.hdesc = (const int4*)c->m_hdesc.p;
         md.cA_hi = (const uint4*)c->m_chi.p;
         md.cA_lo = (const uint4*)c->m_clo.p;
-        md.clsu = (const int32_t*)c->m_clsu.p;
+        md.spl = (const uint32_t*)c->spl.p;
+        md.rowmask = (const uint64_t*)c->m_rowmask.p;
+        md.spl_ld = spl_ld;
+        md.S = mt.S;
+        md.U = mt.U;
+        md.q = mt.q;
+        md.ntiles = (total + 8 * mt.S - 1) / (8 * mt.S) * mt.U;
         int ksmax = 1;
         for (int v : mt.ks) ksmax = std::max(ksmax, v);
-        md.kwin = 32 * ksmax;
+        md.ksmax = ksmax;
         for (int f = 0; f < kMaxFolds; ++f) {
             md.c_base[f] = mt.c_base[f];
             md.c_mt[f] = mt.c_mt[f];
             md.c_ks[f] = mt.c_ks[f];
         }
-        md.S = mt.S;
-        md.U = mt.U;
-        md.q = mt.q;
-        mfma_nwg = (total + 8 * mt.S - 1) / (8 * mt.S) * mt.U;
-        {
-            size_t lds = mfma_lds_bytes(nh, fs, md.kwin, &md.nbp_max, &md.dft_tiles_max, md);
-            if (lds > 160 * 1024)
-                return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");
-        }
-        if (total + 8 * mt.S >= ((int64_t)1 << 32))
-            return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
+        const MfmaLds lay = mfma_lds_plan(nh, fs, md);
+        if (lay.total > 160 * 1024)
+            return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");
+        // one persistent workgroup per compute unit, dealt over the 8 XCDs
+        const int ncu = c->ncu > 0 ? c->ncu : 256;
+        md.nxcd = (ncu % 8 == 0) ? 8 : 1;
+        mfma_nwg = (int)std::min<int64_t>(ncu, md.ntiles);
+        if (mfma_nwg % md.nxcd != 0) md.nxcd = 1;
     }
     const int64_t Bc = mfma ? nchunks
                             : std::max<int64_t>(1, std::min<int64_t>(
@@ -614,7 +626,8 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             o.medcut = medcut;
             HIPCHK(c, hipEventRecord(E[0], c->st));
             if (fused) {
-                HIPCHK(c, launch_guard(c->st, sp, tb, g, b, (uint8_t*)c->rowok.p));
+                if (mfma) HIPCHK(c, launch_rowmask(c->st, sp, tb, g, nchunks, (uint64_t*)c->m_rowmask.p));
+                else HIPCHK(c, launch_guard(c->st, sp, tb, g, b, (uint8_t*)c->rowok.p));
                 if (mfma) {
                     std::vector<uint8_t> desc(sizeof(FoldSet) + sizeof(MfmaDev));
                     std::memcpy(desc.data(), &fs, sizeof(FoldSet));

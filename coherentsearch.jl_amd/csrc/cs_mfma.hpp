@@ -5,36 +5,52 @@
 
 namespace cs {
 
+// Zero bins on each side of the pre-split spectrum planes: every 16-trial
+// interpolation window (at most 64 bins) starting in [-kSpecPad, namps] reads
+// inside the allocation.
+constexpr int kSpecPad = 64;
+
 struct MfmaDev {
-    const uint4* wA_hi;      // interpolation weight tiles, [tile][64 lanes] x 8 bf16
+    const uint4* wA_hi;       // interpolation weight tiles, [tile][64 lanes] x 8 bf16
     const uint4* wA_lo;
-    const float2* gA;        // [class][16] conj(A(dr)) per trial of the group
-    const int32_t* clsmap;   // [(h-1)*q + residue] -> class
-    const int32_t* cls_tile; // [class] -> first tile
-    const int32_t* ks;       // [h-1] -> K steps
-    const int32_t* clsu;     // [(h-1)*U + u] -> {first tile, class}
-    const uint4* cA_hi;      // c2r DFT tiles, [tile][64 lanes] x 8 bf16
+    const float2* gA;         // [class][16] conj(A(dr)) per trial of the group
+    const int4* hdesc;        // [(h-1)*U + u]: {first bin of trial 16u (clamped),
+                              //  bin advance per occurrence, first weight tile,
+                              //  class | K steps << 28}
+    const uint4* cA_hi;       // c2r DFT tiles, [tile][64 lanes] x 8 bf16
     const uint4* cA_lo;
+    const uint32_t* spl;      // split spectrum: [plane re/im][kSpecPad + bin], (hi | lo << 16)
+    const uint64_t* rowmask;  // [global block]: bit h-1 set if harmonic row h is kept
+    int64_t spl_ld;           // plane stride (namps + 2 * kSpecPad)
+    int64_t S;                // occurrence stride, lcm(16, q) trials
+    int64_t ntiles;           // (ceil(total / (8 S))) * U tiles of 128 trials
     int32_t c_base[kMaxFolds];
     int32_t c_mt[kMaxFolds];
     int32_t c_ks[kMaxFolds];
-    int64_t S;               // occurrence stride, lcm(16, q) trials
-    int32_t U;               // S / 16
+    int32_t U;                // S / 16
     int32_t q;
-    int32_t kwin;            // staged bins per (harmonic, occurrence): 32 * max K steps
-    int32_t nbp_max;         // set by the launcher
-    int32_t dft_tiles_max;   // set by the launcher
-    int32_t pad;
+    int32_t ksmax;            // largest interpolation K steps (1 or 2)
+    int32_t nbp;              // profile row stride in LDS (floats)
+    int32_t dft_tiles_max;    // largest per-fold DFT tile count (hi or lo)
+    int32_t nxcd;             // XCDs the persistent grid is spread over
+};
+
+struct MfmaLds {
+    size_t ftp, region, dft, total;
 };
 
 bool mfma_supported(const FoldSet& fs, int nharms);
-size_t mfma_lds_bytes(int nharms, const FoldSet& fs, int kwin, int* nbp_max, int* dft_max,
-                      const MfmaDev& md);
-// md.nbp_max / md.dft_tiles_max must be filled by mfma_lds_bytes; fs_dev and
-// md_dev are device copies of fs and md.
+// Fills md.nbp / md.dft_tiles_max and returns the LDS plan.
+MfmaLds mfma_lds_plan(int nharms, const FoldSet& fs, MfmaDev& md);
+// Pre-split spectrum planes (2 * (namps + 2 * kSpecPad) words at spl).
+hipError_t launch_split_spectrum(hipStream_t st, const SpecDev& sp, uint32_t* spl);
+// Per-block harmonic row guards as bit masks (nharms <= 64).
+hipError_t launch_rowmask(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
+                          int64_t nblocks, uint64_t* rowmask);
+// fs_dev and md_dev are device copies of fs and md; nwg persistent workgroups.
 hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
                              const FoldSet& fs, const BatchDev& b, const FusedDev& fu,
                              const MfmaDev& md, const FoldSet* fs_dev, const MfmaDev* md_dev,
-                             int64_t nwg);
+                             int nwg);
 
 }  // namespace cs

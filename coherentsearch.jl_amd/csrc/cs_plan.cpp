@@ -264,11 +264,34 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
             t.clsu[2 * ((size_t)(h - 1) * t.U + u) + 1] = c;
         }
     }
-    // c2r: even/odd DFT tiles
+    // per (harmonic, group) descriptors of the persistent kernel: first bin of
+    // trial 16u (clamped: rows that far out fail the guard), bin advance per
+    // occurrence (S/q periods of h*pnum), first weight tile, class | K steps << 28
+    t.hdesc.assign((size_t)nh * t.U * 4, 0);
+    for (int h = 1; h <= nh; ++h) {
+        const HarmDev& d = pl.harm[h - 1];
+        const int64_t hp = d.hp;
+        for (int u = 0; u < t.U; ++u) {
+            const int64_t T = (int64_t)16 * u * hp;
+            const int64_t qint = T / q, res = T % q;
+            const int p = pl.row[d.row_off + res];
+            int64_t b0 = d.rfloor0 + qint + pl.carry[d.tab_off + p] + 1 - m / 2;
+            b0 = std::max<int64_t>(-(1 << 30), std::min<int64_t>(b0, 1 << 30));
+            const size_t e = 4 * ((size_t)(h - 1) * t.U + u);
+            const int cls = t.clsu[2 * ((size_t)(h - 1) * t.U + u) + 1];
+            t.hdesc[e + 0] = (int32_t)b0;
+            t.hdesc[e + 1] = (int32_t)(hp * (t.S / q));
+            t.hdesc[e + 2] = t.clsu[2 * ((size_t)(h - 1) * t.U + u)];
+            t.hdesc[e + 3] = cls | (t.ks[h - 1] << 28);
+        }
+    }
+    // c2r: even/odd DFT tiles; row i = 4g + v of row tile a computes profile
+    // entry n = g*R + 4a + v (R = 4 * row tiles), so lane group g of a trial
+    // column ends up holding the contiguous run n = g*R .. g*R + R - 1
     int ctile = 0;
     for (size_t f = 0; f < fold_nbins.size() && f < 8; ++f) {
         const int NB = fold_nbins[f], H = NB / 2;
-        const int mt = (H + 1 + 15) / 16, ks = (H + 31) / 32;
+        const int mt = (H + 1 + 15) / 16, ks = (H + 31) / 32, R = 4 * mt;
         t.c_base[f] = ctile;
         t.c_mt[f] = mt;
         t.c_ks[f] = ks;
@@ -276,7 +299,8 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
             for (int kk = 0; kk < ks; ++kk)
                 for (int gsel = 0; gsel < 2; ++gsel) {
                     for (int l = 0; l < 64; ++l) {
-                        int n = a * 16 + l % 16;
+                        const int i = l % 16;
+                        const int n = (i / 4) * R + 4 * a + i % 4;
                         for (int e = 0; e < 8; ++e) {
                             int h = kk * 32 + (l / 16) * 8 + e + 1;
                             double v = 0.0;

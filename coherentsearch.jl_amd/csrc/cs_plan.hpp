@@ -63,7 +63,8 @@ std::vector<int> ladder_boxcar_widths(int nbins, int k, const std::vector<int>& 
 //    inverse DFT, rows n = 0..H, columns h = 1..H:
 //      Ccos[n][h] = c_h cos(2 pi h n / NB),  Csin[n][h] = -c_h sin(2 pi h n / NB)
 //    (c_h = 2, c_H = 1, Csin[.][H] = 0), so x[n] = E + O and x[NB-n] = E - O
-//    with E = Ccos . Re X and O = Csin . Im X; split likewise.
+//    with E = Ccos . Re X and O = Csin . Im X; split likewise.  Row i = 4g + v
+//    of row tile a is entry n = g*4*mt + 4a + v (mt row tiles).
 struct MfmaTables {
     bool ok = false;
     std::string why;
@@ -76,7 +77,9 @@ struct MfmaTables {
     std::vector<int32_t> cls_tile;        // [class] -> first tile
     std::vector<int32_t> ks;              // [h-1] -> K steps (1 or 2)
     std::vector<int32_t> clsu;            // [(h-1)*U + u] -> {first tile, class} of the group at 16u
-    std::vector<uint16_t> cA_hi, cA_lo;   // c2r tiles: [(a*ks + kk)*2 + {cos, sin}]
+    std::vector<int32_t> hdesc;           // [(h-1)*U + u] x {first bin of trial 16u, bin advance
+                                          //  per occurrence, first tile, class | K steps << 28}
+    std::vector<uint16_t> cA_hi, cA_lo;   // c2r tiles: [(a*ks + kk)*2 + {cos, sin}], rows permuted
     int32_t c_base[8] = {0}, c_mt[8] = {0}, c_ks[8] = {0};
 };
 
