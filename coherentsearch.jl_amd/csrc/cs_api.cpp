@@ -51,7 +51,7 @@ struct cs_ctx {
     std::map<int, DBuf> twid;
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
-    DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask;
+    DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol;
     DBuf spl;                  // pre-split spectrum planes of the MFMA mode
     bool spl_valid = false;    // spl matches the current spectrum
     int ncu = 0;               // compute units of the device
@@ -258,7 +258,7 @@ int cs_ctx_destroy(cs_ctx* c) {
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
                     &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
-                    &c->m_desc, &c->m_rowmask, &c->spl};
+                    &c->m_desc, &c->m_rowmask, &c->m_scol, &c->spl};
     for (DBuf* b : bufs) release(*b);
     for (auto& kv : c->twid) release(kv.second);
     for (hipEvent_t ev : c->ev) (void)hipEventDestroy(ev);
@@ -537,6 +537,14 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             md.c_mt[f] = mt.c_mt[f];
             md.c_ks[f] = mt.c_ks[f];
         }
+        {   // sampled columns of a full block, per fold
+            std::vector<int32_t> sc((size_t)nf * Nprof);
+            for (int f = 0; f < nf; ++f)
+                for (int64_t j = 0; j < Nprof; ++j)
+                    sc[(size_t)f * Nprof + j] = mfma_sample_column(j, Nprof, fs.f[f].nbins, fs.cap[f]);
+            if ((s = upload(c, c->m_scol, sc))) return s;
+            md.scol = (const int32_t*)c->m_scol.p;
+        }
         const MfmaLds lay = mfma_lds_plan(nh, fs, md);
         if (lay.total > 160 * 1024)
             return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");
@@ -566,7 +574,10 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
     FusedDev fu{};
     if (fused) {
         if ((s = ensure(c, c->mraw, (size_t)nf * BT * sizeof(float)))) return s;
-        if ((s = ensure(c, c->samples, (size_t)Bc * nf * kSigmaSamples * sizeof(float)))) return s;
+        const int64_t sld = mfma ? kSigmaSamplesPadded : kSigmaSamples;
+        if ((s = ensure(c, c->samples, (size_t)Bc * nf * sld * sizeof(float)))) return s;
+        fu.sld = sld;
+        fu.salign = mfma ? 16 : 1;
         if ((s = ensure(c, c->rowok, (size_t)Bc * nh))) return s;
         fu.mraw = (float*)c->mraw.p;
         fu.ldm = BT;

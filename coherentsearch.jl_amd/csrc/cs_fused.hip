@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g,
     for (int f = wv; f < fs.nf; f += 4) {
         const FoldDev& fd = fs.f[f];
         const int64_t nv = valid ? b.nvalid[cl * fs.nf + f] : 0;
-        float* samp = fu.samples + ((int64_t)cl * fs.nf + f) * kSigmaSamples;
+        float* samp = fu.samples + ((int64_t)cl * fs.nf + f) * fu.sld;
         float* mo = fu.mraw + (int64_t)f * fu.ldm + jl;
         const bool small = fd.widths[fd.nw - 1] <= 6;
         switch (fd.nbins) {
@@ -263,12 +263,20 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
             if (ncol > nv) ncol = nv;
             ns = ncol * nbins < cap ? ncol * nbins : cap;
         }
-        const float* S = fu.samples + ((int64_t)cl * nf + f) * kSigmaSamples;
+        const float* S = fu.samples + ((int64_t)cl * nf + f) * fu.sld;
+        // sample s is bin s % nbins of sampled column s / nbins; columns start at
+        // multiples of nbins rounded up to salign
+        const int nbs = (nbins + fu.salign - 1) / fu.salign * fu.salign;
         uint32_t keys[kSPer];
 #pragma unroll
         for (int r = 0; r < kSPer; ++r) {
-            int64_t s = threadIdx.x + (int64_t)kST * r;
-            keys[r] = s < ns ? __float_as_uint(S[s]) : 0xFFFFFFFFu;
+            const int s = threadIdx.x + kST * r;
+            int pos = s;
+            if (nbs != nbins) {
+                const int cc = s / nbins;
+                pos = cc * nbs + (s - cc * nbins);
+            }
+            keys[r] = s < ns ? __float_as_uint(S[pos]) : 0xFFFFFFFFu;
         }
         const uint32_t half = (uint32_t)(ns / 2);
         const uint32_t up = select_kth(keys, half, hist, sh);

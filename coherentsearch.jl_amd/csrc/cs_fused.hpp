@@ -7,7 +7,10 @@ namespace cs {
 struct FusedDev {
     float* mraw;            // [fold][batch column]: zero-baseline bound before sigma
     int64_t ldm;            // leading dimension of mraw
-    float* samples;         // [(block_local * nf + fold) * 8192 + slot]: sigma samples |x|
+    float* samples;         // [(block_local * nf + fold) * sld + slot]: sigma samples |x|
+    int64_t sld;            // samples per (block, fold) region
+    int32_t salign;         // sampled columns start at multiples of round_up(nbins, salign)
+    int32_t pad;
     const uint8_t* rowok;   // [block_local * nharms + h - 1]: harmonic row kept for the block
 };
 

@@ -31,12 +31,22 @@ namespace cs {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint4 uint4_a4 __attribute__((aligned(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// explicit address spaces for pointers handed to out-of-line device functions
+// (a generic pointer would turn their LDS reads into FLAT accesses)
+#define CS_LDS __attribute__((address_space(3)))
+#define CS_GLB __attribute__((address_space(1)))
+#define CS_CONST __attribute__((address_space(4)))
 
 constexpr int kOcc8 = 8;          // occurrences per tile
 constexpr int kWT = 128;          // trials per tile
 constexpr int kThreads = 512;     // 8 waves
+constexpr int kStageFloats = 1024;  // per-wave LDS staging of sampled columns
+
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
+__device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v) { return __builtin_bit_cast(bf16x8, v); }
 
 // (hi | lo << 16) words -> hi and lo operand vectors (element e from word e)
 __device__ __forceinline__ void unpack8(const uint32_t* w, bf16x8& hi, bf16x8& lo) {
@@ -73,9 +83,29 @@ __device__ __forceinline__ float pair_swap(float x) {
                                                               0xF, 0xF, true));
 }
 
-// column swizzle of the stacked-amplitude planes: spreads the rows one c2r
-// operand read touches over distinct LDS banks
-__device__ __forceinline__ int swz(int r) { return ((r >> 3) & 3) << 4; }
+// Workgroup barrier that orders LDS only.  No wave of the fused kernel reads
+// global memory another wave wrote, so the barrier need not wait for this
+// wave's outstanding global stores (bounds, sigma samples), which a full
+// __syncthreads() release fence would.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Read-only table entry at a wave-uniform index, read through the constant
+// address space so it becomes a scalar load (the kernel's global stores
+// cannot alias it).
+__device__ __forceinline__ int4 ld_const4(const int4* p, size_t i) {
+    typedef int i32x4v __attribute__((ext_vector_type(4)));
+    const i32x4v v = ((const CS_CONST i32x4v*)p)[i];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
+// Row stride (words) of the stacked-amplitude planes.  A c2r operand read
+// touches rows 8k apart in the four lane groups of a column; with 130-word
+// rows those land 16k banks apart, distinct for k = 1, 2, 3, 5, 6.
+constexpr int kFtpLd = 130;
 
 // ---------------------------------------------------------------------------
 // Profile sequences from registers (cs_geo.hpp)
@@ -194,17 +224,73 @@ __device__ __forceinline__ void bank_max(const float (&Xs)[G::X], int g, float (
     }
 }
 
+// Sample slot column of block trial j (_block_sigma's whole-column subsample,
+// search.jl:504-538): j itself when all nv columns fit in cap samples, else
+// c when j = floor(c * nv / ncol) for some c < ncol = clamp(cap / nbins, 1, nv);
+// -1 if the column is not sampled.
+__host__ __device__ inline int sample_column(int64_t j, int64_t nv, int nbins, int64_t cap) {
+    if ((int64_t)nbins * nv <= cap) return (int)j;
+    int64_t ncol = cap / nbins;
+    if (ncol < 1) ncol = 1;
+    if (ncol > nv) ncol = nv;
+    const int64_t c = (j * ncol + nv - 1) / nv;
+    return (c < ncol && (c * nv) / ncol == j) ? (int)c : -1;
+}
+
+// One sampled column's share of this lane group (runs n = g*R .. g*R + R - 1
+// of P and M) into a column slot block: P[n] at n, M[n] at H + n, P[H] in
+// place of M[0]; whole CH-word chunks.
+template <int H, int R, int ASN>
+__device__ __forceinline__ void put_column(__attribute__((address_space(ASN))) float* base, int g,
+                                           const float (&P)[R], const float (&M)[R], float pH) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(ASN))) f32x4 V4;
+    typedef __attribute__((address_space(ASN))) f2 V2;
+    constexpr int CH = (H % 4 == 0) ? 4 : (H % 2 == 0) ? 2 : 1;
+#pragma unroll
+    for (int c0 = 0; c0 < R; c0 += CH) {
+        const int n0 = g * R + c0;
+        if (n0 + CH <= H) {
+            float pv[CH], mv[CH];
+#pragma unroll
+            for (int e = 0; e < CH; ++e) {
+                pv[e] = fabsf(P[c0 + e]);
+                mv[e] = fabsf(M[c0 + e]);
+            }
+            if (c0 == 0) mv[0] = n0 == 0 ? fabsf(pH) : mv[0];
+            if constexpr (CH == 4) {
+                *(V4*)(base + n0) = f32x4{pv[0], pv[1], pv[2], pv[3]};
+                *(V4*)(base + H + n0) = f32x4{mv[0], mv[1], mv[2], mv[3]};
+            } else if constexpr (CH == 2) {
+                *(V2*)(base + n0) = f2{pv[0], pv[1]};
+                *(V2*)(base + H + n0) = f2{mv[0], mv[1]};
+            } else {
+                base[n0] = pv[0];
+                base[H + n0] = mv[0];
+            }
+        }
+    }
+}
+
 // One fold of one 16-trial column block: split-BF16 c2r into registers, sigma
 // samples, zero-baseline bound max_w max_p boxsum / sqrt(w) over the fold's
 // bank (all widths <= WM).  Called by the whole wave; `active` gates the
 // outputs of the lane's trial column.
+// What one fold pass hands back: the bound of the lane's trial column, the
+// staging cursor after the fold's sampled columns, and where they go.
+struct FoldOut {
+    float bound;
+    int stg_end;          // staging floats in use after this fold
+    uint64_t dst;         // global address of the staged columns (0: none staged)
+};
+
 template <int NB, int WM>
-__device__ __noinline__ void fold_pass(const uint32_t* __restrict__ fre,
-                                       const uint32_t* __restrict__ fim,
-                                       const uint4* __restrict__ thi, const uint4* __restrict__ tlo,
-                                       int k, int tcol, uint32_t bankmask, bool active,
-                                       int64_t jcol, int64_t nv, int32_t cap, float* samp,
-                                       float* mo) {
+__device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fre,
+                                       const CS_LDS uint32_t* __restrict__ fim,
+                                       const CS_LDS u32x4* __restrict__ thi,
+                                       const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
+                                       uint32_t bankmask, bool active, int scol,
+                                       CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
     constexpr int EXT = WM <= 6 ? 5 : WM - 1;
     using G = geo::Geo<NB, EXT>;
     constexpr int H = G::H, MT = G::MT, R = G::R, KS = G::KS, X = G::X;
@@ -220,8 +306,7 @@ __device__ __noinline__ void fold_pass(const uint32_t* __restrict__ fre,
             const int hh = kk * 32 + g * 8 + e + 1;
             uint32_t a = 0u, bq = 0u;
             if (hh <= H) {
-                const int r = hh * k - 1;
-                const int idx = r * kWT + (tcol ^ swz(r));
+                const int idx = (hh * k - 1) * kFtpLd + tcol;
                 a = fre[idx];
                 bq = fim[idx];
             }
@@ -250,28 +335,37 @@ __device__ __noinline__ void fold_pass(const uint32_t* __restrict__ fre,
             M[4 * a + v] = E[v] - O[v];
         }
     }
-    // sigma samples (_block_sigma, search.jl:504-538): whole columns
-    if (active) {
-        const int64_t NN = (int64_t)NB * nv;
-        int64_t slot = -1, take = NB;
-        if (NN <= cap) {
-            slot = jcol * NB;
-        } else {
-            int64_t ncol = cap / NB;
-            if (ncol < 1) ncol = 1;
-            if (ncol > nv) ncol = nv;
-            const int64_t c = (jcol * ncol + nv - 1) / nv;
-            if (c < ncol && (c * nv) / ncol == jcol) {
-                slot = c * NB;
-                take = cap - slot < NB ? cap - slot : NB;
-            }
-        }
-        if (slot >= 0) {
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-                const int n = g * R + i;
-                if (n <= H && n < take) samp[slot + n] = fabsf(P[i]);
-                if (n >= 1 && n <= H - 1 && NB - n < take) samp[slot + NB - n] = fabsf(M[i]);
+    // sigma samples (_block_sigma, search.jl:504-538): the whole profile of a
+    // sampled column goes to its NB slots.  The median only sees the multiset,
+    // so the slots hold P[n] at n (n < H), the Nyquist bin P[H] at H and M[n]
+    // at H + n, and columns start every NBS = round_up(NB, 16) slots.  A wave's
+    // sampled columns are consecutive slots of one region: they are staged in
+    // the wave's LDS buffer and the caller writes them out after its last fold,
+    // so no global store is pending across the fold calls.  Columns that are
+    // not consecutive (a wave across a block boundary) or do not fit are
+    // stored directly.
+    FoldOut out{-INFINITY, stg_off, 0ull};
+    {
+        constexpr int NBS = (NB + 15) / 16 * 16;
+        const float pH = __shfl(P[H % R], col + 16 * (H / R), 64);
+        const bool smp = active && scol >= 0;
+        const uint64_t bal = __ballot(smp && g == 0);   // one bit per sampled column
+        if (bal != 0) {
+            const int first = __builtin_ctzll(bal);
+            const int c_first = __shfl(scol, first, 64);
+            const uint64_t sa = (uint64_t)(uintptr_t)samp;
+            const uint64_t s_first =
+                ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(sa >> 32), first, 64) << 32) |
+                (uint32_t)__shfl((int)(uint32_t)sa, first, 64);
+            const int rank = __builtin_popcountll(bal & ((1ull << col) - 1ull));
+            const bool contig = !smp || (scol == c_first + rank && sa == s_first);
+            const int need = __builtin_popcountll(bal) * NBS;
+            if (__ballot(!contig) == 0 && stg_off + need <= kStageFloats) {
+                if (smp) put_column<H, R, 3>(stg + stg_off + rank * NBS, g, P, M, pH);
+                out.stg_end = stg_off + need;
+                out.dst = s_first + (uint64_t)c_first * NBS * sizeof(float);
+            } else if (smp) {
+                put_column<H, R, 1>(samp + (int64_t)scol * NBS, g, P, M, pH);
             }
         }
     }
@@ -303,7 +397,8 @@ __device__ __noinline__ void fold_pass(const uint32_t* __restrict__ fre,
         if (bankmask & (1u << a)) res = fmaxf(res, m[a] * (1.0f / sqrtf((float)bank_w(a))));
     res = fmaxf(res, __shfl_xor(res, 16, 64));
     res = fmaxf(res, __shfl_xor(res, 32, 64));
-    if (active && g == 0) *mo = res;
+    out.bound = res;
+    return out;
 }
 
 #define CS_MFMA_SIZES(X) X(16) X(20) X(24) X(30) X(32) X(40) X(60) X(64) X(120) X(128)
@@ -315,9 +410,9 @@ __host__ __device__ constexpr bool mfma_size(int nb) {
 
 __host__ __device__ inline MfmaLds mfma_lds_layout(int nharms, int dft_tiles) {
     MfmaLds L;
-    L.ftp = (size_t)2 * nharms * kWT * 4;
-    L.region = 0;
+    L.ftp = (size_t)2 * nharms * kFtpLd * 4;
     L.dft = (size_t)dft_tiles * 2 * 64 * 16;
+    L.region = (size_t)(kThreads / 64) * kStageFloats * 4;   // per-wave sample staging
     L.total = L.ftp + L.region + L.dft;
     return L;
 }
@@ -335,12 +430,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const MfmaDev& md = *mdd;
     extern __shared__ __align__(16) char smem[];
     const int nh = tb.nharms;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar branches)
     uint32_t* fre = (uint32_t*)smem;   // [h][128 trials], (hi | lo << 16)
-    uint32_t* fim = fre + nh * kWT;
+    uint32_t* fim = fre + nh * kFtpLd;
     const int ntc = md.dft_tiles_max;  // DFT tiles of all folds
-    uint4* dhi = (uint4*)(smem + (size_t)2 * nh * kWT * 4);
+    uint4* dhi = (uint4*)(smem + (size_t)2 * nh * kFtpLd * 4);
     uint4* dlo = dhi + (size_t)ntc * 64;
+    const size_t lay_stage = (size_t)2 * nh * kFtpLd * 4 + (size_t)ntc * 2 * 64 * 16;
     for (int i = tid; i < ntc * 64; i += kThreads) {
         dhi[i] = md.cA_hi[i];
         dlo[i] = md.cA_lo[i];
@@ -370,20 +467,48 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const int64_t bmin = -kSpecPad, bmax = sp.namps + kSpecPad - 32 * KS;
     const uint32_t np = (uint32_t)g.Nprof;
 
-    for (int64_t tile = t_first; tile < t_end; tile += t_step) {
-        const int64_t st = tile / U;
-        const int u = (int)(tile - st * U);
-        const int64_t tbase = st * kOcc8 * S + 16 * (int64_t)u;
-        const int64_t occ = (int64_t)kOcc8 * st + o;
-
-        // ---------------- interpolation: harmonics wv+1, wv+9, ... of this wave ----------
-        int4 hd[8];
+    // Everything a tile reads from global memory before its compute: the
+    // c2r trial's valid counts and sampled columns for all folds, the
+    // harmonic descriptors, row masks and every interpolation window.  It is
+    // fetched for the next tile before this tile's outputs are stored, so the
+    // loads are not ordered behind those stores.
+    int64_t st = 0, tbase = 0, occ = 0, t = 0, jj = 0;
+    int u = 0, cl = 0;
+    bool inb = false;
+    int32_t nvf[kMaxFolds], scf[kMaxFolds];
+    int4 hd[8];
+    uint64_t rmv[4];
+    uint4 wa[8], wb[8];
+    auto fetch = [&](int64_t tile) {
+        st = tile / U;
+        u = (int)(tile - st * U);
+        tbase = st * kOcc8 * S + 16 * (int64_t)u;
+        occ = (int64_t)kOcc8 * st + o;
+        t = tbase + (int64_t)wv * S + (lane & 15);
+        inb = t < g.total;
+        jj = 0;
+        cl = 0;
+        if (inb) {
+            const uint32_t rel = (uint32_t)(t - g.origin);
+            const uint32_t cg = rel / np;
+            jj = rel - cg * np;
+            cl = (int)((int64_t)cg - b.c_first);
+        }
+#pragma unroll
+        for (int f = 0; f < kMaxFolds; ++f) {
+            nvf[f] = 0;
+            scf[f] = -1;
+            if (f < fs.nf && inb) {
+                nvf[f] = b.nvalid[cl * fs.nf + f];
+                scf[f] = md.scol[(size_t)f * np + jj];
+            }
+        }
+        // slots past nharms repeat the last harmonic: loads stay unconditional
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int h = wv + 1 + 8 * s;
-            if (h <= nh) hd[s] = md.hdesc[(size_t)(h - 1) * U + u];
+            hd[s] = ld_const4(md.hdesc, (size_t)((h <= nh ? h : nh) - 1) * U + u);
         }
-        uint64_t rmv[4];
         {
             const int64_t t0 = tbase + o * S + 4 * g4;
             const uint32_t rel = (uint32_t)(t0 - g.origin);
@@ -394,35 +519,46 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 rmv[v] = (t0 + v < g.total) ? md.rowmask[cgv] : 0ull;
             }
         }
-        // operands of one harmonic in flight: window words, weight tiles, conj(A)
-        struct Op {
-            uint4 wa, wb, ah, al, q0, q1;
-        };
-        Op ring[3];
-        auto issue = [&](int s, Op& x) {
+        // every window of this wave's harmonics (L2 / MALL / HBM)
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
             const int4 d = hd[s];
             int64_t b0 = (int64_t)d.x + (int64_t)d.y * occ;
             b0 = b0 < bmin ? bmin : (b0 > bmax ? bmax : b0);
             const uint32_t* p = plane + b0 + 8 * g4;
-            x.wa = *(const uint4_a4*)p;
-            x.wb = *(const uint4_a4*)(p + 4);
+            wa[s] = *(const uint4_a4*)p;
+            wb[s] = *(const uint4_a4*)(p + 4);
+        }
+    };
+    if (t_first < t_end) fetch(t_first);
+
+    for (int64_t tile = t_first; tile < t_end; tile += t_step) {
+        // ---------------- interpolation: harmonics wv+1, wv+9, ... of this wave ----------
+        // weight tiles and conj(A) (L2 resident) run two harmonics ahead
+        struct Op {
+            uint4 ah, al, q0, q1;
+        };
+        Op ring[3];
+        auto issue = [&](int s, Op& x) {
+            const int4 d = hd[s];
             x.ah = md.wA_hi[(size_t)d.z * 64 + lane];
             x.al = md.wA_lo[(size_t)d.z * 64 + lane];
             const uint4* gp = (const uint4*)(md.gA + (size_t)(d.w & 0x0fffffff) * 16 + 4 * g4);
             x.q0 = gp[0];
             x.q1 = gp[1];
         };
-        if (wv + 1 <= nh) issue(0, ring[0]);
-        if (wv + 9 <= nh) issue(1, ring[1]);
+        issue(0, ring[0]);
+        issue(1, ring[1]);
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int h = wv + 1 + 8 * s;
-            if (h > nh) break;
-            if (s + 2 < 8 && h + 16 <= nh) issue(s + 2, ring[(s + 2) % 3]);
+            if (s + 2 < 8) issue(s + 2, ring[(s + 2) % 3]);
+            if (h > nh) continue;
             const Op& x = ring[s % 3];
             bf16x8 bh, bl;
             {
-                const uint32_t w[8] = {x.wa.x, x.wa.y, x.wa.z, x.wa.w, x.wb.x, x.wb.y, x.wb.z, x.wb.w};
+                const uint32_t w[8] = {wa[s].x, wa[s].y, wa[s].z, wa[s].w,
+                                       wb[s].x, wb[s].y, wb[s].z, wb[s].w};
                 unpack8(w, bh, bl);
             }
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
@@ -456,59 +592,86 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 val = ((rmv[v] >> (h - 1)) & 1ull) ? val : 0.0f;
                 ow[v] = split_word(val);
             }
-            const int r = h - 1;
-            uint32_t* dst = (c ? fim : fre) + r * kWT + ((o * 16 + 4 * g4) ^ swz(r));
-            *(uint4*)dst = make_uint4(ow[0], ow[1], ow[2], ow[3]);
+            uint32_t* dst = (c ? fim : fre) + (h - 1) * kFtpLd + o * 16 + 4 * g4;
+            *(uint2*)dst = make_uint2(ow[0], ow[1]);
+            *(uint2*)(dst + 2) = make_uint2(ow[2], ow[3]);
         }
-        __syncthreads();
+        lds_barrier();
 
         // ---------------- c2r + bound, fold by fold, wave-local --------------------------
         {
-            const int col = lane & 15;
-            const int tcol = wv * 16 + col;
-            const int64_t t = tbase + (int64_t)wv * S + col;
-            const bool inb = t < g.total;
-            int64_t jj = 0;
-            int cl = 0;
-            if (inb) {
-                const uint32_t rel = (uint32_t)(t - g.origin);
-                const uint32_t cg = rel / np;
-                jj = rel - cg * np;
-                cl = (int)((int64_t)cg - b.c_first);
-            }
+            const int tcol = wv * 16 + (lane & 15);
             const int nf = fs.nf;
-            for (int f = 0; f < nf; ++f) {
+            const CS_LDS uint32_t* lre = (const CS_LDS uint32_t*)fre;
+            const CS_LDS uint32_t* lim = (const CS_LDS uint32_t*)fim;
+            CS_LDS float* lstg = (CS_LDS float*)(smem + lay_stage) + wv * kStageFloats;
+            float bnd[kMaxFolds];
+            int seg_beg[kMaxFolds], seg_end[kMaxFolds];
+            uint64_t seg_dst[kMaxFolds];
+            int cursor = 0;
+#pragma unroll
+            for (int f = 0; f < kMaxFolds; ++f) {
+                bnd[f] = -INFINITY;
+                seg_beg[f] = seg_end[f] = cursor;
+                seg_dst[f] = 0ull;
+                if (f >= nf) continue;
                 const int NB = fs.f[f].nbins;
-                const int64_t nv = inb ? (int64_t)b.nvalid[cl * nf + f] : 0;
+                const int64_t nv = nvf[f];
                 const bool act = inb && jj < nv;
-                float* mo = fu.mraw + (int64_t)f * fu.ldm + (inb ? (t - b.t_base) : 0);
-                if (inb && !act && g4 == 0) *mo = -INFINITY;
-                float* samp = fu.samples + ((int64_t)cl * nf + f) * kSigmaSamples;
-                const uint4* th = dhi + (size_t)md.c_base[f] * 64;
-                const uint4* tl = dlo + (size_t)md.c_base[f] * 64;
+                // sampled column of this trial (sample_column for blocks that are
+                // not full, else the host table)
+                int sc = scf[f];
+                if (act && nv != np) sc = sample_column(jj, nv, NB, fs.cap[f]);
+                CS_GLB float* gsamp =
+                    (CS_GLB float*)(fu.samples + ((int64_t)cl * nf + f) * fu.sld);
+                const CS_LDS u32x4* th = (const CS_LDS u32x4*)(dhi + (size_t)md.c_base[f] * 64);
+                const CS_LDS u32x4* tl = (const CS_LDS u32x4*)(dlo + (size_t)md.c_base[f] * 64);
                 const int k = fs.f[f].k;
                 const uint32_t bm = fs.bankmask[f];
-                const int32_t cap = fs.cap[f];
                 const bool narrow = (bm >> 5) == 0;   // bank within widths 1, 2, 3, 4, 6
+                FoldOut fo{-INFINITY, cursor, 0ull};
                 switch (NB) {
 #define X(SZ)                                                                                   \
     case SZ:                                                                                    \
         if constexpr (GateCT<SZ>::wmax <= 6 || !WIDE)                                           \
-            fold_pass<SZ, 6>(fre, fim, th, tl, k, tcol, bm, act, jj, nv, cap, samp, mo);        \
+            fo = fold_pass<SZ, 6>(lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, lstg, cursor); \
         else if (narrow)                                                                        \
-            fold_pass<SZ, 6>(fre, fim, th, tl, k, tcol, bm, act, jj, nv, cap, samp, mo);        \
+            fo = fold_pass<SZ, 6>(lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, lstg, cursor); \
         else                                                                                    \
-            fold_pass<SZ, GateCT<SZ>::wmax>(fre, fim, th, tl, k, tcol, bm, act, jj, nv, cap,    \
-                                            samp, mo);                                          \
+            fo = fold_pass<SZ, GateCT<SZ>::wmax>(lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, \
+                                                 lstg, cursor);                                 \
         break;
                     CS_MFMA_SIZES(X)
 #undef X
                     default:
                         break;
                 }
+                bnd[f] = act ? fo.bound : -INFINITY;
+                seg_end[f] = __builtin_amdgcn_readfirstlane(fo.stg_end);
+                seg_dst[f] = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(fo.dst >> 32)) << 32) |
+                             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)fo.dst);
+                cursor = seg_end[f];
+            }
+            // next tile's loads go out before this tile's stores
+            const int64_t t_out = t;
+            const bool inb_out = inb;
+            if (tile + t_step < t_end) fetch(tile + t_step);
+            // outputs of the tile: bounds, then the staged sigma samples
+            if (inb_out && g4 == 0) {
+#pragma unroll
+                for (int f = 0; f < kMaxFolds; ++f)
+                    if (f < nf) fu.mraw[(int64_t)f * fu.ldm + (t_out - b.t_base)] = bnd[f];
+            }
+#pragma unroll
+            for (int f = 0; f < kMaxFolds; ++f) {
+                if (f >= nf || seg_dst[f] == 0ull) continue;
+                const int nq = (seg_end[f] - seg_beg[f]) / 4;
+                CS_GLB f32x4* dst = (CS_GLB f32x4*)(uintptr_t)seg_dst[f];
+                const CS_LDS f32x4* src = (const CS_LDS f32x4*)(lstg + seg_beg[f]);
+                for (int q = lane; q < nq; q += 64) dst[q] = src[q];
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 }
 
@@ -549,6 +712,10 @@ bool mfma_supported(const FoldSet& fs, int nharms) {
         if ((fs.bankmask[f] >> bank_count(lim)) != 0) return false;
     }
     return true;
+}
+
+int mfma_sample_column(int64_t j, int64_t nv, int nbins, int64_t cap) {
+    return sample_column(j, nv, nbins, cap);
 }
 
 MfmaLds mfma_lds_plan(int nharms, const FoldSet& fs, MfmaDev& md) {

@@ -9,6 +9,9 @@ namespace cs {
 // interpolation window (at most 64 bins) starting in [-kSpecPad, namps] reads
 // inside the allocation.
 constexpr int kSpecPad = 64;
+// Sample region per (block, fold) of the MFMA mode: at most cap / nbins columns
+// of round_up(nbins, 16) slots, <= 8192 * 32 / 20 < 16384.
+constexpr int kSigmaSamplesPadded = 16384;
 
 struct MfmaDev {
     const uint4* wA_hi;       // interpolation weight tiles, [tile][64 lanes] x 8 bf16
@@ -21,6 +24,7 @@ struct MfmaDev {
     const uint4* cA_lo;
     const uint32_t* spl;      // split spectrum: [plane re/im][kSpecPad + bin], (hi | lo << 16)
     const uint64_t* rowmask;  // [global block]: bit h-1 set if harmonic row h is kept
+    const int32_t* scol;      // [fold][block trial]: sampled column of a full block, or -1
     int64_t spl_ld;           // plane stride (namps + 2 * kSpecPad)
     int64_t S;                // occurrence stride, lcm(16, q) trials
     int64_t ntiles;           // (ceil(total / (8 S))) * U tiles of 128 trials
@@ -40,6 +44,8 @@ struct MfmaLds {
 };
 
 bool mfma_supported(const FoldSet& fs, int nharms);
+// Sampled column of block trial j (whole-column subsample of the sigma estimate), -1 if none.
+int mfma_sample_column(int64_t j, int64_t nv, int nbins, int64_t cap);
 // Fills md.nbp / md.dft_tiles_max and returns the LDS plan.
 MfmaLds mfma_lds_plan(int nharms, const FoldSet& fs, MfmaDev& md);
 // Pre-split spectrum planes (2 * (namps + 2 * kSpecPad) words at spl).
