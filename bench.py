@@ -7,8 +7,10 @@ unit-power Gaussian bins, element 0 = 0, seed 2001) with von Mises pulsars at
 m=16, hidr=0.5, decimation_set(60, 6) = folds 1..6, fundamentals 0.1-125 Hz,
 threshold 8, blocks of 2048 trials: 32,186,485 trial fundamentals.
 
-One step = one full search over the band, amplitudes already resident in HBM,
-ending with the raw candidate list on the host.  value = trials / s.
+One step = one full search over the band in the split-BF16 fused mode (the
+default --mode), amplitudes already resident in HBM, ending with the raw
+candidate list on the host.  value = trials / s.  The plain FP32 GPU mode is
+timed in the same run for the ratio the metric names.
 
 Also reported: per-stage device time (HIP events on the engine's stream), the
 roofline of the dominant kernel, and the CPU baseline (the oracle's OpenMP
@@ -29,7 +31,7 @@ for _p in (os.path.join(ROOT, "coherentsearch.jl_amd"), os.path.join(ROOT, "orac
 
 METRIC = "coherent trials/s @2^25 pts nharms=60: split-BF16+batched vs plain FP32 GPU"
 HBM_PEAK_GBS = 8000.0
-FP32_VALU_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
 def log(*a):
@@ -68,12 +70,46 @@ def cpu_baseline(cfg, amps, params, r_lo, total, target_s=12.0):
                       f"of the same spectrum and parameters, {dt:.1f} s wall"}
 
 
+def timed_search(eng, r_lo, r_hi, threshold, steps, warmup, label):
+    """Warm up, then time `steps` full searches; every C-ABI call returns after
+    its stream has drained, so the host clock brackets the device work."""
+    for _ in range(warmup):
+        raw = eng.search_raw(r_lo, r_hi, threshold)
+    stage = {}
+    t0 = time.perf_counter()
+    for i in range(steps):
+        raw = eng.search_raw(r_lo, r_hi, threshold)
+        tm = eng.timings()
+        for k, v in tm.items():
+            stage[k] = stage.get(k, 0.0) + v
+        log(f"[bench] {label} step {i + 1}/{steps}: {len(raw)} raw candidates, "
+            f"device {tm['total']:.2f} ms")
+    wall = time.perf_counter() - t0
+    for k in stage:
+        stage[k] /= steps
+    return raw, wall, stage
+
+
+def load_traffic(kernel_tag):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (profiles/traffic_<tag>.json),
+    or None when that measurement is absent."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel_tag}.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="plain", choices=["plain", "fused", "fused_valu"])
+    ap.add_argument("--mode", default="fused", choices=["plain", "fused", "fused_valu"])
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="searches timed in the plain FP32 mode for the ratio (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
@@ -101,25 +137,14 @@ def main():
                                                             cfg["hifreq"], 100, 2048)
     eng = cs.Engine(params, Nprof, device=0)
     eng.set_mode(args.mode)
-    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))   # resident in HBM from here on
-
-    # Every C-ABI call returns only after its stream has drained, so the host
-    # clock around the calls brackets the device work exactly.
-    log(f"[bench] {total} trials, {nchunks} blocks; warmup {args.warmup}")
-    for _ in range(args.warmup):
-        raw = eng.search_raw(r_lo, r_hi, cfg["threshold"])
-    stage = {}
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        raw = eng.search_raw(r_lo, r_hi, cfg["threshold"])
-        tm = eng.timings()
-        for k, v in tm.items():
-            stage[k] = stage.get(k, 0.0) + v
-        log(f"[bench] step {i + 1}/{args.steps}: {len(raw)} raw candidates, device {tm['total']:.2f} ms")
-    wall = time.perf_counter() - t0
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))   # resident in HBM from here on
+    upload_s = time.perf_counter() - t0
+
+    log(f"[bench] {total} trials, {nchunks} blocks, mode {args.mode}; warmup {args.warmup}")
+    raw, wall, stage = timed_search(eng, r_lo, r_hi, cfg["threshold"], args.steps, args.warmup,
+                                    args.mode)
     ms_per_step = 1e3 * wall / args.steps
-    for k in stage:
-        stage[k] /= args.steps
     value = total / (wall / args.steps)
 
     # detections of the three injected pulsars (fold-aware, before dedup)
@@ -132,14 +157,16 @@ def main():
                 best = max(best, float(rec["metric"]))
         found[str(f0)] = round(best, 3)
 
-    # roofline of the dominant stage (per launch: one launch per batch of blocks)
+    # roofline of the dominant kernel.  One launch covers the whole search (one
+    # batch); its device time is the HIP-event interval on the engine's stream.
     nb = max(1.0, stage.get("batches", 1.0))
     dom = max(("interp", "c2r", "sigma", "gate", "rescan", "fused"), key=lambda k: stage.get(k, 0.0))
     trials_per_launch = total / nb
-    if dom == "interp":
-        # algorithmic HBM bytes per trial: 4*(nharms+1)*hidr gathered + the FP32
-        # stack written (8 B per harmonic)
-        bytes_per_trial = 4 * (params.nharms + 1) * params.hidr + 8 * params.nharms
+    # algorithmic HBM bytes per trial: harmonic h streams its band once,
+    # 8 * sum_h h * lodr = 4 * (nharms + 1) * hidr bytes per fundamental
+    spec_bytes = 4 * (params.nharms + 1) * params.hidr
+    if dom == "interp":      # plus the FP32 stack written
+        bytes_per_trial = spec_bytes + 8 * params.nharms
     elif dom == "c2r":
         nbins_sum = sum(2 * (params.nharms // k) for k in params.decimations)
         bytes_per_trial = 8 * sum(params.nharms // k for k in params.decimations) + 4 * nbins_sum
@@ -147,12 +174,34 @@ def main():
         nbins_sum = sum(2 * (params.nharms // k) for k in params.decimations)
         bytes_per_trial = 4 * nbins_sum
     else:
-        bytes_per_trial = 4 * (params.nharms + 1) * params.hidr
+        bytes_per_trial = spec_bytes
     launch_s = stage[dom] / nb / 1e3
     achieved = bytes_per_trial * trials_per_launch / launch_s / 1e9
+    tag = {"fused": "fused_mfma_c2" if args.mode == "fused" else "fused_valu_c2"}.get(dom, dom + "_c2")
+    traffic = load_traffic(tag)
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_trial": bytes_per_trial}
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic.get("bytes_per_launch") if traffic else None,
+            "bytes_per_trial": bytes_per_trial, "launch_ms": round(launch_s * 1e3, 3)}
+    # the interpolation contraction on the matrix cores: 4 * m * nharms useful
+    # FLOP per trial (real x complex MACs), against the dense bf16 peak
+    flop_per_trial = 4 * params.m * params.nharms
+    mfma = None
+    if dom == "fused" and args.mode == "fused":
+        tf = flop_per_trial * trials_per_launch / launch_s / 1e12
+        mfma = {"achieved": round(tf, 2), "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tf / BF16_MFMA_PEAK_TFLOPS, 5), "flop_per_trial": flop_per_trial}
+
+    plain = None
+    if args.plain_steps > 0 and args.mode != "plain":
+        eng.set_mode("plain")
+        praw, pwall, pstage = timed_search(eng, r_lo, r_hi, cfg["threshold"], args.plain_steps, 1,
+                                           "plain")
+        pms = 1e3 * pwall / args.plain_steps
+        plain = {"value": round(total / (pwall / args.plain_steps), 1), "unit": "trials/s",
+                 "ms_per_step": round(pms, 3), "raw_candidates": int(len(praw)),
+                 "stages_ms": {k: round(v, 3) for k, v in pstage.items()}}
+        eng.set_mode(args.mode)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -170,8 +219,13 @@ def main():
                                "blocks of 2048", "mode": args.mode, "trials": total,
                    "blocks": nchunks},
         "stages_ms": {k: round(v, 3) for k, v in stage.items()},
+        "upload_s": round(upload_s, 4),
+        "raw_candidates": int(len(raw)),
         "detections": found,
         "roofline": roof,
+        "mfma": mfma,
+        "plain_fp32_gpu": plain,
+        "speedup_vs_plain_fp32_gpu": (round(value / plain["value"], 3) if plain else None),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

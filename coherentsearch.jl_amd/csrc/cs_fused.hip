@@ -210,11 +210,19 @@ __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t kth, uint
         const uint32_t dmask = (1u << bits[pass]) - 1u;
         for (int i = threadIdx.x; i < 2048; i += kST) hist[i] = 0;
         __syncthreads();
+        if (pass == 0) {
+            // the leading digit (exponent and top mantissa bits) is shared by
+            // many lanes: one LDS atomic per distinct digit of the wave
 #pragma unroll
-        for (int r = 0; r < kSPer; ++r) {
-            uint32_t key = keys[r];
-            bool act = (key & mask) == prefix;
-            hist_add(hist, (key >> s_) & dmask, act, bits[pass]);
+            for (int r = 0; r < kSPer; ++r) hist_add(hist, (keys[r] >> s_) & dmask, true, bits[pass]);
+        } else {
+            // later passes see only the keys of the selected bucket, spread
+            // over its digits: plain LDS atomics
+#pragma unroll
+            for (int r = 0; r < kSPer; ++r) {
+                const uint32_t key = keys[r];
+                if ((key & mask) == prefix) atomicAdd(&hist[(key >> s_) & dmask], 1u);
+            }
         }
         __syncthreads();
         uint32_t a[4], loc = 0;
