@@ -1,0 +1,183 @@
+// Host check of the split-BF16 MFMA operand tables (coherentsearch.jl_amd/csrc/
+// cs_plan.cpp build_mfma_tables) against quantities recomputed here from the
+// interpolation plans alone:
+//  1. window starts: for every (harmonic, 16-trial group) the first bin of any
+//     occurrence st*8S + o*S + 16u equals the table's b0(16u) + adv*(8st + o),
+//     where the direct value is floor(h r_t) + 1 - m/2 from the exact integer
+//     state (t*hp) div/mod q and the carry table;
+//  2. weight tiles: row i of a group's tile applied to a window starting at the
+//     group's first bin reproduces trial i's own m-tap FP32 weights at its own
+//     first bin (up to the bf16 hi+lo split), and conj(A) of the class is the
+//     trial's A;
+//  3. c2r tiles: the permuted even/odd tiles (hi + lo) reproduce the numpy
+//     irfft convention (unnormalised) computed directly in FP64.
+// Prints OK on success.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "cs_plan.hpp"
+
+using namespace cs;
+
+static float bf2f(uint16_t h) {
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+static int bad = 0;
+#define CHECK(cond, ...)                      \
+    do {                                      \
+        if (!(cond)) {                        \
+            if (bad < 10) printf(__VA_ARGS__); \
+            ++bad;                            \
+        }                                     \
+    } while (0)
+
+// exact first bin and phase of trial t for harmonic plan d
+static int64_t first_bin(const Plans& pl, const HarmDev& d, int64_t t, int* phase) {
+    const __int128 T = (__int128)t * d.hp;
+    const int64_t qint = (int64_t)(T / d.q), res = (int64_t)(T % d.q);
+    const int p = pl.row[d.row_off + res];
+    *phase = p;
+    return d.rfloor0 + qint + pl.carry[d.tab_off + p] + 1 - pl.m / 2;
+}
+
+static void check_config(int nharms, int m, double hidr, double r_lo, const std::vector<int>& nbins,
+                         unsigned seed) {
+    Plans pl = build_plans(nharms, m, hidr, r_lo);
+    if (!pl.rational) {
+        printf("config nharms=%d: not rational\n", nharms);
+        ++bad;
+        return;
+    }
+    MfmaTables t = build_mfma_tables(pl, nbins);
+    if (!t.ok) {
+        printf("tables failed: %s\n", t.why.c_str());
+        ++bad;
+        return;
+    }
+    srand(seed);
+    // 1 + 2: descriptors and weight tiles
+    std::vector<double> xr(4096);
+    for (int h = 1; h <= nharms; ++h) {
+        const HarmDev& d = pl.harm[h - 1];
+        for (int u = 0; u < t.U; ++u) {
+            const int32_t* e = &t.hdesc[4 * ((size_t)(h - 1) * t.U + u)];
+            const int64_t b0u = e[0], adv = e[1];
+            const int tile = e[2], cls = e[3] & 0x0fffffff, ks = e[3] >> 28;
+            CHECK(ks == t.ks[h - 1], "h=%d ks %d vs %d\n", h, ks, t.ks[h - 1]);
+            for (int trial = 0; trial < 6; ++trial) {
+                const int64_t st = rand() % 4000, o = rand() % 8;
+                const int64_t tt = st * 8 * t.S + o * t.S + 16 * (int64_t)u;
+                int p;
+                const int64_t direct = first_bin(pl, d, tt, &p);
+                CHECK(direct == b0u + adv * (8 * st + o),
+                      "h=%d u=%d st=%lld o=%lld: first bin %lld vs %lld\n", h, u, (long long)st,
+                      (long long)o, (long long)direct, (long long)(b0u + adv * (8 * st + o)));
+            }
+            // tile rows against each trial's own weights
+            for (auto& v : xr) v = (double)(rand() % 20001 - 10000) / 10000.0;
+            int p0;
+            const int64_t g0 = first_bin(pl, d, 16 * (int64_t)u, &p0);
+            for (int i = 0; i < 16; ++i) {
+                int p;
+                const int64_t bi = first_bin(pl, d, 16 * (int64_t)u + i, &p);
+                const int64_t off = bi - g0;
+                CHECK(off >= 0 && off + m <= 32 * ks, "h=%d u=%d i=%d offset %lld\n", h, u, i,
+                      (long long)off);
+                double want = 0.0, scale = 0.0;
+                for (int tap = 0; tap < m; ++tap) {
+                    const double w = pl.W32[(size_t)(d.tab_off + p) * m + tap];
+                    want += w * xr[off + tap];
+                    scale += std::fabs(w * xr[off + tap]);
+                }
+                double got = 0.0;
+                for (int kk = 0; kk < ks; ++kk)
+                    for (int kg = 0; kg < 4; ++kg)
+                        for (int el = 0; el < 8; ++el) {
+                            const size_t idx = (((size_t)(tile + kk) * 64) + (kg * 16 + i)) * 8 + el;
+                            const double w = (double)bf2f(t.wA_hi[idx]) + (double)bf2f(t.wA_lo[idx]);
+                            got += w * xr[kk * 32 + kg * 8 + el];
+                        }
+                CHECK(std::fabs(got - want) <= 1e-5 * scale + 1e-12,
+                      "h=%d u=%d i=%d tile row %g vs %g\n", h, u, i, got, want);
+                const float ar = t.gA[2 * ((size_t)cls * 16 + i)];
+                const float ai = t.gA[2 * ((size_t)cls * 16 + i) + 1];
+                CHECK(ar == pl.A32[2 * (size_t)(d.tab_off + p)] &&
+                          ai == pl.A32[2 * (size_t)(d.tab_off + p) + 1],
+                      "h=%d u=%d i=%d conj(A) mismatch\n", h, u, i);
+            }
+        }
+    }
+    // 3: c2r tiles, with the row permutation of each lane group
+    for (size_t f = 0; f < nbins.size(); ++f) {
+        const int NB = nbins[f], H = NB / 2, mt = t.c_mt[f], ks = t.c_ks[f], R = 4 * mt;
+        std::vector<double> Xr(H + 1), Xi(H + 1), want(NB), got(NB, 0.0);
+        for (int h = 0; h <= H; ++h) {
+            Xr[h] = (double)(rand() % 2001 - 1000) / 1000.0;
+            Xi[h] = (double)(rand() % 2001 - 1000) / 1000.0;
+        }
+        Xr[0] = Xi[0] = 0.0;   // row 0 (DC) of the stack is zero
+        for (int n = 0; n < NB; ++n) {   // numpy irfft * NB: DC/Nyquist imaginary parts ignored
+            double s = Xr[0] + ((n % 2) ? -Xr[H] : Xr[H]);
+            for (int h = 1; h < H; ++h) {
+                const double a = 2.0 * M_PI * (double)((int64_t)h * n % NB) / NB;
+                s += 2.0 * (Xr[h] * std::cos(a) - Xi[h] * std::sin(a));
+            }
+            want[n] = s;
+        }
+        std::vector<double> E(16 * mt, 0.0), O(16 * mt, 0.0);
+        for (int a = 0; a < mt; ++a)
+            for (int kk = 0; kk < ks; ++kk)
+                for (int sel = 0; sel < 2; ++sel) {
+                    const size_t tile = (size_t)t.c_base[f] + (a * ks + kk) * 2 + sel;
+                    for (int l = 0; l < 64; ++l)
+                        for (int el = 0; el < 8; ++el) {
+                            const int i = l % 16, hh = kk * 32 + (l / 16) * 8 + el + 1;
+                            if (hh > H) continue;
+                            const size_t idx = (tile * 64 + l) * 8 + el;
+                            const double c = (double)bf2f(t.cA_hi[idx]) + (double)bf2f(t.cA_lo[idx]);
+                            (sel ? O : E)[a * 16 + i] += c * (sel ? Xi[hh] : Xr[hh]);
+                        }
+                }
+        // row i = 4g + v of tile a is entry n = g*R + 4a + v
+        std::vector<int> seen(NB, 0);
+        for (int a = 0; a < mt; ++a)
+            for (int i = 0; i < 16; ++i) {
+                const int n = (i / 4) * R + 4 * a + i % 4;
+                if (n > H) continue;
+                const double e = E[a * 16 + i], o = O[a * 16 + i];
+                got[n] = e + o;
+                seen[n]++;
+                if (n > 0 && n < H) {
+                    got[NB - n] = e - o;
+                    seen[NB - n]++;
+                }
+            }
+        double scale = 0.0;
+        for (int h = 1; h <= H; ++h) scale += std::fabs(Xr[h]) + std::fabs(Xi[h]);
+        for (int n = 0; n < NB; ++n) {
+            CHECK(seen[n] == 1, "NB=%d bin %d produced %d times\n", NB, n, seen[n]);
+            CHECK(std::fabs(got[n] - want[n]) <= 2e-5 * 2.0 * scale,
+                  "NB=%d bin %d: %g vs %g\n", NB, n, got[n], want[n]);
+        }
+    }
+}
+
+int main() {
+    // 2^25 points, dt 64 us: nharms 60, folds 120..20 (r_lo = 0.1 Hz * T)
+    const double T2 = 33554432.0 * 64e-6;
+    check_config(60, 16, 0.5, 0.1 * T2, {120, 60, 40, 30, 24, 20}, 11);
+    // 2^20 points: nharms 8, one fold; and a 2^24-point nharms=32 search
+    check_config(8, 16, 0.5, 1.0 * 1048576 * 64e-6, {16}, 12);
+    check_config(32, 16, 0.5, 0.1 * 16777216 * 64e-6, {64, 32, 20, 16}, 13);
+    // wider windows (two K steps): m = 32
+    check_config(32, 32, 0.5, 4000.0, {64, 32, 20, 16}, 14);
+    printf(bad ? "FAIL %d\n" : "OK\n", bad);
+    return bad ? 1 : 0;
+}
