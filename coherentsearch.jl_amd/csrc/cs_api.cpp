@@ -55,6 +55,18 @@ struct cs_ctx {
     DBuf spl;                  // pre-split spectrum planes of the MFMA mode
     bool spl_valid = false;    // spl matches the current spectrum
     int ncu = 0;               // compute units of the device
+    // host-built tables are kept while the grid they were built for is unchanged
+    bool tab_valid = false;    // plans on the device for (tab_rlo, tab_f64)
+    double tab_rlo = 0.0;
+    bool tab_f64 = false;
+    TabDev tab{};
+    bool mfma_valid = false;   // MFMA operand tables for (mfma_rlo, mfma_nbs)
+    double mfma_rlo = 0.0;
+    std::vector<int> mfma_nbs;
+    MfmaDev mfma{};
+    bool nv_valid = false;     // per-block valid counts for (nv_rlo, nv_total, nv_N)
+    double nv_rlo = 0.0;
+    int64_t nv_total = 0, nv_N = 0;
     std::vector<hipEvent_t> ev;
     double timings[8] = {0};
 
@@ -110,6 +122,13 @@ int upload(cs_ctx* c, DBuf& b, const std::vector<T>& v) {
 
 // Upload the interpolation tables for a grid anchored at r_lo and fill `tb`.
 int set_tables(cs_ctx* c, double r_lo, bool weights_f64, TabDev& tb, int64_t last_trial) {
+    if (last_trial >= ((int64_t)1 << 32))
+        return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
+    if (c->tab_valid && c->tab_rlo == r_lo && c->tab_f64 == weights_f64) {
+        tb = c->tab;
+        return CS_OK;
+    }
+    c->tab_valid = false;
     Plans pl = build_plans(c->nharms, c->m, c->hidr, r_lo);
     if (pl.rational) {
         for (const HarmDev& d : pl.harm)
@@ -138,6 +157,10 @@ int set_tables(cs_ctx* c, double r_lo, bool weights_f64, TabDev& tb, int64_t las
     tb.nharms = c->nharms;
     tb.weights_f64 = weights_f64 ? 1 : 0;
     tb.pad = 0;
+    c->tab = tb;
+    c->tab_rlo = r_lo;
+    c->tab_f64 = weights_f64;
+    c->tab_valid = true;
     return CS_OK;
 }
 
@@ -499,16 +522,51 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
     if (mfma) {
         std::vector<int> nbs;
         for (int f = 0; f < nf; ++f) nbs.push_back(fs.f[f].nbins);
-        MfmaTables mt = build_mfma_tables(build_plans(nh, c->m, c->hidr, r_lo), nbs);
-        if (!mt.ok) return c->fail(CS_EINVAL, "fused MFMA mode: " + mt.why);
-        if (total + 8 * mt.S >= ((int64_t)1 << 32))
+        if (!(c->mfma_valid && c->mfma_rlo == r_lo && c->mfma_nbs == nbs)) {
+            c->mfma_valid = false;
+            MfmaTables mt = build_mfma_tables(build_plans(nh, c->m, c->hidr, r_lo), nbs);
+            if (!mt.ok) return c->fail(CS_EINVAL, "fused MFMA mode: " + mt.why);
+            if ((s = upload(c, c->m_whi, mt.wA_hi))) return s;
+            if ((s = upload(c, c->m_wlo, mt.wA_lo))) return s;
+            if ((s = upload(c, c->m_gA, mt.gA))) return s;
+            if ((s = upload(c, c->m_hdesc, mt.hdesc))) return s;
+            if ((s = upload(c, c->m_chi, mt.cA_hi))) return s;
+            if ((s = upload(c, c->m_clo, mt.cA_lo))) return s;
+            MfmaDev& mc = c->mfma;
+            mc = MfmaDev{};
+            mc.wA_hi = (const uint4*)c->m_whi.p;
+            mc.wA_lo = (const uint4*)c->m_wlo.p;
+            mc.gA = (const float2*)c->m_gA.p;
+            mc.hdesc = (const int4*)c->m_hdesc.p;
+            mc.cA_hi = (const uint4*)c->m_chi.p;
+            mc.cA_lo = (const uint4*)c->m_clo.p;
+            mc.S = mt.S;
+            mc.U = mt.U;
+            mc.q = mt.q;
+            int ksmax = 1;
+            for (int v : mt.ks) ksmax = std::max(ksmax, v);
+            mc.ksmax = ksmax;
+            for (int f = 0; f < kMaxFolds; ++f) {
+                mc.c_base[f] = mt.c_base[f];
+                mc.c_mt[f] = mt.c_mt[f];
+                mc.c_ks[f] = mt.c_ks[f];
+            }
+            {   // sampled columns of a full block, per fold
+                std::vector<int32_t> sc((size_t)nf * Nprof);
+                for (int f = 0; f < nf; ++f)
+                    for (int64_t j = 0; j < Nprof; ++j)
+                        sc[(size_t)f * Nprof + j] =
+                            mfma_sample_column(j, Nprof, fs.f[f].nbins, fs.cap[f]);
+                if ((s = upload(c, c->m_scol, sc))) return s;
+                mc.scol = (const int32_t*)c->m_scol.p;
+            }
+            c->mfma_rlo = r_lo;
+            c->mfma_nbs = nbs;
+            c->mfma_valid = true;
+        }
+        md = c->mfma;
+        if (total + 8 * md.S >= ((int64_t)1 << 32))
             return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
-        if ((s = upload(c, c->m_whi, mt.wA_hi))) return s;
-        if ((s = upload(c, c->m_wlo, mt.wA_lo))) return s;
-        if ((s = upload(c, c->m_gA, mt.gA))) return s;
-        if ((s = upload(c, c->m_hdesc, mt.hdesc))) return s;
-        if ((s = upload(c, c->m_chi, mt.cA_hi))) return s;
-        if ((s = upload(c, c->m_clo, mt.cA_lo))) return s;
         if ((s = ensure(c, c->m_rowmask, (size_t)nchunks * sizeof(uint64_t)))) return s;
         const int64_t spl_ld = c->namps + 2 * kSpecPad;
         if (!c->spl_valid) {
@@ -516,35 +574,10 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             HIPCHK(c, launch_split_spectrum(c->st, spec_of(c), (uint32_t*)c->spl.p));
             c->spl_valid = true;
         }
-        md.wA_hi = (const uint4*)c->m_whi.p;
-        md.wA_lo = (const uint4*)c->m_wlo.p;
-        md.gA = (const float2*)c->m_gA.p;
-        md.hdesc = (const int4*)c->m_hdesc.p;
-        md.cA_hi = (const uint4*)c->m_chi.p;
-        md.cA_lo = (const uint4*)c->m_clo.p;
         md.spl = (const uint32_t*)c->spl.p;
         md.rowmask = (const uint64_t*)c->m_rowmask.p;
         md.spl_ld = spl_ld;
-        md.S = mt.S;
-        md.U = mt.U;
-        md.q = mt.q;
-        md.ntiles = (total + 8 * mt.S - 1) / (8 * mt.S) * mt.U;
-        int ksmax = 1;
-        for (int v : mt.ks) ksmax = std::max(ksmax, v);
-        md.ksmax = ksmax;
-        for (int f = 0; f < kMaxFolds; ++f) {
-            md.c_base[f] = mt.c_base[f];
-            md.c_mt[f] = mt.c_mt[f];
-            md.c_ks[f] = mt.c_ks[f];
-        }
-        {   // sampled columns of a full block, per fold
-            std::vector<int32_t> sc((size_t)nf * Nprof);
-            for (int f = 0; f < nf; ++f)
-                for (int64_t j = 0; j < Nprof; ++j)
-                    sc[(size_t)f * Nprof + j] = mfma_sample_column(j, Nprof, fs.f[f].nbins, fs.cap[f]);
-            if ((s = upload(c, c->m_scol, sc))) return s;
-            md.scol = (const int32_t*)c->m_scol.p;
-        }
+        md.ntiles = (total + 8 * md.S - 1) / (8 * md.S) * md.U;
         const MfmaLds lay = mfma_lds_plan(nh, fs, md);
         if (lay.total > 160 * 1024)
             return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");
@@ -561,15 +594,23 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
     fs.ldp = BT;
 
     // per-block valid counts for every fold (host, exact FP64 predicate)
-    std::vector<int32_t> nv((size_t)nchunks * nf);
-    const double nyq = (double)c->N / 2.0;
-    for (int64_t cc = 0; cc < nchunks; ++cc) {
-        int64_t i0 = cc * Nprof;
-        int64_t n = std::min(Nprof, total - i0);
-        double rstart = r_lo + (double)i0 * lodr;
-        for (int f = 0; f < nf; ++f) nv[cc * nf + f] = (int32_t)nvalid_of(ks[f], rstart, lodr, n, nyq);
+    if (!(c->nv_valid && c->nv_rlo == r_lo && c->nv_total == total && c->nv_N == c->N)) {
+        c->nv_valid = false;
+        std::vector<int32_t> nv((size_t)nchunks * nf);
+        const double nyq = (double)c->N / 2.0;
+        for (int64_t cc = 0; cc < nchunks; ++cc) {
+            int64_t i0 = cc * Nprof;
+            int64_t n = std::min(Nprof, total - i0);
+            double rstart = r_lo + (double)i0 * lodr;
+            for (int f = 0; f < nf; ++f)
+                nv[cc * nf + f] = (int32_t)nvalid_of(ks[f], rstart, lodr, n, nyq);
+        }
+        if ((s = upload(c, c->nvalid, nv))) return s;
+        c->nv_rlo = r_lo;
+        c->nv_total = total;
+        c->nv_N = c->N;
+        c->nv_valid = true;
     }
-    if ((s = upload(c, c->nvalid, nv))) return s;
     if ((s = ensure(c, c->sigma, (size_t)nchunks * nf * sizeof(double)))) return s;
     FusedDev fu{};
     if (fused) {

@@ -93,6 +93,16 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// An LDS pointer the optimiser cannot see through: keeps a constant base
+// from being propagated into out-of-line callees, which would then fetch the
+// dynamic-LDS base from its offset table at every access.
+template <class T>
+__device__ __forceinline__ CS_LDS T* opaque_lds(CS_LDS T* p) {
+    uint32_t v = (uint32_t)(uintptr_t)p;
+    asm volatile("" : "+s"(v));
+    return (CS_LDS T*)(uintptr_t)v;
+}
+
 // Read-only table entry at a wave-uniform index, read through the constant
 // address space so it becomes a scalar load (the kernel's global stores
 // cannot alias it).
@@ -296,22 +306,21 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
     constexpr int H = G::H, MT = G::MT, R = G::R, KS = G::KS, X = G::X;
     const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
 
-    // B operands: harmonic rows h*k (h = 1..H) of this lane's trial column
+    // B operands: harmonic rows h*k (h = 1..H) of this lane's trial column;
+    // stack row h*k sits at plane row h*k - 1
+    const int rstride = k * kFtpLd, rbase = tcol - kFtpLd;
     bf16x8 rh[KS], rl[KS], ih[KS], il[KS];
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) {
         uint32_t wr[8], wi[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int hh = kk * 32 + g * 8 + e + 1;
-            uint32_t a = 0u, bq = 0u;
-            if (hh <= H) {
-                const int idx = (hh * k - 1) * kFtpLd + tcol;
-                a = fre[idx];
-                bq = fim[idx];
-            }
-            wr[e] = a;
-            wi[e] = bq;
+            // rows past H carry exactly zero c2r coefficients, so any finite
+            // row serves: clamp to row H and read unconditionally
+            const int hh = min(kk * 32 + g * 8 + e + 1, H);
+            const int idx = hh * rstride + rbase;
+            wr[e] = fre[idx];
+            wi[e] = fim[idx];
         }
         unpack8(wr, rh[kk], rl[kk]);
         unpack8(wi, ih[kk], il[kk]);
@@ -602,9 +611,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         {
             const int tcol = wv * 16 + (lane & 15);
             const int nf = fs.nf;
-            const CS_LDS uint32_t* lre = (const CS_LDS uint32_t*)fre;
-            const CS_LDS uint32_t* lim = (const CS_LDS uint32_t*)fim;
-            CS_LDS float* lstg = (CS_LDS float*)(smem + lay_stage) + wv * kStageFloats;
+            const CS_LDS uint32_t* lre = opaque_lds((CS_LDS uint32_t*)fre);
+            const CS_LDS uint32_t* lim = opaque_lds((CS_LDS uint32_t*)fim);
+            CS_LDS float* lstg = opaque_lds((CS_LDS float*)(smem + lay_stage) + wv * kStageFloats);
             float bnd[kMaxFolds];
             int seg_beg[kMaxFolds], seg_end[kMaxFolds];
             uint64_t seg_dst[kMaxFolds];
@@ -624,8 +633,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 if (act && nv != np) sc = sample_column(jj, nv, NB, fs.cap[f]);
                 CS_GLB float* gsamp =
                     (CS_GLB float*)(fu.samples + ((int64_t)cl * nf + f) * fu.sld);
-                const CS_LDS u32x4* th = (const CS_LDS u32x4*)(dhi + (size_t)md.c_base[f] * 64);
-                const CS_LDS u32x4* tl = (const CS_LDS u32x4*)(dlo + (size_t)md.c_base[f] * 64);
+                const CS_LDS u32x4* th = opaque_lds((CS_LDS u32x4*)(dhi + (size_t)md.c_base[f] * 64));
+                const CS_LDS u32x4* tl = opaque_lds((CS_LDS u32x4*)(dlo + (size_t)md.c_base[f] * 64));
                 const int k = fs.f[f].k;
                 const uint32_t bm = fs.bankmask[f];
                 const bool narrow = (bm >> 5) == 0;   // bank within widths 1, 2, 3, 4, 6

@@ -10,7 +10,9 @@
 //     first bin (up to the bf16 hi+lo split), and conj(A) of the class is the
 //     trial's A;
 //  3. c2r tiles: the permuted even/odd tiles (hi + lo) reproduce the numpy
-//     irfft convention (unnormalised) computed directly in FP64.
+//     irfft convention (unnormalised) computed directly in FP64, and every
+//     coefficient of a stack row past H is exactly zero (the kernel reads a
+//     clamped row for those slots).
 // Prints OK on success.
 #include <cmath>
 #include <cstdio>
@@ -139,8 +141,15 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
                     for (int l = 0; l < 64; ++l)
                         for (int el = 0; el < 8; ++el) {
                             const int i = l % 16, hh = kk * 32 + (l / 16) * 8 + el + 1;
-                            if (hh > H) continue;
                             const size_t idx = (tile * 64 + l) * 8 + el;
+                            if (hh > H) {
+                                // the kernel reads row H for these slots: the
+                                // coefficients must be exactly zero
+                                CHECK(t.cA_hi[idx] == 0 && t.cA_lo[idx] == 0,
+                                      "NB=%d tile %zu lane %d el %d: nonzero past H\n", NB, tile, l,
+                                      el);
+                                continue;
+                            }
                             const double c = (double)bf2f(t.cA_hi[idx]) + (double)bf2f(t.cA_lo[idx]);
                             (sel ? O : E)[a * 16 + i] += c * (sel ? Xi[hh] : Xr[hh]);
                         }
