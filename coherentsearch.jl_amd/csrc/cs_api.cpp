@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "cs_fused.hpp"
+#include "cs_geo.hpp"
 #include "cs_mfma.hpp"
 #include "cs_gpu.hpp"
 #include "cs_plan.hpp"
@@ -60,7 +61,7 @@ struct cs_ctx {
     double tab_rlo = 0.0;
     bool tab_f64 = false;
     TabDev tab{};
-    bool mfma_valid = false;   // MFMA operand tables for (mfma_rlo, mfma_nbs)
+    bool mfma_valid = false;   // MFMA operand tables for mfma_rlo and mfma_nbs (lengths, then widest widths)
     double mfma_rlo = 0.0;
     std::vector<int> mfma_nbs;
     MfmaDev mfma{};
@@ -520,11 +521,16 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
     MfmaDev md{};
     int mfma_nwg = 0;
     if (mfma) {
-        std::vector<int> nbs;
-        for (int f = 0; f < nf; ++f) nbs.push_back(fs.f[f].nbins);
-        if (!(c->mfma_valid && c->mfma_rlo == r_lo && c->mfma_nbs == nbs)) {
+        std::vector<int> nbs, wms, key;
+        for (int f = 0; f < nf; ++f) {
+            nbs.push_back(fs.f[f].nbins);
+            wms.push_back(fold_wm(fs.f[f].nbins, fs.bankmask[f]));
+        }
+        key = nbs;
+        key.insert(key.end(), wms.begin(), wms.end());
+        if (!(c->mfma_valid && c->mfma_rlo == r_lo && c->mfma_nbs == key)) {
             c->mfma_valid = false;
-            MfmaTables mt = build_mfma_tables(build_plans(nh, c->m, c->hidr, r_lo), nbs);
+            MfmaTables mt = build_mfma_tables(build_plans(nh, c->m, c->hidr, r_lo), nbs, wms);
             if (!mt.ok) return c->fail(CS_EINVAL, "fused MFMA mode: " + mt.why);
             if ((s = upload(c, c->m_whi, mt.wA_hi))) return s;
             if ((s = upload(c, c->m_wlo, mt.wA_lo))) return s;
@@ -561,7 +567,7 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
                 mc.scol = (const int32_t*)c->m_scol.p;
             }
             c->mfma_rlo = r_lo;
-            c->mfma_nbs = nbs;
+            c->mfma_nbs = key;
             c->mfma_valid = true;
         }
         md = c->mfma;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "cs_geo.hpp"
 #include "cs_gpu.hpp"
 
 namespace cs {
@@ -135,16 +136,6 @@ __device__ __forceinline__ void block_of(const GridDev& g, int64_t t, int64_t& c
     cg = (t - g.origin) / g.Nprof;
     first = g.origin + cg * g.Nprof;
     last = min(first + g.Nprof, g.total) - 1;
-}
-
-__host__ __device__ constexpr int bank_w(int a) {
-    return a == 0 ? 1 : a == 1 ? 2 : a == 2 ? 3 : a == 3 ? 4 : a == 4 ? 6 : a == 5 ? 9
-         : a == 6 ? 13 : a == 7 ? 19 : a == 8 ? 28 : a == 9 ? 42 : a == 10 ? 63 : 94;
-}
-__host__ __device__ constexpr int bank_count(int lim) {
-    int n = 0;
-    for (int a = 0; a < 12; ++a) if (bank_w(a) <= lim) n = a + 1;
-    return n;
 }
 
 template <int NB>

@@ -20,14 +20,90 @@
 // other lanes of the same column; this header gives, at compile time, where
 // every entry lives.  Entries no valid window reads are left as the lane's
 // own (finite) registers.
+//
+// Profile lengths up to 64 use the direct layout instead (DGeo below): one
+// real c2r with Re and Im stacked along K, bins in ascending order per lane.
 #pragma once
 
 #ifndef CS_HD
+#if defined(__HIPCC__) || defined(__HIP__)
 #define CS_HD __host__ __device__
+#else
+#define CS_HD
+#endif
 #endif
 
 namespace cs {
+
+// Boxcar width ladder of the fused kernels: bank entry a has width bank_w(a)
+// (1, 2, 3, 4, 6, 9, 13, ...); bank_count(lim) entries have width <= lim.
+CS_HD constexpr int bank_w(int a) {
+    return a == 0 ? 1 : a == 1 ? 2 : a == 2 ? 3 : a == 3 ? 4 : a == 4 ? 6 : a == 5 ? 9
+         : a == 6 ? 13 : a == 7 ? 19 : a == 8 ? 28 : a == 9 ? 42 : a == 10 ? 63 : 94;
+}
+CS_HD constexpr int bank_count(int lim) {
+    int n = 0;
+    for (int a = 0; a < 12; ++a) if (bank_w(a) <= lim) n = a + 1;
+    return n;
+}
+// widest bank width the kernels support for a profile of nb bins (0.3 nb)
+CS_HD constexpr int gate_wmax(int nb) {
+    return bank_w(bank_count((3 * nb) / 10 < 1 ? 1 : (3 * nb) / 10) - 1);
+}
+// widest width a fold pass evaluates: 6 for banks within widths 1..6, else
+// the profile length's widest supported width
+CS_HD constexpr int fold_wm(int nb, unsigned bankmask) {
+    return (bankmask >> 5) == 0 || gate_wmax(nb) <= 6 ? 6 : gate_wmax(nb);
+}
+
 namespace geo {
+
+// ---------------------------------------------------------------------------
+// Direct layout (profile lengths NB <= 64).  The c2r tile row i = 4g + v of
+// row tile a computes bin (g*RD + 4a + v) mod NB directly, from Re X(1..H)
+// in K slots 0..HP-1 and Im X(1..H-1) in K slots HP..2HP-1, so lane group g
+// of a trial column holds registers X[r] = bin (g*RD + r) mod NB, r < 4*MT:
+// its RD window starts followed by the first bins of the next group's run.
+// A window of width w <= WM at start r < RD reads X[r .. r + w - 1]; entries
+// r >= 4*MT come from the next lane group, register r - RD (group 3 reads
+// group 0's register r - RD + SH, SH = 4*RD - NB, because of the wrap).
+// Group 3 owns only RD - SH starts.
+// ---------------------------------------------------------------------------
+CS_HD constexpr bool c2r_direct(int nb) { return nb <= 64; }
+
+template <int NB, int WM>
+struct DGeo {
+    static constexpr int H = NB / 2;
+    static constexpr int RD = (NB + 3) / 4;            // starts per lane group
+    static constexpr int SH = 4 * RD - NB;             // 0 unless NB % 4 != 0
+    static constexpr int EXT = WM - 1;                 // entries past a start
+    static constexpr int MT = (RD + 3) / 4 > (EXT + SH + 3) / 4 ? (RD + 3) / 4
+                                                               : (EXT + SH + 3) / 4;
+    static constexpr int NR = 4 * MT;                  // registers from the MFMAs
+    static constexpr int X = RD + EXT;                 // sequence length
+    static constexpr int NF = X > NR ? X - NR : 0;     // entries fetched from the next group
+    static constexpr int KD = H <= 16 ? 1 : H <= 32 ? 2 : 4;   // K steps
+    static constexpr int HP = 16 * KD;                 // K slots per (Re | Im) half
+    // bin held by register r of lane group g
+    static constexpr int bin(int g, int r) { return (g * RD + r) % NB; }
+    // does lane group g own the window start at its register r (r < RD)?
+    static constexpr bool owns(int g, int r) { return r < RD && g * RD + r < NB; }
+    // register of the next lane group that holds sequence entry r >= NR of group g
+    static constexpr int next_reg(int g, int r) { return g == 3 ? r - RD + SH : r - RD; }
+};
+
+// c2r coefficient of K slot kidx for bin n of a direct-layout tile (numpy
+// irfft convention, unnormalised; DC excluded, Nyquist imaginary part ignored):
+// returns {re/im selector 0/1, harmonic h}; the value is
+//   re: (h < H ? 2 : 1) cos(2 pi h n / NB) for h <= H, else 0
+//   im: -2 sin(2 pi h n / NB) for h < H, else 0
+struct KSlot {
+    int im;
+    int h;
+};
+CS_HD constexpr KSlot dslot(int kidx, int hp) {
+    return kidx < hp ? KSlot{0, kidx + 1} : KSlot{1, kidx - hp + 1};
+}
 
 struct Src {
     int lane;   // lane group 0..3 of the column that holds the value

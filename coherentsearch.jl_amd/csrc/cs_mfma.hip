@@ -7,9 +7,10 @@
 // by an exact integer number of bins per occurrence:
 //
 //   interpolation  D[16 trials][8 occ x (re, im)] = Wx[16][K] . Win[K][16]
-//   c2r (per fold) E[n][128] = Ccos[n][h] . Re F[h][128]
+//   c2r, NB <= 64  x[n][128] = C[n][Re h | Im h] . F[h][128]   (direct)
+//   c2r, NB > 64   E[n][128] = Ccos[n][h] . Re F[h][128]
 //                  O[n][128] = Csin[n][h] . Im F[h][128]
-//                  bin n = E + O, bin NB - n = E - O
+//                  bin n = E + O, bin NB - n = E - O             (even/odd)
 //
 // each a v_mfma_f32_16x16x32_bf16 chain on bf16 hi/lo splits (hi.hi + hi.lo +
 // lo.hi, FP32 accumulation).  The spectrum is kept pre-split as (hi | lo << 16)
@@ -17,9 +18,11 @@
 // byte-permutes; the stacked amplitudes go to LDS pre-split the same way.  The
 // c2r rows are permuted so each of the four lanes of a trial column holds a
 // contiguous run of the profile, and the boxcar bound is formed from registers
-// (cs_geo.hpp).  LDS holds the stacked amplitudes and every fold's DFT tiles;
-// a workgroup loops over tiles (XCD-contiguous ranges), two barriers per tile.
-// Index and phase bookkeeping stay exact integer arithmetic.
+// (cs_geo.hpp): in the direct layout a lane's run already carries most of its
+// window extension, in the even/odd layout runs are stitched across lanes.
+// LDS holds the stacked amplitudes and every fold's DFT tiles; a workgroup
+// loops over tiles (XCD-contiguous ranges), two barriers per tile.  Index and
+// phase bookkeeping stay exact integer arithmetic.
 #include <hip/hip_runtime.h>
 
 #include "cs_device.hpp"
@@ -42,7 +45,7 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 constexpr int kOcc8 = 8;          // occurrences per tile
 constexpr int kWT = 128;          // trials per tile
 constexpr int kThreads = 512;     // 8 waves
-constexpr int kStageFloats = 1024;  // per-wave LDS staging of sampled columns
+constexpr int kStageFloats = 768;   // per-wave LDS staging of sampled columns
 
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -294,6 +297,40 @@ struct FoldOut {
     uint64_t dst;         // global address of the staged columns (0: none staged)
 };
 
+// sigma samples (_block_sigma, search.jl:504-538): the whole profile of a
+// sampled column goes to its NB slots; columns start every NBS slots.  The
+// median only sees the multiset, so the order of a column's slots is free.
+// A wave's sampled columns are consecutive slots of one region: they are
+// staged in the wave's LDS buffer and the caller writes them out after its
+// last fold, so no global store is pending across the fold calls.  Columns
+// that are not consecutive (a wave across a block boundary) or do not fit
+// are stored directly.  putl / putg write this lane group's share of the
+// column whose slot block starts at their argument.
+template <int NBS, class PutL, class PutG>
+__device__ __forceinline__ void stage_samples(FoldOut& out, bool active, int scol, int g, int col,
+                                              CS_GLB float* samp, CS_LDS float* stg, int stg_off,
+                                              PutL putl, PutG putg) {
+    const bool smp = active && scol >= 0;
+    const uint64_t bal = __ballot(smp && g == 0);   // one bit per sampled column
+    if (bal == 0) return;
+    const int first = __builtin_ctzll(bal);
+    const int c_first = __shfl(scol, first, 64);
+    const uint64_t sa = (uint64_t)(uintptr_t)samp;
+    const uint64_t s_first =
+        ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(sa >> 32), first, 64) << 32) |
+        (uint32_t)__shfl((int)(uint32_t)sa, first, 64);
+    const int rank = __builtin_popcountll(bal & ((1ull << col) - 1ull));
+    const bool contig = !smp || (scol == c_first + rank && sa == s_first);
+    const int need = __builtin_popcountll(bal) * NBS;
+    if (__ballot(!contig) == 0 && stg_off + need <= kStageFloats) {
+        if (smp) putl(stg + stg_off + rank * NBS);
+        out.stg_end = stg_off + need;
+        out.dst = s_first + (uint64_t)c_first * NBS * sizeof(float);
+    } else if (smp) {
+        putg(samp + (int64_t)scol * NBS);
+    }
+}
+
 template <int NB, int WM>
 __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fre,
                                        const CS_LDS uint32_t* __restrict__ fim,
@@ -344,39 +381,15 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
             M[4 * a + v] = E[v] - O[v];
         }
     }
-    // sigma samples (_block_sigma, search.jl:504-538): the whole profile of a
-    // sampled column goes to its NB slots.  The median only sees the multiset,
-    // so the slots hold P[n] at n (n < H), the Nyquist bin P[H] at H and M[n]
-    // at H + n, and columns start every NBS = round_up(NB, 16) slots.  A wave's
-    // sampled columns are consecutive slots of one region: they are staged in
-    // the wave's LDS buffer and the caller writes them out after its last fold,
-    // so no global store is pending across the fold calls.  Columns that are
-    // not consecutive (a wave across a block boundary) or do not fit are
-    // stored directly.
+    // sigma samples: P[n] at slot n (n < H), the Nyquist bin P[H] at H and
+    // M[n] at H + n
     FoldOut out{-INFINITY, stg_off, 0ull};
     {
-        constexpr int NBS = (NB + 15) / 16 * 16;
         const float pH = __shfl(P[H % R], col + 16 * (H / R), 64);
-        const bool smp = active && scol >= 0;
-        const uint64_t bal = __ballot(smp && g == 0);   // one bit per sampled column
-        if (bal != 0) {
-            const int first = __builtin_ctzll(bal);
-            const int c_first = __shfl(scol, first, 64);
-            const uint64_t sa = (uint64_t)(uintptr_t)samp;
-            const uint64_t s_first =
-                ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(sa >> 32), first, 64) << 32) |
-                (uint32_t)__shfl((int)(uint32_t)sa, first, 64);
-            const int rank = __builtin_popcountll(bal & ((1ull << col) - 1ull));
-            const bool contig = !smp || (scol == c_first + rank && sa == s_first);
-            const int need = __builtin_popcountll(bal) * NBS;
-            if (__ballot(!contig) == 0 && stg_off + need <= kStageFloats) {
-                if (smp) put_column<H, R, 3>(stg + stg_off + rank * NBS, g, P, M, pH);
-                out.stg_end = stg_off + need;
-                out.dst = s_first + (uint64_t)c_first * NBS * sizeof(float);
-            } else if (smp) {
-                put_column<H, R, 1>(samp + (int64_t)scol * NBS, g, P, M, pH);
-            }
-        }
+        stage_samples<(NB + 15) / 16 * 16>(
+            out, active, scol, g, col, samp, stg, stg_off,
+            [&](CS_LDS float* d) { put_column<H, R, 3>(d, g, P, M, pH); },
+            [&](CS_GLB float* d) { put_column<H, R, 1>(d, g, P, M, pH); });
     }
     // boxcar bound
     float m[12];
@@ -408,6 +421,187 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
     res = fmaxf(res, __shfl_xor(res, 32, 64));
     out.bound = res;
     return out;
+}
+
+// One sampled column's share of lane group g in the direct layout: registers
+// X[r] = bin g*RD + r for the starts the group owns, at slots g*RD + r.
+template <int RD, int SH, int ASN>
+__device__ __forceinline__ void put_run(__attribute__((address_space(ASN))) float* base, int g,
+                                        const float* Xs) {
+    typedef float f2 __attribute__((ext_vector_type(2)));
+    typedef __attribute__((address_space(ASN))) f32x4 V4;
+    typedef __attribute__((address_space(ASN))) f2 V2;
+    constexpr int CH = (RD % 4 == 0) ? 4 : (RD % 2 == 0) ? 2 : 1;
+    constexpr int OWN3 = RD - SH;   // starts of group 3
+    base += g * RD;
+#pragma unroll
+    for (int c0 = 0; c0 < RD; c0 += CH) {
+        float v[CH];
+#pragma unroll
+        for (int e = 0; e < CH; ++e) v[e] = fabsf(Xs[c0 + e]);
+        const bool whole = c0 + CH <= OWN3 || g != 3;
+        if (whole) {
+            if constexpr (CH == 4) *(V4*)(base + c0) = f32x4{v[0], v[1], v[2], v[3]};
+            else if constexpr (CH == 2) *(V2*)(base + c0) = f2{v[0], v[1]};
+            else base[c0] = v[0];
+        } else {
+#pragma unroll
+            for (int e = 0; e < CH; ++e)
+                if (c0 + e < OWN3) base[c0 + e] = v[e];
+        }
+    }
+}
+
+// One fold of one 16-trial column block in the direct layout (NB <= 64,
+// cs_geo.hpp DGeo): split-BF16 c2r straight to bins, lane group g holding
+// bins g*RD .. g*RD + 4*MT - 1 (mod NB), the rest of its window extension
+// read from the next lane group; sigma samples; the bound
+// max_w max_p boxsum / sqrt(w) over the fold's bank (widths <= WM).
+template <int NB, int WM>
+__device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ fre,
+                                            const CS_LDS uint32_t* __restrict__ fim,
+                                            const CS_LDS u32x4* __restrict__ thi,
+                                            const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
+                                            uint32_t bankmask, bool active, int scol,
+                                            CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+    using D = geo::DGeo<NB, WM>;
+    constexpr int H = D::H, RD = D::RD, SH = D::SH, MT = D::MT, NR = D::NR, X = D::X,
+                  NF = D::NF, KD = D::KD;
+    static_assert(KD <= 2, "the direct layout covers H <= 32");
+    const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
+    const int rstride = k * kFtpLd, rbase = tcol - kFtpLd;   // stack row h*k at plane row h*k - 1
+
+    // B operands: K slot kk*32 + 8g + e holds Re h (first half) or Im h
+    // (second half); slots past the last harmonic read row H (zero coefficients)
+    bf16x8 bh[KD], bl[KD];
+    if constexpr (KD == 1) {
+        const CS_LDS uint32_t* pl = g < 2 ? fre : fim;
+        uint32_t w[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) w[e] = pl[min(8 * (g & 1) + e + 1, H) * rstride + rbase];
+        unpack8(w, bh[0], bl[0]);
+    } else {
+        uint32_t wr[8], wi[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int idx = min(8 * g + e + 1, H) * rstride + rbase;
+            wr[e] = fre[idx];
+            wi[e] = fim[idx];
+        }
+        unpack8(wr, bh[0], bl[0]);
+        unpack8(wi, bh[1], bl[1]);
+    }
+    float Xs[X];
+#pragma unroll
+    for (int a = 0; a < MT; ++a) {
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int kk = 0; kk < KD; ++kk) {
+            const int tt = a * KD + kk;
+            acc = mfma3(as_bf16x8(thi[tt * 64 + lane]), as_bf16x8(tlo[tt * 64 + lane]), bh[kk],
+                        bl[kk], acc);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Xs[4 * a + v] = acc[v];
+    }
+    // window extension past the MFMA registers: the next lane group's
+    // registers (group 0 serves group 3 across the wrap)
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+        const int r = NR + i;
+        float offer = Xs[r - RD];
+        if constexpr (SH != 0) offer = g == 0 ? Xs[r - RD + SH] : offer;
+        Xs[r] = __shfl(offer, (lane + 16) & 63, 64);
+    }
+
+    FoldOut out{-INFINITY, stg_off, 0ull};
+    stage_samples<(NB + 15) / 16 * 16>(
+        out, active, scol, g, col, samp, stg, stg_off,
+        [&](CS_LDS float* d) { put_run<RD, SH, 3>(d, g, Xs); },
+        [&](CS_GLB float* d) { put_run<RD, SH, 1>(d, g, Xs); });
+
+    // boxcar bound over the starts this lane group owns
+    float m[12];
+#pragma unroll
+    for (int a = 0; a < 12; ++a) m[a] = -INFINITY;
+    const bool last = g == 3;
+    if constexpr (WM <= 6) {
+#pragma unroll
+        for (int r = 0; r < RD; ++r) {
+            float s1 = Xs[r];
+            float s2 = Xs[r] + Xs[r + 1];
+            float s3 = s2 + Xs[r + 2];
+            float s4 = s2 + (Xs[r + 2] + Xs[r + 3]);
+            float s6 = s4 + (Xs[r + 4] + Xs[r + 5]);
+            if (SH != 0 && r >= RD - SH) {
+                s1 = last ? -INFINITY : s1;
+                s2 = last ? -INFINITY : s2;
+                s3 = last ? -INFINITY : s3;
+                s4 = last ? -INFINITY : s4;
+                s6 = last ? -INFINITY : s6;
+            }
+            m[0] = fmaxf(m[0], s1);
+            m[1] = fmaxf(m[1], s2);
+            m[2] = fmaxf(m[2], s3);
+            m[3] = fmaxf(m[3], s4);
+            m[4] = fmaxf(m[4], s6);
+        }
+    } else {
+        float ps[X + 1];
+        ps[0] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < X; ++i) ps[i + 1] = ps[i] + Xs[i];
+#pragma unroll
+        for (int a = 0; a < bank_count(WM); ++a) {
+            const int w = bank_w(a);
+#pragma unroll
+            for (int r = 0; r < RD; ++r) {
+                float d = ps[r + w] - ps[r];
+                if (SH != 0 && r >= RD - SH) d = last ? -INFINITY : d;
+                m[a] = fmaxf(m[a], d);
+            }
+        }
+    }
+    float res = -INFINITY;
+#pragma unroll
+    for (int a = 0; a < bank_count(WM <= 6 ? 6 : WM); ++a)
+        if (bankmask & (1u << a)) res = fmaxf(res, m[a] * (1.0f / sqrtf((float)bank_w(a))));
+    res = fmaxf(res, __shfl_xor(res, 16, 64));
+    res = fmaxf(res, __shfl_xor(res, 32, 64));
+    out.bound = res;
+    return out;
+}
+
+// The pass for a fold of NB bins: the direct layout up to 64 bins, the
+// even/odd layout above; widths up to 6 unless the fold's bank is wider.
+template <int NB, bool WIDE>
+__device__ __forceinline__ FoldOut run_fold(bool narrow, const CS_LDS uint32_t* fre,
+                                            const CS_LDS uint32_t* fim, const CS_LDS u32x4* thi,
+                                            const CS_LDS u32x4* tlo, int k, int tcol,
+                                            uint32_t bankmask, bool active, int scol,
+                                            CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+    constexpr int WW = gate_wmax(NB);
+    if constexpr (geo::c2r_direct(NB)) {
+        if constexpr (WW <= 6 || !WIDE)
+            return fold_direct<NB, 6>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                      stg, stg_off);
+        else if (narrow)
+            return fold_direct<NB, 6>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                      stg, stg_off);
+        else
+            return fold_direct<NB, WW>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                       stg, stg_off);
+    } else {
+        if constexpr (WW <= 6 || !WIDE)
+            return fold_pass<NB, 6>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
+                                    stg_off);
+        else if (narrow)
+            return fold_pass<NB, 6>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
+                                    stg_off);
+        else
+            return fold_pass<NB, WW>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                     stg, stg_off);
+    }
 }
 
 #define CS_MFMA_SIZES(X) X(16) X(20) X(24) X(30) X(32) X(40) X(60) X(64) X(120) X(128)
@@ -640,15 +834,10 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 const bool narrow = (bm >> 5) == 0;   // bank within widths 1, 2, 3, 4, 6
                 FoldOut fo{-INFINITY, cursor, 0ull};
                 switch (NB) {
-#define X(SZ)                                                                                   \
-    case SZ:                                                                                    \
-        if constexpr (GateCT<SZ>::wmax <= 6 || !WIDE)                                           \
-            fo = fold_pass<SZ, 6>(lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, lstg, cursor); \
-        else if (narrow)                                                                        \
-            fo = fold_pass<SZ, 6>(lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, lstg, cursor); \
-        else                                                                                    \
-            fo = fold_pass<SZ, GateCT<SZ>::wmax>(lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, \
-                                                 lstg, cursor);                                 \
+#define X(SZ)                                                                                 \
+    case SZ:                                                                                  \
+        fo = run_fold<SZ, WIDE>(narrow, lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, lstg, \
+                                cursor);                                                      \
         break;
                     CS_MFMA_SIZES(X)
 #undef X
@@ -729,7 +918,8 @@ int mfma_sample_column(int64_t j, int64_t nv, int nbins, int64_t cap) {
 
 MfmaLds mfma_lds_plan(int nharms, const FoldSet& fs, MfmaDev& md) {
     int tot = 0;
-    for (int f = 0; f < fs.nf; ++f) tot += md.c_mt[f] * md.c_ks[f] * 2;
+    for (int f = 0; f < fs.nf; ++f)
+        tot += md.c_mt[f] * md.c_ks[f] * (geo::c2r_direct(fs.f[f].nbins) ? 1 : 2);
     md.dft_tiles_max = tot;
     md.nbp = 0;
     return mfma_lds_layout(nharms, tot);

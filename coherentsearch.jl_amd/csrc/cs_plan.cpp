@@ -10,6 +10,7 @@
 // (directinterp.jl:237-282) are a reordering of exactly those FMAs and need no
 // counterpart here.
 #include "cs_plan.hpp"
+#include "cs_geo.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -185,7 +186,8 @@ static void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
     lo = f2bf(x - bf2f(hi));
 }
 
-MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins) {
+MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins,
+                             const std::vector<int>& fold_wm) {
     MfmaTables t;
     if (!pl.rational) { t.why = "trial step is not a small rational"; return t; }
     const int m = pl.m, nh = pl.nharms;
@@ -285,12 +287,44 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
             t.hdesc[e + 3] = cls | (t.ks[h - 1] << 28);
         }
     }
-    // c2r: even/odd DFT tiles; row i = 4g + v of row tile a computes profile
-    // entry n = g*R + 4a + v (R = 4 * row tiles), so lane group g of a trial
-    // column ends up holding the contiguous run n = g*R .. g*R + R - 1
     int ctile = 0;
     for (size_t f = 0; f < fold_nbins.size() && f < 8; ++f) {
         const int NB = fold_nbins[f], H = NB / 2;
+        if (geo::c2r_direct(NB)) {
+            // direct tiles: row i = 4g + v of row tile a computes bin
+            // (g*RD + 4a + v) mod NB from K slots (Re h | Im h)
+            const int wm = f < fold_wm.size() ? fold_wm[f] : 6;
+            const int RD = (NB + 3) / 4, SH = 4 * RD - NB, ext = wm - 1;
+            const int mt = std::max((RD + 3) / 4, (ext + SH + 3) / 4);
+            const int kd = H <= 16 ? 1 : H <= 32 ? 2 : 4, hp = 16 * kd;
+            t.c_base[f] = ctile;
+            t.c_mt[f] = mt;
+            t.c_ks[f] = kd;
+            for (int a = 0; a < mt; ++a)
+                for (int kk = 0; kk < kd; ++kk) {
+                    for (int l = 0; l < 64; ++l) {
+                        const int i = l % 16;
+                        const int n = ((i / 4) * RD + 4 * a + i % 4) % NB;
+                        for (int e = 0; e < 8; ++e) {
+                            const geo::KSlot ks = geo::dslot(kk * 32 + (l / 16) * 8 + e, hp);
+                            const int64_t ee = ((int64_t)ks.h * n) % NB;
+                            const double ang = 2.0 * (double)ee / (double)NB;
+                            double v = 0.0;
+                            if (ks.im == 0 && ks.h <= H) v = (ks.h < H ? 2.0 : 1.0) * cospi_d(ang);
+                            else if (ks.im == 1 && ks.h < H) v = -2.0 * sinpi_d(ang);
+                            uint16_t hi, lo;
+                            split_bf16((float)v, hi, lo);
+                            t.cA_hi.push_back(hi);
+                            t.cA_lo.push_back(lo);
+                        }
+                    }
+                    ++ctile;
+                }
+            continue;
+        }
+        // even/odd DFT tiles; row i = 4g + v of row tile a computes profile
+        // entry n = g*R + 4a + v (R = 4 * row tiles), so lane group g of a
+        // trial column ends up holding the contiguous run n = g*R .. g*R + R - 1
         const int mt = (H + 1 + 15) / 16, ks = (H + 31) / 32, R = 4 * mt;
         t.c_base[f] = ctile;
         t.c_mt[f] = mt;

@@ -59,12 +59,17 @@ std::vector<int> ladder_boxcar_widths(int nbins, int k, const std::vector<int>& 
 //  * interpolation: for each harmonic h and each residue class of 16-trial
 //    groups, the 16 x (m + delta) weight block Wx[i][k] = W[k - delta_i, p_i],
 //    split into bf16 hi + lo, plus the per-trial conj(A(dr)) factors;
-//  * c2r: for each fold, the even/odd halves of the numpy-convention real
-//    inverse DFT, rows n = 0..H, columns h = 1..H:
+//  * c2r, profile lengths NB > 64: the even/odd halves of the numpy-convention
+//    real inverse DFT, rows n = 0..H, columns h = 1..H:
 //      Ccos[n][h] = c_h cos(2 pi h n / NB),  Csin[n][h] = -c_h sin(2 pi h n / NB)
 //    (c_h = 2, c_H = 1, Csin[.][H] = 0), so x[n] = E + O and x[NB-n] = E - O
 //    with E = Ccos . Re X and O = Csin . Im X; split likewise.  Row i = 4g + v
-//    of row tile a is entry n = g*4*mt + 4a + v (mt row tiles).
+//    of row tile a is entry n = g*4*mt + 4a + v (mt row tiles); tiles
+//    [(a*ks + kk)*2 + {cos, sin}].
+//  * c2r, NB <= 64 (direct layout, cs_geo.hpp DGeo): one real DFT with
+//    K slots (Re h = 1..H | Im h = 1..H-1), row i = 4g + v of row tile a is
+//    bin (g*RD + 4a + v) mod NB, RD = ceil(NB/4); tiles [a*ks + kk].  The row
+//    tile count depends on the fold's widest boxcar (fold_wm).
 struct MfmaTables {
     bool ok = false;
     std::string why;
@@ -79,7 +84,7 @@ struct MfmaTables {
     std::vector<int32_t> clsu;            // [(h-1)*U + u] -> {first tile, class} of the group at 16u
     std::vector<int32_t> hdesc;           // [(h-1)*U + u] x {first bin of trial 16u, bin advance
                                           //  per occurrence, first tile, class | K steps << 28}
-    std::vector<uint16_t> cA_hi, cA_lo;   // c2r tiles: [(a*ks + kk)*2 + {cos, sin}], rows permuted
+    std::vector<uint16_t> cA_hi, cA_lo;   // c2r tiles of every fold, rows permuted
     int32_t c_base[8] = {0}, c_mt[8] = {0}, c_ks[8] = {0};
 };
 
@@ -89,6 +94,8 @@ struct FoldDev {
     int32_t widths[kMaxWidths];
 };
 
-MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins);
+// fold_wm[f]: widest boxcar the fold's pass evaluates (geo fold_wm)
+MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins,
+                             const std::vector<int>& fold_wm);
 
 }  // namespace cs

@@ -1,8 +1,9 @@
 // Host check of the register geometry used by the split-BF16 bound
 // (coherentsearch.jl_amd/csrc/cs_geo.hpp): for every supported profile length,
-// rebuild each lane's XP/XM sequences from emulated lane registers and compare
-// every boxcar sum and per-width maximum with a direct circular boxcar over
-// the profile.  Integer-valued profiles make the sums exact.
+// rebuild each lane's XP/XM sequences (even/odd layout) or its direct-layout
+// run and extension from emulated lane registers and compare every boxcar sum
+// and per-width maximum with a direct circular boxcar over the profile.
+// Integer-valued profiles make the sums exact.
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -90,6 +91,70 @@ static int check_nb(unsigned seed) {
     return bad;
 }
 
+// Direct layout (DGeo): lane group g's MFMA registers hold bins
+// (g*RD + r) mod NB (r < NR); entries r >= NR are fetched the way the kernel
+// does it, every lane reading its next group, which offers register r - RD
+// (group 0 offers r - RD + SH).  Every window the lanes form must equal the
+// circular boxcar, every bin must start exactly one window, and the sample
+// writes (registers r < RD of the starts a group owns, at slot g*RD + r) must
+// cover every bin exactly once with its own value.
+template <int NB, int WM>
+static int check_direct(unsigned seed) {
+    using D = DGeo<NB, WM>;
+    constexpr int RD = D::RD, NR = D::NR, X = D::X, SH = D::SH;
+    static_assert(D::EXT + SH <= NR && D::EXT <= NR, "extension source outside the MFMA registers");
+    int nwid = 0;
+    while (nwid < 9 && kW[nwid] <= WM) ++nwid;
+    int bad = 0;
+    srand(seed);
+    for (int trial = 0; trial < 50; ++trial) {
+        std::vector<long> x(NB);
+        for (int b = 0; b < NB; ++b) x[b] = (rand() % 2001) - 1000;
+        long reg[4][X];
+        for (int g = 0; g < 4; ++g)
+            for (int r = 0; r < NR; ++r) reg[g][r] = x[D::bin(g, r)];
+        for (int g = 0; g < 4; ++g)
+            for (int r = NR; r < X; ++r) {
+                const int sg = (g + 1) % 4;                         // the lane read
+                const int sr = sg == 0 ? r - RD + SH : r - RD;      // what it offers
+                if (sr < 0 || sr >= NR) {
+                    printf("NB=%d WM=%d: offer register %d out of range\n", NB, WM, sr);
+                    return bad + 1;
+                }
+                if (sr != D::next_reg(g, r)) ++bad;
+                reg[g][r] = reg[sg][sr];
+            }
+        std::vector<int> starts(NB, 0), slots(NB, 0);
+        for (int g = 0; g < 4; ++g)
+            for (int r = 0; r < RD; ++r) {
+                if (!D::owns(g, r)) continue;
+                const int bin = g * RD + r;
+                starts[bin] += 1;
+                slots[bin] += 1;
+                if (reg[g][r] != x[bin]) ++bad;   // sample slot value
+                for (int a = 0; a < nwid; ++a) {
+                    long s = 0, ref = 0;
+                    for (int j = 0; j < kW[a]; ++j) {
+                        s += reg[g][r + j];
+                        ref += x[(bin + j) % NB];
+                    }
+                    if (s != ref) {
+                        if (bad < 5)
+                            printf("direct NB=%d WM=%d g=%d r=%d w=%d: %ld vs %ld\n", NB, WM, g, r,
+                                   kW[a], s, ref);
+                        ++bad;
+                    }
+                }
+            }
+        for (int b = 0; b < NB; ++b)
+            if (starts[b] != 1 || slots[b] != 1) {
+                if (bad < 5) printf("direct NB=%d bin %d started %d times\n", NB, b, starts[b]);
+                ++bad;
+            }
+    }
+    return bad;
+}
+
 template <int NB, int EXT>
 static void rounds() {
     using G = Geo<NB, EXT>;
@@ -110,6 +175,11 @@ int main() {
 #undef X
 #define X(NB, EXT) bad += check_nb<NB, EXT>(seed++);
     CS_GEO_CASES(X)
+#undef X
+    // direct layout: every length up to 64, narrow and full banks
+#define X(NB, WM) bad += check_direct<NB, WM>(seed++);
+    X(16, 6) X(20, 6) X(24, 6) X(30, 6) X(32, 6) X(40, 6) X(60, 6) X(64, 6)
+    X(16, 4) X(30, 9) X(32, 9) X(40, 9) X(60, 13) X(64, 19)
 #undef X
     printf(bad ? "FAIL %d\n" : "OK\n", bad);
     return bad ? 1 : 0;

@@ -9,10 +9,11 @@
 //     group's first bin reproduces trial i's own m-tap FP32 weights at its own
 //     first bin (up to the bf16 hi+lo split), and conj(A) of the class is the
 //     trial's A;
-//  3. c2r tiles: the permuted even/odd tiles (hi + lo) reproduce the numpy
+//  3. c2r tiles: the permuted even/odd tiles (NB > 64) and the direct-layout
+//     tiles (NB <= 64, every register of every lane group) reproduce the numpy
 //     irfft convention (unnormalised) computed directly in FP64, and every
-//     coefficient of a stack row past H is exactly zero (the kernel reads a
-//     clamped row for those slots).
+//     coefficient of a dead K slot is exactly zero (the kernel reads a clamped
+//     row for those slots).
 // Prints OK on success.
 #include <cmath>
 #include <cstdio>
@@ -50,14 +51,16 @@ static int64_t first_bin(const Plans& pl, const HarmDev& d, int64_t t, int* phas
 }
 
 static void check_config(int nharms, int m, double hidr, double r_lo, const std::vector<int>& nbins,
-                         unsigned seed) {
+                         unsigned seed, const std::vector<int>& wms = {}) {
     Plans pl = build_plans(nharms, m, hidr, r_lo);
     if (!pl.rational) {
         printf("config nharms=%d: not rational\n", nharms);
         ++bad;
         return;
     }
-    MfmaTables t = build_mfma_tables(pl, nbins);
+    std::vector<int> wm = wms;
+    if (wm.empty()) wm.assign(nbins.size(), 6);
+    MfmaTables t = build_mfma_tables(pl, nbins, wm);
     if (!t.ok) {
         printf("tables failed: %s\n", t.why.c_str());
         ++bad;
@@ -133,6 +136,49 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
             }
             want[n] = s;
         }
+        double scale = 0.0;
+        for (int h = 1; h <= H; ++h) scale += std::fabs(Xr[h]) + std::fabs(Xi[h]);
+        if (NB <= 64) {
+            // direct layout: K slot kk*32 + 8(l/16) + e is Re h (first HP slots)
+            // or Im h (next HP), the kernel reading row min(h, H); lane group g,
+            // register r = 4a + v holds bin (g*RD + r) mod NB
+            const int RD = (NB + 3) / 4, SH = 4 * RD - NB, ext = wm[f] - 1;
+            const int mt = std::max((RD + 3) / 4, (ext + SH + 3) / 4);
+            const int kd = H <= 16 ? 1 : H <= 32 ? 2 : 4, hp = 16 * kd;
+            CHECK(t.c_mt[f] == mt && t.c_ks[f] == kd, "NB=%d direct tile counts %d %d\n", NB,
+                  t.c_mt[f], t.c_ks[f]);
+            std::vector<double> acc(16 * mt, 0.0);
+            for (int a = 0; a < mt; ++a)
+                for (int kk = 0; kk < kd; ++kk) {
+                    const size_t tile = (size_t)t.c_base[f] + a * kd + kk;
+                    for (int l = 0; l < 64; ++l)
+                        for (int el = 0; el < 8; ++el) {
+                            const int i = l % 16, kidx = kk * 32 + (l / 16) * 8 + el;
+                            const bool im = kidx >= hp;
+                            const int h = (im ? kidx - hp : kidx) + 1;
+                            const size_t idx = (tile * 64 + l) * 8 + el;
+                            const double c = (double)bf2f(t.cA_hi[idx]) + (double)bf2f(t.cA_lo[idx]);
+                            const bool live = im ? h < H : h <= H;
+                            if (!live) {
+                                CHECK(t.cA_hi[idx] == 0 && t.cA_lo[idx] == 0,
+                                      "NB=%d direct tile %zu lane %d el %d: nonzero dead slot\n",
+                                      NB, tile, l, el);
+                                continue;
+                            }
+                            const int hr = std::min(h, H);
+                            acc[a * 16 + i] += c * (im ? Xi[hr] : Xr[hr]);
+                        }
+                }
+            for (int g = 0; g < 4; ++g)
+                for (int r = 0; r < 4 * mt; ++r) {
+                    const int a = r / 4, v = r % 4, i = 4 * g + v;
+                    const int n = (g * RD + r) % NB;
+                    CHECK(std::fabs(acc[a * 16 + i] - want[n]) <= 2e-5 * 2.0 * scale,
+                          "NB=%d direct g=%d r=%d bin %d: %g vs %g\n", NB, g, r, n,
+                          acc[a * 16 + i], want[n]);
+                }
+            continue;
+        }
         std::vector<double> E(16 * mt, 0.0), O(16 * mt, 0.0);
         for (int a = 0; a < mt; ++a)
             for (int kk = 0; kk < ks; ++kk)
@@ -168,8 +214,6 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
                     seen[NB - n]++;
                 }
             }
-        double scale = 0.0;
-        for (int h = 1; h <= H; ++h) scale += std::fabs(Xr[h]) + std::fabs(Xi[h]);
         for (int n = 0; n < NB; ++n) {
             CHECK(seen[n] == 1, "NB=%d bin %d produced %d times\n", NB, n, seen[n]);
             CHECK(std::fabs(got[n] - want[n]) <= 2e-5 * 2.0 * scale,
@@ -187,6 +231,10 @@ int main() {
     check_config(32, 16, 0.5, 0.1 * 16777216 * 64e-6, {64, 32, 20, 16}, 13);
     // wider windows (two K steps): m = 32
     check_config(32, 32, 0.5, 4000.0, {64, 32, 20, 16}, 14);
+    // wide banks (more direct row tiles): nharms 60, folds 120, 60, 40, 30
+    check_config(60, 32, 0.5, 0.1 * T2, {120, 60, 40, 30}, 15, {28, 13, 9, 9});
+    check_config(32, 16, 0.5, 0.1 * 16777216 * 64e-6, {64, 32, 16}, 16, {19, 9, 6});
+    check_config(12, 16, 0.5, 1000.0, {24}, 17);
     printf(bad ? "FAIL %d\n" : "OK\n", bad);
     return bad ? 1 : 0;
 }
