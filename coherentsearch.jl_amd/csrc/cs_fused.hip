@@ -155,26 +155,14 @@ __global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g,
 
 // ---------------------------------------------------------------------------
 // Exact selection of the median |x| of one (block, fold)'s samples: 3-pass
-// radix select on the FP32 bit patterns with wave-aggregated LDS histogram
-// updates (lanes holding the same digit add once), then the lower central
-// order statistic from a count and a max.
+// radix select on the FP32 bit patterns (12, 10, 10 bits) with plain LDS
+// histogram atomics, then the lower central order statistic from a count and
+// a max.  The 12-bit leading digit (sign, exponent, three mantissa bits)
+// spreads a wave's keys over enough bins that same-address atomics stay few.
 // ---------------------------------------------------------------------------
 constexpr int kST = 512;
 constexpr int kSPer = 16;
-
-__device__ __forceinline__ void hist_add(uint32_t* hist, uint32_t digit, bool active, int nbits) {
-    const int lane = threadIdx.x & 63;
-    unsigned long long peers = __ballot(active);
-    if (peers == 0) return;
-    for (int bt = 0; bt < nbits; ++bt) {
-        unsigned long long v = __ballot(active && ((digit >> bt) & 1u));
-        peers &= ((digit >> bt) & 1u) ? v : ~v;
-    }
-    if (active) {
-        int leader = __ffsll((long long)peers) - 1;
-        if (lane == leader) atomicAdd(&hist[digit], (uint32_t)__popcll(peers));
-    }
-}
+constexpr int kHist = 4096;
 
 __device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -203,41 +191,35 @@ __device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* sh) {
 __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t kth, uint32_t* hist,
                                uint32_t* sh) {
     uint32_t prefix = 0, mask = 0;
-    const int shifts[3] = {21, 10, 0};
-    const int bits[3] = {11, 11, 10};
+    const int shifts[3] = {20, 10, 0};
+    const int bits[3] = {12, 10, 10};
+    constexpr int PQ = kHist / kST;   // bins per thread in the scan
+#pragma unroll
     for (int pass = 0; pass < 3; ++pass) {
         const int s_ = shifts[pass];
         const uint32_t dmask = (1u << bits[pass]) - 1u;
-        for (int i = threadIdx.x; i < 2048; i += kST) hist[i] = 0;
+        for (int i = threadIdx.x; i < kHist; i += kST) hist[i] = 0;
         __syncthreads();
-        if (pass == 0) {
-            // the leading digit (exponent and top mantissa bits) is shared by
-            // many lanes: one LDS atomic per distinct digit of the wave
+        // pass 0 counts every key; later passes only the selected bucket's
 #pragma unroll
-            for (int r = 0; r < kSPer; ++r) hist_add(hist, (keys[r] >> s_) & dmask, true, bits[pass]);
-        } else {
-            // later passes see only the keys of the selected bucket, spread
-            // over its digits: plain LDS atomics
-#pragma unroll
-            for (int r = 0; r < kSPer; ++r) {
-                const uint32_t key = keys[r];
-                if ((key & mask) == prefix) atomicAdd(&hist[(key >> s_) & dmask], 1u);
-            }
+        for (int r = 0; r < kSPer; ++r) {
+            const uint32_t key = keys[r];
+            if ((key & mask) == prefix) atomicAdd(&hist[(key >> s_) & dmask], 1u);
         }
         __syncthreads();
-        uint32_t a[4], loc = 0;
+        uint32_t a[PQ], loc = 0;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { a[q] = hist[4 * threadIdx.x + q]; loc += a[q]; }
+        for (int q = 0; q < PQ; ++q) { a[q] = hist[PQ * threadIdx.x + q]; loc += a[q]; }
         uint32_t ex = blk_excl_scan(loc, sh + 16);
         if (ex <= kth && kth < ex + loc) {
             uint32_t c = ex;
             int d = 0;
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
+            for (int q = 0; q < PQ; ++q) {
                 if (kth >= c && kth < c + a[q]) { d = q; break; }
                 c += a[q];
             }
-            sh[0] = 4 * threadIdx.x + d;
+            sh[0] = PQ * threadIdx.x + d;
             sh[1] = kth - c;
         }
         __syncthreads();
@@ -252,7 +234,7 @@ __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t kth, uint
 
 __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev b, FusedDev fu,
                                                 OutDev o) {
-    __shared__ uint32_t hist[2048];
+    __shared__ uint32_t hist[kHist];
     __shared__ uint32_t sh[64];
     const int cl = blockIdx.x, f = blockIdx.y, nf = fs.nf;
     const int64_t cg = b.c_first + cl;

@@ -355,7 +355,7 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
             // rows past H carry exactly zero c2r coefficients, so any finite
             // row serves: clamp to row H and read unconditionally
             const int hh = min(kk * 32 + g * 8 + e + 1, H);
-            const int idx = hh * rstride + rbase;
+            const int idx = (int)__umul24((unsigned)hh, (unsigned)rstride) + rbase;
             wr[e] = fre[idx];
             wi[e] = fim[idx];
         }
@@ -469,7 +469,8 @@ __device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ 
                   NF = D::NF, KD = D::KD;
     static_assert(KD <= 2, "the direct layout covers H <= 32");
     const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
-    const int rstride = k * kFtpLd, rbase = tcol - kFtpLd;   // stack row h*k at plane row h*k - 1
+    // stack row h*k at plane row h*k - 1 (rows < 64, strides < 2^24: 24-bit products)
+    const int rstride = k * kFtpLd, rbase = tcol - kFtpLd;
 
     // B operands: K slot kk*32 + 8g + e holds Re h (first half) or Im h
     // (second half); slots past the last harmonic read row H (zero coefficients)
@@ -478,13 +479,14 @@ __device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ 
         const CS_LDS uint32_t* pl = g < 2 ? fre : fim;
         uint32_t w[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) w[e] = pl[min(8 * (g & 1) + e + 1, H) * rstride + rbase];
+        for (int e = 0; e < 8; ++e)
+            w[e] = pl[(int)__umul24((unsigned)min(8 * (g & 1) + e + 1, H), (unsigned)rstride) + rbase];
         unpack8(w, bh[0], bl[0]);
     } else {
         uint32_t wr[8], wi[8];
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
-            const int idx = min(8 * g + e + 1, H) * rstride + rbase;
+            const int idx = (int)__umul24((unsigned)min(8 * g + e + 1, H), (unsigned)rstride) + rbase;
             wr[e] = fre[idx];
             wi[e] = fim[idx];
         }
@@ -616,7 +618,8 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int nharms, int dft_tiles) {
     L.ftp = (size_t)2 * nharms * kFtpLd * 4;
     L.dft = (size_t)dft_tiles * 2 * 64 * 16;
     L.region = (size_t)(kThreads / 64) * kStageFloats * 4;   // per-wave sample staging
-    L.total = L.ftp + L.region + L.dft;
+    L.seg = (size_t)(kThreads / 64) * kMaxFolds * 16;        // per-wave {beg, end, dst} per fold
+    L.total = L.ftp + L.region + L.dft + L.seg;
     return L;
 }
 
@@ -641,6 +644,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     uint4* dhi = (uint4*)(smem + (size_t)2 * nh * kFtpLd * 4);
     uint4* dlo = dhi + (size_t)ntc * 64;
     const size_t lay_stage = (size_t)2 * nh * kFtpLd * 4 + (size_t)ntc * 2 * 64 * 16;
+    const size_t lay_seg = lay_stage + (size_t)(kThreads / 64) * kStageFloats * 4;
     for (int i = tid; i < ntc * 64; i += kThreads) {
         dhi[i] = md.cA_hi[i];
         dlo[i] = md.cA_lo[i];
@@ -679,7 +683,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     int u = 0, cl = 0;
     bool inb = false;
     int32_t nvf[kMaxFolds], scf[kMaxFolds];
-    int4 hd[8];
     uint64_t rmv[4];
     uint4 wa[8], wb[8];
     auto fetch = [&](int64_t tile) {
@@ -706,12 +709,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 scf[f] = md.scol[(size_t)f * np + jj];
             }
         }
-        // slots past nharms repeat the last harmonic: loads stay unconditional
-#pragma unroll
-        for (int s = 0; s < 8; ++s) {
-            const int h = wv + 1 + 8 * s;
-            hd[s] = ld_const4(md.hdesc, (size_t)((h <= nh ? h : nh) - 1) * U + u);
-        }
         {
             const int64_t t0 = tbase + o * S + 4 * g4;
             const uint32_t rel = (uint32_t)(t0 - g.origin);
@@ -722,10 +719,12 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 rmv[v] = (t0 + v < g.total) ? md.rowmask[cgv] : 0ull;
             }
         }
-        // every window of this wave's harmonics (L2 / MALL / HBM)
+        // every window of this wave's harmonics (L2 / MALL / HBM); slots past
+        // nharms repeat the last harmonic, so the loads stay unconditional
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            const int4 d = hd[s];
+            const int h = wv + 1 + 8 * s;
+            const int4 d = ld_const4(md.hdesc, (size_t)((h <= nh ? h : nh) - 1) * U + u);
             int64_t b0 = (int64_t)d.x + (int64_t)d.y * occ;
             b0 = b0 < bmin ? bmin : (b0 > bmax ? bmax : b0);
             const uint32_t* p = plane + b0 + 8 * g4;
@@ -738,12 +737,16 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     for (int64_t tile = t_first; tile < t_end; tile += t_step) {
         // ---------------- interpolation: harmonics wv+1, wv+9, ... of this wave ----------
         // weight tiles and conj(A) (L2 resident) run two harmonics ahead
+        // the harmonic descriptors (scalar loads) ride in the ring with them
         struct Op {
             uint4 ah, al, q0, q1;
+            int4 d;
         };
         Op ring[3];
         auto issue = [&](int s, Op& x) {
-            const int4 d = hd[s];
+            const int h = wv + 1 + 8 * s;
+            const int4 d = ld_const4(md.hdesc, (size_t)((h <= nh ? h : nh) - 1) * U + u);
+            x.d = d;
             x.ah = md.wA_hi[(size_t)d.z * 64 + lane];
             x.al = md.wA_lo[(size_t)d.z * 64 + lane];
             const uint4* gp = (const uint4*)(md.gA + (size_t)(d.w & 0x0fffffff) * 16 + 4 * g4);
@@ -767,15 +770,15 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
             acc = mfma3(as_bf16x8(x.ah), as_bf16x8(x.al), bh, bl, acc);
             if constexpr (KS == 2) {
-                if ((hd[s].w >> 28) == 2) {   // second 32-bin K step
-                    int64_t b0 = (int64_t)hd[s].x + (int64_t)hd[s].y * occ;
+                if ((x.d.w >> 28) == 2) {   // second 32-bin K step
+                    int64_t b0 = (int64_t)x.d.x + (int64_t)x.d.y * occ;
                     b0 = b0 < bmin ? bmin : (b0 > bmax ? bmax : b0);
                     const uint32_t* p = plane + b0 + 32 + 8 * g4;
                     const uint4 x0 = *(const uint4_a4*)p, x1 = *(const uint4_a4*)(p + 4);
                     const uint32_t w[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
                     bf16x8 ch, cl;
                     unpack8(w, ch, cl);
-                    const size_t ti = ((size_t)hd[s].z + 1) * 64 + lane;
+                    const size_t ti = ((size_t)x.d.z + 1) * 64 + lane;
                     acc = mfma3(as_bf16x8(md.wA_hi[ti]), as_bf16x8(md.wA_lo[ti]), ch, cl, acc);
                 }
             }
@@ -808,15 +811,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const CS_LDS uint32_t* lre = opaque_lds((CS_LDS uint32_t*)fre);
             const CS_LDS uint32_t* lim = opaque_lds((CS_LDS uint32_t*)fim);
             CS_LDS float* lstg = opaque_lds((CS_LDS float*)(smem + lay_stage) + wv * kStageFloats);
+            // where each fold's staged columns sit and go: LDS, not registers,
+            // because they live across every fold call
+            CS_LDS u32x4* segtab = opaque_lds((CS_LDS u32x4*)(smem + lay_seg) + wv * kMaxFolds);
             float bnd[kMaxFolds];
-            int seg_beg[kMaxFolds], seg_end[kMaxFolds];
-            uint64_t seg_dst[kMaxFolds];
             int cursor = 0;
 #pragma unroll
             for (int f = 0; f < kMaxFolds; ++f) {
                 bnd[f] = -INFINITY;
-                seg_beg[f] = seg_end[f] = cursor;
-                seg_dst[f] = 0ull;
                 if (f >= nf) continue;
                 const int NB = fs.f[f].nbins;
                 const int64_t nv = nvf[f];
@@ -845,10 +847,11 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                         break;
                 }
                 bnd[f] = act ? fo.bound : -INFINITY;
-                seg_end[f] = __builtin_amdgcn_readfirstlane(fo.stg_end);
-                seg_dst[f] = ((uint64_t)__builtin_amdgcn_readfirstlane((int)(fo.dst >> 32)) << 32) |
-                             (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)fo.dst);
-                cursor = seg_end[f];
+                const int end = __builtin_amdgcn_readfirstlane(fo.stg_end);
+                if (lane == 0)
+                    segtab[f] = u32x4{(uint32_t)cursor, (uint32_t)end, (uint32_t)fo.dst,
+                                      (uint32_t)(fo.dst >> 32)};
+                cursor = end;
             }
             // next tile's loads go out before this tile's stores
             const int64_t t_out = t;
@@ -860,12 +863,16 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 for (int f = 0; f < kMaxFolds; ++f)
                     if (f < nf) fu.mraw[(int64_t)f * fu.ldm + (t_out - b.t_base)] = bnd[f];
             }
-#pragma unroll
-            for (int f = 0; f < kMaxFolds; ++f) {
-                if (f >= nf || seg_dst[f] == 0ull) continue;
-                const int nq = (seg_end[f] - seg_beg[f]) / 4;
-                CS_GLB f32x4* dst = (CS_GLB f32x4*)(uintptr_t)seg_dst[f];
-                const CS_LDS f32x4* src = (const CS_LDS f32x4*)(lstg + seg_beg[f]);
+            for (int f = 0; f < nf; ++f) {
+                const u32x4 e = segtab[f];
+                const uint64_t dsta =
+                    ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
+                    (uint32_t)__builtin_amdgcn_readfirstlane((int)e.z);
+                if (dsta == 0ull) continue;
+                const int beg = __builtin_amdgcn_readfirstlane((int)e.x);
+                const int nq = (__builtin_amdgcn_readfirstlane((int)e.y) - beg) / 4;
+                CS_GLB f32x4* dst = (CS_GLB f32x4*)(uintptr_t)dsta;
+                const CS_LDS f32x4* src = (const CS_LDS f32x4*)(lstg + beg);
                 for (int q = lane; q < nq; q += 64) dst[q] = src[q];
             }
         }
