@@ -42,6 +42,11 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 #define CS_GLB __attribute__((address_space(1)))
 #define CS_CONST __attribute__((address_space(4)))
 
+template <bool B>
+struct BoolC {
+    static constexpr bool value = B;
+};
+
 constexpr int kOcc8 = 8;          // occurrences per tile
 constexpr int kWT = 128;          // trials per tile
 constexpr int kThreads = 512;     // 8 waves
@@ -66,11 +71,14 @@ __device__ __forceinline__ void unpack8(const uint32_t* w, bf16x8& hi, bf16x8& l
     lo = as_bf16x8(l);
 }
 
+// x -> (hi | lo << 16): hi = bf16(x), lo = bf16(x - hi), both round to
+// nearest even; two packed conversions, the second re-rounding x to hi
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t split_word(float x) {
-    const __bf16 h = (__bf16)x;
-    const __bf16 l = (__bf16)(x - (float)h);
-    return (uint32_t)__builtin_bit_cast(uint16_t, h) |
-           ((uint32_t)__builtin_bit_cast(uint16_t, l) << 16);
+    const uint32_t hh = __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{x, x}, bf16x2));
+    const float hf = __builtin_bit_cast(float, hh << 16);
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{x, x - hf}, bf16x2));
 }
 
 __device__ __forceinline__ f32x4 mfma3(bf16x8 ah, bf16x8 al, bf16x8 bh, bf16x8 bl, f32x4 acc) {
@@ -684,6 +692,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     bool inb = false;
     int32_t nvf[kMaxFolds], scf[kMaxFolds];
     uint64_t rmv[4];
+    bool rfull = false;   // every harmonic row of the tile passes its guard (wave-uniform)
     uint4 wa[8], wb[8];
     auto fetch = [&](int64_t tile) {
         st = tile / U;
@@ -718,6 +727,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 const uint32_t cgv = cg + (rem + v >= np ? 1u : 0u);
                 rmv[v] = (t0 + v < g.total) ? md.rowmask[cgv] : 0ull;
             }
+            const uint64_t need = nh >= 64 ? ~0ull : (1ull << nh) - 1ull;
+            rfull = __all(((rmv[0] & rmv[1] & rmv[2] & rmv[3]) & need) == need);
         }
         // every window of this wave's harmonics (L2 / MALL / HBM); slots past
         // nharms repeat the last harmonic, so the loads stay unconditional
@@ -749,10 +760,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             x.d = d;
             x.ah = md.wA_hi[(size_t)d.z * 64 + lane];
             x.al = md.wA_lo[(size_t)d.z * 64 + lane];
-            const uint4* gp = (const uint4*)(md.gA + (size_t)(d.w & 0x0fffffff) * 16 + 4 * g4);
+            const uint4* gp =
+                (const uint4*)(md.gA + ((size_t)(d.w & 0x0fffffff) * 2 + c) * 16 + 4 * g4);
             x.q0 = gp[0];
             x.q1 = gp[1];
         };
+        // two copies of the loop: with the row guard applied, and without it
+        // for tiles whose rows all pass (the usual case)
+        auto interp = [&](auto masked) {
         issue(0, ring[0]);
         issue(1, ring[1]);
 #pragma unroll
@@ -789,19 +804,22 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             uint32_t ow[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
+                // gA[k, g] * complex(ar, ai): the even lane holds re and forms
+                // ar re - ai im, the odd lane holds im and forms ar im + ai re
+                // (each one rounding per product and one for the sum)
                 const float mine = acc[v], oth = pair_swap(mine);
-                const float re = c ? oth : mine, im = c ? mine : oth;
-                const float ar = gA[2 * v], ai = gA[2 * v + 1];
-                // gA[k, g] * complex(ar, ai), product order of the reference
-                const float p1 = ar * re, p2 = ai * im, p3 = ar * im, p4 = ai * re;
-                float val = c ? (p3 + p4) : (p1 - p2);
-                val = ((rmv[v] >> (h - 1)) & 1ull) ? val : 0.0f;
+                float val = mine * gA[2 * v] + oth * gA[2 * v + 1];   // Im A pre-signed
+                if constexpr (decltype(masked)::value)
+                    val = ((rmv[v] >> (h - 1)) & 1ull) ? val : 0.0f;
                 ow[v] = split_word(val);
             }
             uint32_t* dst = (c ? fim : fre) + (h - 1) * kFtpLd + o * 16 + 4 * g4;
             *(uint2*)dst = make_uint2(ow[0], ow[1]);
             *(uint2*)(dst + 2) = make_uint2(ow[2], ow[3]);
         }
+        };
+        if (rfull) interp(BoolC<false>{});
+        else interp(BoolC<true>{});
         lds_barrier();
 
         // ---------------- c2r + bound, fold by fold, wave-local --------------------------

@@ -16,7 +16,8 @@ constexpr int kSigmaSamplesPadded = 16384;
 struct MfmaDev {
     const uint4* wA_hi;       // interpolation weight tiles, [tile][64 lanes] x 8 bf16
     const uint4* wA_lo;
-    const float2* gA;         // [class][16] conj(A(dr)) per trial of the group
+    const float2* gA;         // [class][parity][16] conj(A(dr)) per trial of the group, Im
+                              // negated for parity 0 (the lanes that hold Re)
     const int4* hdesc;        // [(h-1)*U + u]: {first bin of trial 16u (clamped),
                               //  bin advance per occurrence, first weight tile,
                               //  class | K steps << 28}

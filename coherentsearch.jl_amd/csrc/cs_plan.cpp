@@ -230,10 +230,13 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
         for (size_t c = 0; c < starts.size(); ++c) {
             t.clsmap[(size_t)(h - 1) * q + starts[c]] = ncls;
             t.cls_tile.push_back(ntile);
-            for (int i = 0; i < 16; ++i) {
-                t.gA.push_back(pl.A32[2 * (size_t)(d.tab_off + P[c][i])]);
-                t.gA.push_back(pl.A32[2 * (size_t)(d.tab_off + P[c][i]) + 1]);
-            }
+            // parity 0 (lanes holding Re) carries -Im A, parity 1 Im A
+            for (int par = 0; par < 2; ++par)
+                for (int i = 0; i < 16; ++i) {
+                    const float ai = pl.A32[2 * (size_t)(d.tab_off + P[c][i]) + 1];
+                    t.gA.push_back(pl.A32[2 * (size_t)(d.tab_off + P[c][i])]);
+                    t.gA.push_back(par ? ai : -ai);
+                }
             for (int kk = 0; kk < ks; ++kk) {
                 for (int l = 0; l < 64; ++l) {
                     int i = l % 16;

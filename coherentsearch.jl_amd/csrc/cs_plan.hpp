@@ -77,7 +77,7 @@ struct MfmaTables {
     int32_t U = 0;                  // S / 16
     int32_t q = 0;
     std::vector<uint16_t> wA_hi, wA_lo;   // [tile][64 lanes][8]
-    std::vector<float> gA;                // [class][16][2]
+    std::vector<float> gA;                // [class][parity][16][2]: (Re A, -Im A | Im A)
     std::vector<int32_t> clsmap;          // [(h-1)*q + residue] -> class, -1 if no group starts there
     std::vector<int32_t> cls_tile;        // [class] -> first tile
     std::vector<int32_t> ks;              // [h-1] -> K steps (1 or 2)

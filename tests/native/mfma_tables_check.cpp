@@ -8,7 +8,7 @@
 //  2. weight tiles: row i of a group's tile applied to a window starting at the
 //     group's first bin reproduces trial i's own m-tap FP32 weights at its own
 //     first bin (up to the bf16 hi+lo split), and conj(A) of the class is the
-//     trial's A;
+//     trial's A (Im negated in the copy for the lanes that hold Re);
 //  3. c2r tiles: the permuted even/odd tiles (NB > 64) and the direct-layout
 //     tiles (NB <= 64, every register of every lane group) reproduce the numpy
 //     irfft convention (unnormalised) computed directly in FP64, and every
@@ -111,11 +111,14 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
                         }
                 CHECK(std::fabs(got - want) <= 1e-5 * scale + 1e-12,
                       "h=%d u=%d i=%d tile row %g vs %g\n", h, u, i, got, want);
-                const float ar = t.gA[2 * ((size_t)cls * 16 + i)];
-                const float ai = t.gA[2 * ((size_t)cls * 16 + i) + 1];
-                CHECK(ar == pl.A32[2 * (size_t)(d.tab_off + p)] &&
-                          ai == pl.A32[2 * (size_t)(d.tab_off + p) + 1],
-                      "h=%d u=%d i=%d conj(A) mismatch\n", h, u, i);
+                for (int par = 0; par < 2; ++par) {
+                    const float ar = t.gA[2 * (((size_t)cls * 2 + par) * 16 + i)];
+                    const float ai = t.gA[2 * (((size_t)cls * 2 + par) * 16 + i) + 1];
+                    const float want_ai = pl.A32[2 * (size_t)(d.tab_off + p) + 1];
+                    CHECK(ar == pl.A32[2 * (size_t)(d.tab_off + p)] &&
+                              ai == (par ? want_ai : -want_ai),
+                          "h=%d u=%d i=%d parity %d: conj(A) mismatch\n", h, u, i, par);
+                }
             }
         }
     }
