@@ -114,6 +114,26 @@ __device__ __forceinline__ CS_LDS T* opaque_lds(CS_LDS T* p) {
     return (CS_LDS T*)(uintptr_t)v;
 }
 
+// Global-memory view of a pointer read from a device-side descriptor.  The
+// compiler cannot infer the address space of such a pointer and would emit
+// FLAT accesses, which also count against the LDS/scalar wait counter, so a
+// later scalar-load wait would wait for them too.
+template <class T>
+__device__ __forceinline__ const CS_GLB T* glb(const T* p) {
+    return (const CS_GLB T*)p;
+}
+// 16-byte global loads through such a pointer (p 16-byte aligned, or 4-byte
+// aligned for ldg4_a4)
+__device__ __forceinline__ uint4 ldg4(const void* p) {
+    const u32x4 v = *(const CS_GLB u32x4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ uint4 ldg4_a4(const void* p) {
+    typedef u32x4 u32x4_a4 __attribute__((aligned(4)));
+    const u32x4_a4 v = *(const CS_GLB u32x4_a4*)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // Read-only table entry at a wave-uniform index, read through the constant
 // address space so it becomes a scalar load (the kernel's global stores
 // cannot alias it).
@@ -654,12 +674,13 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const size_t lay_stage = (size_t)2 * nh * kFtpLd * 4 + (size_t)ntc * 2 * 64 * 16;
     const size_t lay_seg = lay_stage + (size_t)(kThreads / 64) * kStageFloats * 4;
     for (int i = tid; i < ntc * 64; i += kThreads) {
-        dhi[i] = md.cA_hi[i];
-        dlo[i] = md.cA_lo[i];
+        dhi[i] = ldg4(md.cA_hi + i);
+        dlo[i] = ldg4(md.cA_lo + i);
     }
 
     // tile schedule: contiguous tile ranges per XCD (workgroups are dealt to
-    // XCDs round-robin), so neighbouring tiles share that XCD's L2
+    // XCDs round-robin), so tiles of neighbouring 16-trial groups, whose
+    // windows overlap, run together and share that XCD's L2
     int64_t t_first, t_end, t_step;
     {
         const int G = gridDim.x, nx = md.nxcd;
@@ -678,25 +699,51 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const int j16 = lane & 15, o = j16 >> 1, c = j16 & 1, g4 = lane >> 4;
     const int64_t S = md.S;
     const int U = md.U;
-    const uint32_t* plane = md.spl + (size_t)c * md.spl_ld + kSpecPad;
+    const CS_GLB uint32_t* plane = glb(md.spl) + (size_t)c * md.spl_ld + kSpecPad;
     const int64_t bmin = -kSpecPad, bmax = sp.namps + kSpecPad - 32 * KS;
     const uint32_t np = (uint32_t)g.Nprof;
 
     // Everything a tile reads from global memory before its compute: the
-    // c2r trial's valid counts and sampled columns for all folds, the
-    // harmonic descriptors, row masks and every interpolation window.  It is
-    // fetched for the next tile before this tile's outputs are stored, so the
-    // loads are not ordered behind those stores.
+    // harmonic descriptors' window starts, the c2r trial's valid counts and
+    // sampled columns for all folds, the row masks and every interpolation
+    // window.  It is fetched for the next tile before this tile's outputs are
+    // stored, so the loads are not ordered behind those stores.  Only the
+    // descriptors are waited for here (the window addresses need them); every
+    // other load is unconditional, from a clamped index, and its validity is
+    // applied where it is used.  The table bases sit in vector registers: the
+    // loads index them per lane anyway, and a scalar copy would be re-read
+    // from the descriptor block at every use under scalar-register pressure.
+    const int32_t* nvb = b.nvalid;
+    const int32_t* scb = md.scol;
+    const uint64_t* rmb = md.rowmask;
+    const int4* hdb = md.hdesc;
+    asm volatile("" : "+v"(nvb), "+v"(scb), "+v"(rmb), "+v"(hdb));
+    const int nfv = fs.nf;
+    const uint32_t cg_last = (uint32_t)((g.total - 1 - g.origin) / g.Nprof);
     int64_t st = 0, tbase = 0, occ = 0, t = 0, jj = 0;
     int u = 0, cl = 0;
     bool inb = false;
     int32_t nvf[kMaxFolds], scf[kMaxFolds];
     uint64_t rmv[4];
-    bool rfull = false;   // every harmonic row of the tile passes its guard (wave-uniform)
     uint4 wa[8], wb[8];
+    // window start (x) and advance (y) of this wave's harmonics, one value
+    // per lane (lane 2s + c holds component c of harmonic slot s; slots past
+    // nharms repeat the last harmonic), loaded one tile ahead of their use
+    auto load_desc = [&](int64_t tile) {
+        const int uu = (int)((uint32_t)tile % (uint32_t)U);   // tile counts stay below 2^31
+        const int s = (lane >> 1) & 7, h = wv + 1 + 8 * s;
+        return glb((const int*)hdb)[((size_t)((h <= nh ? h : nh) - 1) * U + uu) * 4 + (lane & 1)];
+    };
+    int dnext = t_first < t_end ? load_desc(t_first) : 0;
     auto fetch = [&](int64_t tile) {
-        st = tile / U;
-        u = (int)(tile - st * U);
+        const uint32_t tl = (uint32_t)tile;
+        const uint32_t stu = tl / (uint32_t)U;
+        st = stu;
+        u = (int)(tl - stu * (uint32_t)U);
+        const int dreg = dnext;
+        // unconditional (past the range it reloads the last tile's): a load
+        // under a branch leaves the merged wait state pending at once
+        dnext = load_desc(tile + t_step < t_end ? tile + t_step : tile);
         tbase = st * kOcc8 * S + 16 * (int64_t)u;
         occ = (int64_t)kOcc8 * st + o;
         t = tbase + (int64_t)wv * S + (lane & 15);
@@ -711,12 +758,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         }
 #pragma unroll
         for (int f = 0; f < kMaxFolds; ++f) {
-            nvf[f] = 0;
-            scf[f] = -1;
-            if (f < fs.nf && inb) {
-                nvf[f] = b.nvalid[cl * fs.nf + f];
-                scf[f] = md.scol[(size_t)f * np + jj];
-            }
+            const int ff = f < nfv ? f : nfv - 1;
+            nvf[f] = glb(nvb)[cl * nfv + ff];
+            scf[f] = glb(scb)[(size_t)ff * np + jj];
         }
         {
             const int64_t t0 = tbase + o * S + 4 * g4;
@@ -725,22 +769,18 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const uint32_t cgv = cg + (rem + v >= np ? 1u : 0u);
-                rmv[v] = (t0 + v < g.total) ? md.rowmask[cgv] : 0ull;
+                rmv[v] = glb(rmb)[cgv < cg_last ? cgv : cg_last];
             }
-            const uint64_t need = nh >= 64 ? ~0ull : (1ull << nh) - 1ull;
-            rfull = __all(((rmv[0] & rmv[1] & rmv[2] & rmv[3]) & need) == need);
         }
-        // every window of this wave's harmonics (L2 / MALL / HBM); slots past
-        // nharms repeat the last harmonic, so the loads stay unconditional
+        // every window of this wave's harmonics (L2 / MALL / HBM)
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            const int h = wv + 1 + 8 * s;
-            const int4 d = ld_const4(md.hdesc, (size_t)((h <= nh ? h : nh) - 1) * U + u);
-            int64_t b0 = (int64_t)d.x + (int64_t)d.y * occ;
+            int64_t b0 = (int64_t)__builtin_amdgcn_readlane(dreg, 2 * s) +
+                         (int64_t)__builtin_amdgcn_readlane(dreg, 2 * s + 1) * occ;
             b0 = b0 < bmin ? bmin : (b0 > bmax ? bmax : b0);
-            const uint32_t* p = plane + b0 + 8 * g4;
-            wa[s] = *(const uint4_a4*)p;
-            wb[s] = *(const uint4_a4*)(p + 4);
+            const CS_GLB uint32_t* p = plane + b0 + 8 * g4;
+            wa[s] = ldg4_a4((const void*)p);
+            wb[s] = ldg4_a4((const void*)(p + 4));
         }
     };
     if (t_first < t_end) fetch(t_first);
@@ -758,12 +798,11 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const int h = wv + 1 + 8 * s;
             const int4 d = ld_const4(md.hdesc, (size_t)((h <= nh ? h : nh) - 1) * U + u);
             x.d = d;
-            x.ah = md.wA_hi[(size_t)d.z * 64 + lane];
-            x.al = md.wA_lo[(size_t)d.z * 64 + lane];
-            const uint4* gp =
-                (const uint4*)(md.gA + ((size_t)(d.w & 0x0fffffff) * 2 + c) * 16 + 4 * g4);
-            x.q0 = gp[0];
-            x.q1 = gp[1];
+            x.ah = ldg4(md.wA_hi + (size_t)d.z * 64 + lane);
+            x.al = ldg4(md.wA_lo + (size_t)d.z * 64 + lane);
+            const float2* gp = md.gA + ((size_t)(d.w & 0x0fffffff) * 2 + c) * 16 + 4 * g4;
+            x.q0 = ldg4(gp);
+            x.q1 = ldg4(gp + 2);
         };
         // two copies of the loop: with the row guard applied, and without it
         // for tiles whose rows all pass (the usual case)
@@ -788,13 +827,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 if ((x.d.w >> 28) == 2) {   // second 32-bin K step
                     int64_t b0 = (int64_t)x.d.x + (int64_t)x.d.y * occ;
                     b0 = b0 < bmin ? bmin : (b0 > bmax ? bmax : b0);
-                    const uint32_t* p = plane + b0 + 32 + 8 * g4;
-                    const uint4 x0 = *(const uint4_a4*)p, x1 = *(const uint4_a4*)(p + 4);
+                    const CS_GLB uint32_t* p = plane + b0 + 32 + 8 * g4;
+                    const uint4 x0 = ldg4_a4((const void*)p), x1 = ldg4_a4((const void*)(p + 4));
                     const uint32_t w[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
                     bf16x8 ch, cl;
                     unpack8(w, ch, cl);
                     const size_t ti = ((size_t)x.d.z + 1) * 64 + lane;
-                    acc = mfma3(as_bf16x8(md.wA_hi[ti]), as_bf16x8(md.wA_lo[ti]), ch, cl, acc);
+                    acc = mfma3(as_bf16x8(ldg4(md.wA_hi + ti)), as_bf16x8(ldg4(md.wA_lo + ti)), ch, cl,
+                                acc);
                 }
             }
             const float gA[8] = {__builtin_bit_cast(float, x.q0.x), __builtin_bit_cast(float, x.q0.y),
@@ -818,7 +858,10 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             *(uint2*)(dst + 2) = make_uint2(ow[2], ow[3]);
         }
         };
-        if (rfull) interp(BoolC<false>{});
+        // every harmonic row of the tile passes its guard (wave-uniform); the
+        // row masks were loaded with the previous tile's prefetch
+        const uint64_t need = nh >= 64 ? ~0ull : (1ull << nh) - 1ull;
+        if (__all(((rmv[0] & rmv[1] & rmv[2] & rmv[3]) & need) == need)) interp(BoolC<false>{});
         else interp(BoolC<true>{});
         lds_barrier();
 

@@ -36,7 +36,7 @@ struct MfmaDev {
     int32_t q;
     int32_t ksmax;            // largest interpolation K steps (1 or 2)
     int32_t nbp;              // profile row stride in LDS (floats)
-    int32_t dft_tiles_max;    // largest per-fold DFT tile count (hi or lo)
+    int32_t dft_tiles_max;    // c2r DFT tiles of all folds (hi or lo)
     int32_t nxcd;             // XCDs the persistent grid is spread over
 };
 
