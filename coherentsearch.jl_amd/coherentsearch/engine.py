@@ -118,7 +118,7 @@ class Engine:
                   "decim_metrics")
         return out[:nv.value]
 
-    def block_metrics(self, rfund: Sequence[float], kernel: str = "direct",
+    def block_metrics(self, rfund: Sequence[float], kernel: str = "fft",
                       widths: Optional[Sequence[int]] = None) -> np.ndarray:
         if kernel not in ("fft", "direct"):
             raise ValueError(f"kernel must be :fft or :direct, got :{kernel}")
@@ -214,16 +214,17 @@ def search(ft: FFTFile, params: Optional[SearchParams] = None, lofreq: float = 0
     return cands
 
 
-def block_metrics(ft: FFTFile, rfund, params: SearchParams, kernel: str = "direct",
+def block_metrics(ft: FFTFile, rfund, params: SearchParams, kernel: str = "fft",
                   widths=None, engine: Optional[Engine] = None) -> np.ndarray:
-    """block_metrics (src/search.jl:1061-1079); the device provides :direct."""
+    """block_metrics (src/search.jl:1061-1079): kernel :fft (the reference's
+    default) or :direct, both on the device in FP64."""
     eng = engine or Engine(params, 2048)
     eng.set_spectrum(ft)
     return eng.block_metrics(rfund, kernel=kernel, widths=widths)
 
 
 def search_block(ft: FFTFile, rfund, params: SearchParams, threshold: Optional[float] = None,
-                 kernel: str = "direct", engine: Optional[Engine] = None) -> List[Candidate]:
+                 kernel: str = "fft", engine: Optional[Engine] = None) -> List[Candidate]:
     """search_block (src/search.jl:1087-1097)."""
     thr = params.threshold if threshold is None else threshold
     m = block_metrics(ft, rfund, params, kernel=kernel, engine=engine)

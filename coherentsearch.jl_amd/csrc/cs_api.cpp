@@ -52,6 +52,7 @@ struct cs_ctx {
     std::map<int, DBuf> twid;
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
+    DBuf bandtab;   // block_metrics(:fft): per-harmonic lobin, then fine-grid length
     DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol;
     DBuf spl;                  // pre-split spectrum planes of the MFMA mode
     bool spl_valid = false;    // spl matches the current spectrum
@@ -280,7 +281,7 @@ int cs_ctx_destroy(cs_ctx* c) {
     if (c->st) (void)hipStreamSynchronize(c->st);
     DBuf* bufs[] = {&c->amps, &c->harm, &c->row, &c->carry, &c->W32, &c->A32, &c->W64, &c->A64,
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
-                    &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples,
+                    &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples, &c->bandtab,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
                     &c->m_desc, &c->m_rowmask, &c->m_scol, &c->spl};
     for (DBuf* b : bufs) release(*b);
@@ -421,15 +422,15 @@ int cs_block_metrics(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel,
                      const int32_t* widths, int32_t nwidths, double* out) {
     if (!c) return CS_EINVAL;
     if (kernel != 0 && kernel != 1) return c->fail(CS_EINVAL, "kernel must be :fft or :direct");
-    if (kernel == 0)
-        return c->fail(CS_EINVAL, "kernel :fft (FFT-correlation) is not available on the device path");
     if (L < 1) return CS_OK;
     if (widths && nwidths < 1) return c->fail(CS_EINVAL, "empty width bank");
     int s = check_spectrum(c);
     if (s) return s;
     HIPCHK(c, hipSetDevice(c->device));
     const int nh = c->nharms, nbins = 2 * nh, m2 = c->m / 2;
+    // per-harmonic band and row guard (reference_profiles, search.jl:1027-1034)
     std::vector<uint8_t> ok(nh, 0);
+    std::vector<int64_t> lob(nh, 0), kgrid(nh, 1);
     const int64_t Nhalf = c->N / 2;
     for (int h = 1; h <= nh; ++h) {
         double rmin = rfund[0] * h, rmax = rfund[L - 1] * h;
@@ -437,13 +438,26 @@ int cs_block_metrics(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel,
         int64_t hibin = (int64_t)std::ceil(rmax) + 1;
         int64_t numbins = hibin - lobin;
         ok[h - 1] = (lobin >= m2 && (lobin + numbins + m2) <= c->namps && hibin < Nhalf) ? 1 : 0;
+        lob[h - 1] = lobin;
+        kgrid[h - 1] = numbins * c->numbetween;
     }
+    if (kernel == 0 && c->numbetween < 1) return c->fail(CS_EINVAL, "numbetween must be >= 1");
     if ((s = upload(c, c->rowok, ok))) return s;
     std::vector<double> rv(rfund, rfund + L);
     if ((s = upload(c, c->rfund, rv))) return s;
     if ((s = ensure(c, c->ftp, (size_t)nh * L * sizeof(double2)))) return s;
-    HIPCHK(c, launch_point_interp_f64(c->st, spec_of(c), (const double*)c->rfund.p, L, nh, c->m,
-                                      (const uint8_t*)c->rowok.p, (double2*)c->ftp.p, L));
+    if (kernel == 1) {
+        HIPCHK(c, launch_point_interp_f64(c->st, spec_of(c), (const double*)c->rfund.p, L, nh, c->m,
+                                          (const uint8_t*)c->rowok.p, (double2*)c->ftp.p, L));
+    } else {
+        std::vector<int64_t> tab(lob);
+        tab.insert(tab.end(), kgrid.begin(), kgrid.end());
+        if ((s = upload(c, c->bandtab, tab))) return s;
+        const int64_t* dl = (const int64_t*)c->bandtab.p;
+        HIPCHK(c, launch_point_interp_fft(c->st, spec_of(c), (const double*)c->rfund.p, L, nh, c->m,
+                                          c->numbetween, dl, dl + nh, (const uint8_t*)c->rowok.p,
+                                          (double2*)c->ftp.p, L));
+    }
     std::vector<int> ws;
     if (widths) ws.assign(widths, widths + nwidths);
     else ws = ladder(c, nbins, 1);

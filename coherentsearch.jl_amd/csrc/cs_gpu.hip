@@ -737,6 +737,76 @@ __global__ __launch_bounds__(256) void k_point_interp(SpecDev sp, const double* 
     out[(int64_t)(h - 1) * ld + j] = res;
 }
 
+// FP64 Eqn. 30 at fine-grid point lobin + i/nb (fourierinterp.jl:23-74): the
+// FFT correlation of finterp_fft (fourierinterp.jl:195-214) produces exactly
+// this sum at each grid point, the kernel taps b - r in [-m/2, m/2 - 1/nb]
+// reducing to Eqn. 30's j = -m/2+1 .. m/2 (and to the bin itself at offset 0).
+__device__ double2 e30_grid(const SpecDev& sp, int64_t lobin, int64_t i, int nb, int m) {
+    const int64_t q = i / nb, rem = i - q * nb;
+    const int64_t b0 = lobin + q;
+    if (rem == 0) {
+        if (b0 < 0 || b0 >= sp.namps) return make_double2(NAN, NAN);
+        const float2 v = sp.amps[b0];
+        return make_double2((double)v.x, (double)v.y);
+    }
+    const double dr = (double)rem / (double)nb;
+    const int64_t lo0 = b0 + 1 - m / 2;
+    if (lo0 < 0 || lo0 + m > sp.namps) return make_double2(NAN, NAN);
+    double sr = 0.0, si = 0.0;
+    for (int k = 0; k < m; ++k) {
+        const double off = dr - (double)(-(m / 2) + 1 + k);
+        const double s = sinc_d(off);
+        const double cr = s * cospi(off), ci = s * sinpi(off);
+        const float2 bv = sp.amps[lo0 + k];
+        sr += cr * (double)bv.x + ci * (double)bv.y;
+        si += cr * (double)bv.y - ci * (double)bv.x;
+    }
+    return make_double2(sr, si);
+}
+
+// block_metrics(kernel = :fft): harmonic h of trial j is the fine-grid
+// interpolation of finterp_fft(lobin_h, numbins_h, nb) at r = rfund[j] * h,
+// linear between the two neighbouring grid points and clamped to the grid ends
+// (uniform_linear_interp, search.jl:974-986); rows failing the guard stay 0.
+__global__ __launch_bounds__(256) void k_point_interp_fft(SpecDev sp, const double* __restrict__ rfund,
+                                                          int64_t L, int m, int nb,
+                                                          const int64_t* __restrict__ lob,
+                                                          const int64_t* __restrict__ kgrid,
+                                                          const uint8_t* rowok, double2* out,
+                                                          int64_t ld) {
+    const int h = blockIdx.y + 1;
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= L) return;
+    double2 res = make_double2(0.0, 0.0);
+    if (rowok[h - 1]) {
+        const int64_t lobin = lob[h - 1], K = kgrid[h - 1];
+        const double r = rfund[j] * (double)h;
+        const double p = (r - (double)lobin) * (double)nb;
+        if (p <= 0.0) {
+            res = e30_grid(sp, lobin, 0, nb, m);
+        } else if (p >= (double)(K - 1)) {
+            res = e30_grid(sp, lobin, K - 1, nb, m);
+        } else {
+            const int64_t i0 = (int64_t)floor(p);
+            const double f = p - (double)i0;
+            const double2 a = e30_grid(sp, lobin, i0, nb, m), c = e30_grid(sp, lobin, i0 + 1, nb, m);
+            res = make_double2(a.x * (1.0 - f) + c.x * f, a.y * (1.0 - f) + c.y * f);
+        }
+    }
+    out[(int64_t)(h - 1) * ld + j] = res;
+}
+
+hipError_t launch_point_interp_fft(hipStream_t st, const SpecDev& sp, const double* rfund,
+                                   int64_t L, int nharms, int m, int nb, const int64_t* lob,
+                                   const int64_t* kgrid, const uint8_t* rowok, double2* out,
+                                   int64_t ld) {
+    if (L <= 0) return hipSuccess;
+    dim3 grid((unsigned)((L + 255) / 256), (unsigned)nharms);
+    hipLaunchKernelGGL(k_point_interp_fft, grid, dim3(256), 0, st, sp, rfund, L, m, nb, lob, kgrid,
+                       rowok, out, ld);
+    return hipGetLastError();
+}
+
 hipError_t launch_point_interp_f64(hipStream_t st, const SpecDev& sp, const double* rfund,
                                    int64_t L, int nharms, int m, const uint8_t* rowok,
                                    double2* out, int64_t ld) {

@@ -111,5 +111,11 @@ hipError_t launch_exact_f64(hipStream_t st, const double* prof, int64_t ldp, int
 hipError_t launch_point_interp_f64(hipStream_t st, const SpecDev& sp, const double* rfund,
                                    int64_t L, int nharms, int m, const uint8_t* rowok,
                                    double2* out, int64_t ld);
+// block_metrics(:fft): linear interpolation between fine-grid points
+// lob[h-1] + i/nb (i < kgrid[h-1]) of FP64 Eqn. 30
+hipError_t launch_point_interp_fft(hipStream_t st, const SpecDev& sp, const double* rfund,
+                                   int64_t L, int nharms, int m, int nb, const int64_t* lob,
+                                   const int64_t* kgrid, const uint8_t* rowok, double2* out,
+                                   int64_t ld);
 
 }  // namespace cs

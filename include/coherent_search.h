@@ -92,8 +92,10 @@ int cs_search(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
               cs_cand *out, int64_t cap, int64_t *n_out);
 
 /* block_metrics(ft, rfund, params; kernel, widths) (src/search.jl:1061-1079):
- * kernel 1 = :direct (Eqn. 30 point by point, FP64); kernel 0 = :fft is not
- * available on this device path and returns CS_EINVAL.  widths == NULL means
+ * kernel 1 = :direct (Eqn. 30 point by point, FP64); kernel 0 = :fft
+ * (finterp_fft onto the numbetween grid, then clamped linear interpolation,
+ * search.jl:974-986, 1036-1040; the grid values are FP64 Eqn. 30, which the FFT
+ * correlation reproduces up to its rounding).  widths == NULL means
  * ladder_boxcar_widths(2*nharms, 1, params). */
 int cs_block_metrics(cs_ctx *ctx, const double *rfund, int64_t L,
                      int32_t kernel, const int32_t *widths, int32_t nwidths,

@@ -4,6 +4,8 @@ same seeded inputs.
 Bars: integer work (trial indices, folds, blocks, row-zeroing) bit-exact; the
 FP32-weight interpolation bit-exact (same FMA chain); FP64 bridge metrics
 (chunk_metrics / decim / block_metrics :direct) within 1e-12 relative;
+block_metrics :fft within 1e-9 relative of the oracle's FFT correlation (the
+device evaluates the same grid values as direct FP64 sums);
 search candidates: identical (t, k) sets away from threshold and S/N within
 1e-4 relative (the FP32 profile stage moves the block sigma by ~1e-7).
 """
@@ -126,12 +128,54 @@ def test_block_metrics_direct_vs_oracle(cs, O, nharms, m, widths):
     assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-10
 
 
-def test_block_metrics_fft_kernel_rejected(cs):
-    N = 1 << 16
-    eng, p = _engine(cs, nharms=8, m=16)
-    eng.set_spectrum(_ft(cs, noise_amps(N // 2, 1), N))
-    with pytest.raises(ValueError):
-        eng.block_metrics(1000.0 + np.arange(16) / 16.0, "fft")
+@pytest.mark.parametrize("nharms,m,widths", [(8, 16, None), (16, 32, None), (32, 16, [1, 2, 3, 4, 6])])
+def test_block_metrics_fft_vs_oracle(cs, O, nharms, m, widths):
+    """kernel=:fft (the reference's default, search.jl:1036-1040): the device's
+    FP64 Eqn. 30 grid values with clamped linear interpolation against the
+    oracle's FFT correlation (finterp_fft, fourierinterp.jl:195-214).  The two
+    differ only by FFT rounding."""
+    N = 1 << 18
+    amps = noise_amps(N // 2, seed=23)
+    eng, p = _engine(cs, nharms=nharms, m=m)
+    eng.set_spectrum(_ft(cs, amps, N))
+    rf = 10010.3 + np.arange(200) * (0.37 / nharms)
+    got = eng.block_metrics(rf, "fft", widths)
+    want = O.block_metrics(O.make_params(nharms=nharms, m=m), amps, N, rf, "fft", widths)
+    assert np.max(np.abs(got - want)) / np.max(np.abs(want)) < 1e-9
+
+
+def test_block_metrics_fft_on_grid_equals_direct(cs):
+    """When every harmonic frequency r*h lies on the 1/numbetween grid the
+    linear interpolation weight is zero and :fft is Eqn. 30 itself, so it equals
+    :direct (a property of the method, independent of the oracle)."""
+    N = 1 << 18
+    amps = noise_amps(N // 2, seed=29)
+    eng, p = _engine(cs, nharms=16, m=16)
+    eng.set_spectrum(_ft(cs, amps, N))
+    rf = 9000.0 + np.arange(128) / 16.0
+    a = eng.block_metrics(rf, "fft")
+    d = eng.block_metrics(rf, "direct")
+    assert np.max(np.abs(a - d)) / np.max(np.abs(d)) < 1e-12
+    # off the grid the two methods differ (linear interpolation error ~1e-2)
+    rf2 = rf + 0.5 / 16.0
+    assert np.max(np.abs(eng.block_metrics(rf2, "fft") - eng.block_metrics(rf2, "direct"))) > 1e-6
+
+
+def test_search_block_default_fft(cs, O, small_spectrum):
+    """search_block keeps the trials whose :fft metric exceeds the threshold
+    (search.jl:1087-1097), on an injected pulsar."""
+    amps, N, dt = small_spectrum
+    T = N * dt
+    p = cs.SearchParams(nharms=16, m=16)
+    ft = _ft(cs, amps, N, dt)
+    rf = 13.7 * T + (np.arange(-64, 64) * (0.5 / 16))
+    cands = cs.search_block(ft, rf, p, threshold=6.0)
+    want = O.block_metrics(O.make_params(nharms=16, m=16), amps, N, rf, "fft")
+    keep = [float(r) for r, v in zip(rf, want) if v > 6.0]
+    assert len(keep) > 0
+    assert [c.r for c in cands] == keep
+    got = np.array([c.metric for c in cands])
+    assert np.allclose(got, want[want > 6.0], rtol=1e-9, atol=0)
 
 
 def _compare_search(cands, want, threshold, tol=1e-4):
