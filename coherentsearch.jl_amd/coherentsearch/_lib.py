@@ -17,6 +17,7 @@ HEADER = os.path.join(_HERE, "..", "..", "include", "coherent_search.h")
 
 CS_OK, CS_EINVAL, CS_EDIM, CS_ENOSPC, CS_EDEVICE, CS_ENOMEM = range(6)
 CS_MODE_PLAIN, CS_MODE_FUSED, CS_MODE_FUSED_VALU = 0, 1, 2
+CS_TUNE_BATCH_BLOCKS, CS_TUNE_WORKGROUPS = 1, 2
 
 
 class DimensionMismatch(ValueError):
@@ -84,6 +85,7 @@ def lib():
     L.cs_decim_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, C.c_int32, dp, i64p]
     L.cs_interp_rows.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, C.c_int32, dp]
     L.cs_set_mode.argtypes = [vp, C.c_int32]
+    L.cs_set_tuning.argtypes = [vp, C.c_int32, C.c_int64]
     L.cs_last_timings.argtypes = [vp, dp]
     L.cs_plan_tables.argtypes = [C.POINTER(CsParams), C.c_double, C.c_int32, i64p, i32p,
                                  C.POINTER(C.c_uint8), fp, fp]

@@ -82,6 +82,16 @@ class Engine:
              "fused_valu": _lib.CS_MODE_FUSED_VALU}[mode]
         self._chk(_lib.lib().cs_set_mode(self._h, m), "cs_set_mode")
 
+    def set_tuning(self, batch_blocks: Optional[int] = None, workgroups: Optional[int] = None):
+        """Launch shape of the split-BF16 search (0 or None leaves the default);
+        results do not depend on it."""
+        if batch_blocks is not None:
+            self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_BATCH_BLOCKS,
+                                               int(batch_blocks)), "cs_set_tuning")
+        if workgroups is not None:
+            self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_WORKGROUPS, int(workgroups)),
+                      "cs_set_tuning")
+
     # -- entry points ------------------------------------------------------------
     def search_raw(self, r_lo: float, r_hi: float, threshold: float) -> np.ndarray:
         cap = 1 << 16

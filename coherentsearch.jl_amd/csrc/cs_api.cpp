@@ -29,6 +29,9 @@ struct DBuf {
 constexpr int kBankW[12] = {1, 2, 3, 4, 6, 9, 13, 19, 28, 42, 63, 94};
 constexpr int64_t kBatchTrials = (int64_t)1 << 21;   // plain path: trials per batch (HBM scratch bound)
 constexpr int64_t kBatchTrialsFused = (int64_t)1 << 25;   // fused path: ~1 GB of bounds + samples per batch
+// split-BF16 path: blocks per batch (6 folds: 6.4 GB of sigma samples and
+// 0.8 GB of bounds at 2048-trial blocks), one batch at 2^25 points
+constexpr int64_t kBatchBlocksMfma = 16384;
 constexpr int kMaxHarm = 64;                         // device path: profiles of <= 128 bins
 
 }  // namespace
@@ -57,6 +60,8 @@ struct cs_ctx {
     DBuf spl;                  // pre-split spectrum planes of the MFMA mode
     bool spl_valid = false;    // spl matches the current spectrum
     int ncu = 0;               // compute units of the device
+    int64_t tune_batch = 0;    // CS_TUNE_BATCH_BLOCKS (0: default)
+    int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
     // host-built tables are kept while the grid they were built for is unchanged
     bool tab_valid = false;    // plans on the device for (tab_rlo, tab_f64)
     double tab_rlo = 0.0;
@@ -297,6 +302,20 @@ int cs_set_mode(cs_ctx* c, int32_t mode) {
     if (mode != CS_MODE_PLAIN && mode != CS_MODE_FUSED && mode != CS_MODE_FUSED_VALU)
         return c->fail(CS_EINVAL, "unknown mode");
     c->mode = mode;
+    return CS_OK;
+}
+
+int cs_set_tuning(cs_ctx* c, int32_t knob, int64_t value) {
+    if (!c) return CS_EINVAL;
+    if (value < 0) return c->fail(CS_EINVAL, "tuning value must be >= 0");
+    if (knob == CS_TUNE_BATCH_BLOCKS) {
+        c->tune_batch = value;
+    } else if (knob == CS_TUNE_WORKGROUPS) {
+        if (value > 65536) return c->fail(CS_EINVAL, "at most 65536 workgroups");
+        c->tune_wg = (int)value;
+    } else {
+        return c->fail(CS_EINVAL, "unknown tuning knob");
+    }
     return CS_OK;
 }
 
@@ -601,15 +620,14 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         const MfmaLds lay = mfma_lds_plan(nh, fs, md);
         if (lay.total > 160 * 1024)
             return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");
-        // one persistent workgroup per compute unit, dealt over the 8 XCDs
-        const int ncu = c->ncu > 0 ? c->ncu : 256;
-        md.nxcd = (ncu % 8 == 0) ? 8 : 1;
-        mfma_nwg = (int)std::min<int64_t>(ncu, md.ntiles);
-        if (mfma_nwg % md.nxcd != 0) md.nxcd = 1;
+        // one persistent workgroup per compute unit (dealt over the 8 XCDs),
+        // unless tuned
+        mfma_nwg = c->tune_wg > 0 ? c->tune_wg : (c->ncu > 0 ? c->ncu : 256);
     }
-    const int64_t Bc = mfma ? nchunks
-                            : std::max<int64_t>(1, std::min<int64_t>(
-                                  nchunks, (fused ? kBatchTrialsFused : kBatchTrials) / Nprof));
+    const int64_t Bc =
+        mfma ? std::min<int64_t>(nchunks, c->tune_batch > 0 ? c->tune_batch : kBatchBlocksMfma)
+             : std::max<int64_t>(1, std::min<int64_t>(
+                   nchunks, (fused ? kBatchTrialsFused : kBatchTrials) / Nprof));
     const int64_t BT = Bc * Nprof;
     fs.ldp = BT;
 
@@ -669,6 +687,15 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
     uint32_t* cand_count = surv_count + 1;
     double medcut = (c->medmargin == INFINITY) ? -INFINITY : threshold - c->medmargin;
 
+    if (mfma) {
+        // row masks of every block, and the fold and table descriptors, once
+        // per search (the batches only differ in their by-value BatchDev)
+        HIPCHK(c, launch_rowmask(c->st, sp, tb, g, nchunks, (uint64_t*)c->m_rowmask.p));
+        std::vector<uint8_t> desc(sizeof(FoldSet) + sizeof(MfmaDev));
+        std::memcpy(desc.data(), &fs, sizeof(FoldSet));
+        std::memcpy(desc.data() + sizeof(FoldSet), &md, sizeof(MfmaDev));
+        if ((s = upload(c, c->m_desc, desc))) return s;
+    }
     for (int attempt = 0; attempt < 2; ++attempt) {
         if ((s = ensure(c, c->cands, (size_t)cand_cap * sizeof(cs_cand_dev)))) return s;
         HIPCHK(c, launch_reset_u32(c->st, surv_count, 2));
@@ -686,6 +713,13 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             b.pad = 0;
             b.nvalid = (const int32_t*)c->nvalid.p + c0 * nf;
             b.sigma = (double*)c->sigma.p + c0 * nf;
+            b.tile0 = b.ntile = 0;
+            if (mfma) {   // super-tiles (8 occurrences, 8 S trials) touching the batch
+                const int64_t st_lo = t_base / (8 * md.S);
+                const int64_t st_hi = (t_base + nt + 8 * md.S - 1) / (8 * md.S);
+                b.tile0 = st_lo * md.U;
+                b.ntile = (st_hi - st_lo) * md.U;
+            }
             OutDev o;
             o.surv = (uint64_t*)c->surv.p;
             o.surv_count = surv_count;
@@ -698,17 +732,13 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
             o.medcut = medcut;
             HIPCHK(c, hipEventRecord(E[0], c->st));
             if (fused) {
-                if (mfma) HIPCHK(c, launch_rowmask(c->st, sp, tb, g, nchunks, (uint64_t*)c->m_rowmask.p));
-                else HIPCHK(c, launch_guard(c->st, sp, tb, g, b, (uint8_t*)c->rowok.p));
+                if (!mfma) HIPCHK(c, launch_guard(c->st, sp, tb, g, b, (uint8_t*)c->rowok.p));
                 if (mfma) {
-                    std::vector<uint8_t> desc(sizeof(FoldSet) + sizeof(MfmaDev));
-                    std::memcpy(desc.data(), &fs, sizeof(FoldSet));
-                    std::memcpy(desc.data() + sizeof(FoldSet), &md, sizeof(MfmaDev));
-                    if ((s = upload(c, c->m_desc, desc))) return s;
+                    const int nwg = (int)std::min<int64_t>(mfma_nwg, b.ntile);
                     HIPCHK(c, launch_fused_mfma(c->st, sp, tb, g, fs, b, fu, md,
                                                 (const FoldSet*)c->m_desc.p,
                                                 (const MfmaDev*)((const uint8_t*)c->m_desc.p + sizeof(FoldSet)),
-                                                mfma_nwg));
+                                                nwg));
                 }
                 else HIPCHK(c, launch_fused(c->st, sp, tb, g, fs, b, fu));
                 HIPCHK(c, hipEventRecord(E[1], c->st));

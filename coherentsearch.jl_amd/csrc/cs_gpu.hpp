@@ -61,6 +61,8 @@ struct BatchDev {
     int32_t pad;
     const int32_t* nvalid;   // [block_local * nf + f]
     double* sigma;           // [block_local * nf + f]
+    int64_t tile0;      // split-BF16 kernel: first 128-trial tile touching the batch
+    int64_t ntile;      //   and the number of tiles that do
 };
 
 struct OutDev {

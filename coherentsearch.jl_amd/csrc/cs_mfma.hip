@@ -683,15 +683,15 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     // windows overlap, run together and share that XCD's L2
     int64_t t_first, t_end, t_step;
     {
-        const int G = gridDim.x, nx = md.nxcd;
-        if (nx > 1 && G % nx == 0) {
+        const int G = gridDim.x, nx = (G % 8 == 0) ? 8 : 1;
+        if (nx > 1) {
             const int x = blockIdx.x % nx, l = blockIdx.x / nx;
-            t_first = md.ntiles * x / nx + l;
-            t_end = md.ntiles * (x + 1) / nx;
+            t_first = b.tile0 + b.ntile * x / nx + l;
+            t_end = b.tile0 + b.ntile * (x + 1) / nx;
             t_step = G / nx;
         } else {
-            t_first = blockIdx.x;
-            t_end = md.ntiles;
+            t_first = b.tile0 + blockIdx.x;
+            t_end = b.tile0 + b.ntile;
             t_step = G;
         }
     }
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         tbase = st * kOcc8 * S + 16 * (int64_t)u;
         occ = (int64_t)kOcc8 * st + o;
         t = tbase + (int64_t)wv * S + (lane & 15);
-        inb = t < g.total;
+        inb = t >= b.t_base && t < b.t_base + b.nt;   // trials of this batch only
         jj = 0;
         cl = 0;
         if (inb) {

@@ -28,7 +28,7 @@ struct MfmaDev {
     const int32_t* scol;      // [fold][block trial]: sampled column of a full block, or -1
     int64_t spl_ld;           // plane stride (namps + 2 * kSpecPad)
     int64_t S;                // occurrence stride, lcm(16, q) trials
-    int64_t ntiles;           // (ceil(total / (8 S))) * U tiles of 128 trials
+    int64_t ntiles;           // (ceil(total / (8 S))) * U tiles of 128 trials in the search
     int32_t c_base[kMaxFolds];
     int32_t c_mt[kMaxFolds];
     int32_t c_ks[kMaxFolds];
@@ -37,7 +37,6 @@ struct MfmaDev {
     int32_t ksmax;            // largest interpolation K steps (1 or 2)
     int32_t nbp;              // profile row stride in LDS (floats)
     int32_t dft_tiles_max;    // c2r DFT tiles of all folds (hi or lo)
-    int32_t nxcd;             // XCDs the persistent grid is spread over
 };
 
 struct MfmaLds {

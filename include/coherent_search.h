@@ -33,6 +33,10 @@ extern "C" {
 #define CS_MODE_FUSED 1       /* split-BF16 MFMA interpolation and c2r, fused pipeline */
 #define CS_MODE_FUSED_VALU 2  /* fused pipeline with FP32 VALU interpolation and c2r */
 
+/* Tuning knobs of cs_set_tuning (0 restores the default). */
+#define CS_TUNE_BATCH_BLOCKS 1  /* split-BF16 mode: blocks per batch (default 16384) */
+#define CS_TUNE_WORKGROUPS 2    /* split-BF16 mode: persistent workgroups (default: one per CU) */
+
 /* SearchParams (src/search.jl:131-144).  `decimations` is pointer + length
  * (the reference's Vector{Int}); precision 0 = :f64, 1 = :f32 (the profile
  * stage of the reference; the GPU search computes its profile stage in FP32
@@ -123,6 +127,12 @@ int cs_interp_rows(cs_ctx *ctx, double r_lo, int64_t t0, int64_t n,
 
 /* Execution mode for cs_search (CS_MODE_PLAIN / CS_MODE_FUSED / CS_MODE_FUSED_VALU). */
 int cs_set_mode(cs_ctx *ctx, int32_t mode);
+
+/* Launch-shape knob of cs_search (CS_TUNE_*), value >= 0 (0: default).  The
+ * reference has no such knob; its counterpart is the chunk-invariance property
+ * the search keeps (test/test_search.jl:234-276): every knob changes only how
+ * the trials are scheduled, never a result bit. */
+int cs_set_tuning(cs_ctx *ctx, int32_t knob, int64_t value);
 
 /* Per-stage device time of the last cs_search, milliseconds, HIP events on the
  * context's stream: [0] interp, [1] c2r, [2] sigma, [3] gate, [4] rescan,

@@ -178,6 +178,28 @@ def test_search_block_default_fft(cs, O, small_spectrum):
     assert np.allclose(got, want[want > 6.0], rtol=1e-9, atol=0)
 
 
+@pytest.mark.parametrize("batch,wg", [(3, 0), (1, 37), (5, 8), (2, 1)])
+def test_fused_launch_shape_invariance(cs, small_spectrum, batch, wg):
+    """The split-BF16 search returns bit-identical raw candidates whatever the
+    batch size and persistent grid: batches of 500-trial blocks cut 512-trial
+    super-tiles, whose trials outside the batch are left to the neighbouring
+    batch (the chunk-invariance property, test/test_search.jl:234-276)."""
+    _say(f"launch shape batch={batch} wg={wg}")
+    amps, N, dt = small_spectrum
+    T = N * dt
+    eng, p = _engine(cs, nharms=32, m=16, decimations=cs.decimation_set(32, 4), blocksize=500)
+    eng.set_spectrum(_ft(cs, amps, N, dt))
+    eng.set_mode("fused")
+    ref = eng.search_raw(5.0 * T, 30.0 * T, 6.0)
+    eng.set_tuning(batch_blocks=batch, workgroups=wg)
+    got = eng.search_raw(5.0 * T, 30.0 * T, 6.0)
+    nb = eng.timings()["batches"]
+    eng.set_tuning(batch_blocks=0, workgroups=0)
+    assert len(ref) > 0
+    assert nb > 1
+    assert np.array_equal(ref, got)
+
+
 def _compare_search(cands, want, threshold, tol=1e-4):
     """(t, k) sets equal apart from trials within tol of threshold; metrics agree."""
     g = {(int(c["t"]), int(c["k"])): c for c in cands}
