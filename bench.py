@@ -1,11 +1,14 @@
 #!/usr/bin/env python3
 """Coherent-search throughput on one MI355X.
 
-Workload: a 2^25-sample synthetic PRESTO spectrum (dt = 64 us, complex64
-unit-power Gaussian bins, element 0 = 0, seed 2001) with von Mises pulsars at
-0.7, 61.3 and 412.9 Hz (duty 3/10/5 %, S/N 10/15/20); search nharms=60,
-m=16, hidr=0.5, decimation_set(60, 6) = folds 1..6, fundamentals 0.1-125 Hz,
-threshold 8, blocks of 2048 trials: 32,186,485 trial fundamentals.
+Workload (default, --config c2): a 2^25-sample synthetic PRESTO spectrum
+(dt = 64 us, complex64 unit-power Gaussian bins, element 0 = 0, seed 2001) with
+von Mises pulsars at 0.7, 61.3 and 412.9 Hz (duty 3/10/5 %, S/N 10/15/20);
+search nharms=60, m=16, hidr=0.5, decimation_set(60, 6) = folds 1..6,
+fundamentals 0.1-125 Hz, threshold 8, blocks of 2048 trials: 32,186,485 trial
+fundamentals.  --config c1 (2^20 samples, nharms 8) and --config c5 (2^27
+samples with red noise, 128,745,940 trials, several batches) run the same
+measurement on the other single-GPU workloads.
 
 One step = one full search over the band in the split-BF16 fused mode (the
 default --mode), amplitudes already resident in HBM, ending with the raw
@@ -112,6 +115,7 @@ def main():
                     help="searches timed in the plain FP32 mode for the ratio (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c5"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -121,10 +125,11 @@ def main():
     import numpy as np
 
     import coherentsearch as cs
-    from coherentsearch.synth import config_c2, make_config_spectrum
+    from coherentsearch import synth
+    from coherentsearch.synth import make_config_spectrum
 
-    cfg = config_c2()
-    log(f"[bench] generating the 2^25-sample spectrum (seed {cfg['seed']})")
+    cfg = {"c1": synth.config_c1, "c2": synth.config_c2, "c5": synth.config_c5}[args.config]()
+    log(f"[bench] generating the 2^{int(math.log2(cfg['N']))}-sample spectrum (seed {cfg['seed']})")
     t0 = time.perf_counter()
     amps = make_config_spectrum(cfg)
     log(f"[bench] spectrum ready in {time.perf_counter() - t0:.1f} s")
@@ -147,9 +152,9 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = total / (wall / args.steps)
 
-    # detections of the three injected pulsars (fold-aware, before dedup)
+    # detections of the injected pulsars (fold-aware, before dedup)
     found = {}
-    for f0 in (0.7, 61.3, 412.9):
+    for f0 in [p.freq for p in cfg["pulsars"]]:
         best = 0.0
         for rec in raw:
             r = cs.candidate_r(r_lo, lodr, Nprof, int(rec["t"]), int(rec["k"]))
@@ -177,8 +182,8 @@ def main():
         bytes_per_trial = spec_bytes
     launch_s = stage[dom] / nb / 1e3
     achieved = bytes_per_trial * trials_per_launch / launch_s / 1e9
-    tag = {"fused": "fused_mfma_c2" if args.mode == "fused" else "fused_valu_c2"}.get(dom, dom + "_c2")
-    traffic = load_traffic(tag)
+    tag = {"fused": "fused_mfma" if args.mode == "fused" else "fused_valu"}.get(dom, dom)
+    traffic = load_traffic(f"{tag}_{args.config}")
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic.get("bytes_per_launch") if traffic else None,
@@ -198,9 +203,22 @@ def main():
         praw, pwall, pstage = timed_search(eng, r_lo, r_hi, cfg["threshold"], args.plain_steps, 1,
                                            "plain")
         pms = 1e3 * pwall / args.plain_steps
+        # agreement of the two modes' raw candidates: (t, k) sets equal away from
+        # the threshold, metrics within the S/N tolerance
+        a = {(int(r["t"]), int(r["k"])): float(r["metric"]) for r in raw}
+        b = {(int(r["t"]), int(r["k"])): float(r["metric"]) for r in praw}
+        both = set(a) & set(b)
+        odd = [a.get(k, b.get(k)) for k in set(a) ^ set(b)]
         plain = {"value": round(total / (pwall / args.plain_steps), 1), "unit": "trials/s",
                  "ms_per_step": round(pms, 3), "raw_candidates": int(len(praw)),
-                 "stages_ms": {k: round(v, 3) for k, v in pstage.items()}}
+                 "stages_ms": {k: round(v, 3) for k, v in pstage.items()},
+                 "agreement": {"common": len(both), "only_one_mode": len(odd),
+                               "max_dist_of_those_from_threshold": (
+                                   round(max(abs(v - cfg["threshold"]) for v in odd), 6)
+                                   if odd else None),
+                               "max_rel_metric_diff": (
+                                   float(f"{max(abs(a[k] - b[k]) / abs(b[k]) for k in both):.3e}")
+                                   if both else None)}}
         eng.set_mode(args.mode)
 
     cpu = None
@@ -214,10 +232,14 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "2^25-sample .fft (dt 64 us, seed 2001, 3 von Mises pulsars), "
-                               "nharms=60, m=16, decimation_set(60,6), 0.1-125 Hz, threshold 8, "
-                               "blocks of 2048", "mode": args.mode, "trials": total,
-                   "blocks": nchunks},
+        "config": {"workload": (
+                       f"2^{int(math.log2(N))}-sample .fft (dt 64 us, seed {cfg['seed']}, "
+                       f"{len(cfg['pulsars'])} von Mises pulsars"
+                       + (", red noise below 1 Hz, median-normalised" if cfg.get("red_fc") else "")
+                       + f"), nharms={params.nharms}, m=16, decimation_set({params.nharms},"
+                       f"{cfg['maxdecim']}), {cfg['lofreq']}-{cfg['hifreq']} Hz, threshold "
+                       f"{cfg['threshold']:g}, blocks of 2048"),
+                   "name": args.config, "mode": args.mode, "trials": total, "blocks": nchunks},
         "stages_ms": {k: round(v, 3) for k, v in stage.items()},
         "upload_s": round(upload_s, 4),
         "raw_candidates": int(len(raw)),
