@@ -4,7 +4,7 @@ folding, batched c2r, boxcar matched-filter S/N) on gfx950, behind the C ABI of
 include/coherent_search.h."""
 from .dedup import harmonically_related, remove_duplicates, remove_harmonics
 from .engine import (Engine, SearchCache, block_metrics, chunk_metrics, raw_to_candidates,
-                     search, search_block)
+                     search, search_block, search_files)
 from .fileio import FFTFile, SimpleInf, freqs, write_fft
 from .params import (Candidate, SearchParams, boxcar_widths, candidate_r, decimation_set,
                      ladder_boxcar_widths, proftype, trial_grid)
@@ -13,6 +13,7 @@ __all__ = [
     "Candidate", "Engine", "FFTFile", "SearchCache", "SearchParams", "SimpleInf",
     "block_metrics", "boxcar_widths", "candidate_r", "chunk_metrics", "decimation_set",
     "freqs", "harmonically_related", "ladder_boxcar_widths", "proftype", "raw_to_candidates",
-    "remove_duplicates", "remove_harmonics", "search", "search_block", "trial_grid",
+    "remove_duplicates", "remove_harmonics", "search", "search_block", "search_files",
+    "trial_grid",
     "write_fft",
 ]

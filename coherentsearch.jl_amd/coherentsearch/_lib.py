@@ -51,6 +51,17 @@ class CsCand(C.Structure):
     ]
 
 
+class CsFile(C.Structure):
+    _fields_ = [
+        ("amps", C.POINTER(C.c_float)),
+        ("namps", C.c_int64),
+        ("N", C.c_int64),
+        ("T", C.c_double),
+        ("r_lo", C.c_double),
+        ("r_hi", C.c_double),
+    ]
+
+
 CAND_DTYPE = np.dtype([("t", "<i8"), ("k", "<i4"), ("nharm", "<i4"),
                        ("block", "<i8"), ("metric", "<f8")])
 
@@ -81,6 +92,8 @@ def lib():
     L.cs_search.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.POINTER(CsCand),
                             C.c_int64, i64p]
     L.cs_block_metrics.argtypes = [vp, dp, C.c_int64, C.c_int32, i32p, C.c_int32, dp]
+    L.cs_search_files.argtypes = [vp, C.c_int32, C.POINTER(CsFile), C.c_double,
+                                  C.POINTER(CsCand), C.c_int64, i64p, i64p]
     L.cs_chunk_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_double, C.c_int32, dp]
     L.cs_decim_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, C.c_int32, dp, i64p]
     L.cs_interp_rows.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, C.c_int32, dp]

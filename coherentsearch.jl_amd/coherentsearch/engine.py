@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -105,6 +105,33 @@ class Engine:
                 continue
             self._chk(st, "cs_search")
             return np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
+
+    def search_files(self, files: Sequence[Tuple[FFTFile, float, float]],
+                     threshold: float) -> List[np.ndarray]:
+        """Raw candidates of several spectra, each (ft, r_lo, r_hi), from one
+        cs_search_files call; the next file uploads while one is searched."""
+        keep, recs = [], (_lib.CsFile * len(files))()
+        for i, (ft, r_lo, r_hi) in enumerate(files):
+            a = np.ascontiguousarray(np.asarray(ft.amps, dtype=np.complex64))
+            keep.append(a)
+            recs[i] = _lib.CsFile(a.view(np.float32).ctypes.data_as(C.POINTER(C.c_float)), len(a),
+                                  int(ft.N), float(ft.T), float(r_lo), float(r_hi))
+        counts = np.zeros(len(files), dtype=np.int64)
+        cap = 1 << 16
+        while True:
+            out = (_lib.CsCand * cap)()
+            n = C.c_int64()
+            st = _lib.lib().cs_search_files(self._h, len(files), recs, float(threshold), out, cap,
+                                            counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            C.byref(n))
+            self._spec_key = None   # the context holds no spectrum afterwards
+            if st == _lib.CS_ENOSPC and n.value > cap:
+                cap = n.value
+                continue
+            self._chk(st, "cs_search_files")
+            allc = np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
+            offs = np.concatenate([[0], np.cumsum(counts)])
+            return [allc[offs[i]:offs[i + 1]] for i in range(len(files))]
 
     def timings(self) -> dict:
         a = np.zeros(8)
@@ -222,6 +249,40 @@ def search(ft: FFTFile, params: Optional[SearchParams] = None, lofreq: float = 0
     if harm_remove:
         cands = remove_harmonics(cands, numharm=numharm, tol=harm_tol)
     return cands
+
+
+def search_files(fts: Sequence[FFTFile], params: Optional[SearchParams] = None,
+                 lofreq: float = 0.1, hifreq: float = 100.0, lobin: int = 100,
+                 blocksize: int = 2048, threshold: Optional[float] = None, remove: bool = True,
+                 dr_tol: float = 1.0, harm_remove: bool = True, numharm: int = 16,
+                 harm_tol: float = 1.0, device: int = 0,
+                 mode: str = "fused") -> List[List[Candidate]]:
+    """search over several files with one device context, as the CLI's
+    search_one loop shares one SearchCache (src/cli.jl:231-289); each file's
+    list equals search(ft, params; ...) on that file.  The next file's upload
+    runs while a file is searched."""
+    params = SearchParams() if params is None else params
+    thr = params.threshold if threshold is None else float(threshold)
+    grids = [trial_grid(ft.T, params, lofreq, hifreq, lobin, blocksize) for ft in fts]
+    out: List[List[Candidate]] = [[] for _ in fts]
+    todo = [i for i, g in enumerate(grids) if g[3] > 0]
+    if not todo:
+        return out
+    eng = Engine(params, grids[todo[0]][4], device)
+    try:
+        eng.set_mode(mode)
+        raws = eng.search_files([(fts[i], grids[i][0], grids[i][1]) for i in todo], thr)
+    finally:
+        eng.close()
+    for i, raw in zip(todo, raws):
+        r_lo, r_hi, lodr, total, Nprof, nchunks = grids[i]
+        cands = raw_to_candidates(raw, fts[i], r_lo, lodr, Nprof)
+        cands = remove_duplicates(cands, dr_tol=dr_tol) if remove else \
+            sorted(cands, key=lambda c: c.freq)
+        if harm_remove:
+            cands = remove_harmonics(cands, numharm=numharm, tol=harm_tol)
+        out[i] = cands
+    return out
 
 
 def block_metrics(ft: FFTFile, rfund, params: SearchParams, kernel: str = "fft",

@@ -7,8 +7,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "cs_fused.hpp"
@@ -61,6 +63,14 @@ struct cs_ctx {
     bool spl_valid = false;    // spl matches the current spectrum
     int ncu = 0;               // compute units of the device
     int64_t tune_batch = 0;    // CS_TUNE_BATCH_BLOCKS (0: default)
+    // multi-file search: spectrum view (a device slot instead of `amps`), two
+    // device slots, two pinned staging buffers and the upload stream
+    const void* amps_view = nullptr;
+    DBuf slot[2];
+    void* pin[2] = {nullptr, nullptr};
+    size_t pin_bytes = 0;
+    hipStream_t cst = nullptr;
+    hipEvent_t up[2] = {nullptr, nullptr};
     int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
     // host-built tables are kept while the grid they were built for is unchanged
     bool tab_valid = false;    // plans on the device for (tab_rlo, tab_f64)
@@ -191,7 +201,7 @@ int get_twid(cs_ctx* c, int nbins, const double2** out) {
 
 SpecDev spec_of(cs_ctx* c) {
     SpecDev sp;
-    sp.amps = (const float2*)c->amps.p;
+    sp.amps = (const float2*)(c->amps_view ? c->amps_view : c->amps.p);
     sp.namps = c->namps;
     sp.N = c->N;
     sp.Nhalf = c->N / 2;
@@ -230,7 +240,8 @@ int64_t nvalid_of(int k, double rstart, double lodr, int64_t n, double nyq) {
 }
 
 int check_spectrum(cs_ctx* c) {
-    if (!c->amps.p || c->namps <= 0) return c->fail(CS_EINVAL, "no spectrum set (cs_set_spectrum)");
+    if ((!c->amps.p && !c->amps_view) || c->namps <= 0)
+        return c->fail(CS_EINVAL, "no spectrum set (cs_set_spectrum)");
     return CS_OK;
 }
 
@@ -288,8 +299,13 @@ int cs_ctx_destroy(cs_ctx* c) {
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
                     &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples, &c->bandtab,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
-                    &c->m_desc, &c->m_rowmask, &c->m_scol, &c->spl};
+                    &c->m_desc, &c->m_rowmask, &c->m_scol, &c->spl, &c->slot[0], &c->slot[1]};
     for (DBuf* b : bufs) release(*b);
+    for (void* p : c->pin)
+        if (p) (void)hipHostFree(p);
+    for (hipEvent_t e : c->up)
+        if (e) (void)hipEventDestroy(e);
+    if (c->cst) (void)hipStreamDestroy(c->cst);
     for (auto& kv : c->twid) release(kv.second);
     for (hipEvent_t ev : c->ev) (void)hipEventDestroy(ev);
     if (c->st) (void)hipStreamDestroy(c->st);
@@ -334,6 +350,7 @@ static int set_spectrum_impl(cs_ctx* c, const void* src, hipMemcpyKind kind, int
     if (s) return s;
     HIPCHK(c, hipMemcpyAsync(c->amps.p, src, (size_t)namps * sizeof(float2), kind, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
+    c->amps_view = nullptr;
     c->namps = namps;
     c->N = N;
     c->T = T;
@@ -806,6 +823,141 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         return CS_OK;
     }
     return c->fail(CS_ENOSPC, "candidate buffer could not be sized");
+}
+
+static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, double threshold,
+                             cs_cand* out, int64_t cap, int64_t* counts, int64_t* n_out) {
+    *n_out = 0;
+    if (nfiles <= 0) return CS_OK;
+    if (cap < 0 || (cap > 0 && !out)) return c->fail(CS_EINVAL, "bad candidate buffer");
+    int64_t maxn = 0;
+    for (int32_t i = 0; i < nfiles; ++i) {
+        const cs_file& f = files[i];
+        if (!f.amps || f.namps <= 0 || f.N <= 0 || !(f.T > 0))
+            return c->fail(CS_EINVAL, "file " + std::to_string(i) + ": invalid spectrum");
+        maxn = std::max(maxn, f.namps);
+        counts[i] = 0;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    const size_t maxb = (size_t)maxn * sizeof(float2);
+    int s;
+    for (int k = 0; k < 2; ++k)
+        if ((s = ensure(c, c->slot[k], maxb))) return s;
+    if (c->pin_bytes < maxb) {
+        for (void*& p : c->pin) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+        }
+        c->pin_bytes = 0;
+        for (void*& p : c->pin)
+            if (hipHostMalloc(&p, maxb, hipHostMallocDefault) != hipSuccess || !p)
+                return c->fail(CS_ENOMEM, "pinned staging allocation failed");
+        c->pin_bytes = maxb;
+    }
+    if (!c->cst) HIPCHK(c, hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking));
+    for (hipEvent_t& e : c->up)
+        if (!e) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+
+    // files of equal geometry run back to back, so the plan and table caches
+    // (keyed on r_lo and the trial count) are reused; results do not depend on
+    // the order
+    std::vector<int32_t> order(nfiles);
+    for (int32_t i = 0; i < nfiles; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        const cs_file &x = files[a], &y = files[b];
+        if (x.N != y.N) return x.N < y.N;
+        if (x.r_lo != y.r_lo) return x.r_lo < y.r_lo;
+        return x.r_hi < y.r_hi;
+    });
+    // staging of file k (worker thread): a multi-threaded copy into pinned
+    // slot k & 1, then its upload on the copy stream.  Slot k & 1 was last read
+    // by the search of file k - 2, which has completed.
+    auto stage = [c, files, &order](int k) -> hipError_t {
+        const cs_file& f = files[order[k]];
+        const int sl = k & 1;
+        const size_t bytes = (size_t)f.namps * sizeof(float2);
+        const unsigned hw = std::thread::hardware_concurrency();
+        const int nt = (int)std::max<size_t>(1, std::min<size_t>({(size_t)(hw ? hw : 4), 8,
+                                                                   bytes >> 22}));
+        std::vector<std::thread> pool;
+        const size_t part = (bytes + nt - 1) / nt;
+        for (int t = 0; t < nt; ++t) {
+            const size_t b0 = std::min(bytes, part * t), b1 = std::min(bytes, b0 + part);
+            pool.emplace_back([=] {
+                std::memcpy((char*)c->pin[sl] + b0, (const char*)f.amps + b0, b1 - b0);
+            });
+        }
+        for (auto& th : pool) th.join();
+        hipError_t e = hipSetDevice(c->device);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(c->slot[sl].p, c->pin[sl], bytes, hipMemcpyHostToDevice, c->cst);
+        if (e == hipSuccess) e = hipEventRecord(c->up[sl], c->cst);
+        return e;
+    };
+    std::vector<std::vector<cs_cand>> res(nfiles);
+    std::vector<double> tsum(8, 0.0);
+    HIPCHK(c, stage(0));
+    int rc = CS_OK;
+    for (int32_t k = 0; k < nfiles && rc == CS_OK; ++k) {
+        std::future<hipError_t> next;
+        if (k + 1 < nfiles) next = std::async(std::launch::async, stage, k + 1);
+        const cs_file& f = files[order[k]];
+        hipError_t e = hipStreamWaitEvent(c->st, c->up[k & 1], 0);
+        if (e == hipSuccess) {
+            c->amps_view = c->slot[k & 1].p;
+            c->namps = f.namps;
+            c->N = f.N;
+            c->T = f.T;
+            c->spl_valid = false;
+            std::vector<cs_cand> buf(1024);
+            int64_t n = 0;
+            int st = cs_search(c, f.r_lo, f.r_hi, threshold, buf.data(), (int64_t)buf.size(), &n);
+            if (st == CS_ENOSPC && n > (int64_t)buf.size()) {
+                buf.resize((size_t)n);
+                st = cs_search(c, f.r_lo, f.r_hi, threshold, buf.data(), (int64_t)buf.size(), &n);
+            }
+            if (st != CS_OK) rc = st;
+            buf.resize((size_t)std::max<int64_t>(0, std::min<int64_t>(n, (int64_t)buf.size())));
+            res[order[k]] = std::move(buf);
+            for (int i = 0; i < 7; ++i) tsum[i] += c->timings[i];
+            tsum[7] += c->timings[7];
+        } else {
+            rc = c->fail(CS_EDEVICE, std::string("upload wait: ") + hipGetErrorString(e));
+        }
+        if (next.valid()) {
+            const hipError_t ne = next.get();
+            if (ne != hipSuccess && rc == CS_OK)
+                rc = c->fail(CS_EDEVICE, std::string("upload: ") + hipGetErrorString(ne));
+        }
+    }
+    (void)hipStreamSynchronize(c->cst);
+    c->amps_view = nullptr;
+    c->namps = 0;
+    c->spl_valid = false;
+    for (int i = 0; i < 8; ++i) c->timings[i] = tsum[i];
+    if (rc != CS_OK) return rc;
+    int64_t total = 0;
+    for (int32_t i = 0; i < nfiles; ++i) {
+        counts[i] = (int64_t)res[i].size();
+        for (const cs_cand& cd : res[i]) {
+            if (total < cap) out[total] = cd;
+            ++total;
+        }
+    }
+    *n_out = total;
+    return total > cap ? CS_ENOSPC : CS_OK;
+}
+
+int cs_search_files(cs_ctx* c, int32_t nfiles, const cs_file* files, double threshold,
+                    cs_cand* out, int64_t cap, int64_t* counts, int64_t* n_out) {
+    if (!c || !n_out || (nfiles > 0 && (!files || !counts))) return CS_EINVAL;
+    try {
+        return search_files_impl(c, nfiles, files, threshold, out, cap, counts, n_out);
+    } catch (const std::exception& e) {
+        c->amps_view = nullptr;
+        c->namps = 0;
+        return c->fail(CS_ENOMEM, std::string("multi-file search: ") + e.what());
+    }
 }
 
 }  // extern "C"

@@ -95,6 +95,26 @@ int cs_set_spectrum_device(cs_ctx *ctx, const void *d_amps, int64_t namps,
 int cs_search(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
               cs_cand *out, int64_t cap, int64_t *n_out);
 
+/* One spectrum of a multi-file search: host complex64 amplitudes (namps
+ * (re, im) pairs, as cs_set_spectrum), the .inf N and T, and the band. */
+typedef struct cs_file {
+    const float *amps;
+    int64_t namps;
+    int64_t N;
+    double T;
+    double r_lo;
+    double r_hi;
+} cs_file;
+
+/* search over several files with one context, as the CLI's search_one loop
+ * shares one SearchCache (src/cli.jl:231-289, search.jl:2002-2111 per file).
+ * Each file's raw candidates equal those of cs_set_spectrum + cs_search on it;
+ * they are written in file order, counts[i] per file, *n_out in total
+ * (CS_ENOSPC if > cap).  The next file's upload (pinned staging, second
+ * stream) runs while the current file is searched. */
+int cs_search_files(cs_ctx *ctx, int32_t nfiles, const cs_file *files, double threshold,
+                    cs_cand *out, int64_t cap, int64_t *counts, int64_t *n_out);
+
 /* block_metrics(ft, rfund, params; kernel, widths) (src/search.jl:1061-1079):
  * kernel 1 = :direct (Eqn. 30 point by point, FP64); kernel 0 = :fft
  * (finterp_fft onto the numbetween grid, then clamped linear interpolation,

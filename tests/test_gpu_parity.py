@@ -200,6 +200,41 @@ def test_fused_launch_shape_invariance(cs, small_spectrum, batch, wg):
     assert np.array_equal(ref, got)
 
 
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+def test_search_files_equals_single_file_runs(cs, mode):
+    """Several spectra (three lengths, one repeated, one with a pulsar) through
+    one cs_search_files call give, file by file, the raw candidates of
+    cs_set_spectrum + cs_search on that file (the multi-file .cohout equals
+    separate runs, test/test_cli.jl:65-85)."""
+    _say(f"search_files mode={mode}")
+    from coherentsearch.synth import Pulsar, make_spectrum
+    dt = 64e-6
+    specs = [(1 << 17, 41), (1 << 18, 42), (1 << 16, 43), (1 << 18, 44), (1 << 17, 41)]
+    fts = []
+    for N, seed in specs:
+        amps = make_spectrum(N, dt, seed=seed,
+                             pulsars=[Pulsar(21.3, 0.06, 20.0)] if seed == 42 else [])
+        fts.append(_ft(cs, amps, N, dt))
+    p = cs.SearchParams(nharms=32, m=16, decimations=cs.decimation_set(32, 4))
+    eng = cs.Engine(p, 1024)
+    eng.set_mode(mode)
+    jobs = [(ft, 5.0 * ft.T, 40.0 * ft.T) for ft in fts]
+    multi = eng.search_files(jobs, 6.5)
+    assert len(multi) == len(fts)
+    for (ft, r_lo, r_hi), got in zip(jobs, multi):
+        eng.set_spectrum(ft)
+        want = eng.search_raw(r_lo, r_hi, 6.5)
+        assert np.array_equal(got, want)
+    assert len(multi[1]) > 0          # the file with the pulsar
+    assert np.array_equal(multi[0], multi[4])
+    # and through the host mirror, per file as search() does
+    lists = cs.search_files(fts, p, lofreq=5.0, hifreq=40.0, threshold=6.5, blocksize=1024,
+                            mode=mode)
+    one = cs.search(fts[1], p, lofreq=5.0, hifreq=40.0, threshold=6.5, blocksize=1024, mode=mode)
+    assert [(c.r, c.metric) for c in lists[1]] == [(c.r, c.metric) for c in one]
+    assert any(abs(c.freq - 21.3) < 2.0 / fts[1].T for c in lists[1])
+
+
 def _compare_search(cands, want, threshold, tol=1e-4):
     """(t, k) sets equal apart from trials within tol of threshold; metrics agree."""
     g = {(int(c["t"]), int(c["k"])): c for c in cands}
