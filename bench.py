@@ -8,7 +8,8 @@ search nharms=60, m=16, hidr=0.5, decimation_set(60, 6) = folds 1..6,
 fundamentals 0.1-125 Hz, threshold 8, blocks of 2048 trials: 32,186,485 trial
 fundamentals.  --config c1 (2^20 samples, nharms 8) and --config c5 (2^27
 samples with red noise, 128,745,940 trials, several batches) run the same
-measurement on the other single-GPU workloads.
+measurement on the other single-GPU workloads; --config c3 times a 64-file
+DM sweep through one multi-file call (uploads included and overlapped).
 
 One step = one full search over the band in the split-BF16 fused mode (the
 default --mode), amplitudes already resident in HBM, ending with the raw
@@ -105,6 +106,81 @@ def load_traffic(kernel_tag):
         return None
 
 
+def run_c3(args, cs, nfiles):
+    """The DM-sweep workload: nfiles spectra of 2^22..2^25 samples (one pulsar
+    each, seeds 3000 + i), nharms 32, decimation_set(32, 4), 0.1-125 Hz,
+    threshold 8.  value = total trials / wall of one cs_search_files call
+    (uploads from host memory included, overlapped with the searches)."""
+    import numpy as np
+
+    from coherentsearch.synth import config_c3_file, make_config_spectrum
+    t0 = time.perf_counter()
+    cfgs = [config_c3_file(i) for i in range(nfiles)]
+    fts = []
+    for i, c in enumerate(cfgs):
+        fts.append(cs.FFTFile.from_arrays(make_config_spectrum(c), c["N"], c["dt"]))
+        if i % 8 == 7:
+            log(f"[bench] c3: {i + 1}/{nfiles} spectra ready ({time.perf_counter() - t0:.0f} s)")
+    params = cs.SearchParams(nharms=32, m=16, decimations=cs.decimation_set(32, 4),
+                             threshold=8.0)
+    grids = [cs.trial_grid(ft.T, params, 0.1, 125.0, 100, 2048) for ft in fts]
+    total = sum(g[3] for g in grids)
+    jobs = [(ft, g[0], g[1]) for ft, g in zip(fts, grids)]
+    eng = cs.Engine(params, 2048, device=0)
+    eng.set_mode(args.mode)
+    for _ in range(args.warmup):
+        eng.search_files(jobs[:4], 8.0)
+    walls, dev = [], 0.0
+    for i in range(args.steps):
+        t1 = time.perf_counter()
+        multi = eng.search_files(jobs, 8.0)
+        walls.append(time.perf_counter() - t1)
+        dev = eng.timings()["total"]
+        log(f"[bench] c3 step {i + 1}/{args.steps}: {walls[-1] * 1e3:.1f} ms for {nfiles} files, "
+            f"device {dev:.1f} ms")
+    wall = sum(walls) / len(walls)
+    # the same files one by one: upload (pageable) then search, no overlap
+    t1 = time.perf_counter()
+    seq, same = [], True
+    for (ft, r_lo, r_hi), got in zip(jobs, multi):
+        eng.set_spectrum(ft)
+        want = eng.search_raw(r_lo, r_hi, 8.0)
+        same = same and bool(np.array_equal(got, want))
+        seq.append(len(want))
+    seq_wall = time.perf_counter() - t1
+    found = 0
+    for c, ft, g, raw in zip(cfgs, fts, grids, multi):
+        f0 = c["pulsars"][0].freq
+        r_lo, r_hi, lodr, tot, Nprof, nch = g
+        if not (0.1 <= f0 <= 125.0):
+            continue
+        for rec in raw:
+            r = cs.candidate_r(r_lo, lodr, Nprof, int(rec["t"]), int(rec["k"]))
+            if abs(r / ft.T - f0) < 3.0 / ft.T:
+                found += 1
+                break
+    in_band = sum(1 for c in cfgs if 0.1 <= c["pulsars"][0].freq <= 125.0)
+    line = {
+        "metric": METRIC, "value": round(total / wall, 1), "unit": "trials/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{nfiles}-file DM sweep: 2^22..2^25-sample .fft files (seeds "
+                               f"3000+i, one von Mises pulsar each), nharms=32, m=16, "
+                               f"decimation_set(32,4), 0.1-125 Hz, threshold 8, blocks of 2048; "
+                               f"host amplitudes, uploads inside the timed region",
+                   "name": "c3", "mode": args.mode, "files": nfiles, "trials": total},
+        "device_ms_per_step": round(dev, 3),
+        "device_only_rate": round(total / (dev / 1e3), 1) if dev > 0 else None,
+        "sequential_upload_then_search_ms": round(1e3 * seq_wall, 3),
+        "overlap_gain": round(seq_wall / wall, 3),
+        "per_file_equal_to_single_runs": same,
+        "raw_candidates": int(sum(len(r) for r in multi)),
+        "pulsars_detected": f"{found}/{in_band} in band",
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -115,7 +191,8 @@ def main():
                     help="searches timed in the plain FP32 mode for the ratio (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--c3-files", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -128,6 +205,9 @@ def main():
     from coherentsearch import synth
     from coherentsearch.synth import make_config_spectrum
 
+    if args.config == "c3":
+        run_c3(args, cs, args.c3_files)
+        return
     cfg = {"c1": synth.config_c1, "c2": synth.config_c2, "c5": synth.config_c5}[args.config]()
     log(f"[bench] generating the 2^{int(math.log2(cfg['N']))}-sample spectrum (seed {cfg['seed']})")
     t0 = time.perf_counter()
