@@ -99,6 +99,8 @@ def lib():
     L.cs_interp_rows.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, C.c_int32, dp]
     L.cs_set_mode.argtypes = [vp, C.c_int32]
     L.cs_set_tuning.argtypes = [vp, C.c_int32, C.c_int64]
+    L.cs_autotune.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_int32,
+                              i64p, C.POINTER(C.c_int32)]
     L.cs_last_timings.argtypes = [vp, dp]
     L.cs_plan_tables.argtypes = [C.POINTER(CsParams), C.c_double, C.c_int32, i64p, i32p,
                                  C.POINTER(C.c_uint8), fp, fp]

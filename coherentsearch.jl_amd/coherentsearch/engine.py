@@ -92,6 +92,17 @@ class Engine:
             self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_WORKGROUPS, int(workgroups)),
                       "cs_set_tuning")
 
+    def autotune(self, r_lo: float, r_hi: float, threshold: float,
+                 cache_path: Optional[str] = None, reps: int = 2) -> Tuple[int, int]:
+        """Time the split-BF16 search's launch shapes for this band and keep the
+        fastest (cs_autotune); returns (blocks per batch, workgroups), 0 = default."""
+        b = C.c_int64()
+        w = C.c_int32()
+        self._chk(_lib.lib().cs_autotune(self._h, float(r_lo), float(r_hi), float(threshold),
+                                         cache_path.encode() if cache_path else None, int(reps),
+                                         C.byref(b), C.byref(w)), "cs_autotune")
+        return int(b.value), int(w.value)
+
     # -- entry points ------------------------------------------------------------
     def search_raw(self, r_lo: float, r_hi: float, threshold: float) -> np.ndarray:
         cap = 1 << 16

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <future>
@@ -12,6 +13,8 @@
 #include <string>
 #include <thread>
 #include <vector>
+
+#include <unistd.h>
 
 #include "cs_fused.hpp"
 #include "cs_geo.hpp"
@@ -333,6 +336,132 @@ int cs_set_tuning(cs_ctx* c, int32_t knob, int64_t value) {
         return c->fail(CS_EINVAL, "unknown tuning knob");
     }
     return CS_OK;
+}
+
+// cs_autotune: the cache line of a search geometry (parameters, trial range,
+// spectrum length, device architecture)
+static std::string tune_key(cs_ctx* c, double r_lo, double r_hi) {
+    hipDeviceProp_t prop;
+    const char* arch = hipGetDeviceProperties(&prop, c->device) == hipSuccess ? prop.gcnArchName : "?";
+    std::string k = "nh=" + std::to_string(c->nharms) + " m=" + std::to_string(c->m) + " dec=";
+    for (int d : c->decims) k += std::to_string(d) + ",";
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), " hidr=%.17g nprof=%lld mode=%d namps=%lld rlo=%.17g rhi=%.17g arch=",
+                  c->hidr, (long long)c->Nprof, c->mode, (long long)c->namps, r_lo, r_hi);
+    k += buf;
+    for (const char* q = arch; *q && *q != ':'; ++q) k += (*q == ' ' || *q == '\t') ? '_' : *q;
+    return k;
+}
+
+static int tune_time(cs_ctx* c, double r_lo, double r_hi, double thr, int reps, double* ms) {
+    std::vector<cs_cand> buf(1 << 12);
+    std::vector<double> t;
+    for (int r = 0; r <= reps; ++r) {   // the first search warms the caches
+        int64_t n = 0;
+        int st = cs_search(c, r_lo, r_hi, thr, buf.data(), (int64_t)buf.size(), &n);
+        if (st == CS_ENOSPC) {
+            buf.resize((size_t)n);
+            st = cs_search(c, r_lo, r_hi, thr, buf.data(), (int64_t)buf.size(), &n);
+        }
+        if (st != CS_OK) return st;
+        if (r > 0) t.push_back(c->timings[6]);
+    }
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return CS_OK;
+}
+
+int cs_autotune(cs_ctx* c, double r_lo, double r_hi, double threshold, const char* cache_path,
+                int32_t reps, int64_t* out_batch, int32_t* out_wg) {
+    if (!c) return CS_EINVAL;
+    int s = check_spectrum(c);
+    if (s) return s;
+    if (reps < 1 || reps > 50) return c->fail(CS_EINVAL, "reps must be in 1..50");
+    auto report = [&]() {
+        if (out_batch) *out_batch = c->tune_batch;
+        if (out_wg) *out_wg = c->tune_wg;
+        return CS_OK;
+    };
+    if (c->mode != CS_MODE_FUSED) {   // the knobs only shape the split-BF16 search
+        c->tune_batch = 0;
+        c->tune_wg = 0;
+        return report();
+    }
+    const std::string key = tune_key(c, r_lo, r_hi);
+    std::vector<std::string> lines;
+    if (cache_path && *cache_path) {
+        if (FILE* fh = std::fopen(cache_path, "r")) {
+            char ln[1024];
+            while (std::fgets(ln, sizeof(ln), fh)) {
+                std::string l(ln);
+                while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+                if (l.empty()) continue;
+                const size_t tab = l.find('\t');
+                if (tab != std::string::npos && l.compare(0, tab, key) == 0) {
+                    long long b = 0, w = 0;
+                    if (std::sscanf(l.c_str() + tab + 1, "%lld %lld", &b, &w) == 2 && b >= 0 &&
+                        w >= 0 && w <= 65536) {
+                        std::fclose(fh);
+                        c->tune_batch = b;
+                        c->tune_wg = (int)w;
+                        return report();
+                    }
+                    continue;   // malformed entry: measured again and replaced
+                }
+                lines.push_back(l);
+            }
+            std::fclose(fh);
+        }
+    }
+    // candidates: persistent grid sizes around one workgroup per CU, batch
+    // sizes dividing the search into 1, 2 or 4 batches
+    const double lodr = c->hidr / c->nharms;
+    const double span = std::floor((r_hi - r_lo) / lodr);
+    const int64_t total = span >= 0 ? (int64_t)span + 1 : 0;
+    const int64_t nchunks = (total + c->Nprof - 1) / c->Nprof;
+    const int ncu = c->ncu > 0 ? c->ncu : 256;
+    std::vector<int> wgs{0, std::max(8, ncu * 3 / 4 / 8 * 8), std::max(8, ncu / 2 / 8 * 8)};
+    std::vector<int64_t> bats{0};
+    for (int64_t d : {2, 4})
+        if (nchunks >= d) bats.push_back((nchunks + d - 1) / d);
+    double best = INFINITY;
+    int64_t bb = 0;
+    int bw = 0;
+    for (int w : wgs)
+        for (int64_t bt : bats) {
+            c->tune_wg = w;
+            c->tune_batch = bt;
+            double ms = 0.0;
+            if ((s = tune_time(c, r_lo, r_hi, threshold, reps, &ms))) {
+                c->tune_wg = 0;
+                c->tune_batch = 0;
+                return s;
+            }
+            if (ms < best) {
+                best = ms;
+                bb = bt;
+                bw = w;
+            }
+        }
+    c->tune_batch = bb;
+    c->tune_wg = bw;
+    if (cache_path && *cache_path) {   // temp file + rename, as the FFTW wisdom export
+        char val[96];
+        std::snprintf(val, sizeof(val), "\t%lld %d %.4f", (long long)bb, bw, best);
+        lines.push_back(key + val);
+        const std::string tmp = std::string(cache_path) + "." + std::to_string((long long)getpid()) + ".tmp";
+        bool ok = false;
+        if (FILE* fh = std::fopen(tmp.c_str(), "w")) {
+            ok = true;
+            for (const std::string& l : lines) ok = ok && std::fprintf(fh, "%s\n", l.c_str()) > 0;
+            ok = (std::fclose(fh) == 0) && ok;
+        }
+        if (!ok || std::rename(tmp.c_str(), cache_path) != 0) {
+            std::remove(tmp.c_str());
+            return c->fail(CS_EINVAL, std::string("cannot write the tuning cache ") + cache_path);
+        }
+    }
+    return report();
 }
 
 int cs_last_timings(const cs_ctx* c, double* ms8) {

@@ -154,6 +154,17 @@ int cs_set_mode(cs_ctx *ctx, int32_t mode);
  * the trials are scheduled, never a result bit. */
 int cs_set_tuning(cs_ctx *ctx, int32_t knob, int64_t value);
 
+/* Autotuner of the split-BF16 search's launch shape for one search geometry:
+ * times cs_search (device time, median of reps after one warm search) over
+ * a few persistent-grid and batch sizes and keeps the fastest, then reports
+ * it (either output may be NULL).  With cache_path, a line per geometry
+ * (parameters, band, spectrum length, device) is looked up first and written
+ * after measuring, by temp file + rename as the reference's FFTW wisdom
+ * export does (src/wisdom.jl:46-61).  Other modes have no knobs (0, 0). */
+int cs_autotune(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
+                const char *cache_path, int32_t reps, int64_t *best_batch,
+                int32_t *best_workgroups);
+
 /* Per-stage device time of the last cs_search, milliseconds, HIP events on the
  * context's stream: [0] interp, [1] c2r, [2] sigma, [3] gate, [4] rescan,
  * [5] fused, [6] total device, [7] number of batches. */

@@ -200,6 +200,32 @@ def test_fused_launch_shape_invariance(cs, small_spectrum, batch, wg):
     assert np.array_equal(ref, got)
 
 
+def test_autotune_cache_and_invariance(cs, small_spectrum, tmp_path):
+    """cs_autotune picks a launch shape by timing, the tuned search returns the
+    default search's raw candidates bit for bit, and the tuning cache keeps one
+    line per geometry that a later call reads instead of measuring."""
+    _say("autotune")
+    amps, N, dt = small_spectrum
+    T = N * dt
+    eng, p = _engine(cs, nharms=32, m=16, decimations=cs.decimation_set(32, 4), blocksize=500)
+    eng.set_spectrum(_ft(cs, amps, N, dt))
+    eng.set_mode("fused")
+    ref = eng.search_raw(5.0 * T, 30.0 * T, 6.0)
+    path = str(tmp_path / "tune.txt")
+    b, w = eng.autotune(5.0 * T, 30.0 * T, 6.0, cache_path=path, reps=1)
+    assert b >= 0 and 0 <= w <= 65536
+    assert np.array_equal(eng.search_raw(5.0 * T, 30.0 * T, 6.0), ref)
+    lines = open(path).read().splitlines()
+    assert len(lines) == 1 and "\t" in lines[0]
+    eng.set_tuning(batch_blocks=0, workgroups=0)
+    assert eng.autotune(5.0 * T, 30.0 * T, 6.0, cache_path=path) == (b, w)
+    assert open(path).read().splitlines() == lines          # read, not re-measured
+    eng.autotune(5.0 * T, 20.0 * T, 6.0, cache_path=path, reps=1)
+    assert len(open(path).read().splitlines()) == 2
+    eng.set_mode("plain")
+    assert eng.autotune(5.0 * T, 30.0 * T, 6.0) == (0, 0)
+
+
 @pytest.mark.parametrize("mode", ["fused", "plain"])
 def test_search_files_equals_single_file_runs(cs, mode):
     """Several spectra (three lengths, one repeated, one with a pulsar) through
