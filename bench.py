@@ -193,6 +193,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5"])
     ap.add_argument("--c3-files", type=int, default=64)
+    ap.add_argument("--autotune", action="store_true",
+                    help="run cs_autotune before timing (the default launch shape is the one "
+                         "the committed rocprof summaries measure)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
@@ -226,6 +229,12 @@ def main():
     eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))   # resident in HBM from here on
     upload_s = time.perf_counter() - t0
 
+    tuned = None
+    if args.mode == "fused" and args.autotune:
+        t0 = time.perf_counter()
+        tb, tw = eng.autotune(r_lo, r_hi, cfg["threshold"], reps=2)
+        tuned = {"batch_blocks": tb, "workgroups": tw, "seconds": round(time.perf_counter() - t0, 2)}
+        log(f"[bench] autotune: batch {tb or 'default'}, workgroups {tw or 'one per CU'}")
     log(f"[bench] {total} trials, {nchunks} blocks, mode {args.mode}; warmup {args.warmup}")
     raw, wall, stage = timed_search(eng, r_lo, r_hi, cfg["threshold"], args.steps, args.warmup,
                                     args.mode)
@@ -322,6 +331,7 @@ def main():
                    "name": args.config, "mode": args.mode, "trials": total, "blocks": nchunks},
         "stages_ms": {k: round(v, 3) for k, v in stage.items()},
         "upload_s": round(upload_s, 4),
+        "launch_shape": tuned,
         "raw_candidates": int(len(raw)),
         "detections": found,
         "roofline": roof,
