@@ -18,6 +18,7 @@ HEADER = os.path.join(_HERE, "..", "..", "include", "coherent_search.h")
 CS_OK, CS_EINVAL, CS_EDIM, CS_ENOSPC, CS_EDEVICE, CS_ENOMEM = range(6)
 CS_MODE_PLAIN, CS_MODE_FUSED, CS_MODE_FUSED_VALU = 0, 1, 2
 CS_TUNE_BATCH_BLOCKS, CS_TUNE_WORKGROUPS = 1, 2
+CS_TRIAGE_FEATURES, CS_TRIAGE_PARAMS = 32, 2177
 
 
 class DimensionMismatch(ValueError):
@@ -94,6 +95,12 @@ def lib():
     L.cs_block_metrics.argtypes = [vp, dp, C.c_int64, C.c_int32, i32p, C.c_int32, dp]
     L.cs_search_files.argtypes = [vp, C.c_int32, C.POINTER(CsFile), C.c_double,
                                   C.POINTER(CsCand), C.c_int64, i64p, i64p]
+    L.cs_triage_features.argtypes = [vp, C.c_double, C.c_double, fp, C.c_int64, i64p]
+    L.cs_mlp_forward.argtypes = [vp, fp, fp, fp, C.c_int64, fp]
+    L.cs_mlp_grad.argtypes = [vp, fp, fp, fp, fp, C.c_int64, fp]
+    L.cs_search_blocks.argtypes = [vp, C.c_double, C.c_double, C.c_double,
+                                   C.POINTER(C.c_uint8), C.c_int64, C.POINTER(CsCand), C.c_int64,
+                                   i64p]
     L.cs_chunk_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_double, C.c_int32, dp]
     L.cs_decim_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, C.c_int32, dp, i64p]
     L.cs_interp_rows.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, C.c_int32, dp]

@@ -25,6 +25,14 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _fp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
 class Engine:
     """One device context: the reference's SearchCache plus the spectrum held in
     HBM.  Not thread-safe (one per host thread)."""
@@ -143,6 +151,51 @@ class Engine:
             allc = np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
             offs = np.concatenate([[0], np.cumsum(counts)])
             return [allc[offs[i]:offs[i + 1]] for i in range(len(files))]
+
+    # -- block triage ------------------------------------------------------------
+    def triage_features(self, r_lo: float, r_hi: float) -> np.ndarray:
+        """[block][32] features of the band's search blocks (cs_triage_features)."""
+        nb = C.c_int64()
+        st = _lib.lib().cs_triage_features(self._h, float(r_lo), float(r_hi), None, 0, C.byref(nb))
+        if st not in (_lib.CS_OK, _lib.CS_ENOSPC):
+            self._chk(st, "cs_triage_features")
+        out = np.zeros((nb.value, _lib.CS_TRIAGE_FEATURES), dtype=np.float32)
+        if nb.value:
+            self._chk(_lib.lib().cs_triage_features(self._h, float(r_lo), float(r_hi), _fp(out),
+                                                    nb.value, C.byref(nb)), "cs_triage_features")
+        return out
+
+    def mlp_forward(self, params: np.ndarray, norm: np.ndarray, X: np.ndarray) -> np.ndarray:
+        p, nm, x = _f32(params), _f32(norm), _f32(X)
+        out = np.zeros(len(x), dtype=np.float32)
+        self._chk(_lib.lib().cs_mlp_forward(self._h, _fp(p), _fp(nm), _fp(x), len(x), _fp(out)),
+                  "cs_mlp_forward")
+        return out
+
+    def mlp_grad(self, params: np.ndarray, norm: np.ndarray, X: np.ndarray, y: np.ndarray):
+        """(gradient in the params layout, mean binary cross-entropy)"""
+        p, nm, x, yy = _f32(params), _f32(norm), _f32(X), _f32(y)
+        g = np.zeros(_lib.CS_TRIAGE_PARAMS + 1, dtype=np.float32)
+        self._chk(_lib.lib().cs_mlp_grad(self._h, _fp(p), _fp(nm), _fp(x), _fp(yy), len(x),
+                                         _fp(g)), "cs_mlp_grad")
+        return g[:-1], float(g[-1])
+
+    def search_blocks(self, r_lo: float, r_hi: float, threshold: float,
+                      mask: np.ndarray) -> np.ndarray:
+        """Raw candidates of the blocks with mask != 0 only (cs_search_blocks)."""
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
+        cap = 1 << 16
+        while True:
+            out = (_lib.CsCand * cap)()
+            n = C.c_int64()
+            st = _lib.lib().cs_search_blocks(self._h, float(r_lo), float(r_hi), float(threshold),
+                                             m.ctypes.data_as(C.POINTER(C.c_uint8)), len(m), out,
+                                             cap, C.byref(n))
+            if st == _lib.CS_ENOSPC and n.value > cap:
+                cap = n.value
+                continue
+            self._chk(st, "cs_search_blocks")
+            return np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
 
     def timings(self) -> dict:
         a = np.zeros(8)

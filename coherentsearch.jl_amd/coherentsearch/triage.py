@@ -1,0 +1,103 @@
+"""Block triage: rank a band's search blocks by a learned score and search
+only the best ones (no counterpart in the reference search; for callers that
+trade recall for time).
+
+Features: 32 incoherent-power statistics per search block, computed on the
+device (cs_triage_features).  Scorer: an MLP 32 -> 64 (ReLU) -> 1 logit whose
+forward and backward passes run on the device in bf16 with FP32 accumulation
+(cs_mlp_forward / cs_mlp_grad); the Adam update of its 2177 FP32 parameters
+runs here.  Labels: whether a block holds a raw candidate of the full search.
+"""
+from __future__ import annotations
+
+import math
+from typing import List, Optional
+
+import numpy as np
+
+from . import _lib
+
+NF = _lib.CS_TRIAGE_FEATURES
+NH = 64
+NPARAMS = _lib.CS_TRIAGE_PARAMS
+
+
+def block_labels(raw: np.ndarray, nblocks: int) -> np.ndarray:
+    """1 for the blocks holding at least one raw candidate, else 0."""
+    y = np.zeros(nblocks, dtype=np.float32)
+    if len(raw):
+        y[np.unique(raw["block"])] = 1.0
+    return y
+
+
+class BlockTriage:
+    """The scorer's parameters (W1 [64][32], b1, w2, b2) and the feature
+    standardisation (mean, inverse std) fitted on the training features."""
+
+    def __init__(self, seed: int = 0):
+        rng = np.random.default_rng(seed)
+        W1 = rng.standard_normal((NH, NF)) * math.sqrt(2.0 / NF)
+        w2 = rng.standard_normal(NH) * math.sqrt(1.0 / NH)
+        self.params = np.concatenate([W1.ravel(), np.zeros(NH), w2, [0.0]]).astype(np.float32)
+        self.norm = np.concatenate([np.zeros(NF), np.ones(NF)]).astype(np.float32)
+
+    def fit(self, engine, X: np.ndarray, y: np.ndarray, epochs: int = 400, lr: float = 3e-3,
+            pos_fraction: float = 0.25, seed: int = 0) -> List[float]:
+        """Full-batch Adam on the mean binary cross-entropy; the rare positive
+        blocks are repeated until they make up pos_fraction of the rows."""
+        X = np.asarray(X, dtype=np.float32)
+        y = np.asarray(y, dtype=np.float32)
+        mean = X.mean(axis=0)
+        std = X.std(axis=0)
+        self.norm = np.concatenate([mean, 1.0 / np.where(std > 1e-12, std, 1.0)]).astype(np.float32)
+        pos = np.flatnonzero(y > 0.5)
+        rows = np.arange(len(y))
+        if 0 < len(pos) < pos_fraction * len(y):
+            reps = int(math.ceil(pos_fraction * len(y) / len(pos)))
+            rows = np.concatenate([rows] + [pos] * (reps - 1))
+        Xr, yr = X[rows], y[rows]
+        p = self.params.astype(np.float64)
+        m1 = np.zeros_like(p)
+        m2 = np.zeros_like(p)
+        b1, b2, eps = 0.9, 0.999, 1e-8
+        hist = []
+        for it in range(1, epochs + 1):
+            g, loss = engine.mlp_grad(p.astype(np.float32), self.norm, Xr, yr)
+            g = g.astype(np.float64)
+            m1 = b1 * m1 + (1 - b1) * g
+            m2 = b2 * m2 + (1 - b2) * g * g
+            p -= lr * (m1 / (1 - b1 ** it)) / (np.sqrt(m2 / (1 - b2 ** it)) + eps)
+            hist.append(loss)
+        self.params = p.astype(np.float32)
+        return hist
+
+    def logits(self, engine, X: np.ndarray) -> np.ndarray:
+        return engine.mlp_forward(self.params, self.norm, X)
+
+    @staticmethod
+    def select(logits: np.ndarray, fraction: float) -> np.ndarray:
+        """mask of the ceil(fraction * n) highest-scoring blocks"""
+        n = len(logits)
+        k = max(1, min(n, int(math.ceil(fraction * n))))
+        mask = np.zeros(n, dtype=np.uint8)
+        mask[np.argsort(-logits, kind="stable")[:k]] = 1
+        return mask
+
+    def save(self, path: str):
+        np.savez(path, params=self.params, norm=self.norm)
+
+    @classmethod
+    def load(cls, path: str) -> "BlockTriage":
+        d = np.load(path)
+        t = cls()
+        t.params = d["params"].astype(np.float32)
+        t.norm = d["norm"].astype(np.float32)
+        return t
+
+
+def triage_search(engine, model: BlockTriage, r_lo: float, r_hi: float, threshold: float,
+                  fraction: float) -> np.ndarray:
+    """Raw candidates of the top `fraction` of the band's blocks by score."""
+    X = engine.triage_features(r_lo, r_hi)
+    mask = model.select(model.logits(engine, X), fraction)
+    return engine.search_blocks(r_lo, r_hi, threshold, mask)

@@ -21,6 +21,7 @@
 #include "cs_mfma.hpp"
 #include "cs_gpu.hpp"
 #include "cs_plan.hpp"
+#include "cs_triage.hpp"
 
 using namespace cs;
 
@@ -61,6 +62,7 @@ struct cs_ctx {
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
     DBuf bandtab;   // block_metrics(:fft): per-harmonic lobin, then fine-grid length
+    DBuf pyr, tfeat, mlp_w, mlp_x, mlp_p;   // block triage: power pyramid, features, MLP buffers
     DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol;
     DBuf spl;                  // pre-split spectrum planes of the MFMA mode
     bool spl_valid = false;    // spl matches the current spectrum
@@ -301,6 +303,7 @@ int cs_ctx_destroy(cs_ctx* c) {
     DBuf* bufs[] = {&c->amps, &c->harm, &c->row, &c->carry, &c->W32, &c->A32, &c->W64, &c->A64,
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
                     &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples, &c->bandtab,
+                    &c->pyr, &c->tfeat, &c->mlp_w, &c->mlp_x, &c->mlp_p,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
                     &c->m_desc, &c->m_rowmask, &c->m_scol, &c->spl, &c->slot[0], &c->slot[1]};
     for (DBuf* b : bufs) release(*b);
@@ -651,8 +654,10 @@ int cs_plan_tables(const cs_params* p, double r_lo, int32_t h, int64_t* info, in
     return CS_OK;
 }
 
-int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* out, int64_t cap,
-              int64_t* n_out) {
+// cs_search, or with a block mask (one byte per block, nonzero = search it)
+// only the selected blocks: the batches are the runs of selected blocks
+static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, const uint8_t* mask,
+                       int64_t nmask, cs_cand* out, int64_t cap, int64_t* n_out) {
     if (!c || !n_out) return CS_EINVAL;
     *n_out = 0;
     for (double& v : c->timings) v = 0.0;
@@ -820,7 +825,21 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         rowo += fs.f[f].nbins;
     }
     uint32_t cand_cap = (uint32_t)std::max<int64_t>(1 << 16, std::min<int64_t>(cap, (int64_t)1 << 26));
-    const int64_t nbatch = (nchunks + Bc - 1) / Bc;
+    if (mask && nmask != nchunks)
+        return c->fail(CS_EDIM, "block mask has " + std::to_string(nmask) + " entries for " +
+                                    std::to_string(nchunks) + " blocks");
+    std::vector<std::pair<int64_t, int64_t>> batches;   // (first block, blocks)
+    for (int64_t c0 = 0; c0 < nchunks;) {
+        if (mask && !mask[c0]) {
+            ++c0;
+            continue;
+        }
+        int64_t e = c0;
+        while (e < nchunks && (!mask || mask[e]) && e - c0 < Bc) ++e;
+        batches.emplace_back(c0, e - c0);
+        c0 = e;
+    }
+    const int64_t nbatch = (int64_t)batches.size();
     const int nev = 6;
     while ((int64_t)c->ev.size() < nbatch * nev) {
         hipEvent_t e;
@@ -847,8 +866,8 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         HIPCHK(c, launch_reset_u32(c->st, surv_count, 2));
         for (int64_t bi = 0; bi < nbatch; ++bi) {
             hipEvent_t* E = &c->ev[bi * nev];
-            const int64_t c0 = bi * Bc;
-            const int64_t nb = std::min(Bc, nchunks - c0);
+            const int64_t c0 = batches[bi].first;
+            const int64_t nb = batches[bi].second;
             const int64_t t_base = c0 * Nprof;
             const int64_t nt = std::min(nb * Nprof, total - t_base);
             BatchDev b;
@@ -952,6 +971,101 @@ int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* ou
         return CS_OK;
     }
     return c->fail(CS_ENOSPC, "candidate buffer could not be sized");
+}
+
+int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* out, int64_t cap,
+              int64_t* n_out) {
+    return search_impl(c, r_lo, r_hi, threshold, nullptr, 0, out, cap, n_out);
+}
+
+int cs_search_blocks(cs_ctx* c, double r_lo, double r_hi, double threshold, const uint8_t* mask,
+                     int64_t nmask, cs_cand* out, int64_t cap, int64_t* n_out) {
+    if (!c || !mask) return CS_EINVAL;
+    return search_impl(c, r_lo, r_hi, threshold, mask, nmask, out, cap, n_out);
+}
+
+// ---- block triage -----------------------------------------------------------
+static_assert(CS_TRIAGE_FEATURES == kTriageFeatures && CS_TRIAGE_PARAMS == kTriageParams,
+              "triage sizes of the header and the kernels differ");
+int cs_triage_features(cs_ctx* c, double r_lo, double r_hi, float* out, int64_t cap_blocks,
+                       int64_t* nblocks) {
+    if (!c || !nblocks) return CS_EINVAL;
+    *nblocks = 0;
+    int s = check_spectrum(c);
+    if (s) return s;
+    if (c->nharms > 64) return c->fail(CS_EINVAL, "triage features need nharms <= 64");
+    HIPCHK(c, hipSetDevice(c->device));
+    const double lodr = c->hidr / c->nharms;
+    const double span = std::floor((r_hi - r_lo) / lodr);
+    const int64_t total = span >= 0 ? (int64_t)span + 1 : 0;
+    const int64_t nch = (total + c->Nprof - 1) / c->Nprof;
+    *nblocks = nch;
+    if (nch == 0) return CS_OK;
+    if (!out || cap_blocks < nch) return c->fail(CS_ENOSPC, "feature buffer too small");
+    uint32_t fm = 0;
+    for (int d : c->decims)
+        if (d >= 2 && d <= 6) fm |= 1u << d;
+    const SpecDev sp = spec_of(c);
+    if ((s = ensure(c, c->pyr, (size_t)triage_pyramid_floats(sp.namps) * sizeof(float)))) return s;
+    if ((s = ensure(c, c->tfeat, (size_t)nch * kTriageFeatures * sizeof(float)))) return s;
+    HIPCHK(c, launch_triage_features(c->st, sp.amps, sp.namps, (float*)c->pyr.p, r_lo, lodr,
+                                     c->Nprof, total, c->nharms, fm, nch, (float*)c->tfeat.p));
+    HIPCHK(c, hipMemcpyAsync(out, c->tfeat.p, (size_t)nch * kTriageFeatures * sizeof(float),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
+}
+
+static int mlp_call(cs_ctx* c, const float* params, const float* norm, const float* X,
+                    const float* y, int64_t n, float* logits, float* grad) {
+    if (!c || !params || !norm || (n > 0 && !X)) return CS_EINVAL;
+    if (n < 0) return c->fail(CS_EINVAL, "n must be >= 0");
+    if (n == 0) {
+        if (grad) std::memset(grad, 0, sizeof(float) * (kTriageParams + 1));
+        return CS_OK;
+    }
+    HIPCHK(c, hipSetDevice(c->device));
+    int s;
+    const size_t np = kTriageParams, nx = (size_t)n * kTriageFeatures;
+    if ((s = ensure(c, c->mlp_w, (np + 2 * kTriageFeatures) * sizeof(float)))) return s;
+    if ((s = ensure(c, c->mlp_x, (nx + 2 * (size_t)n) * sizeof(float)))) return s;
+    float* dW = (float*)c->mlp_w.p;
+    float* dX = (float*)c->mlp_x.p;
+    float* dY = dX + nx;
+    float* dZ = dY + n;
+    HIPCHK(c, hipMemcpyAsync(dW, params, np * sizeof(float), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(dW + np, norm, 2 * kTriageFeatures * sizeof(float),
+                             hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(dX, X, nx * sizeof(float), hipMemcpyHostToDevice, c->st));
+    float* dP = nullptr;
+    float* dG = nullptr;
+    if (grad) {
+        HIPCHK(c, hipMemcpyAsync(dY, y, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->st));
+        const size_t rows = (size_t)mlp_partial_rows(n);
+        if ((s = ensure(c, c->mlp_p, (rows + 1) * (np + 1) * sizeof(float)))) return s;
+        dP = (float*)c->mlp_p.p;
+        dG = dP + rows * (np + 1);
+    }
+    HIPCHK(c, launch_mlp(c->st, dW, dW + np, dX, grad ? dY : nullptr, n, logits ? dZ : nullptr, dP,
+                         dG));
+    if (logits)
+        HIPCHK(c, hipMemcpyAsync(logits, dZ, (size_t)n * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    if (grad)
+        HIPCHK(c, hipMemcpyAsync(grad, dG, (np + 1) * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
+}
+
+int cs_mlp_forward(cs_ctx* c, const float* params, const float* norm, const float* X, int64_t n,
+                   float* logits) {
+    if (!logits && n > 0) return CS_EINVAL;
+    return mlp_call(c, params, norm, X, nullptr, n, logits, nullptr);
+}
+
+int cs_mlp_grad(cs_ctx* c, const float* params, const float* norm, const float* X, const float* y,
+                int64_t n, float* grad) {
+    if (!grad || (n > 0 && !y)) return CS_EINVAL;
+    return mlp_call(c, params, norm, X, y, n, nullptr, grad);
 }
 
 static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, double threshold,

@@ -95,6 +95,34 @@ int cs_set_spectrum_device(cs_ctx *ctx, const void *d_amps, int64_t namps,
 int cs_search(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
               cs_cand *out, int64_t cap, int64_t *n_out);
 
+/* Block triage (no counterpart in the reference; for callers that trade
+ * recall for time): 32 incoherent-power features per search block and an MLP
+ * scorer 32 -> 64 (ReLU) -> 1 logit whose forward and backward passes run on
+ * bf16 operands with FP32 accumulation.  params: W1[64][32] (row = hidden
+ * unit), b1[64], w2[64], b2 (FP32); norm: feature mean[32], inverse std[32]. */
+#define CS_TRIAGE_FEATURES 32
+#define CS_TRIAGE_PARAMS 2177
+
+/* Features of every search block of the band r_lo .. r_hi of the current
+ * spectrum, [block][CS_TRIAGE_FEATURES] (*nblocks = block count; CS_ENOSPC if
+ * cap_blocks is smaller). */
+int cs_triage_features(cs_ctx *ctx, double r_lo, double r_hi, float *out, int64_t cap_blocks,
+                       int64_t *nblocks);
+/* logits of n feature rows */
+int cs_mlp_forward(cs_ctx *ctx, const float *params, const float *norm, const float *X,
+                   int64_t n, float *logits);
+/* gradient of the mean binary cross-entropy of n rows with labels y (0/1):
+ * grad[0 .. CS_TRIAGE_PARAMS-1] in the params layout, grad[CS_TRIAGE_PARAMS] =
+ * the loss.  Reductions run in a fixed order (deterministic). */
+int cs_mlp_grad(cs_ctx *ctx, const float *params, const float *norm, const float *X,
+                const float *y, int64_t n, float *grad);
+/* cs_search over the blocks with mask[block] != 0 only (nmask = the band's
+ * block count, CS_EDIM otherwise); each selected block's candidates equal
+ * those cs_search reports for it. */
+int cs_search_blocks(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
+                     const uint8_t *mask, int64_t nmask, cs_cand *out, int64_t cap,
+                     int64_t *n_out);
+
 /* One spectrum of a multi-file search: host complex64 amplitudes (namps
  * (re, im) pairs, as cs_set_spectrum), the .inf N and T, and the band. */
 typedef struct cs_file {

@@ -1,0 +1,129 @@
+"""GPU checks of the block triage and the block-subset search.
+
+Bars: features within FP32 rounding of the FP64 reference (oracle/triage_ref.py);
+MLP logits and gradients against the reference with the device's bf16
+roundings emulated (FP32 versus FP64 accumulation only); gradients
+deterministic; cs_search_blocks bit-identical to cs_search filtered to the
+selected blocks.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cs():
+    import coherentsearch
+    return coherentsearch
+
+
+@pytest.fixture(scope="module")
+def T():
+    import triage_ref
+    return triage_ref
+
+
+def _spectrum(cs, N, seed, pulsars=()):
+    from coherentsearch.synth import make_spectrum
+    return make_spectrum(N, 64e-6, seed=seed, pulsars=list(pulsars))
+
+
+@pytest.mark.parametrize("nharms,decims", [(32, [1, 2, 3, 4]), (60, [1, 2, 3, 4, 5, 6]),
+                                           (8, [1])])
+def test_features_vs_reference(cs, T, nharms, decims):
+    _say(f"triage features nharms={nharms}")
+    from coherentsearch.synth import Pulsar
+    N = 1 << 16
+    amps = _spectrum(cs, N, 5, [Pulsar(31.3, 0.05, 30.0)])
+    p = cs.SearchParams(nharms=nharms, m=16, decimations=decims)
+    eng = cs.Engine(p, 256)
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, 64e-6))
+    lodr = 0.5 / nharms
+    r_lo, r_hi = 60.3, 60.3 + 256 * 5.5 * lodr
+    X = eng.triage_features(r_lo, r_hi)
+    total = int(math.floor((r_hi - r_lo) / lodr)) + 1
+    want = T.block_features(amps, r_lo, lodr, 256, total, nharms, decims)
+    assert X.shape == want.shape == (6, 32)
+    np.testing.assert_allclose(X, want, rtol=2e-5, atol=2e-5)
+
+
+def test_mlp_forward_and_grad_vs_reference(cs, T):
+    _say("mlp forward / grad")
+    rng = np.random.default_rng(7)
+    n = 1000   # not a multiple of the 32-row wave tile
+    params = (rng.standard_normal(T.NPARAMS) * 0.2).astype(np.float32)
+    norm = np.concatenate([rng.standard_normal(32) * 0.1, 0.5 + rng.random(32)]).astype(np.float32)
+    X = rng.standard_normal((n, 32)).astype(np.float32)
+    y = (rng.random(n) > 0.8).astype(np.float32)
+    eng = cs.Engine(cs.SearchParams(nharms=8, m=16), 256)
+    z = eng.mlp_forward(params, norm, X)
+    zr, _ = T.mlp_forward(params, norm, X, round_bf16=True)
+    assert np.max(np.abs(z - zr)) <= 2e-3 * np.max(np.abs(zr))
+    g, loss = eng.mlp_grad(params, norm, X, y)
+    gr, lr_ = T.mlp_grad(params, norm, X, y, round_bf16=True)
+    assert abs(loss - lr_) <= 1e-4 * abs(lr_)
+    assert np.linalg.norm(g - gr) <= 1e-2 * np.linalg.norm(gr)
+    g2, loss2 = eng.mlp_grad(params, norm, X, y)
+    assert np.array_equal(g, g2) and loss == loss2          # fixed-order reductions
+
+
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+def test_search_blocks_equals_filtered_search(cs, small_spectrum, mode):
+    _say(f"search_blocks mode={mode}")
+    amps, N, dt = small_spectrum
+    T_ = N * dt
+    p = cs.SearchParams(nharms=32, m=16, decimations=cs.decimation_set(32, 4))
+    eng = cs.Engine(p, 500)
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
+    eng.set_mode(mode)
+    r_lo, r_hi = 5.0 * T_, 30.0 * T_
+    full = eng.search_raw(r_lo, r_hi, 6.0)
+    nb = len(eng.triage_features(r_lo, r_hi))
+    assert len(full) > 0 and nb > 4
+    assert np.array_equal(eng.search_blocks(r_lo, r_hi, 6.0, np.ones(nb, np.uint8)), full)
+    rng = np.random.default_rng(11)
+    mask = (rng.random(nb) < 0.4).astype(np.uint8)
+    mask[full["block"][0]] = 1
+    got = eng.search_blocks(r_lo, r_hi, 6.0, mask)
+    want = full[mask[full["block"]] == 1]
+    assert len(got) > 0 and np.array_equal(got, want)
+    assert len(eng.search_blocks(r_lo, r_hi, 6.0, np.zeros(nb, np.uint8))) == 0
+    with pytest.raises(ValueError):
+        eng.search_blocks(r_lo, r_hi, 6.0, np.ones(nb + 1, np.uint8))
+
+
+def test_triage_training_separates_blocks(cs):
+    """Full-batch training on one spectrum lowers the loss and ranks the blocks
+    holding candidates above the others."""
+    _say("triage training")
+    from coherentsearch.synth import Pulsar
+    from coherentsearch.triage import BlockTriage, block_labels
+    N, dt = 1 << 19, 64e-6
+    rng = np.random.default_rng(5)
+    psrs = [Pulsar(float(f), float(d), float(s)) for f, d, s in
+            zip(rng.uniform(3.0, 60.0, 12), rng.uniform(0.02, 0.12, 12), rng.uniform(10, 25, 12))]
+    amps = _spectrum(cs, N, 21, psrs)
+    p = cs.SearchParams(nharms=32, m=16, decimations=cs.decimation_set(32, 4))
+    eng = cs.Engine(p, 512)
+    eng.set_mode("fused")
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
+    T_ = N * dt
+    r_lo, r_hi = 2.0 * T_, 70.0 * T_
+    raw = eng.search_raw(r_lo, r_hi, 8.0)
+    X = eng.triage_features(r_lo, r_hi)
+    y = block_labels(raw, len(X))
+    assert 0 < y.sum() < len(y)
+    model = BlockTriage(seed=1)
+    hist = model.fit(eng, X, y, epochs=150, lr=1e-2)
+    assert hist[-1] < 0.7 * hist[0]
+    z = model.logits(eng, X)
+    assert z[y > 0].mean() > z[y == 0].mean() + 1.0

@@ -1,0 +1,83 @@
+"""CPU checks of the block-triage reference (oracle/triage_ref.py).
+
+The triage (features + MLP scorer) is this project's own addition, so its
+reference is pinned by mathematics rather than by reference fixtures:
+* the FP64 gradient equals central finite differences of the loss;
+* bf16 rounding is round-to-nearest-even on known values;
+* the cell maxima equal an independently built max pyramid;
+* power features scale with |X|^2 and the ratio features do not.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import triage_ref as T
+
+
+def _rand_model(rng):
+    params = rng.standard_normal(T.NPARAMS) * 0.3
+    norm = np.concatenate([rng.standard_normal(T.NF) * 0.1, 1.0 + rng.random(T.NF)])
+    return params, norm
+
+
+def test_gradient_matches_finite_differences():
+    rng = np.random.default_rng(1)
+    params, norm = _rand_model(rng)
+    X = rng.standard_normal((40, T.NF))
+    y = (rng.random(40) > 0.5).astype(float)
+    g, loss = T.mlp_grad(params, norm, X, y)
+
+    def L(p):
+        z, _ = T.mlp_forward(p, norm, X)
+        return np.mean(np.maximum(z, 0) - z * y + np.log1p(np.exp(-np.abs(z))))
+
+    assert abs(L(params) - loss) < 1e-12
+    idx = np.concatenate([rng.choice(T.NH * T.NF, 40, replace=False),
+                          T.NH * T.NF + np.arange(0, 2 * T.NH + 1, 7), [T.NPARAMS - 1]])
+    eps = 1e-6
+    for i in idx:
+        e = np.zeros(T.NPARAMS)
+        e[i] = eps
+        fd = (L(params + e) - L(params - e)) / (2 * eps)
+        assert abs(fd - g[i]) <= 1e-7 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_bf16_rounding():
+    vals = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -9, -2.5, 1.0 + 2 ** -7 + 2 ** -8],
+                    dtype=np.float32)
+    want = [1.0, 1.0, 1.0 + 2 ** -7, -2.5, 1.0 + 2 ** -6]   # ties to even
+    assert np.array_equal(T.bf16(vals), np.array(want))
+
+
+def test_cell_maxima_equal_a_numpy_pyramid():
+    rng = np.random.default_rng(2)
+    P = rng.exponential(size=5000).astype(np.float64)
+    levels = {}
+    for s in range(1, 7):
+        w = 1 << s
+        pad = np.concatenate([P, np.zeros((-len(P)) % w)])
+        levels[s] = pad.reshape(-1, w).max(axis=1)
+    for _ in range(400):
+        g = int(rng.integers(2, 65))
+        f = int(rng.integers(0, 5000 // g + 2))
+        s = int(math.floor(math.log2(g)))
+        c0, c1 = (g * f) >> s, (g * f + g - 1) >> s
+        L = levels[s]
+        want = max([L[c] for c in range(c0, c1 + 1) if 0 <= c < len(L)], default=0.0)
+        assert T._q(P, g, f) == want
+
+
+def test_feature_scaling():
+    rng = np.random.default_rng(3)
+    n = 1 << 14
+    amps = ((rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)).astype(np.complex64)
+    args = dict(r_lo=60.0, lodr=1 / 64, Nprof=256, total=256 * 5 + 17, nharms=32,
+                decimations=[1, 2, 3, 4])
+    F1 = T.block_features(amps, **args)
+    F2 = T.block_features(amps * np.float32(2.0), **args)
+    assert F1.shape == (6, 32)
+    power = list(range(0, 28)) + [30]
+    assert np.allclose(F2[:, power], 4.0 * F1[:, power], rtol=1e-12, atol=0)
+    assert np.allclose(F2[:, [28, 31]], F1[:, [28, 31]], rtol=1e-12, atol=0)
+    assert np.all(F1[:, 25] == 0.0)   # fold 6 is not searched
