@@ -181,6 +181,123 @@ def run_c3(args, cs, nfiles):
     print(json.dumps(line), flush=True)
 
 
+def run_triage(args, cs):
+    """Block triage at equal time: train the scorer on a separate 2^23-sample
+    spectrum (40 pulsars, seed 2101) labelled by its full search, then on the
+    2^25-sample workload compare, per block fraction q, the triaged fused
+    search (features + scorer + the top-q blocks) with the plain FP32 search
+    given the same time budget (the blocks it reaches in band order): recall
+    of the full search's raw candidates and of the injected pulsars."""
+    import numpy as np
+
+    from coherentsearch import synth
+    from coherentsearch.synth import Pulsar, make_config_spectrum, make_spectrum
+    from coherentsearch.triage import BlockTriage, block_labels
+
+    cfg = synth.config_c2()
+    params = cs.SearchParams(nharms=60, m=16, decimations=cs.decimation_set(60, 6), threshold=8.0)
+    # training spectrum
+    rng = np.random.default_rng(2101)
+    psrs = [Pulsar(float(math.exp(rng.uniform(math.log(0.2), math.log(120.0)))),
+                   float(rng.uniform(0.02, 0.15)), float(rng.uniform(7.0, 20.0))) for _ in range(40)]
+    Ntr = 1 << 23
+    t0 = time.perf_counter()
+    tr = make_spectrum(Ntr, cfg["dt"], 2101, psrs)
+    log(f"[bench] triage: training spectrum ready ({time.perf_counter() - t0:.0f} s)")
+    eng = cs.Engine(params, 2048, device=0)
+    eng.set_mode("fused")
+    eng.set_spectrum(cs.FFTFile.from_arrays(tr, Ntr, cfg["dt"]))
+    g = cs.trial_grid(Ntr * cfg["dt"], params, 0.1, 125.0, 100, 2048)
+    raw_tr = eng.search_raw(g[0], g[1], 8.0)
+    Xtr = eng.triage_features(g[0], g[1])
+    ytr = block_labels(raw_tr, len(Xtr))
+    model = BlockTriage(seed=0)
+    t0 = time.perf_counter()
+    hist = model.fit(eng, Xtr, ytr, epochs=400, lr=3e-3)
+    fit_s = time.perf_counter() - t0
+    log(f"[bench] triage: trained on {len(Xtr)} blocks ({int(ytr.sum())} positive), "
+        f"loss {hist[0]:.3f} -> {hist[-1]:.3f} in {fit_s:.1f} s")
+    # evaluation spectrum (the 2^25-sample workload)
+    amps = make_config_spectrum(cfg)
+    N, dt = cfg["N"], cfg["dt"]
+    T = N * dt
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
+    r_lo, r_hi, lodr, total, Nprof, nchunks = cs.trial_grid(T, params, cfg["lofreq"], cfg["hifreq"],
+                                                            100, 2048)
+    for _ in range(args.warmup):
+        eng.search_raw(r_lo, r_hi, 8.0)
+    t0 = time.perf_counter()
+    full = eng.search_raw(r_lo, r_hi, 8.0)
+    t_full = time.perf_counter() - t0
+    fullset = {(int(c["t"]), int(c["k"])) for c in full}
+
+    def pulsars_found(raw):
+        n = 0
+        for psr in cfg["pulsars"]:
+            for rec in raw:
+                r = cs.candidate_r(r_lo, lodr, Nprof, int(rec["t"]), int(rec["k"]))
+                if abs(r / T - psr.freq) < 3.0 / T:
+                    n += 1
+                    break
+        return n
+
+    def recall(raw):
+        got = {(int(c["t"]), int(c["k"])) for c in raw}
+        return len(got & fullset) / max(1, len(fullset))
+
+    eng.set_mode("plain")
+    eng.search_raw(r_lo, r_hi, 8.0)
+    t0 = time.perf_counter()
+    eng.search_raw(r_lo, r_hi, 8.0)
+    t_plain = time.perf_counter() - t0
+    curve = []
+    for q in (0.005, 0.01, 0.02, 0.05, 0.1, 0.2):
+        eng.set_mode("fused")
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            X = eng.triage_features(r_lo, r_hi)
+            mask = model.select(model.logits(eng, X), q)
+            raw = eng.search_blocks(r_lo, r_hi, 8.0, mask)
+            ts.append(time.perf_counter() - t0)
+        tq = sorted(ts)[1]
+        # the plain FP32 search with the same time: the blocks it reaches in band order
+        kb = max(1, min(nchunks, int(nchunks * tq / t_plain)))
+        pmask = np.zeros(nchunks, np.uint8)
+        pmask[:kb] = 1
+        eng.set_mode("plain")
+        t0 = time.perf_counter()
+        praw = eng.search_blocks(r_lo, r_hi, 8.0, pmask)
+        tp = time.perf_counter() - t0
+        curve.append({"blocks_fraction": q, "triage_ms": round(1e3 * tq, 3),
+                      "triage_recall": round(recall(raw), 4),
+                      "triage_pulsars": f"{pulsars_found(raw)}/{len(cfg['pulsars'])}",
+                      "plain_same_budget_blocks_fraction": round(kb / nchunks, 4),
+                      "plain_same_budget_ms": round(1e3 * tp, 3),
+                      "plain_same_budget_recall": round(recall(praw), 4),
+                      "plain_same_budget_pulsars": f"{pulsars_found(praw)}/{len(cfg['pulsars'])}"})
+        log(f"[bench] triage q={q}: {curve[-1]}")
+    X = eng.triage_features(r_lo, r_hi)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        eng.triage_features(r_lo, r_hi)
+    t_feat = (time.perf_counter() - t0) / 5
+    line = {
+        "metric": "block-triage recall at equal time (2^25 pts nharms=60)", "value": curve[2]["triage_recall"],
+        "unit": "recall of the full search's raw candidates at 2% of the blocks",
+        "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+        "config": {"workload": "train: 2^23-sample spectrum, 40 pulsars (seed 2101), labels from its full "
+                               "fused search; evaluate: the 2^25-sample nharms=60 workload (seed 2001)",
+                   "name": "triage", "blocks": nchunks, "train_blocks": len(Xtr),
+                   "train_positive_blocks": int(ytr.sum())},
+        "full_fused_ms": round(1e3 * t_full, 3), "full_plain_ms": round(1e3 * t_plain, 3),
+        "features_ms": round(1e3 * t_feat, 3), "fit_s": round(fit_s, 2),
+        "train_loss": [round(hist[0], 4), round(hist[-1], 4)],
+        "full_raw_candidates": len(full), "curve": curve,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,7 +308,7 @@ def main():
                     help="searches timed in the plain FP32 mode for the ratio (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5", "triage"])
     ap.add_argument("--c3-files", type=int, default=64)
     ap.add_argument("--autotune", action="store_true",
                     help="run cs_autotune before timing (the default launch shape is the one "
@@ -210,6 +327,9 @@ def main():
 
     if args.config == "c3":
         run_c3(args, cs, args.c3_files)
+        return
+    if args.config == "triage":
+        run_triage(args, cs)
         return
     cfg = {"c1": synth.config_c1, "c2": synth.config_c2, "c5": synth.config_c5}[args.config]()
     log(f"[bench] generating the 2^{int(math.log2(cfg['N']))}-sample spectrum (seed {cfg['seed']})")
