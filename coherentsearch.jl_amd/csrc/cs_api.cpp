@@ -64,6 +64,10 @@ struct cs_ctx {
     DBuf bandtab;   // block_metrics(:fft): per-harmonic lobin, then fine-grid length
     DBuf pyr, tfeat, mlp_w, mlp_x, mlp_p;   // block triage: power pyramid, features, MLP buffers
     DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol;
+    // split-BF16 batches: identity list (u32 i at i, m_ident_n entries), and
+    // for a block subset its block list, slot table and per-batch tile lists
+    DBuf m_ident, m_sel, m_slot, m_tiles;
+    int64_t m_ident_n = 0;
     DBuf spl;                  // pre-split spectrum planes of the MFMA mode
     bool spl_valid = false;    // spl matches the current spectrum
     int ncu = 0;               // compute units of the device
@@ -305,7 +309,8 @@ int cs_ctx_destroy(cs_ctx* c) {
                     &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples, &c->bandtab,
                     &c->pyr, &c->tfeat, &c->mlp_w, &c->mlp_x, &c->mlp_p,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
-                    &c->m_desc, &c->m_rowmask, &c->m_scol, &c->spl, &c->slot[0], &c->slot[1]};
+                    &c->m_desc, &c->m_rowmask, &c->m_scol, &c->m_ident, &c->m_sel, &c->m_slot,
+                    &c->m_tiles, &c->spl, &c->slot[0], &c->slot[1]};
     for (DBuf* b : bufs) release(*b);
     for (void* p : c->pin)
         if (p) (void)hipHostFree(p);
@@ -828,16 +833,52 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     if (mask && nmask != nchunks)
         return c->fail(CS_EDIM, "block mask has " + std::to_string(nmask) + " entries for " +
                                     std::to_string(nchunks) + " blocks");
-    std::vector<std::pair<int64_t, int64_t>> batches;   // (first block, blocks)
-    for (int64_t c0 = 0; c0 < nchunks;) {
-        if (mask && !mask[c0]) {
-            ++c0;
-            continue;
+    // batches: runs of consecutive selected blocks (first block, blocks);
+    // on the split-BF16 path, groups of up to Bc selected blocks (first
+    // selection position, blocks) with the tiles touching them
+    std::vector<std::pair<int64_t, int64_t>> batches;
+    std::vector<int32_t> selv;                 // selected blocks (mask given)
+    std::vector<std::pair<int64_t, int64_t>> tile_rng;   // per batch: (list offset, tiles)
+    std::vector<uint32_t> tiles;               // tile lists of a block subset
+    if (mfma) {
+        const int64_t sup = 8 * md.S;          // trials per super-tile (U tiles)
+        if (mask) {
+            for (int64_t cg = 0; cg < nchunks; ++cg)
+                if (mask[cg]) selv.push_back((int32_t)cg);
         }
-        int64_t e = c0;
-        while (e < nchunks && (!mask || mask[e]) && e - c0 < Bc) ++e;
-        batches.emplace_back(c0, e - c0);
-        c0 = e;
+        const int64_t nsel = mask ? (int64_t)selv.size() : nchunks;
+        for (int64_t p0 = 0; p0 < nsel; p0 += Bc) {
+            const int64_t nb = std::min(Bc, nsel - p0);
+            batches.emplace_back(p0, nb);
+            if (!mask) {   // a run of blocks: one range of the identity list
+                const int64_t lo = p0 * Nprof, hi = std::min((p0 + nb) * Nprof, total);
+                const int64_t st_lo = lo / sup, st_hi = (hi + sup - 1) / sup;
+                tile_rng.emplace_back(st_lo * md.U, (st_hi - st_lo) * md.U);
+                continue;
+            }
+            const int64_t off = (int64_t)tiles.size();
+            int64_t st_done = -1;              // super-tiles listed so far (ascending)
+            for (int64_t p = p0; p < p0 + nb; ++p) {
+                const int64_t lo = (int64_t)selv[p] * Nprof;
+                const int64_t hi = std::min(lo + Nprof, total);
+                for (int64_t st = std::max(lo / sup, st_done + 1); st <= (hi - 1) / sup; ++st) {
+                    for (int u = 0; u < md.U; ++u) tiles.push_back((uint32_t)(st * md.U + u));
+                    st_done = st;
+                }
+            }
+            tile_rng.emplace_back(off, (int64_t)tiles.size() - off);
+        }
+    } else {
+        for (int64_t c0 = 0; c0 < nchunks;) {
+            if (mask && !mask[c0]) {
+                ++c0;
+                continue;
+            }
+            int64_t e = c0;
+            while (e < nchunks && (!mask || mask[e]) && e - c0 < Bc) ++e;
+            batches.emplace_back(c0, e - c0);
+            c0 = e;
+        }
     }
     const int64_t nbatch = (int64_t)batches.size();
     const int nev = 6;
@@ -856,6 +897,21 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         // row masks of every block, and the fold and table descriptors, once
         // per search (the batches only differ in their by-value BatchDev)
         HIPCHK(c, launch_rowmask(c->st, sp, tb, g, nchunks, (uint64_t*)c->m_rowmask.p));
+        // selection tables: the identity list serves every block selected
+        const int64_t nid = std::max(nchunks, md.ntiles);
+        if (c->m_ident_n < nid) {
+            c->m_ident_n = 0;
+            if ((s = ensure(c, c->m_ident, (size_t)nid * sizeof(uint32_t)))) return s;
+            HIPCHK(c, launch_iota(c->st, (uint32_t*)c->m_ident.p, nid));
+            c->m_ident_n = nid;
+        }
+        if (mask) {
+            if ((s = upload(c, c->m_sel, selv))) return s;
+            if ((s = upload(c, c->m_tiles, tiles))) return s;
+            if ((s = ensure(c, c->m_slot, (size_t)nchunks * sizeof(int32_t)))) return s;
+            HIPCHK(c, launch_slot_table(c->st, (const int32_t*)c->m_sel.p, (int64_t)selv.size(),
+                                        nchunks, (int32_t*)c->m_slot.p));
+        }
         std::vector<uint8_t> desc(sizeof(FoldSet) + sizeof(MfmaDev));
         std::memcpy(desc.data(), &fs, sizeof(FoldSet));
         std::memcpy(desc.data() + sizeof(FoldSet), &md, sizeof(MfmaDev));
@@ -870,20 +926,21 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             const int64_t nb = batches[bi].second;
             const int64_t t_base = c0 * Nprof;
             const int64_t nt = std::min(nb * Nprof, total - t_base);
-            BatchDev b;
+            BatchDev b{};
             b.t_base = t_base;
             b.nt = nt;
             b.c_first = c0;
             b.nblocks = (int32_t)nb;
-            b.pad = 0;
-            b.nvalid = (const int32_t*)c->nvalid.p + c0 * nf;
+            b.nvalid = (const int32_t*)c->nvalid.p;
             b.sigma = (double*)c->sigma.p + c0 * nf;
-            b.tile0 = b.ntile = 0;
-            if (mfma) {   // super-tiles (8 occurrences, 8 S trials) touching the batch
-                const int64_t st_lo = t_base / (8 * md.S);
-                const int64_t st_hi = (t_base + nt + 8 * md.S - 1) / (8 * md.S);
-                b.tile0 = st_lo * md.U;
-                b.ntile = (st_hi - st_lo) * md.U;
+            if (mfma) {   // c0 is a selection position
+                b.t_base = b.nt = b.c_first = 0;
+                b.slot0 = (int32_t)c0;
+                const uint32_t* ident = (const uint32_t*)c->m_ident.p;
+                b.sel = mask ? (const int32_t*)c->m_sel.p : (const int32_t*)ident;
+                b.slot = mask ? (const int32_t*)c->m_slot.p : (const int32_t*)ident;
+                b.tiles = (mask ? (const uint32_t*)c->m_tiles.p : ident) + tile_rng[bi].first;
+                b.ntile = tile_rng[bi].second;
             }
             OutDev o;
             o.surv = (uint64_t*)c->surv.p;

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g,
     const int64_t j = t - first;
     for (int f = wv; f < fs.nf; f += 4) {
         const FoldDev& fd = fs.f[f];
-        const int64_t nv = valid ? b.nvalid[cl * fs.nf + f] : 0;
+        const int64_t nv = valid ? b.nvalid[cg * fs.nf + f] : 0;
         float* samp = fu.samples + ((int64_t)cl * fs.nf + f) * fu.sld;
         float* mo = fu.mraw + (int64_t)f * fu.ldm + jl;
         const bool small = fd.widths[fd.nw - 1] <= 6;
@@ -237,9 +237,9 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
     __shared__ uint32_t hist[kHist];
     __shared__ uint32_t sh[64];
     const int cl = blockIdx.x, f = blockIdx.y, nf = fs.nf;
-    const int64_t cg = b.c_first + cl;
+    const int64_t cg = batch_block(b, cl);
     const int64_t first = g.origin + cg * g.Nprof;
-    const int64_t nv = b.nvalid[cl * nf + f];
+    const int64_t nv = b.nvalid[cg * nf + f];
     const int nbins = fs.f[f].nbins;
     const int64_t cap = fs.cap[f];
     const int64_t NN = (int64_t)nbins * nv;
@@ -305,7 +305,9 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
     // threshold the block's bounds: m0 = bound * invsigma against medcut
     const double invs = sigma > 0 ? 1.0 / sigma : 0.0;
     const bool all = !(o.medcut > -INFINITY) || invs <= 0;
-    const int64_t col0 = first - b.t_base;
+    // bound columns are [local block][trial of the block] (for a contiguous
+    // batch: the trial minus the batch's first trial)
+    const int64_t col0 = (int64_t)cl * g.Nprof;
     for (int64_t j0 = 0; j0 < nv; j0 += kST) {
         int64_t j = j0 + threadIdx.x;
         bool pred = false;

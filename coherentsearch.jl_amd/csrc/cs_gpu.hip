@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sigma32(FoldSet fs, GridDev g, 
     const int cl = blockIdx.x, f = blockIdx.y, nf = fs.nf;
     const int64_t cg = b.c_first + cl;
     const int64_t first = g.origin + cg * g.Nprof;
-    const int64_t nv = b.nvalid[cl * nf + f];
+    const int64_t nv = b.nvalid[cg * nf + f];
     const int nbins = fs.f[f].nbins;
     const int64_t cap = fs.cap[f];
     const int64_t NN = (int64_t)nbins * nv;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(256) void k_gate(const float* __restrict__ prof, in
         int64_t cg, first, last;
         block_of(g, t, cg, first, last);
         int cl = (int)(cg - b.c_first);
-        if (t - first < b.nvalid[cl * nf + fidx]) {
+        if (t - first < b.nvalid[cg * nf + fidx]) {
             double sig = b.sigma[cl * nf + fidx];
             double invs = sig > 0 ? 1.0 / sig : 0.0;
             if (!(o.medcut > -INFINITY) || invs <= 0) {
@@ -342,7 +342,7 @@ __global__ __launch_bounds__(64) void k_gate_generic(const float* __restrict__ p
         int64_t cg, first, last;
         block_of(g, t, cg, first, last);
         int cl = (int)(cg - b.c_first);
-        if (t - first < b.nvalid[cl * nf + fidx]) {
+        if (t - first < b.nvalid[cg * nf + fidx]) {
             double sig = b.sigma[cl * nf + fidx];
             double invs = sig > 0 ? 1.0 / sig : 0.0;
             if (!(o.medcut > -INFINITY) || invs <= 0) {
@@ -512,7 +512,7 @@ __global__ __launch_bounds__(64) void k_rescan(SpecDev sp, TabDev tb, GridDev g,
         const int k = fd.k, Hk = fd.Hk, NB = fd.nbins;
         int64_t cg, first, last;
         block_of(g, t, cg, first, last);
-        int cl = (int)(cg - b.c_first);
+        int cl = batch_slot(b, cg);
         double sig = b.sigma[cl * fs.nf + f];
         double invs = sig > 0 ? 1.0 / sig : 0.0;
         double metric = 0.0;

@@ -52,18 +52,36 @@ struct FoldSet {
     uint32_t bankmask[kMaxFolds];    // widths as a subset of the default bank
 };
 
-// Where a batch sits in the search.
+// Where a batch sits in the search.  A batch is either a run of consecutive
+// blocks (c_first .. c_first + nblocks - 1, trials t_base .. t_base + nt - 1)
+// or, on the split-BF16 path, a list of selected blocks: the search numbers
+// its selected blocks 0, 1, ... (slot[global block], -1 where unselected;
+// sel[position] = global block) and the batch holds positions slot0 ..
+// slot0 + nblocks - 1.  The split-BF16 path always passes slot/sel/tiles
+// (identity tables when every block is selected).
 struct BatchDev {
-    int64_t t_base;     // global trial of batch column 0
-    int64_t nt;         // trials in the batch
-    int64_t c_first;    // global block index of the batch's first block
+    int64_t t_base;     // global trial of batch column 0 (runs of blocks)
+    int64_t nt;         // trials in the batch (runs of blocks)
+    int64_t c_first;    // global block index of the batch's first block (runs)
     int32_t nblocks;    // blocks in the batch
-    int32_t pad;
-    const int32_t* nvalid;   // [block_local * nf + f]
-    double* sigma;           // [block_local * nf + f]
-    int64_t tile0;      // split-BF16 kernel: first 128-trial tile touching the batch
-    int64_t ntile;      //   and the number of tiles that do
+    int32_t slot0;      // selection position of local block 0
+    const int32_t* nvalid;   // [global block * nf + f]
+    double* sigma;           // [local block * nf + f]
+    const int32_t* sel;      // selection position -> global block, or null
+    const int32_t* slot;     // global block -> selection position or -1, or null
+    const uint32_t* tiles;   // split-BF16 kernel: the 128-trial tiles to run
+    int64_t ntile;           //   (tiles touching any block of the batch)
 };
+
+// global block of local block cl, and local block of global block cg (-1 if
+// the block is not in the batch)
+__host__ __device__ inline int64_t batch_block(const BatchDev& b, int cl) {
+    return b.sel ? (int64_t)b.sel[b.slot0 + cl] : b.c_first + cl;
+}
+__host__ __device__ inline int batch_slot(const BatchDev& b, int64_t cg) {
+    const int64_t cl = b.slot ? (int64_t)b.slot[cg] - b.slot0 : cg - b.c_first;
+    return (cl >= 0 && cl < b.nblocks) ? (int)cl : -1;
+}
 
 struct OutDev {
     uint64_t* surv;        // packed (t << 4) | fold

@@ -678,20 +678,21 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         dlo[i] = ldg4(md.cA_lo + i);
     }
 
-    // tile schedule: contiguous tile ranges per XCD (workgroups are dealt to
-    // XCDs round-robin), so tiles of neighbouring 16-trial groups, whose
-    // windows overlap, run together and share that XCD's L2
+    // tile schedule over the batch's tile list (ascending): contiguous list
+    // ranges per XCD (workgroups are dealt to XCDs round-robin), so tiles of
+    // neighbouring 16-trial groups, whose windows overlap, run together and
+    // share that XCD's L2.  `tile` below is a list position.
     int64_t t_first, t_end, t_step;
     {
         const int G = gridDim.x, nx = (G % 8 == 0) ? 8 : 1;
         if (nx > 1) {
             const int x = blockIdx.x % nx, l = blockIdx.x / nx;
-            t_first = b.tile0 + b.ntile * x / nx + l;
-            t_end = b.tile0 + b.ntile * (x + 1) / nx;
+            t_first = b.ntile * x / nx + l;
+            t_end = b.ntile * (x + 1) / nx;
             t_step = G / nx;
         } else {
-            t_first = b.tile0 + blockIdx.x;
-            t_end = b.tile0 + b.ntile;
+            t_first = blockIdx.x;
+            t_end = b.ntile;
             t_step = G;
         }
     }
@@ -717,12 +718,15 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const int32_t* scb = md.scol;
     const uint64_t* rmb = md.rowmask;
     const int4* hdb = md.hdesc;
-    asm volatile("" : "+v"(nvb), "+v"(scb), "+v"(rmb), "+v"(hdb));
+    const int32_t* slb = b.slot;
+    const uint32_t* tlb = b.tiles;
+    asm volatile("" : "+v"(nvb), "+v"(scb), "+v"(rmb), "+v"(hdb), "+v"(slb), "+v"(tlb));
     const int nfv = fs.nf;
     const uint32_t cg_last = (uint32_t)((g.total - 1 - g.origin) / g.Nprof);
     int64_t st = 0, tbase = 0, occ = 0, t = 0, jj = 0;
-    int u = 0, cl = 0;
-    bool inb = false;
+    int u = 0;
+    bool tin = false;   // trial inside the search
+    int32_t slotv = 0;  // selection position of the trial's block (-1: not selected)
     int32_t nvf[kMaxFolds], scf[kMaxFolds];
     uint64_t rmv[4];
     uint4 wa[8], wb[8];
@@ -734,32 +738,43 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         const int s = (lane >> 1) & 7, h = wv + 1 + 8 * s;
         return glb((const int*)hdb)[((size_t)((h <= nh ? h : nh) - 1) * U + uu) * 4 + (lane & 1)];
     };
-    int dnext = t_first < t_end ? load_desc(t_first) : 0;
+    // tile number at list position k (clamped to the last one: past the
+    // range a fetch reloads it), the same address in every lane
+    auto tile_at = [&](int64_t k) { return glb(tlb)[k < t_end ? k : t_end - 1]; };
+    // idc: tile number of the next position to fetch; idn: of the one after
+    // (in flight since the previous fetch)
+    uint32_t idc = 0, idn = 0;
+    int dnext = 0;
+    if (t_first < t_end) {
+        idc = __builtin_amdgcn_readfirstlane(tile_at(t_first));
+        dnext = load_desc(idc);
+        idn = tile_at(t_first + t_step);
+    }
     auto fetch = [&](int64_t tile) {
-        const uint32_t tl = (uint32_t)tile;
+        const uint32_t tl = idc;
         const uint32_t stu = tl / (uint32_t)U;
         st = stu;
         u = (int)(tl - stu * (uint32_t)U);
         const int dreg = dnext;
-        // unconditional (past the range it reloads the last tile's): a load
-        // under a branch leaves the merged wait state pending at once
-        dnext = load_desc(tile + t_step < t_end ? tile + t_step : tile);
+        // unconditional: a load under a branch leaves the merged wait state
+        // pending at once
+        idc = __builtin_amdgcn_readfirstlane(idn);
+        dnext = load_desc(idc);
+        idn = tile_at(tile + 2 * t_step);
         tbase = st * kOcc8 * S + 16 * (int64_t)u;
         occ = (int64_t)kOcc8 * st + o;
         t = tbase + (int64_t)wv * S + (lane & 15);
-        inb = t >= b.t_base && t < b.t_base + b.nt;   // trials of this batch only
-        jj = 0;
-        cl = 0;
-        if (inb) {
-            const uint32_t rel = (uint32_t)(t - g.origin);
-            const uint32_t cg = rel / np;
-            jj = rel - cg * np;
-            cl = (int)((int64_t)cg - b.c_first);
-        }
+        tin = t < g.total;
+        const uint32_t rel = (uint32_t)(t - g.origin);
+        const uint32_t cgr = rel / np;
+        const uint32_t cgc = cgr < cg_last ? cgr : cg_last;
+        jj = rel - cgr * np;
+        // whether the block is in this batch is decided where it is used
+        slotv = glb(slb)[cgc];
 #pragma unroll
         for (int f = 0; f < kMaxFolds; ++f) {
             const int ff = f < nfv ? f : nfv - 1;
-            nvf[f] = glb(nvb)[cl * nfv + ff];
+            nvf[f] = glb(nvb)[(size_t)cgc * nfv + ff];
             scf[f] = glb(scb)[(size_t)ff * np + jj];
         }
         {
@@ -877,6 +892,10 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             CS_LDS u32x4* segtab = opaque_lds((CS_LDS u32x4*)(smem + lay_seg) + wv * kMaxFolds);
             float bnd[kMaxFolds];
             int cursor = 0;
+            // the trial's block in this batch (local block cl)
+            const int clv = slotv - b.slot0;
+            const bool inb = tin && (uint32_t)clv < (uint32_t)b.nblocks;
+            const int cl = inb ? clv : 0;
 #pragma unroll
             for (int f = 0; f < kMaxFolds; ++f) {
                 bnd[f] = -INFINITY;
@@ -915,14 +934,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 cursor = end;
             }
             // next tile's loads go out before this tile's stores
-            const int64_t t_out = t;
-            const bool inb_out = inb;
+            // bound column: [local block][trial of the block]
+            const int64_t col_out = (int64_t)cl * np + jj;
             if (tile + t_step < t_end) fetch(tile + t_step);
             // outputs of the tile: bounds, then the staged sigma samples
-            if (inb_out && g4 == 0) {
+            if (inb && g4 == 0) {
 #pragma unroll
                 for (int f = 0; f < kMaxFolds; ++f)
-                    if (f < nf) fu.mraw[(int64_t)f * fu.ldm + (t_out - b.t_base)] = bnd[f];
+                    if (f < nf) fu.mraw[(int64_t)f * fu.ldm + col_out] = bnd[f];
             }
             for (int f = 0; f < nf; ++f) {
                 const u32x4 e = segtab[f];
@@ -997,6 +1016,39 @@ hipError_t launch_split_spectrum(hipStream_t st, const SpecDev& sp, uint32_t* sp
     const int64_t ld = sp.namps + 2 * kSpecPad;
     hipLaunchKernelGGL(k_split_spectrum, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, st,
                        sp.amps, sp.namps, ld, spl);
+    return hipGetLastError();
+}
+
+__global__ void k_iota(uint32_t* __restrict__ out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (uint32_t)i;
+}
+
+// slot[c] = position of block c in the ascending list sel, or -1
+__global__ void k_slot_table(const int32_t* __restrict__ sel, int64_t nsel, int64_t nblocks,
+                             int32_t* __restrict__ slot) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nblocks) return;
+    int64_t lo = 0, hi = nsel;   // first position with sel >= c
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (sel[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    slot[c] = (lo < nsel && sel[lo] == c) ? (int32_t)lo : -1;
+}
+
+hipError_t launch_iota(hipStream_t st, uint32_t* out, int64_t n) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iota, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_slot_table(hipStream_t st, const int32_t* sel, int64_t nsel, int64_t nblocks,
+                             int32_t* slot) {
+    if (nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slot_table, dim3((unsigned)((nblocks + 255) / 256)), dim3(256), 0, st, sel,
+                       nsel, nblocks, slot);
     return hipGetLastError();
 }
 

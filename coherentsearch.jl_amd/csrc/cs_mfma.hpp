@@ -53,6 +53,11 @@ hipError_t launch_split_spectrum(hipStream_t st, const SpecDev& sp, uint32_t* sp
 // Per-block harmonic row guards as bit masks (nharms <= 64).
 hipError_t launch_rowmask(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
                           int64_t nblocks, uint64_t* rowmask);
+// out[i] = i for i < n (identity block selection and tile lists).
+hipError_t launch_iota(hipStream_t st, uint32_t* out, int64_t n);
+// slot[c] = position of block c in the ascending block list sel[0, nsel), or -1.
+hipError_t launch_slot_table(hipStream_t st, const int32_t* sel, int64_t nsel, int64_t nblocks,
+                             int32_t* slot);
 // fs_dev and md_dev are device copies of fs and md; nwg persistent workgroups.
 hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
                              const FoldSet& fs, const BatchDev& b, const FusedDev& fu,
