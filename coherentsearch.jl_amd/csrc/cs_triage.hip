@@ -16,8 +16,11 @@
 //   [27]     mean_f Q(nharms, f)
 //   [28]     log2(max(rs, 1)) / 20
 //   [29]     fraction of f with S_nharms / nharms > 1.5
-//   [30]     [6] - [13]
-//   [31]     max_f P(f) / max([26], 1e-30)
+//   [30]     median_f P(f)                       (local noise level)
+//   [31]     [6] / max(median_f S_nharms / nharms, 1e-30)   (peak contrast)
+// The medians are the lower medians over the sampled fundamentals
+// f = floor(rs) + i * stride, stride = ceil(nfund / 512) (every fundamental
+// when a block has at most 512).
 #include <hip/hip_runtime.h>
 
 #include "cs_triage.hpp"
@@ -93,13 +96,44 @@ __device__ __forceinline__ float wave_max(float v) {
     return v;
 }
 
+constexpr int kMedSamples = 512;   // fundamentals sampled for the medians
+
+// lower median ((n - 1) / 2-th smallest) of n <= kMedSamples non-negative
+// floats in LDS, one wave; n wave-uniform.  Up to 64 values: the lane whose
+// rank (ties broken by position) is (n - 1) / 2.  More: the smallest key x with
+// #(keys <= x) > (n - 1) / 2, keys ordered as their bits.
+__device__ float wave_lower_median(const float* v, int n, int lane) {
+    if (n <= 0) return 0.0f;
+    const int k = (n - 1) / 2;
+    if (n <= 64) {
+        const float mine = lane < n ? v[lane] : 0.0f;
+        int rank = 0;
+        for (int j = 0; j < n; ++j) {
+            const float o = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine), j));
+            rank += (o < mine || (o == mine && j < lane)) ? 1 : 0;
+        }
+        const uint64_t hit = __ballot(lane < n && rank == k);
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(mine), __ffsll((long long)hit) - 1));
+    }
+    uint32_t lo = 0, hi = 0x7F800000u;   // +inf bounds every non-negative float
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        float cnt = 0.0f;
+        for (int i = lane; i < n; i += 64) cnt += (__float_as_uint(v[i]) <= mid) ? 1.0f : 0.0f;
+        if (wave_sum(cnt) > (float)k) hi = mid;
+        else lo = mid + 1;
+    }
+    return __uint_as_float(lo);
+}
+
 // one wave per search block; lanes over its integer fundamentals
 __global__ __launch_bounds__(256) void k_triage_features(PyrDev p, double r_lo, double lodr,
                                                          int64_t Nprof, int64_t total, int nh,
                                                          uint32_t foldmask, int64_t nblocks,
                                                          float* __restrict__ out) {
+    __shared__ float med_p[4][kMedSamples], med_s[4][kMedSamples];
     const int64_t blk = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     if (blk >= nblocks) return;
     const int64_t i0 = blk * Nprof;
     const int64_t n = min(Nprof, total - i0);
@@ -107,12 +141,14 @@ __global__ __launch_bounds__(256) void k_triage_features(PyrDev p, double r_lo, 
     const double re = rs + (double)(n - 1) * lodr;
     const int64_t f0 = (int64_t)floor(rs), f1 = (int64_t)floor(re);
     const int64_t nfund = f1 - f0 + 1;
+    const int64_t stride = (nfund + kMedSamples - 1) / kMedSamples;
+    const int nsamp = (int)((nfund + stride - 1) / stride);
     int Hs[7] = {1, 2, 4, 8, 16, 32, nh};
     for (int j = 0; j < 7; ++j) Hs[j] = min(Hs[j], nh);
     float smax[7], ssum[7], ssq[7], kmax[5];
     for (int j = 0; j < 7; ++j) smax[j] = ssum[j] = ssq[j] = 0.0f;
     for (int j = 0; j < 5; ++j) kmax[j] = 0.0f;
-    float pmean = 0.0f, pmax = 0.0f, qtop = 0.0f, over = 0.0f;
+    float pmean = 0.0f, qtop = 0.0f, over = 0.0f;
     for (int64_t base = 0; base < nfund; base += 64) {
         const int64_t idx = base + lane;
         const bool on = idx < nfund;
@@ -140,11 +176,19 @@ __global__ __launch_bounds__(256) void k_triage_features(PyrDev p, double r_lo, 
                 if ((foldmask >> k) & 1u) kmax[k - 2] = fmaxf(kmax[k - 2], Sk[k - 2] / (float)(nh / k));
             const float p1 = power_at(p, f);
             pmean += p1;
-            pmax = fmaxf(pmax, p1);
             qtop += qlast;
             over += (S[6] / (float)Hs[6] > 1.5f) ? 1.0f : 0.0f;
+            if (idx % stride == 0) {
+                med_p[wv][idx / stride] = p1;
+                med_s[wv][idx / stride] = S[6] / (float)Hs[6];
+            }
         }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const float pmed = wave_lower_median(med_p[wv], nsamp, lane);
+    const float smed = wave_lower_median(med_s[wv], nsamp, lane);
     const float inv = 1.0f / (float)nfund;
     float F[32];
     for (int j = 0; j < 7; ++j) {
@@ -159,8 +203,8 @@ __global__ __launch_bounds__(256) void k_triage_features(PyrDev p, double r_lo, 
     F[27] = wave_sum(qtop) * inv;
     F[28] = (float)(log2(fmax(rs, 1.0)) / 20.0);
     F[29] = wave_sum(over) * inv;
-    F[30] = F[6] - F[13];
-    F[31] = wave_max(pmax) / fmaxf(F[26], 1e-30f);
+    F[30] = pmed;
+    F[31] = F[6] / fmaxf(smed, 1e-30f);
     if (lane < 32) {
         float v = 0.0f;
         for (int j = 0; j < 32; ++j) v = (lane == j) ? F[j] : v;

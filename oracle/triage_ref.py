@@ -8,7 +8,8 @@ scorer are this project's own, so this module restates their definitions
   r_lo + t * lodr (blocks of Nprof trials), from |X|^2 and the max over the
   2^s-bin cells (s = floor(log2 g)) covering harmonic g's bin range
   [g f, g f + g - 1] of each integer fundamental f, evaluated here as a
-  plain max over the bins of those cells;
+  plain max over the bins of those cells; features 30 and 31 are lower
+  medians over every stride-th fundamental, stride = ceil(nfund / 512);
 * mlp_forward / mlp_grad: 32 -> 64 (ReLU) -> 1 logit and the gradient of
   the mean binary cross-entropy, in FP64, optionally with the operands
   rounded to bf16 where the device rounds them (inputs after
@@ -19,6 +20,7 @@ import math
 import numpy as np
 
 NF, NH = 32, 64
+MED_SAMPLES = 512   # fundamentals sampled (evenly strided) for the block medians
 NPARAMS = NH * NF + 2 * NH + 1
 
 
@@ -89,9 +91,16 @@ def block_features(amps, r_lo, lodr, Nprof, total, nharms, decimations):
         F[27] = qtop.mean()
         F[28] = math.log2(max(rs, 1.0)) / 20.0
         F[29] = np.mean(S[:, 6] > 1.5)
-        F[30] = F[6] - F[13]
-        F[31] = p1.max() / max(F[26], 1e-30)
+        stride = -(-len(fs) // MED_SAMPLES)
+        F[30] = lower_median(p1[::stride])
+        F[31] = F[6] / max(lower_median(S[::stride, 6]), 1e-30)
     return out
+
+
+def lower_median(v):
+    """the ((n - 1) // 2)-th smallest value (0 for no values)"""
+    v = np.sort(np.asarray(v, dtype=np.float64))
+    return float(v[(len(v) - 1) // 2]) if len(v) else 0.0
 
 
 def _std_inputs(X, norm, round_bf16):

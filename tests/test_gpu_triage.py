@@ -37,21 +37,24 @@ def _spectrum(cs, N, seed, pulsars=()):
     return make_spectrum(N, 64e-6, seed=seed, pulsars=list(pulsars))
 
 
-@pytest.mark.parametrize("nharms,decims", [(32, [1, 2, 3, 4]), (60, [1, 2, 3, 4, 5, 6]),
-                                           (8, [1])])
-def test_features_vs_reference(cs, T, nharms, decims):
-    _say(f"triage features nharms={nharms}")
+@pytest.mark.parametrize("nharms,decims,nprof", [(32, [1, 2, 3, 4], 256),
+                                                 (60, [1, 2, 3, 4, 5, 6], 256), (8, [1], 256),
+                                                 (8, [1], 16384)])
+def test_features_vs_reference(cs, T, nharms, decims, nprof):
+    """nprof 16384 at nharms 8: 1024 fundamentals per block, so the medians
+    sample every second one"""
+    _say(f"triage features nharms={nharms} nprof={nprof}")
     from coherentsearch.synth import Pulsar
     N = 1 << 16
     amps = _spectrum(cs, N, 5, [Pulsar(31.3, 0.05, 30.0)])
     p = cs.SearchParams(nharms=nharms, m=16, decimations=decims)
-    eng = cs.Engine(p, 256)
+    eng = cs.Engine(p, nprof)
     eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, 64e-6))
     lodr = 0.5 / nharms
-    r_lo, r_hi = 60.3, 60.3 + 256 * 5.5 * lodr
+    r_lo, r_hi = 60.3, 60.3 + nprof * 5.5 * lodr
     X = eng.triage_features(r_lo, r_hi)
     total = int(math.floor((r_hi - r_lo) / lodr)) + 1
-    want = T.block_features(amps, r_lo, lodr, 256, total, nharms, decims)
+    want = T.block_features(amps, r_lo, lodr, nprof, total, nharms, decims)
     assert X.shape == want.shape == (6, 32)
     np.testing.assert_allclose(X, want, rtol=2e-5, atol=2e-5)
 

@@ -81,3 +81,22 @@ def test_feature_scaling():
     assert np.allclose(F2[:, power], 4.0 * F1[:, power], rtol=1e-12, atol=0)
     assert np.allclose(F2[:, [28, 31]], F1[:, [28, 31]], rtol=1e-12, atol=0)
     assert np.all(F1[:, 25] == 0.0)   # fold 6 is not searched
+
+
+def test_lower_median_and_sampling():
+    rng = np.random.default_rng(4)
+    for n in (1, 2, 7, 64, 513):
+        v = rng.exponential(size=n)
+        want = np.median(v) if n % 2 else np.sort(v)[n // 2 - 1]
+        assert T.lower_median(v) == want
+    assert T.lower_median([]) == 0.0
+    # a block of 1024 fundamentals samples every second one for its medians
+    n = 1 << 15
+    amps = ((rng.standard_normal(n) + 1j * rng.standard_normal(n)) / np.sqrt(2)).astype(np.complex64)
+    lodr, Nprof = 1 / 16, 16384
+    F = T.block_features(amps, r_lo=40.0, lodr=lodr, Nprof=Nprof, total=Nprof, nharms=8,
+                         decimations=[1])
+    P = (amps.real.astype(np.float32) ** 2 + amps.imag.astype(np.float32) ** 2).astype(np.float64)
+    fs = np.arange(40, 40 + 1024)
+    assert F[0, 30] == np.sort(P[fs[::2]])[255]
+    assert F[0, 30] != np.sort(P[fs])[511]   # the full median differs on this seed
