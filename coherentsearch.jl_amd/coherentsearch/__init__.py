@@ -2,6 +2,8 @@
 CoherentSearch.jl (Eqn-30 interpolation, harmonic stack with decimation
 folding, batched c2r, boxcar matched-filter S/N) on gfx950, behind the C ABI of
 include/coherent_search.h."""
+from .candidate import (candidate_profile, measure_ducy, output_path, report_candidates,
+                        rotate_to_peak, write_candidates)
 from .dedup import harmonically_related, remove_duplicates, remove_harmonics
 from .engine import (Engine, SearchCache, block_metrics, chunk_metrics, raw_to_candidates,
                      search, search_block, search_files)
@@ -11,7 +13,9 @@ from .params import (Candidate, SearchParams, boxcar_widths, candidate_r, decima
 
 __all__ = [
     "Candidate", "Engine", "FFTFile", "SearchCache", "SearchParams", "SimpleInf",
-    "block_metrics", "boxcar_widths", "candidate_r", "chunk_metrics", "decimation_set",
+    "block_metrics", "boxcar_widths", "candidate_profile", "candidate_r", "chunk_metrics",
+    "decimation_set", "measure_ducy", "output_path", "report_candidates", "rotate_to_peak",
+    "write_candidates",
     "freqs", "harmonically_related", "ladder_boxcar_widths", "proftype", "raw_to_candidates",
     "remove_duplicates", "remove_harmonics", "search", "search_block", "search_files",
     "trial_grid",

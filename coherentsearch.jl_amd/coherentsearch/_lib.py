@@ -102,6 +102,8 @@ def lib():
                                    C.POINTER(C.c_uint8), C.c_int64, C.POINTER(CsCand), C.c_int64,
                                    i64p]
     L.cs_chunk_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_double, C.c_int32, dp]
+    L.cs_candidate_profiles.argtypes = [vp, dp, i32p, C.c_int64, C.c_int32, dp, C.c_int64, i32p]
+    L.cs_measure_ducy.argtypes = [vp, dp, i32p, C.c_int64, C.c_int32, dp]
     L.cs_decim_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, C.c_int32, dp, i64p]
     L.cs_interp_rows.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, C.c_int32, dp]
     L.cs_set_mode.argtypes = [vp, C.c_int32]

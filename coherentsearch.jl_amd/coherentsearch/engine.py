@@ -25,6 +25,10 @@ def _dp(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
+def _i32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
 def _fp(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
@@ -233,6 +237,37 @@ class Engine:
         self._chk(_lib.lib().cs_block_metrics(self._h, _dp(r), len(r),
                                               1 if kernel == "direct" else 0, wp, nw, _dp(out)),
                   "block_metrics")
+        return out
+
+    def candidate_profiles(self, r: Sequence[float], nharm: Sequence[int], m: int = 64):
+        """candidate_profile (src/candidate.jl:37-61) for every (r, nharm):
+        list of unrotated FP64 profiles of 2H bins (H harmonics below Nyquist)."""
+        rv = np.ascontiguousarray(np.asarray(r, dtype=np.float64))
+        hv = np.ascontiguousarray(np.asarray(nharm, dtype=np.int32))
+        if len(rv) != len(hv):
+            raise ValueError("r and nharm differ in length")
+        n = len(rv)
+        if n == 0:
+            return []
+        ld = max(2, 2 * int(hv.max()))
+        prof = np.zeros((n, ld))
+        ln = np.zeros(n, dtype=np.int32)
+        self._chk(_lib.lib().cs_candidate_profiles(self._h, _dp(rv), _i32p(hv), n, int(m),
+                                                   _dp(prof), ld, _i32p(ln)),
+                  "candidate_profile")
+        return [prof[i, :ln[i]].copy() for i in range(n)]
+
+    def measure_ducy(self, r: Sequence[float], nharm: Sequence[int], m: int = 64) -> np.ndarray:
+        """measure_ducy's duty cycles (src/candidate.jl:82-100): best boxcar
+        width / profile bins, NaN unless nharm >= 1 and r > 0."""
+        rv = np.ascontiguousarray(np.asarray(r, dtype=np.float64))
+        hv = np.ascontiguousarray(np.asarray(nharm, dtype=np.int32))
+        if len(rv) != len(hv):
+            raise ValueError("r and nharm differ in length")
+        out = np.full(len(rv), np.nan)
+        if len(rv):
+            self._chk(_lib.lib().cs_measure_ducy(self._h, _dp(rv), _i32p(hv), len(rv), int(m),
+                                                 _dp(out)), "measure_ducy")
         return out
 
     def interp_rows(self, r_lo: float, t0: int, n: int, weights_f64: bool = False) -> np.ndarray:

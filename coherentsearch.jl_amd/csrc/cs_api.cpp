@@ -62,6 +62,7 @@ struct cs_ctx {
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
     DBuf bandtab;   // block_metrics(:fft): per-harmonic lobin, then fine-grid length
+    DBuf cand_r, cand_h, cand_w, cand_prof;   // candidate profiles: inputs, widths, outputs
     DBuf pyr, tfeat, mlp_w, mlp_x, mlp_p;   // block triage: power pyramid, features, MLP buffers
     DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol;
     // split-BF16 batches: identity list (u32 i at i, m_ident_n entries), and
@@ -307,6 +308,7 @@ int cs_ctx_destroy(cs_ctx* c) {
     DBuf* bufs[] = {&c->amps, &c->harm, &c->row, &c->carry, &c->W32, &c->A32, &c->W64, &c->A64,
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
                     &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples, &c->bandtab,
+                    &c->cand_r, &c->cand_h, &c->cand_w, &c->cand_prof,
                     &c->pyr, &c->tfeat, &c->mlp_w, &c->mlp_x, &c->mlp_p,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
                     &c->m_desc, &c->m_rowmask, &c->m_scol, &c->m_ident, &c->m_sel, &c->m_slot,
@@ -639,6 +641,101 @@ int cs_block_metrics(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel,
     if (s) return s;
     for (int64_t j = 0; j < L; ++j)
         if (std::isnan(out[j])) return c->fail(CS_EDIM, "an interpolation window falls outside the amplitudes");
+    return CS_OK;
+}
+
+// Candidate profiles and best widths (candidate.jl:37-61, 82-100).  H per
+// candidate follows candidate_profile's stopping rule in FP64 on the host;
+// skip[i] marks candidates measure_ducy leaves at NaN (H = -1 on the device).
+static int cand_profiles(cs_ctx* c, const double* r, const int32_t* nharm, int64_t n, int32_t m,
+                         bool ducy_rules, double* prof, int64_t ld, int32_t* len,
+                         std::vector<int32_t>& bw) {
+    if (m % 2 != 0 || m < 2) return c->fail(CS_EINVAL, "m must be even");
+    int s = check_spectrum(c);
+    if (s) return s;
+    HIPCHK(c, hipSetDevice(c->device));
+    const int64_t m2 = m / 2, Nhalf = c->N / 2;
+    std::vector<int32_t> H((size_t)n), woff((size_t)n + 1, 0), wtab;
+    std::map<int, std::pair<int32_t, int32_t>> banks;   // profile bins -> (offset, count)
+    for (int64_t i = 0; i < n; ++i) {
+        if (ducy_rules ? !(nharm[i] >= 1 && r[i] > 0.0) : false) {
+            H[i] = -1;
+            woff[i + 1] = woff[i];
+            continue;
+        }
+        if (nharm[i] < 1) return c->fail(CS_EINVAL, "nharm must be >= 1");
+        int32_t h = 0;
+        for (int32_t g = 1; g <= nharm[i]; ++g) {
+            const double rh = r[i] * (double)g;
+            const int64_t rint = (int64_t)std::floor(rh + 1e-15) + 1;
+            if (!(rint - m2 >= 0 && rint + m2 <= c->namps && rh < (double)Nhalf)) break;
+            h = g;
+        }
+        if (h > kMaxCandH)
+            return c->fail(CS_EINVAL, "candidate profiles are limited to " +
+                                          std::to_string(kMaxCandH) + " harmonics");
+        H[i] = h;
+        const int nb = h >= 1 ? 2 * h : 2;
+        if (prof && ld < nb) return c->fail(CS_EDIM, "profile row stride too small");
+        auto it = banks.find(nb);
+        if (it == banks.end()) {
+            std::vector<int> ws = boxcar_widths(nb, c->fsp, c->maxfrac);
+            if (ws.empty() || ws.back() > nb)
+                return c->fail(CS_EINVAL, "boxcar width exceeds the profile length");
+            it = banks.emplace(nb, std::make_pair((int32_t)wtab.size(), (int32_t)ws.size())).first;
+            wtab.insert(wtab.end(), ws.begin(), ws.end());
+        }
+        woff[i + 1] = woff[i] + it->second.second;
+        if (len) len[i] = nb;
+    }
+    // per-candidate width lists (each candidate's bank copied in order)
+    std::vector<int32_t> wcat;
+    wcat.reserve((size_t)woff[n]);
+    for (int64_t i = 0; i < n; ++i) {
+        if (H[i] < 0) continue;
+        const auto& b = banks[H[i] >= 1 ? 2 * H[i] : 2];
+        wcat.insert(wcat.end(), wtab.begin() + b.first, wtab.begin() + b.first + b.second);
+    }
+    if (wcat.empty()) wcat.push_back(1);
+    std::vector<double> rv(r, r + n);
+    if ((s = upload(c, c->cand_r, rv))) return s;
+    std::vector<int32_t> hw(H);
+    hw.insert(hw.end(), woff.begin(), woff.end());
+    hw.insert(hw.end(), wcat.begin(), wcat.end());
+    if ((s = upload(c, c->cand_h, hw))) return s;
+    if ((s = ensure(c, c->cand_w, (size_t)n * sizeof(int32_t)))) return s;
+    if (prof && (s = ensure(c, c->cand_prof, (size_t)n * ld * sizeof(double)))) return s;
+    const int32_t* hd = (const int32_t*)c->cand_h.p;
+    HIPCHK(c, launch_cand_profiles(c->st, spec_of(c), (const double*)c->cand_r.p, hd, n, m,
+                                   hd + 2 * n + 1, hd + n, prof ? (double*)c->cand_prof.p : nullptr,
+                                   ld, (int32_t*)c->cand_w.p));
+    bw.resize((size_t)n);
+    HIPCHK(c, hipMemcpyAsync(bw.data(), c->cand_w.p, (size_t)n * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, c->st));
+    if (prof)
+        HIPCHK(c, hipMemcpyAsync(prof, c->cand_prof.p, (size_t)n * ld * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
+}
+
+int cs_candidate_profiles(cs_ctx* c, const double* r, const int32_t* nharm, int64_t n, int32_t m,
+                          double* prof, int64_t ld, int32_t* len) {
+    if (!c || (n > 0 && (!r || !nharm))) return CS_EINVAL;
+    if (n <= 0) return CS_OK;
+    std::vector<int32_t> bw;
+    return cand_profiles(c, r, nharm, n, m, false, prof, ld, len, bw);
+}
+
+int cs_measure_ducy(cs_ctx* c, const double* r, const int32_t* nharm, int64_t n, int32_t m,
+                    double* ducy) {
+    if (!c || (n > 0 && (!r || !nharm || !ducy))) return CS_EINVAL;
+    if (n <= 0) return CS_OK;
+    std::vector<int32_t> len((size_t)n, 0), bw;
+    int s = cand_profiles(c, r, nharm, n, m, true, nullptr, 0, len.data(), bw);
+    if (s) return s;
+    for (int64_t i = 0; i < n; ++i)
+        ducy[i] = len[i] > 0 ? (double)bw[i] / (double)len[i] : NAN;
     return CS_OK;
 }
 

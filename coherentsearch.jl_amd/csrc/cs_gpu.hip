@@ -816,4 +816,122 @@ hipError_t launch_point_interp_f64(hipStream_t st, const SpecDev& sp, const doub
     return hipGetLastError();
 }
 
+// Candidate profiles (candidate_profile, candidate.jl:37-61) and their best
+// boxcar width (boxcar_best_width, search.jl:617-638), one 64-lane group per
+// candidate: the first H harmonics at r h by FP64 fourier_interp with an m-bin
+// kernel, a zero DC term, the FP64 real inverse DFT into 2H bins scaled by
+// 1/(2H); H = 0 gives the two-bin zero profile.  Then, over the fold's width
+// bank, the median-baseline circular boxcar maximum, the narrowest width
+// winning ties (strict >, ascending widths).
+__global__ __launch_bounds__(64) void k_cand_profiles(SpecDev sp, const double* __restrict__ r,
+                                                      const int32_t* __restrict__ H, int64_t n,
+                                                      int m, const int32_t* __restrict__ wtab,
+                                                      const int32_t* __restrict__ woff,
+                                                      double* __restrict__ prof_out, int64_t ld,
+                                                      int32_t* __restrict__ best_w) {
+    __shared__ double2 X[kMaxCandH + 1];
+    __shared__ double2 tw[2 * kMaxCandH];
+    __shared__ double prof[2 * kMaxCandH];
+    __shared__ double srt[2 * kMaxCandH];
+    __shared__ double ps[4 * kMaxCandH + 2];
+    const int lane = threadIdx.x;
+    for (int64_t c = blockIdx.x; c < n; c += gridDim.x) {
+        const int Hc = H[c];
+        if (Hc < 0) {   // not measured
+            if (lane == 0) best_w[c] = 0;
+            continue;
+        }
+        const int NB = Hc >= 1 ? 2 * Hc : 2;
+        const double rc = r[c];
+        for (int h = lane; h <= Hc; h += 64) {
+            double2 v = make_double2(0.0, 0.0);
+            if (h >= 1) {   // fourier_interp(r h, amps, m), fourierinterp.jl:66-74
+                const double rh = rc * (double)h;
+                const double dr = rh - floor(rh);
+                const int64_t lo0 = (int64_t)floor(rh + 1e-15) + 1 - m / 2;
+                double sr = 0.0, si = 0.0;
+                for (int i = 0; i < m; ++i) {
+                    const double off = dr - (double)(-(m / 2) + 1 + i);
+                    const double s = sinc_d(off);
+                    const double cr = s * cospi(off), ci = s * sinpi(off);
+                    const float2 bv = sp.amps[lo0 + i];
+                    sr += cr * (double)bv.x + ci * (double)bv.y;
+                    si += cr * (double)bv.y - ci * (double)bv.x;
+                }
+                v = make_double2(sr, si);
+            }
+            X[h] = v;
+        }
+        for (int e = lane; e < NB; e += 64) {
+            double sn, cs_;
+            sincospi(2.0 * (double)e / (double)NB, &sn, &cs_);
+            tw[e] = make_double2(cs_, sn);
+        }
+        __syncthreads();
+        if (Hc >= 1) {
+            group_c2r_f64(X, NB, tw, prof);
+            const double sc = 1.0 / (double)NB;
+            for (int i = lane; i < NB; i += 64) prof[i] *= sc;
+        } else {
+            for (int i = lane; i < NB; i += 64) prof[i] = 0.0;
+        }
+        __syncthreads();
+        if (prof_out)
+            for (int i = lane; i < NB; i += 64) prof_out[c * ld + i] = prof[i];
+        // boxcar_best_width: median baseline, circular prefix sums
+        for (int i = lane; i < NB; i += 64) {
+            const double v = prof[i];
+            int rk = 0;
+            for (int q = 0; q < NB; ++q) {
+                const double u = prof[q];
+                rk += (u < v) || (u == v && q < i);
+            }
+            srt[rk] = v;
+        }
+        __syncthreads();
+        const double med = (NB & 1) ? srt[NB / 2] : 0.5 * (srt[NB / 2 - 1] + srt[NB / 2]);
+        const int32_t* ws = wtab + woff[c];
+        const int nw = woff[c + 1] - woff[c];
+        const int wmax = ws[nw - 1];
+        if (lane == 0) {
+            ps[0] = 0.0;
+            for (int i = 1; i <= NB + wmax; ++i) {
+                const int idx = i > NB ? i - NB : i;
+                ps[i] = ps[i - 1] + (prof[idx - 1] - med);
+            }
+        }
+        __syncthreads();
+        double best = -INFINITY;
+        int bw = ws[0];
+        for (int a = 0; a < nw; ++a) {
+            const int w = ws[a];
+            const double invsw = 1.0 / sqrt((double)w);
+            double mx = -INFINITY;
+            for (int p = lane; p < NB; p += 64) {
+                const double d = ps[p + w] - ps[p];
+                mx = d > mx ? d : mx;
+            }
+            mx = wave_max(mx);
+            const double cand = mx * invsw;
+            if (cand > best) {
+                best = cand;
+                bw = w;
+            }
+        }
+        if (lane == 0) best_w[c] = bw;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_cand_profiles(hipStream_t st, const SpecDev& sp, const double* r,
+                                const int32_t* H, int64_t n, int m, const int32_t* wtab,
+                                const int32_t* woff, double* prof_out, int64_t ld,
+                                int32_t* best_w) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)(n < 65536 ? n : 65536);
+    hipLaunchKernelGGL(k_cand_profiles, dim3(grid), dim3(64), 0, st, sp, r, H, n, m, wtab, woff,
+                       prof_out, ld, best_w);
+    return hipGetLastError();
+}
+
 }  // namespace cs

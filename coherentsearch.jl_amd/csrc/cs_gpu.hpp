@@ -137,5 +137,13 @@ hipError_t launch_point_interp_fft(hipStream_t st, const SpecDev& sp, const doub
                                    int64_t L, int nharms, int m, int nb, const int64_t* lob,
                                    const int64_t* kgrid, const uint8_t* rowok, double2* out,
                                    int64_t ld);
+// candidate profiles: H[c] harmonics (-1: skip), the width bank of candidate c
+// at wtab[woff[c] .. woff[c+1]) (widths <= 2 H[c]); profiles to prof_out[c *
+// ld ..] when non-null, best widths to best_w
+constexpr int kMaxCandH = 512;
+hipError_t launch_cand_profiles(hipStream_t st, const SpecDev& sp, const double* r,
+                                const int32_t* H, int64_t n, int m, const int32_t* wtab,
+                                const int32_t* woff, double* prof_out, int64_t ld,
+                                int32_t* best_w);
 
 }  // namespace cs

@@ -48,7 +48,6 @@ struct BoolC {
 };
 
 constexpr int kOcc8 = 8;          // occurrences per tile
-constexpr int kWT = 128;          // trials per tile
 constexpr int kThreads = 512;     // 8 waves
 constexpr int kStageFloats = 768;   // per-wave LDS staging of sampled columns
 

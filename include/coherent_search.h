@@ -153,6 +153,22 @@ int cs_block_metrics(cs_ctx *ctx, const double *rfund, int64_t L,
                      int32_t kernel, const int32_t *widths, int32_t nwidths,
                      double *out);
 
+/* candidate_profile(ft, r, nharm; m) (src/candidate.jl:37-61) for n
+ * candidates: harmonics h = 1, 2, ... of r[i] by FP64 fourier_interp with an
+ * m-bin kernel, stopping at the first whose window leaves the amplitudes or
+ * whose r h reaches N/2; the H used give a 2H-bin irfft profile (2 zero bins
+ * when H = 0), not rotated.  prof (optional) receives row i at prof[i * ld],
+ * len[i] its length (ld >= 2 * nharm[i]).  CS_EINVAL for nharm < 1, odd m,
+ * or more than 512 harmonics. */
+int cs_candidate_profiles(cs_ctx *ctx, const double *r, const int32_t *nharm, int64_t n,
+                          int32_t m, double *prof, int64_t ld, int32_t *len);
+/* measure_ducy (src/candidate.jl:82-100): ducy[i] = best boxcar width /
+ * profile bins of candidate_profile(r[i], nharm[i]; m), over
+ * boxcar_widths(bins; params' fsp and maxfrac) (boxcar_best_width,
+ * src/search.jl:617-638); NaN unless nharm[i] >= 1 and r[i] > 0. */
+int cs_measure_ducy(cs_ctx *ctx, const double *r, const int32_t *nharm, int64_t n, int32_t m,
+                    double *ducy);
+
 /* chunk_metrics(ft, params, rstart, n; lodr, weights) (src/search.jl:1740-1755):
  * the production machinery over one block at t0 = 0 with the plans anchored
  * at rstart, exact scan, k = 1.  `lodr` is accepted for signature parity and,

@@ -469,3 +469,38 @@ def search_region(params, amps, N, r_lo, total, Nprof=2048, c_begin=0, c_end=Non
         break
     arr = np.frombuffer(out, dtype=CAND_DTYPE, count=n.value).copy()
     return np.sort(arr, order=("t", "k"))
+
+
+def candidate_profile(amps, N, r, nharm, m=64):
+    """candidate_profile, candidate.jl:37-61: harmonics h = 1, 2, ... of r by
+    fourier_interp (m-bin kernel) until the first whose window [rint - m/2,
+    rint + m/2] leaves the amplitudes or whose r h reaches N/2 (rint =
+    floor(r h + 1e-15) + 1); irfft of [0, X_1 .. X_H] into 2H bins, or two zero
+    bins when H = 0."""
+    if nharm < 1:
+        raise ValueError("nharm must be >= 1")
+    if m % 2:
+        raise ValueError("m must be even")
+    a = np.asarray(amps, dtype=np.complex64).astype(np.complex128)
+    m2, nhalf = m // 2, int(N) // 2
+    X = []
+    for h in range(1, nharm + 1):
+        rh = r * h
+        rint = math.floor(rh + 1e-15) + 1
+        if not (rint - m2 >= 0 and rint + m2 <= len(a) and rh < nhalf):
+            break
+        X.append(fourier_interp(rh, a, m))
+    if not X:
+        return np.zeros(2)
+    stack = np.zeros(len(X) + 1, dtype=np.complex128)
+    stack[1:] = X
+    return np.fft.irfft(stack, 2 * len(X))
+
+
+def measure_ducy(amps, N, r, nharm, fsp=1.5, maxfrac=0.3, m=64):
+    """measure_ducy's duty cycle of one candidate, candidate.jl:82-100: NaN
+    unless nharm >= 1 and r > 0, else boxcar_best_width of its profile."""
+    if not (nharm >= 1 and r > 0):
+        return float("nan")
+    prof = candidate_profile(amps, N, r, nharm, m)
+    return boxcar_best_width(prof, fsp, maxfrac)[1]
