@@ -298,6 +298,58 @@ def run_triage(args, cs):
     print(json.dumps(line), flush=True)
 
 
+def run_cands(args, cs):
+    """Post-detection throughput: measure_ducy (candidate_profile at m = 64 plus
+    the best boxcar width) of 4096 candidates, fundamentals uniform over the
+    2^25-sample workload's band, nharm 60 (Nyquist-capped where needed), on the
+    device against the oracle on a bounded sample of the same candidates."""
+    import numpy as np
+
+    from coherentsearch import synth
+    from coherentsearch.synth import make_config_spectrum
+
+    import oracle as O
+
+    cfg = synth.config_c2()
+    amps = make_config_spectrum(cfg)
+    N, dt = cfg["N"], cfg["dt"]
+    T = N * dt
+    params = cs.SearchParams(nharms=60, m=16, decimations=cs.decimation_set(60, 6))
+    eng = cs.Engine(params, 2048, device=0)
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
+    n = 4096
+    rng = np.random.default_rng(7)
+    r = rng.uniform(cfg["lofreq"] * T, cfg["hifreq"] * T, n)
+    nh = np.full(n, 60, dtype=np.int32)
+    for _ in range(max(1, args.warmup)):
+        eng.measure_ducy(r, nh)
+    ts = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        d = eng.measure_ducy(r, nh)
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[len(ts) // 2]
+    # oracle on a bounded sample
+    k, t0, agree = 0, time.perf_counter(), 0
+    while k < n and time.perf_counter() - t0 < min(args.cpu_seconds, 10.0):
+        want = O.measure_ducy(amps, N, float(r[k]), 60, params.boxcar_fsp, params.boxcar_maxfrac)
+        agree += int(want == d[k])
+        k += 1
+    tc = time.perf_counter() - t0
+    line = {"metric": "candidate duty cycles/s (measure_ducy, nharm=60, m=64)",
+            "value": round(n / t, 1), "unit": "candidates/s", "n_gpus": 1, "steps": len(ts),
+            "warmup": args.warmup, "ms_per_step": round(1e3 * t, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "4096 candidates, fundamentals uniform over 0.1-125 Hz of the "
+                                   "2^25-sample spectrum (seed 2001), nharm 60, m 64",
+                       "name": "cands"},
+            "oracle_agreement": f"{agree}/{k}",
+            "cpu_baseline": {"value": round(k / tc, 1), "unit": "candidates/s", "cores": 1,
+                             "kind": "port", "sample": f"the first {k} of the candidates, "
+                                                       f"{tc:.1f} s wall"}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -308,7 +360,7 @@ def main():
                     help="searches timed in the plain FP32 mode for the ratio (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5", "triage"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5", "triage", "cands"])
     ap.add_argument("--c3-files", type=int, default=64)
     ap.add_argument("--autotune", action="store_true",
                     help="run cs_autotune before timing (the default launch shape is the one "
@@ -330,6 +382,9 @@ def main():
         return
     if args.config == "triage":
         run_triage(args, cs)
+        return
+    if args.config == "cands":
+        run_cands(args, cs)
         return
     cfg = {"c1": synth.config_c1, "c2": synth.config_c2, "c5": synth.config_c5}[args.config]()
     log(f"[bench] generating the 2^{int(math.log2(cfg['N']))}-sample spectrum (seed {cfg['seed']})")
