@@ -8,11 +8,16 @@ from .dedup import harmonically_related, remove_duplicates, remove_harmonics
 from .engine import (Engine, SearchCache, block_metrics, chunk_metrics, raw_to_candidates,
                      search, search_block, search_files)
 from .fileio import FFTFile, SimpleInf, freqs, write_fft
+from .metricstats import (BlockMetricStats, MetricHistogram, MetricNorm, MetricStats,
+                          build_metricnorm, hist_quantile, metricstats_summary,
+                          metricstats_windows)
 from .params import (Candidate, SearchParams, boxcar_widths, candidate_r, decimation_set,
                      ladder_boxcar_widths, proftype, trial_grid)
 
 __all__ = [
-    "Candidate", "Engine", "FFTFile", "SearchCache", "SearchParams", "SimpleInf",
+    "BlockMetricStats", "Candidate", "Engine", "FFTFile", "MetricHistogram", "MetricNorm",
+    "MetricStats", "SearchCache", "SearchParams", "SimpleInf", "build_metricnorm",
+    "hist_quantile", "metricstats_summary", "metricstats_windows",
     "block_metrics", "boxcar_widths", "candidate_profile", "candidate_r", "chunk_metrics",
     "decimation_set", "measure_ducy", "output_path", "report_candidates", "rotate_to_peak",
     "write_candidates",

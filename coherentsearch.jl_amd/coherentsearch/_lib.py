@@ -66,6 +66,20 @@ class CsFile(C.Structure):
 CAND_DTYPE = np.dtype([("t", "<i8"), ("k", "<i4"), ("nharm", "<i4"),
                        ("block", "<i8"), ("metric", "<f8")])
 
+
+class CsStatsCfg(C.Structure):
+    _fields_ = [("hist_lo", C.c_double), ("hist_hi", C.c_double), ("hist_nb", C.c_int32),
+                ("nwin", C.c_int32)]
+
+
+BLOCK_STAT_DTYPE = np.dtype([("block", "<i8"), ("k", "<i4"), ("Hk", "<i4"), ("nbins", "<i4"),
+                             ("pad", "<i4"), ("n", "<i8"), ("ngoodbins", "<f8"), ("flo", "<f8"),
+                             ("fhi", "<f8"), ("min", "<f8"), ("median", "<f8"), ("mean", "<f8"),
+                             ("std", "<f8"), ("max", "<f8")])
+HIST_ACC_DTYPE = np.dtype([("k", "<i4"), ("win", "<i4"), ("under", "<i8"), ("over", "<i8"),
+                           ("total", "<i8"), ("flo", "<f8"), ("fhi", "<f8"), ("sum", "<f8"),
+                           ("sumsq", "<f8"), ("vmin", "<f8"), ("vmax", "<f8")])
+
 _LIB = None
 
 
@@ -102,6 +116,13 @@ def lib():
                                    C.POINTER(C.c_uint8), C.c_int64, C.POINTER(CsCand), C.c_int64,
                                    i64p]
     L.cs_chunk_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_double, C.c_int32, dp]
+    L.cs_search_stats.argtypes = [vp, C.c_double, C.c_double, C.c_double,
+                                  C.POINTER(CsStatsCfg), C.POINTER(CsCand), C.c_int64, i64p]
+    L.cs_stats_blocks.argtypes = [vp, vp, C.c_int64, i64p]
+    L.cs_stats_hists.argtypes = [vp, i64p, vp, C.POINTER(C.c_int32)]
+    L.cs_stats_metrics.argtypes = [vp, C.c_int32, dp, C.c_int64]
+    L.cs_search_normalized.argtypes = [vp, C.c_double, dp, dp, C.POINTER(CsCand), C.c_int64,
+                                       i64p]
     L.cs_candidate_profiles.argtypes = [vp, dp, i32p, C.c_int64, C.c_int32, dp, C.c_int64, i32p]
     L.cs_measure_ducy.argtypes = [vp, dp, i32p, C.c_int64, C.c_int32, dp]
     L.cs_decim_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, C.c_int32, dp, i64p]

@@ -129,6 +129,60 @@ class Engine:
             self._chk(st, "cs_search")
             return np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
 
+    def _cands(self, call, what):
+        cap = 1 << 16
+        while True:
+            out = (_lib.CsCand * cap)()
+            n = C.c_int64()
+            st = call(out, cap, n)
+            if st == _lib.CS_ENOSPC and n.value > cap:
+                cap = n.value
+                continue
+            self._chk(st, what)
+            return np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
+
+    def search_stats(self, r_lo: float, r_hi: float, threshold: float, ms) -> np.ndarray:
+        """cs_search_stats: raw candidates as search_raw, with every trial's
+        metric exact; appends the block records and histograms to the
+        MetricStats `ms` (the collecting _search_region!, search.jl:1768-1894)."""
+        from .metricstats import fill_from_device
+        cfg = _lib.CsStatsCfg(float(ms.hist_lo), float(ms.hist_hi), int(ms.hist_nb), int(ms.nwin))
+        raw = self._cands(lambda out, cap, n: _lib.lib().cs_search_stats(
+            self._h, float(r_lo), float(r_hi), float(threshold), C.byref(cfg), out, cap,
+            C.byref(n)), "cs_search_stats")
+        nb = C.c_int64()
+        self._chk(_lib.lib().cs_stats_blocks(self._h, None, 0, C.byref(nb)), "cs_stats_blocks")
+        blocks = np.zeros(nb.value, dtype=_lib.BLOCK_STAT_DTYPE)
+        self._chk(_lib.lib().cs_stats_blocks(self._h, blocks.ctypes.data, nb.value, C.byref(nb)),
+                  "cs_stats_blocks")
+        nf = C.c_int32()
+        self._chk(_lib.lib().cs_stats_hists(self._h, None, None, C.byref(nf)), "cs_stats_hists")
+        counts = np.zeros((nf.value, ms.nwin, int(ms.hist_nb)), dtype=np.int64)
+        acc = np.zeros((nf.value, ms.nwin), dtype=_lib.HIST_ACC_DTYPE)
+        self._chk(_lib.lib().cs_stats_hists(self._h, counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            acc.ctypes.data, C.byref(nf)), "cs_stats_hists")
+        nh = self.params.nharms
+        fill_from_device(ms, blocks, counts, acc, {int(a["k"]): nh // int(a["k"]) for a in acc[:, 0]})
+        return raw
+
+    def stats_metrics(self, k: int, total: int) -> np.ndarray:
+        """The last search_stats call's metric of every trial of fold k (NaN
+        where the fold has no valid trial)."""
+        out = np.zeros(int(total))
+        self._chk(_lib.lib().cs_stats_metrics(self._h, int(k), _dp(out), int(total)),
+                  "cs_stats_metrics")
+        return out
+
+    def search_normalized(self, threshold: float, norm) -> np.ndarray:
+        """cs_search_normalized: the --normalize detection pass over the last
+        search_stats call's metrics, candidates carrying z."""
+        ks = sorted(norm.loc.keys())
+        loc = np.ascontiguousarray(np.concatenate([norm.loc[k] for k in ks]), dtype=np.float64)
+        scale = np.ascontiguousarray(np.concatenate([norm.scale[k] for k in ks]), dtype=np.float64)
+        return self._cands(lambda out, cap, n: _lib.lib().cs_search_normalized(
+            self._h, float(threshold), _dp(loc), _dp(scale), out, cap, C.byref(n)),
+            "cs_search_normalized")
+
     def search_files(self, files: Sequence[Tuple[FFTFile, float, float]],
                      threshold: float) -> List[np.ndarray]:
         """Raw candidates of several spectra, each (ft, r_lo, r_hi), from one
@@ -319,13 +373,14 @@ def search(ft: FFTFile, params: Optional[SearchParams] = None, lofreq: float = 0
     """search(ft, params; ...) (src/search.jl:2002-2111) on the GPU.
 
     ``wisdom``/``wisdom_file`` are FFTW plan caches with no device counterpart
-    and are accepted for signature parity.  ``metricstats`` and ``normalize``
-    (the opt-in diagnostics) are not provided by the device path."""
+    and are accepted for signature parity.  ``metricstats`` (a MetricStats)
+    collects the every-trial statistics; ``normalize`` runs the two-pass
+    search (search.jl:2076-2095): statistics, the MetricNorm model, then the
+    detection over the normalised metrics (kept on the device from pass 1)."""
+    from .metricstats import MetricStats, build_metricnorm
     params = SearchParams() if params is None else params
     if progress not in ("none", "text", "bar"):
         raise ValueError(f"progress must be :none, :text or :bar, got :{progress}")
-    if metricstats is not None or normalize:
-        raise NotImplementedError("metricstats / normalize are not provided by the device path")
     thr = params.threshold if threshold is None else float(threshold)
     r_lo, r_hi, lodr, total, Nprof, nchunks = trial_grid(ft.T, params, lofreq, hifreq, lobin,
                                                          blocksize)
@@ -335,7 +390,14 @@ def search(ft: FFTFile, params: Optional[SearchParams] = None, lofreq: float = 0
         Engine(params, Nprof, device)
     eng.set_mode(mode)
     eng.set_spectrum(ft)
-    raw = eng.search_raw(r_lo, r_hi, thr)
+    if normalize:
+        ms = metricstats if metricstats is not None else MetricStats()
+        eng.search_stats(r_lo, r_hi, math.inf, ms)
+        raw = eng.search_normalized(thr, build_metricnorm(ms))
+    elif metricstats is not None:
+        raw = eng.search_stats(r_lo, r_hi, thr, metricstats)
+    else:
+        raw = eng.search_raw(r_lo, r_hi, thr)
     cands = raw_to_candidates(raw, ft, r_lo, lodr, Nprof)
     if verbose:
         import sys

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <future>
 #include <map>
@@ -21,6 +22,7 @@
 #include "cs_mfma.hpp"
 #include "cs_gpu.hpp"
 #include "cs_plan.hpp"
+#include "cs_stats.hpp"
 #include "cs_triage.hpp"
 
 using namespace cs;
@@ -63,6 +65,21 @@ struct cs_ctx {
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
     DBuf bandtab;   // block_metrics(:fft): per-harmonic lobin, then fine-grid length
     DBuf cand_r, cand_h, cand_w, cand_prof;   // candidate profiles: inputs, widths, outputs
+    // metric statistics: every metric of the last statistics search
+    // ([fold][trial]), histograms, block records, windows, normalisation tables
+    bool stats_on = false;
+    DBuf st_metric, st_cnt, st_blk, st_win, st_norm;
+    struct {
+        bool valid = false;
+        int nf = 0, nwin = 0, nb = 0;
+        double lo = 0, hi = 0, r_lo = 0, lodr = 0, T = 0;
+        int64_t total = 0, Nprof = 0, nchunks = 0, N = 0;
+        std::vector<int> ks, Hks;            // folds in search order
+        std::vector<double> edges;           // [fold][nwin + 1]
+        std::vector<int32_t> win, nv;        // [fold][block], [block][fold]
+        std::vector<BlockStatDev> blk;       // [fold][block]
+        std::vector<uint32_t> cnt;           // [fold][window][bin]
+    } sh;
     DBuf pyr, tfeat, mlp_w, mlp_x, mlp_p;   // block triage: power pyramid, features, MLP buffers
     DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol;
     // split-BF16 batches: identity list (u32 i at i, m_ident_n entries), and
@@ -309,6 +326,7 @@ int cs_ctx_destroy(cs_ctx* c) {
                     &c->ftp, &c->prof, &c->nvalid, &c->sigma, &c->surv, &c->counters, &c->cands,
                     &c->widths, &c->rfund, &c->rowok, &c->mout, &c->mraw, &c->samples, &c->bandtab,
                     &c->cand_r, &c->cand_h, &c->cand_w, &c->cand_prof,
+                    &c->st_metric, &c->st_cnt, &c->st_blk, &c->st_win, &c->st_norm,
                     &c->pyr, &c->tfeat, &c->mlp_w, &c->mlp_x, &c->mlp_p,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
                     &c->m_desc, &c->m_rowmask, &c->m_scol, &c->m_ident, &c->m_sel, &c->m_slot,
@@ -758,8 +776,92 @@ int cs_plan_tables(const cs_params* p, double r_lo, int32_t h, int64_t* info, in
 
 // cs_search, or with a block mask (one byte per block, nonzero = search it)
 // only the selected blocks: the batches are the runs of selected blocks
+// Statistics request of cs_search_stats: histogram range and bins, windows.
+struct StatsReq {
+    double lo, hi;
+    int nb, nwin;
+};
+
+// _logedges, search.jl:1406-1411: nwin log-spaced window edges over [a, b]
+static std::vector<double> log_edges(double a, double b, int nwin) {
+    a = std::max(a, std::numeric_limits<double>::min());
+    b = std::max(b, a * (1 + 1e-9));
+    const double la = std::log10(a), lb = std::log10(b);
+    std::vector<double> e((size_t)nwin + 1);
+    for (int i = 0; i <= nwin; ++i) e[i] = std::pow(10.0, la + (lb - la) * i / nwin);
+    return e;
+}
+
+// _window_index, search.jl:1415-1416 (0-based): last edge <= f, clamped
+static int window_index(const double* e, int nwin, double f) {
+    int le = 0;
+    while (le <= nwin && e[le] <= f) ++le;
+    return std::min(std::max(le - 1, 0), nwin - 1);
+}
+
+// After a statistics search: windows of every (fold, block), the block
+// records and the (fold, window) histograms, copied to the host.
+static int finish_stats(cs_ctx* c, const StatsReq& rq, const FoldSet& fs, const std::vector<int>& ks,
+                        double r_lo, double r_hi, double lodr, int64_t total, int64_t Nprof,
+                        int64_t nchunks) {
+    auto& sh = c->sh;
+    const int nf = fs.nf, nwin = rq.nwin, nb = rq.nb;
+    const double T = c->T;
+    sh.nf = nf;
+    sh.nwin = nwin;
+    sh.nb = nb;
+    sh.lo = rq.lo;
+    sh.hi = rq.hi;
+    sh.r_lo = r_lo;
+    sh.lodr = lodr;
+    sh.T = T;
+    sh.total = total;
+    sh.Nprof = Nprof;
+    sh.nchunks = nchunks;
+    sh.N = c->N;
+    sh.ks = ks;
+    sh.Hks.assign(nf, 0);
+    sh.edges.assign((size_t)nf * (nwin + 1), 0.0);
+    sh.win.assign((size_t)nf * nchunks, 0);
+    for (int f = 0; f < nf; ++f) {
+        const int k = ks[f];
+        sh.Hks[f] = fs.f[f].Hk;
+        const std::vector<double> e = log_edges(k * r_lo / T, k * r_hi / T, nwin);
+        std::copy(e.begin(), e.end(), sh.edges.begin() + (size_t)f * (nwin + 1));
+        for (int64_t cc = 0; cc < nchunks; ++cc) {
+            const int64_t i0 = cc * Nprof, n = std::min(Nprof, total - i0);
+            const double rstart = r_lo + (double)i0 * lodr;
+            const double rmean = rstart + (double)(n - 1) * lodr / 2;
+            const double fb = k == 1 ? rmean / T : k * rmean / T;
+            sh.win[(size_t)f * nchunks + cc] = window_index(e.data(), nwin, fb);
+        }
+    }
+    int s;
+    if ((s = upload(c, c->st_win, sh.win))) return s;
+    if ((s = ensure(c, c->st_cnt, (size_t)nf * nwin * nb * sizeof(uint32_t)))) return s;
+    if ((s = ensure(c, c->st_blk, (size_t)nf * nchunks * sizeof(BlockStatDev)))) return s;
+    HIPCHK(c, hipMemsetAsync(c->st_cnt.p, 0, (size_t)nf * nwin * nb * sizeof(uint32_t), c->st));
+    HIPCHK(c, launch_block_stats(c->st, (const double*)c->st_metric.p, total,
+                                 (const int32_t*)c->nvalid.p, nf, Nprof, nchunks,
+                                 (const int32_t*)c->st_win.p, nwin, rq.lo, rq.hi, nb,
+                                 (uint32_t*)c->st_cnt.p, (BlockStatDev*)c->st_blk.p));
+    sh.blk.resize((size_t)nf * nchunks);
+    sh.cnt.resize((size_t)nf * nwin * nb);
+    sh.nv.resize((size_t)nchunks * nf);
+    HIPCHK(c, hipMemcpyAsync(sh.blk.data(), c->st_blk.p, sh.blk.size() * sizeof(BlockStatDev),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(sh.cnt.data(), c->st_cnt.p, sh.cnt.size() * sizeof(uint32_t),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipMemcpyAsync(sh.nv.data(), c->nvalid.p, sh.nv.size() * sizeof(int32_t),
+                             hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    sh.valid = true;
+    return CS_OK;
+}
+
 static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, const uint8_t* mask,
-                       int64_t nmask, cs_cand* out, int64_t cap, int64_t* n_out) {
+                       int64_t nmask, cs_cand* out, int64_t cap, int64_t* n_out,
+                       const StatsReq* sreq = nullptr) {
     if (!c || !n_out) return CS_EINVAL;
     *n_out = 0;
     for (double& v : c->timings) v = 0.0;
@@ -989,6 +1091,18 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     uint32_t* surv_count = (uint32_t*)c->counters.p;
     uint32_t* cand_count = surv_count + 1;
     double medcut = (c->medmargin == INFINITY) ? -INFINITY : threshold - c->medmargin;
+    // statistics: every valid metric exact and kept, [fold][trial]
+    struct StatsGuard {
+        cs_ctx* c;
+        ~StatsGuard() { c->stats_on = false; }
+    } sguard{c};
+    if (sreq) {
+        c->sh.valid = false;
+        medcut = -INFINITY;
+        if ((s = ensure(c, c->st_metric, (size_t)nf * total * sizeof(double)))) return s;
+        HIPCHK(c, hipMemsetAsync(c->st_metric.p, 0xFF, (size_t)nf * total * sizeof(double), c->st));
+        c->stats_on = true;
+    }
 
     if (mfma) {
         // row masks of every block, and the fold and table descriptors, once
@@ -1046,7 +1160,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             o.cand_cap = cand_cap;
             o.cands = (cs_cand_dev*)c->cands.p;
             o.cand_count = cand_count;
-            o.mout = nullptr;
+            o.mall = c->stats_on ? (double*)c->st_metric.p : nullptr;
+            o.mall_ld = total;
             o.threshold = threshold;
             o.medcut = medcut;
             HIPCHK(c, hipEventRecord(E[0], c->st));
@@ -1110,6 +1225,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             c->timings[6] += tot;
         }
         c->timings[7] = (double)nbatch;
+        if (sreq && (s = finish_stats(c, *sreq, fs, ks, r_lo, r_hi, lodr, total, Nprof, nchunks)))
+            return s;
         *n_out = nc;
         if (out) {
             int64_t w = std::min<int64_t>(nc, cap);
@@ -1136,6 +1253,205 @@ int cs_search_blocks(cs_ctx* c, double r_lo, double r_hi, double threshold, cons
                      int64_t nmask, cs_cand* out, int64_t cap, int64_t* n_out) {
     if (!c || !mask) return CS_EINVAL;
     return search_impl(c, r_lo, r_hi, threshold, mask, nmask, out, cap, n_out);
+}
+
+// ---- metric statistics and the normalised pass --------------------------------
+int cs_search_stats(cs_ctx* c, double r_lo, double r_hi, double threshold, const cs_stats_cfg* cfg,
+                    cs_cand* out, int64_t cap, int64_t* n_out) {
+    if (!c || !cfg || !n_out) return CS_EINVAL;
+    if (!(cfg->hist_hi > cfg->hist_lo))
+        return c->fail(CS_EINVAL, "histogram hi must exceed lo");
+    if (cfg->hist_nb < 1) return c->fail(CS_EINVAL, "histogram bin count must be >= 1");
+    if (cfg->hist_nb > kStatsMaxBins)
+        return c->fail(CS_EINVAL, "histograms are limited to " + std::to_string(kStatsMaxBins) + " bins");
+    StatsReq rq{cfg->hist_lo, cfg->hist_hi, cfg->hist_nb, std::max(1, (int)cfg->nwin)};
+    c->sh.valid = false;
+    return search_impl(c, r_lo, r_hi, threshold, nullptr, 0, out, cap, n_out, &rq);
+}
+
+// folds of the last statistics search in ascending k (the reported order)
+static std::vector<int> folds_by_k(const cs_ctx* c) {
+    std::vector<int> o(c->sh.nf);
+    for (int f = 0; f < c->sh.nf; ++f) o[f] = f;
+    std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return c->sh.ks[a] < c->sh.ks[b]; });
+    return o;
+}
+
+int cs_stats_blocks(cs_ctx* c, cs_block_stat* out, int64_t cap, int64_t* n) {
+    if (!c || !n) return CS_EINVAL;
+    const auto& sh = c->sh;
+    if (!sh.valid) return c->fail(CS_EINVAL, "no statistics search has run");
+    const std::vector<int> ord = folds_by_k(c);
+    const double nyq2 = (double)sh.N / 2;
+    int64_t w = 0;
+    for (int64_t cc = 0; cc < sh.nchunks; ++cc) {
+        const int64_t i0 = cc * sh.Nprof, nblk = std::min(sh.Nprof, sh.total - i0);
+        const double rstart = sh.r_lo + (double)i0 * sh.lodr;
+        const double rmean = rstart + (double)(nblk - 1) * sh.lodr / 2;
+        for (int f : ord) {
+            const BlockStatDev& b = sh.blk[(size_t)f * sh.nchunks + cc];
+            const int k = sh.ks[f], Hk = sh.Hks[f];
+            if (b.n == 0) continue;   // no valid trial (k > 1 past Nyquist): no record
+            if (out && w < cap) {
+                cs_block_stat& r = out[w];
+                r.block = cc;
+                r.k = k;
+                r.Hk = Hk;
+                r.nbins = 2 * Hk;
+                r.pad = 0;
+                r.n = b.n;
+                // chunk_ngoodbins (search.jl:206-207) at k rmean
+                r.ngoodbins = std::min(nyq2 / (k == 1 ? rmean : k * rmean), (double)Hk);
+                if (k == 1) {
+                    r.flo = rstart / sh.T;
+                    r.fhi = (rstart + (double)(nblk - 1) * sh.lodr) / sh.T;
+                } else {
+                    r.flo = k * rstart / sh.T;
+                    r.fhi = k * (rstart + (double)(b.n - 1) * sh.lodr) / sh.T;
+                }
+                r.min = b.vmin;
+                r.median = b.median;
+                r.mean = b.mean;
+                r.std = b.stdv;
+                r.max = b.vmax;
+            }
+            ++w;
+        }
+    }
+    *n = w;
+    if (out && w > cap) return c->fail(CS_ENOSPC, "block statistics buffer too small");
+    return CS_OK;
+}
+
+int cs_stats_hists(cs_ctx* c, int64_t* counts, cs_hist_acc* acc, int32_t* nfolds) {
+    if (!c) return CS_EINVAL;
+    const auto& sh = c->sh;
+    if (!sh.valid) return c->fail(CS_EINVAL, "no statistics search has run");
+    if (nfolds) *nfolds = sh.nf;
+    const std::vector<int> ord = folds_by_k(c);
+    for (int o = 0; o < sh.nf; ++o) {
+        const int f = ord[o];
+        const double* e = sh.edges.data() + (size_t)f * (sh.nwin + 1);
+        for (int w = 0; w < sh.nwin; ++w) {
+            if (counts)
+                for (int i = 0; i < sh.nb; ++i)
+                    counts[((size_t)o * sh.nwin + w) * sh.nb + i] =
+                        sh.cnt[((size_t)f * sh.nwin + w) * sh.nb + i];
+            if (!acc) continue;
+            cs_hist_acc& a = acc[(size_t)o * sh.nwin + w];
+            a.k = sh.ks[f];
+            a.win = w + 1;
+            a.under = a.over = a.total = 0;
+            a.flo = e[w];
+            a.fhi = e[w + 1];
+            a.sum = a.sumsq = 0.0;
+            a.vmin = INFINITY;
+            a.vmax = -INFINITY;
+            for (int64_t cc = 0; cc < sh.nchunks; ++cc) {
+                if (sh.win[(size_t)f * sh.nchunks + cc] != w) continue;
+                const BlockStatDev& b = sh.blk[(size_t)f * sh.nchunks + cc];
+                if (b.n == 0) continue;
+                a.under += b.under;
+                a.over += b.over;
+                a.total += b.n;
+                a.sum += b.sum;
+                a.sumsq += b.sumsq;
+                a.vmin = std::min(a.vmin, b.vmin);
+                a.vmax = std::max(a.vmax, b.vmax);
+            }
+        }
+    }
+    return CS_OK;
+}
+
+int cs_stats_metrics(cs_ctx* c, int32_t k, double* out, int64_t n) {
+    if (!c || !out) return CS_EINVAL;
+    const auto& sh = c->sh;
+    if (!sh.valid) return c->fail(CS_EINVAL, "no statistics search has run");
+    if (n != sh.total) return c->fail(CS_EDIM, "metric buffer must hold every trial of the search");
+    for (int f = 0; f < sh.nf; ++f)
+        if (sh.ks[f] == k) {
+            HIPCHK(c, hipSetDevice(c->device));
+            HIPCHK(c, hipMemcpy(out, (const double*)c->st_metric.p + (size_t)f * sh.total,
+                                (size_t)sh.total * sizeof(double), hipMemcpyDeviceToHost));
+            return CS_OK;
+        }
+    return c->fail(CS_EINVAL, "k is not a searched decimation");
+}
+
+int cs_search_normalized(cs_ctx* c, double threshold, const double* loc, const double* scale,
+                         cs_cand* out, int64_t cap, int64_t* n_out) {
+    if (!c || !loc || !scale || !n_out) return CS_EINVAL;
+    *n_out = 0;
+    const auto& sh = c->sh;
+    if (!sh.valid) return c->fail(CS_EINVAL, "no statistics search has run");
+    HIPCHK(c, hipSetDevice(c->device));
+    const std::vector<int> ord = folds_by_k(c);
+    std::vector<double> tab((size_t)sh.nf * (sh.nwin + 1) + 2 * (size_t)sh.nf * sh.nwin);
+    double* ed = tab.data();
+    double* lc = ed + (size_t)sh.nf * (sh.nwin + 1);
+    double* sc = lc + (size_t)sh.nf * sh.nwin;
+    std::copy(sh.edges.begin(), sh.edges.end(), ed);
+    for (int o = 0; o < sh.nf; ++o)
+        for (int w = 0; w < sh.nwin; ++w) {
+            lc[(size_t)ord[o] * sh.nwin + w] = loc[(size_t)o * sh.nwin + w];
+            sc[(size_t)ord[o] * sh.nwin + w] = scale[(size_t)o * sh.nwin + w];
+        }
+    int s;
+    if ((s = upload(c, c->st_norm, tab))) return s;
+    NormDev nd{};
+    nd.metrics = (const double*)c->st_metric.p;
+    nd.ld = sh.total;
+    nd.total = sh.total;
+    nd.Nprof = sh.Nprof;
+    nd.r_lo = sh.r_lo;
+    nd.lodr = sh.lodr;
+    nd.T = sh.T;
+    nd.threshold = threshold;
+    nd.edges = (const double*)c->st_norm.p;
+    nd.loc = nd.edges + (size_t)sh.nf * (sh.nwin + 1);
+    nd.scale = nd.loc + (size_t)sh.nf * sh.nwin;
+    nd.nwin = sh.nwin;
+    nd.nf = sh.nf;
+    for (int f = 0; f < sh.nf; ++f) {
+        nd.k[f] = sh.ks[f];
+        nd.Hk[f] = sh.Hks[f];
+    }
+    uint32_t cand_cap = (uint32_t)std::max<int64_t>(1 << 16, std::min<int64_t>(cap, (int64_t)1 << 26));
+    if ((s = ensure(c, c->counters, 4 * sizeof(uint32_t)))) return s;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if ((s = ensure(c, c->cands, (size_t)cand_cap * sizeof(cs_cand_dev)))) return s;
+        uint32_t* cnt = (uint32_t*)c->counters.p + 1;
+        HIPCHK(c, launch_reset_u32(c->st, cnt, 1));
+        HIPCHK(c, launch_normalize(c->st, nd, (cs_cand_dev*)c->cands.p, cnt, cand_cap));
+        uint32_t ncand = 0;
+        HIPCHK(c, hipMemcpyAsync(&ncand, cnt, sizeof(uint32_t), hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (ncand > cand_cap && attempt == 0) {
+            cand_cap = ncand + 1024;
+            continue;
+        }
+        const int64_t nc = std::min<int64_t>(ncand, cand_cap);
+        std::vector<cs_cand_dev> hc((size_t)nc);
+        if (nc)
+            HIPCHK(c, hipMemcpy(hc.data(), c->cands.p, (size_t)nc * sizeof(cs_cand_dev),
+                                hipMemcpyDeviceToHost));
+        std::sort(hc.begin(), hc.end(), [](const cs_cand_dev& a, const cs_cand_dev& b) {
+            return a.t != b.t ? a.t < b.t : a.k < b.k;
+        });
+        *n_out = nc;
+        if (out)
+            for (int64_t i = 0; i < std::min<int64_t>(nc, cap); ++i) {
+                out[i].t = hc[i].t;
+                out[i].k = hc[i].k;
+                out[i].nharm = hc[i].nharm;
+                out[i].block = hc[i].block;
+                out[i].metric = hc[i].metric;
+            }
+        if (nc > cap) return c->fail(CS_ENOSPC, "candidate buffer too small");
+        return CS_OK;
+    }
+    return c->fail(CS_ENOSPC, "candidate buffer could not be sized");
 }
 
 // ---- block triage -----------------------------------------------------------

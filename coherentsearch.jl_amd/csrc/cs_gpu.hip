@@ -531,9 +531,8 @@ __global__ __launch_bounds__(64) void k_rescan(SpecDev sp, TabDev tb, GridDev g,
             metric = group_exact_metric(prof, NB, fd.widths, fd.nw, invs, srt, ps);
         }
         if (lane == 0) {
-            if (o.mout) {
-                o.mout[t - b.t_base] = metric;
-            } else if (metric > o.threshold) {
+            if (o.mall) o.mall[(int64_t)f * o.mall_ld + t] = metric;
+            if (metric > o.threshold) {
                 uint32_t idx = atomicAdd(o.cand_count, 1u);
                 if (idx < o.cand_cap) {
                     cs_cand_dev c;

@@ -90,7 +90,8 @@ struct OutDev {
     uint32_t cand_cap;
     struct cs_cand_dev* cands;
     uint32_t* cand_count;
-    double* mout;          // metrics mode: metric per batch trial (else null)
+    double* mall;          // statistics mode: every rescanned metric at
+    int64_t mall_ld;       //   mall[fold * mall_ld + t] (else null)
     double threshold;
     double medcut;
 };

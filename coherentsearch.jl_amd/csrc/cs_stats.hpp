@@ -1,0 +1,45 @@
+// Metric statistics over every trial (the --metricstats diagnostic) and the
+// normalised second pass (--normalize).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cs_gpu.hpp"
+
+namespace cs {
+
+constexpr int kStatsMaxBins = 12288;   // histogram bins (one LDS copy per workgroup)
+
+// Per (fold, block): exact moments, extremes and median of the block's valid
+// trial metrics, and the trials outside the histogram range.
+struct BlockStatDev {
+    int64_t n, under, over, pad;
+    double vmin, vmax, sum, sumsq, mean, median, stdv;
+};
+
+// metrics[f * ld + first + j], j < nvalid[block * nf + f] (the valid prefix);
+// block b of fold f adds its histogram to counts[(f * nwin + win[f * nblocks +
+// b]) * nb + bin] (bins of width (hi - lo) / nb from lo).
+hipError_t launch_block_stats(hipStream_t st, const double* metrics, int64_t ld,
+                              const int32_t* nvalid, int nf, int64_t Nprof, int64_t nblocks,
+                              const int32_t* win, int nwin, double lo, double hi, int nb,
+                              uint32_t* counts, BlockStatDev* out);
+
+// Normalised detection over the resident metrics: for every valid (fold f,
+// trial t), r = k * (r_lo + i0 lodr + j lodr) (r_lo + ... alone for k = 1),
+// window = last edge <= r / T (clamped), z = (M - loc) / scale, and (t, k,
+// Hk, block, z) is appended when z > threshold.
+struct NormDev {
+    const double* metrics;
+    int64_t ld, total, Nprof;
+    double r_lo, lodr, T, threshold;
+    const double* edges;   // [f][nwin + 1]
+    const double* loc;     // [f][nwin]
+    const double* scale;   // [f][nwin]
+    int nwin, nf;
+    int32_t k[kMaxFolds], Hk[kMaxFolds];
+};
+hipError_t launch_normalize(hipStream_t st, const NormDev& nd, cs_cand_dev* cands,
+                            uint32_t* count, uint32_t cap);
+
+}  // namespace cs

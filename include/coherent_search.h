@@ -143,6 +143,55 @@ typedef struct cs_file {
 int cs_search_files(cs_ctx *ctx, int32_t nfiles, const cs_file *files, double threshold,
                     cs_cand *out, int64_t cap, int64_t *counts, int64_t *n_out);
 
+/* Metric statistics (--metricstats: MetricStats, BlockMetricStats and
+ * MetricHistogram, src/search.jl:1295-1549, filled by _search_region! and
+ * decim_pass!, src/search.jl:1664-1719, 1768-1894).  cs_search_stats runs
+ * cs_search with every valid trial's metric computed exactly (the gate
+ * disabled, as the reference does when collecting) and returns the same
+ * candidates; it keeps every metric on the device and forms, per (block, fold),
+ * the metrics' count, extremes, mean, standard deviation and median, and per
+ * (fold, searched-frequency window) a histogram of hist_nb bins over
+ * [hist_lo, hist_hi).  Windows are nwin log-spaced bands of k * [r_lo, r_hi] / T;
+ * a block's metrics go to the window of k * (its mean trial r) / T. */
+typedef struct cs_stats_cfg {
+    double hist_lo, hist_hi;   /* histogram range (MetricStats defaults 0, 64) */
+    int32_t hist_nb;           /* bins, 1 .. 12288 (default 3200) */
+    int32_t nwin;              /* windows per fold (default 16) */
+} cs_stats_cfg;
+
+typedef struct cs_block_stat {   /* BlockMetricStats (block 0-based) */
+    int64_t block;
+    int32_t k, Hk, nbins, pad;
+    int64_t n;
+    double ngoodbins, flo, fhi, min, median, mean, std, max;
+} cs_block_stat;
+
+typedef struct cs_hist_acc {     /* MetricHistogram accumulators (win 1-based) */
+    int32_t k, win;
+    int64_t under, over, total;
+    double flo, fhi, sum, sumsq, vmin, vmax;
+} cs_hist_acc;
+
+int cs_search_stats(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
+                    const cs_stats_cfg *cfg, cs_cand *out, int64_t cap, int64_t *n_out);
+/* The last statistics search's block records, ordered by (block, k); blocks
+ * of a fold k > 1 with no trial below Nyquist have none.  *n = count
+ * (CS_ENOSPC if out is given and cap is smaller). */
+int cs_stats_blocks(cs_ctx *ctx, cs_block_stat *out, int64_t cap, int64_t *n);
+/* Its histograms, folds in ascending k: counts[fold][win][hist_nb] and
+ * acc[fold][win] (either may be NULL); *nfolds = number of folds. */
+int cs_stats_hists(cs_ctx *ctx, int64_t *counts, cs_hist_acc *acc, int32_t *nfolds);
+/* Its every-trial metrics of fold k (n = the search's trial count; NaN where
+ * the fold has no valid trial). */
+int cs_stats_metrics(cs_ctx *ctx, int32_t k, double *out, int64_t n);
+/* The normalised detection pass over the last statistics search's metrics
+ * (the --normalize second pass, src/search.jl:1629-1634, 1848-1852, 1707-1709):
+ * z = (M - loc) / scale with loc/scale[fold in ascending k][window] for the
+ * window of the trial's own searched frequency; candidates with z > threshold,
+ * their metric z. */
+int cs_search_normalized(cs_ctx *ctx, double threshold, const double *loc,
+                         const double *scale, cs_cand *out, int64_t cap, int64_t *n_out);
+
 /* block_metrics(ft, rfund, params; kernel, widths) (src/search.jl:1061-1079):
  * kernel 1 = :direct (Eqn. 30 point by point, FP64); kernel 0 = :fft
  * (finterp_fft onto the numbetween grid, then clamped linear interpolation,

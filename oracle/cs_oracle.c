@@ -1228,12 +1228,14 @@ static void cv_push(cand_vec *cv, or_cand c) {
     cv->v[cv->n++] = c;
 }
 
-/* _search_region!, search.jl:1768-1894 (no metricstats / normalize). */
+/* _search_region!, search.jl:1768-1894.  mall (optional, [fold][total], fold
+ * order 1 then the decimations as listed) receives every scored metric: the
+ * metricstats branch's raw metrics when called with medcut = -inf. */
 int or_search_region(const or_params *p, const float *amps, int64_t namps,
                      int64_t N, double r_lo, int64_t total, int64_t Nprof,
                      int64_t c_begin, int64_t c_end, double threshold,
                      double medcut, int32_t nthreads, or_cand *out, int64_t cap,
-                     int64_t *n_out) {
+                     int64_t *n_out, double *mall) {
     if (p->m % 2 != 0 || Nprof < 1) return OR_EINVAL;
     int32_t nh = p->nharms, nbins = 2 * nh;
     double lodr = p->hidr / nh;
@@ -1286,6 +1288,8 @@ int or_search_region(const or_params *p, const float *amps, int64_t namps,
             double sig = or_block_sigma(pr, n, 1, nbins, n, capb, 0);
             double invs = sig > 0 ? 1.0 / sig : 0.0;
             or_boxcar_metrics(pr, n, n, wbase, nwbase, nbins, invs, medcut, mv);
+            if (mall)
+                for (int64_t j = 0; j < n; j++) mall[i0 + j] = mv[j];
             for (int64_t j = 0; j < n; j++)
                 if (mv[j] > threshold) {
                     or_cand cc = {i0 + j, 1, nh, c, mv[j]};
@@ -1304,6 +1308,7 @@ int or_search_region(const or_params *p, const float *amps, int64_t namps,
                 for (int64_t j = 0; j < n; j++) {
                     double r_dec = (double)k * (rstart + (double)j * lodr);
                     if (!(r_dec < nyq)) continue;
+                    if (mall) mall[(int64_t)(d + 1) * total + i0 + j] = mv[j];
                     if (mv[j] > threshold) {
                         or_cand cc = {i0 + j, k, Hk, c, mv[j]};
                         cv_push(cv, cc);

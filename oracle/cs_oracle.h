@@ -162,12 +162,13 @@ int or_decim_metrics(const or_params *p, const float *amps, int64_t namps,
  * band: raw candidates (unsorted: per-thread lists concatenated). nthreads <= 0
  * means all OpenMP threads.  Returns OR_ENOSPC if cap is too small (*n_out then
  * holds the count needed).  medcut: pass NaN for the production rule
- * threshold - boxcar_medmargin, or -INFINITY for the exact path everywhere. */
+ * threshold - boxcar_medmargin, or -INFINITY for the exact path everywhere.
+ * mall (NULL or [1 + decimations > 1][total]) receives every scored metric. */
 int or_search_region(const or_params *p, const float *amps, int64_t namps,
                      int64_t N, double r_lo, int64_t total, int64_t Nprof,
                      int64_t c_begin, int64_t c_end, double threshold,
                      double medcut, int32_t nthreads, or_cand *out, int64_t cap,
-                     int64_t *n_out);
+                     int64_t *n_out, double *mall);
 
 #ifdef __cplusplus
 }

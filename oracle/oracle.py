@@ -123,7 +123,7 @@ def lib():
         L.or_search_region.argtypes = [C.POINTER(OrParams), fp, C.c_int64, C.c_int64,
                                        C.c_double, C.c_int64, C.c_int64, C.c_int64,
                                        C.c_int64, C.c_double, C.c_double, C.c_int32,
-                                       C.POINTER(OrCand), C.c_int64, i64p]
+                                       C.POINTER(OrCand), C.c_int64, i64p, dp]
         _LIB = L
     return _LIB
 
@@ -447,21 +447,26 @@ def decim_metrics(params, amps, N, rstart, n, k, weights_f64=False):
 
 
 def search_region(params, amps, N, r_lo, total, Nprof=2048, c_begin=0, c_end=None,
-                  threshold=None, medcut=None, nthreads=0):
+                  threshold=None, medcut=None, nthreads=0, metrics=None):
     """_search_region! (search.jl:1768): raw candidates as a CAND_DTYPE array sorted
-    by (t, k)."""
+    by (t, k).  `metrics`, a float64 array [1 + decimations > 1][total] (NaN
+    filled by the caller), receives every scored metric."""
     a, keep = _amps64(amps)
     thr = params.threshold if threshold is None else float(threshold)
     mc = math.nan if medcut is None else float(medcut)
     nchunks = -(-int(total) // int(Nprof))
     ce = nchunks if c_end is None else int(c_end)
+    mp = None
+    if metrics is not None:
+        assert metrics.dtype == np.float64 and metrics.flags.c_contiguous
+        mp = _dp(metrics)
     cap = 1 << 16
     while True:
         out = (OrCand * cap)()
         n = C.c_int64()
         st = lib().or_search_region(C.byref(params), _fp(a), len(keep), int(N), float(r_lo),
                                     int(total), int(Nprof), int(c_begin), ce, thr, mc,
-                                    int(nthreads), out, cap, C.byref(n))
+                                    int(nthreads), out, cap, C.byref(n), mp)
         if st == 3:
             cap = n.value + 16
             continue
