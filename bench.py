@@ -350,6 +350,86 @@ def run_cands(args, cs):
     print(json.dumps(line), flush=True)
 
 
+def run_stats(args, cs):
+    """--metricstats / --normalize at the 2^25-sample nharms=60 workload: the
+    statistics search (every trial's exact metric, block records, histograms),
+    the normalised pass over the resident metrics, against the plain search;
+    the oracle's every-trial pass on a bounded sample of blocks as the CPU
+    baseline."""
+    import numpy as np
+
+    from coherentsearch import synth
+    from coherentsearch.synth import make_config_spectrum
+
+    import oracle as O
+
+    cfg = synth.config_c2()
+    amps = make_config_spectrum(cfg)
+    N, dt = cfg["N"], cfg["dt"]
+    T = N * dt
+    params = cs.SearchParams(nharms=60, m=16, decimations=cs.decimation_set(60, 6), threshold=8.0)
+    r_lo, r_hi, lodr, total, Nprof, nchunks = cs.trial_grid(T, params, cfg["lofreq"],
+                                                            cfg["hifreq"], 100, 2048)
+    eng = cs.Engine(params, Nprof, device=0)
+    eng.set_mode(args.mode)
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
+    for _ in range(max(1, args.warmup)):
+        eng.search_raw(r_lo, r_hi, 8.0)
+    t0 = time.perf_counter()
+    plain = eng.search_raw(r_lo, r_hi, 8.0)
+    t_search = time.perf_counter() - t0
+    ts, tn = [], []
+    for _ in range(max(1, args.steps)):
+        ms = cs.MetricStats()
+        t0 = time.perf_counter()
+        raw = eng.search_stats(r_lo, r_hi, 8.0, ms)
+        ts.append(time.perf_counter() - t0)
+        norm = cs.build_metricnorm(ms)
+        t0 = time.perf_counter()
+        zc = eng.search_normalized(8.0, norm)
+        tn.append(time.perf_counter() - t0)
+    t_stats, t_norm = sorted(ts)[len(ts) // 2], sorted(tn)[len(tn) // 2]
+    dev = eng.timings()
+    summ = cs.metricstats_summary(ms, faps=(0.1, 0.01, 1e-3))
+    # CPU: the oracle's every-trial pass over a bounded run of blocks
+    op = O.make_params(nharms=60, m=16, numbetween=16, threshold=8.0,
+                       decimations=list(params.decimations))
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    nb_cpu, tc = 0, 0.0
+    while tc < min(args.cpu_seconds, 10.0) and nchunks // 2 + nb_cpu < nchunks:
+        c0 = nchunks // 2 + nb_cpu
+        c1 = min(c0 + cores, nchunks)
+        t0 = time.perf_counter()
+        O.search_region(op, amps, N, r_lo, total, Nprof, c_begin=c0, c_end=c1,
+                        medcut=-math.inf, nthreads=cores)
+        tc += time.perf_counter() - t0
+        nb_cpu += c1 - c0
+    line = {"metric": "metric-statistics search (every trial exact) trials/s @2^25 pts nharms=60",
+            "value": round(total / t_stats, 1), "unit": "trials/s", "n_gpus": 1,
+            "steps": len(ts), "warmup": args.warmup, "ms_per_step": round(1e3 * t_stats, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic",
+            "config": {"workload": "2^25-sample .fft (seed 2001), nharms=60, decimation_set(60,6), "
+                                   "0.1-125 Hz, blocks of 2048, MetricStats() defaults",
+                       "name": "stats", "mode": args.mode, "trials": total, "blocks": nchunks},
+            "candidates_equal_plain_search": bool(np.array_equal(raw, plain)),
+            "gated_search_ms": round(1e3 * t_search, 3),
+            "normalized_pass_ms": round(1e3 * t_norm, 3),
+            "normalized_candidates": int(len(zc)),
+            "stages_ms": {k: round(v, 3) for k, v in dev.items()},
+            "summary": [{"k": r["k"], "ntrials": r["ntrials"], "mean": round(r["mean"], 4),
+                         "fap_1e-3": round(r["fap"][2], 4)} for r in summ],
+            "cpu_baseline": {"value": round(nb_cpu * Nprof / tc, 1), "unit": "trials/s",
+                             "cores": cores, "kind": "port",
+                             "sample": f"{nb_cpu} blocks x {Nprof} trials from the band's middle, "
+                                       f"every metric exact, {tc:.1f} s wall"}}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -360,7 +440,7 @@ def main():
                     help="searches timed in the plain FP32 mode for the ratio (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5", "triage", "cands"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5", "triage", "cands", "stats"])
     ap.add_argument("--c3-files", type=int, default=64)
     ap.add_argument("--autotune", action="store_true",
                     help="run cs_autotune before timing (the default launch shape is the one "
@@ -385,6 +465,9 @@ def main():
         return
     if args.config == "cands":
         run_cands(args, cs)
+        return
+    if args.config == "stats":
+        run_stats(args, cs)
         return
     cfg = {"c1": synth.config_c1, "c2": synth.config_c2, "c5": synth.config_c5}[args.config]()
     log(f"[bench] generating the 2^{int(math.log2(cfg['N']))}-sample spectrum (seed {cfg['seed']})")
