@@ -499,6 +499,23 @@ def main():
     ms_per_step = 1e3 * wall / args.steps
     value = total / (wall / args.steps)
 
+    # the same searches replayed as a captured HIP graph (reported beside the
+    # headline, which keeps the per-stage timers of the eager launches)
+    graph = None
+    if args.steps > 0:
+        eng.set_tuning(graphs=True)
+        graw = eng.search_raw(r_lo, r_hi, cfg["threshold"])     # capture
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            graw = eng.search_raw(r_lo, r_hi, cfg["threshold"])
+        gwall = time.perf_counter() - t0
+        graph = {"ms_per_step": round(1e3 * gwall / args.steps, 3),
+                 "trials_per_s": round(total / (gwall / args.steps), 1),
+                 "device_ms": round(eng.timings()["total"], 3),
+                 "raw_candidates_equal": bool(np.array_equal(graw, raw))}
+        eng.set_tuning(graphs=False)
+        log(f"[bench] graph replay {graph['ms_per_step']} ms/step")
+
     # detections of the injected pulsars (fold-aware, before dedup)
     found = {}
     for f0 in [p.freq for p in cfg["pulsars"]]:
@@ -590,6 +607,7 @@ def main():
         "stages_ms": {k: round(v, 3) for k, v in stage.items()},
         "upload_s": round(upload_s, 4),
         "launch_shape": tuned,
+        "graph_replay": graph,
         "raw_candidates": int(len(raw)),
         "detections": found,
         "roofline": roof,

@@ -94,9 +94,14 @@ class Engine:
              "fused_valu": _lib.CS_MODE_FUSED_VALU}[mode]
         self._chk(_lib.lib().cs_set_mode(self._h, m), "cs_set_mode")
 
-    def set_tuning(self, batch_blocks: Optional[int] = None, workgroups: Optional[int] = None):
-        """Launch shape of the split-BF16 search (0 or None leaves the default);
-        results do not depend on it."""
+    def set_tuning(self, batch_blocks: Optional[int] = None, workgroups: Optional[int] = None,
+                   graphs: Optional[bool] = None):
+        """Launch shape of the split-BF16 search (0 or None leaves the default)
+        and graph replay of the search's launches; results do not depend on
+        them."""
+        if graphs is not None:
+            self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_GRAPHS, int(bool(graphs))),
+                      "cs_set_tuning")
         if batch_blocks is not None:
             self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_BATCH_BLOCKS,
                                                int(batch_blocks)), "cs_set_tuning")

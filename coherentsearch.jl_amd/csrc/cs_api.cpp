@@ -99,6 +99,11 @@ struct cs_ctx {
     hipStream_t cst = nullptr;
     hipEvent_t up[2] = {nullptr, nullptr};
     int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
+    bool use_graph = false;    // CS_TUNE_GRAPHS: replay a captured launch sequence
+    hipGraph_t g_graph = nullptr;
+    hipGraphExec_t g_exec = nullptr;
+    std::vector<uint8_t> g_key;   // every launch argument the graph was captured with
+    int64_t g_captures = 0;
     // host-built tables are kept while the grid they were built for is unchanged
     bool tab_valid = false;    // plans on the device for (tab_rlo, tab_f64)
     double tab_rlo = 0.0;
@@ -339,6 +344,8 @@ int cs_ctx_destroy(cs_ctx* c) {
     if (c->cst) (void)hipStreamDestroy(c->cst);
     for (auto& kv : c->twid) release(kv.second);
     for (hipEvent_t ev : c->ev) (void)hipEventDestroy(ev);
+    if (c->g_exec) (void)hipGraphExecDestroy(c->g_exec);
+    if (c->g_graph) (void)hipGraphDestroy(c->g_graph);
     if (c->st) (void)hipStreamDestroy(c->st);
     delete c;
     return CS_OK;
@@ -360,6 +367,9 @@ int cs_set_tuning(cs_ctx* c, int32_t knob, int64_t value) {
     } else if (knob == CS_TUNE_WORKGROUPS) {
         if (value > 65536) return c->fail(CS_EINVAL, "at most 65536 workgroups");
         c->tune_wg = (int)value;
+    } else if (knob == CS_TUNE_GRAPHS) {
+        if (value > 1) return c->fail(CS_EINVAL, "CS_TUNE_GRAPHS takes 0 or 1");
+        c->use_graph = value == 1;
     } else {
         return c->fail(CS_EINVAL, "unknown tuning knob");
     }
@@ -1105,9 +1115,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     }
 
     if (mfma) {
-        // row masks of every block, and the fold and table descriptors, once
-        // per search (the batches only differ in their by-value BatchDev)
-        HIPCHK(c, launch_rowmask(c->st, sp, tb, g, nchunks, (uint64_t*)c->m_rowmask.p));
+        // the fold and table descriptors, once per search (the batches only
+        // differ in their by-value BatchDev)
         // selection tables: the identity list serves every block selected
         const int64_t nid = std::max(nchunks, md.ntiles);
         if (c->m_ident_n < nid) {
@@ -1128,11 +1137,14 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         std::memcpy(desc.data() + sizeof(FoldSet), &md, sizeof(MfmaDev));
         if ((s = upload(c, c->m_desc, desc))) return s;
     }
-    for (int attempt = 0; attempt < 2; ++attempt) {
-        if ((s = ensure(c, c->cands, (size_t)cand_cap * sizeof(cs_cand_dev)))) return s;
+    // The launch sequence of one attempt: the row masks, the counters, then
+    // every batch's kernels; stage events only when not captured into a graph.
+    auto emit = [&](bool ev) -> int {
+        if (mfma) HIPCHK(c, launch_rowmask(c->st, sp, tb, g, nchunks, (uint64_t*)c->m_rowmask.p));
         HIPCHK(c, launch_reset_u32(c->st, surv_count, 2));
         for (int64_t bi = 0; bi < nbatch; ++bi) {
             hipEvent_t* E = &c->ev[bi * nev];
+            auto rec = [&](int q) { return ev ? hipEventRecord(E[q], c->st) : hipSuccess; };
             const int64_t c0 = batches[bi].first;
             const int64_t nb = batches[bi].second;
             const int64_t t_base = c0 * Nprof;
@@ -1164,7 +1176,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             o.mall_ld = total;
             o.threshold = threshold;
             o.medcut = medcut;
-            HIPCHK(c, hipEventRecord(E[0], c->st));
+            HIPCHK(c, rec(0));
             if (fused) {
                 if (!mfma) HIPCHK(c, launch_guard(c->st, sp, tb, g, b, (uint8_t*)c->rowok.p));
                 if (mfma) {
@@ -1175,27 +1187,86 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
                                                 nwg));
                 }
                 else HIPCHK(c, launch_fused(c->st, sp, tb, g, fs, b, fu));
-                HIPCHK(c, hipEventRecord(E[1], c->st));
-                HIPCHK(c, hipEventRecord(E[2], c->st));
+                HIPCHK(c, rec(1));
+                HIPCHK(c, rec(2));
                 HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
                 HIPCHK(c, launch_select(c->st, fs, g, b, fu, o));
-                HIPCHK(c, hipEventRecord(E[3], c->st));
-                HIPCHK(c, hipEventRecord(E[4], c->st));
+                HIPCHK(c, rec(3));
+                HIPCHK(c, rec(4));
             } else {
                 HIPCHK(c, launch_interp(c->st, sp, tb, g, t_base, nt, false, c->ftp.p, BT));
-                HIPCHK(c, hipEventRecord(E[1], c->st));
+                HIPCHK(c, rec(1));
                 for (int f = 0; f < nf; ++f)
                     HIPCHK(c, launch_c2r_f32(c->st, (const float2*)c->ftp.p, BT, fs.f[f].k,
                                              fs.f[f].nbins, nt, fs.prof[f], BT, fs.twid[f]));
-                HIPCHK(c, hipEventRecord(E[2], c->st));
+                HIPCHK(c, rec(2));
                 HIPCHK(c, launch_sigma_f32(c->st, fs, g, b));
-                HIPCHK(c, hipEventRecord(E[3], c->st));
+                HIPCHK(c, rec(3));
                 HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
                 HIPCHK(c, launch_gate_f32(c->st, fs, g, b, o));
-                HIPCHK(c, hipEventRecord(E[4], c->st));
+                HIPCHK(c, rec(4));
             }
             HIPCHK(c, launch_rescan(c->st, sp, tb, g, fs, b, o, 8192));
-            HIPCHK(c, hipEventRecord(E[5], c->st));
+            HIPCHK(c, rec(5));
+        }
+        return CS_OK;
+    };
+    const bool graph = c->use_graph && !sreq && !mask;
+    std::vector<uint8_t> gkey;
+    if (graph) {   // everything the captured launches bake in
+        auto put = [&](const void* p, size_t n) {
+            const uint8_t* b = (const uint8_t*)p;
+            gkey.insert(gkey.end(), b, b + n);
+        };
+        const void* ptrs[] = {c->surv.p, c->counters.p, c->nvalid.p, c->sigma.p, c->m_rowmask.p,
+                              c->m_desc.p, c->m_ident.p, c->ftp.p, c->prof.p, c->rowok.p,
+                              c->mraw.p, c->samples.p};
+        put(ptrs, sizeof(ptrs));
+        put(&sp, sizeof(sp));
+        put(&tb, sizeof(tb));
+        put(&g, sizeof(g));
+        put(&fs, sizeof(fs));
+        put(&md, sizeof(md));
+        put(&fu, sizeof(fu));
+        const double dv[] = {threshold, medcut};
+        put(dv, sizeof(dv));
+        const int64_t iv[] = {c->mode, mfma_nwg, nbatch, BT, nchunks, total, Nprof};
+        put(iv, sizeof(iv));
+        put(batches.data(), batches.size() * sizeof(batches[0]));
+        put(tile_rng.data(), tile_rng.size() * sizeof(tile_rng[0]));
+    }
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if ((s = ensure(c, c->cands, (size_t)cand_cap * sizeof(cs_cand_dev)))) return s;
+        if (graph) {
+            std::vector<uint8_t> key = gkey;
+            const void* ck[] = {c->cands.p};
+            key.insert(key.end(), (const uint8_t*)ck, (const uint8_t*)ck + sizeof(ck));
+            key.insert(key.end(), (const uint8_t*)&cand_cap, (const uint8_t*)&cand_cap + sizeof(cand_cap));
+            if (!c->g_exec || key != c->g_key) {
+                if (c->g_exec) (void)hipGraphExecDestroy(c->g_exec);
+                if (c->g_graph) (void)hipGraphDestroy(c->g_graph);
+                c->g_exec = nullptr;
+                c->g_graph = nullptr;
+                c->g_key.clear();
+                HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+                const int se = emit(false);
+                hipGraph_t gr = nullptr;
+                const hipError_t ee = hipStreamEndCapture(c->st, &gr);
+                if (se) {
+                    if (gr) (void)hipGraphDestroy(gr);
+                    return se;
+                }
+                HIPCHK(c, ee);
+                c->g_graph = gr;
+                HIPCHK(c, hipGraphInstantiate(&c->g_exec, gr, nullptr, nullptr, 0));
+                c->g_key = key;
+                ++c->g_captures;
+            }
+            HIPCHK(c, hipEventRecord(c->ev[0], c->st));
+            HIPCHK(c, hipGraphLaunch(c->g_exec, c->st));
+            HIPCHK(c, hipEventRecord(c->ev[1], c->st));
+        } else if ((s = emit(true))) {
+            return s;
         }
         HIPCHK(c, hipStreamSynchronize(c->st));
         uint32_t counts[2];
@@ -1212,7 +1283,12 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         std::sort(hc.begin(), hc.end(), [](const cs_cand_dev& a, const cs_cand_dev& b) {
             return a.t != b.t ? a.t < b.t : a.k < b.k;
         });
-        for (int64_t bi = 0; bi < nbatch; ++bi) {
+        if (graph) {   // one interval: the whole replayed sequence
+            float tot = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&tot, c->ev[0], c->ev[1]));
+            c->timings[6] = tot;
+        }
+        for (int64_t bi = 0; bi < nbatch && !graph; ++bi) {
             hipEvent_t* E = &c->ev[bi * nev];
             for (int q = 0; q < 5; ++q) {
                 float ms = 0.f;

@@ -36,6 +36,9 @@ extern "C" {
 /* Tuning knobs of cs_set_tuning (0 restores the default). */
 #define CS_TUNE_BATCH_BLOCKS 1  /* split-BF16 mode: blocks per batch (default 16384) */
 #define CS_TUNE_WORKGROUPS 2    /* split-BF16 mode: persistent workgroups (default: one per CU) */
+#define CS_TUNE_GRAPHS 3        /* 1: cs_search replays its launch sequence as a HIP graph,
+                                   captured again whenever an argument changes (stage
+                                   timers then report only the total); default 0 */
 
 /* SearchParams (src/search.jl:131-144).  `decimations` is pointer + length
  * (the reference's Vector{Int}); precision 0 = :f64, 1 = :f32 (the profile
