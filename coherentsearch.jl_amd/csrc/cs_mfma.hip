@@ -358,6 +358,43 @@ __device__ __forceinline__ void stage_samples(FoldOut& out, bool active, int sco
     }
 }
 
+// Words of eight stacked-amplitude rows h0 + e (e < 8) of one LDS plane,
+// stride rstride words from rbase; rows past H read row H (its c2r
+// coefficients beyond H are exactly zero, so any finite row serves).  The
+// byte addresses advance by one add and are clamped by one min each.
+template <int H>
+__device__ __forceinline__ void gather_rows(const CS_LDS uint32_t* pl, int h0, int rstride,
+                                            int rbase, uint32_t* w) {
+    const uint32_t b = (uint32_t)(uintptr_t)pl;
+    const uint32_t st = 4u * (uint32_t)rstride;
+    uint32_t a = b + 4u * (uint32_t)((int)__umul24((unsigned)h0, (unsigned)rstride) + rbase);
+    const uint32_t cap = b + 4u * (uint32_t)((int)__umul24((unsigned)H, (unsigned)rstride) + rbase);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        w[e] = *(const CS_LDS uint32_t*)(uintptr_t)(a < cap ? a : cap);
+        a += st;
+    }
+}
+
+// The same rows of both planes (im = re + dplane bytes), one address chain.
+template <int H>
+__device__ __forceinline__ void gather_rows2(const CS_LDS uint32_t* re, const CS_LDS uint32_t* im,
+                                             int h0, int rstride, int rbase, uint32_t* wr,
+                                             uint32_t* wi) {
+    const uint32_t b = (uint32_t)(uintptr_t)re;
+    const uint32_t dp = (uint32_t)(uintptr_t)im - b;
+    const uint32_t st = 4u * (uint32_t)rstride;
+    uint32_t a = b + 4u * (uint32_t)((int)__umul24((unsigned)h0, (unsigned)rstride) + rbase);
+    const uint32_t cap = b + 4u * (uint32_t)((int)__umul24((unsigned)H, (unsigned)rstride) + rbase);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const uint32_t x = a < cap ? a : cap;
+        wr[e] = *(const CS_LDS uint32_t*)(uintptr_t)x;
+        wi[e] = *(const CS_LDS uint32_t*)(uintptr_t)(x + dp);
+        a += st;
+    }
+}
+
 template <int NB, int WM>
 __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fre,
                                        const CS_LDS uint32_t* __restrict__ fim,
@@ -377,15 +414,7 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) {
         uint32_t wr[8], wi[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            // rows past H carry exactly zero c2r coefficients, so any finite
-            // row serves: clamp to row H and read unconditionally
-            const int hh = min(kk * 32 + g * 8 + e + 1, H);
-            const int idx = (int)__umul24((unsigned)hh, (unsigned)rstride) + rbase;
-            wr[e] = fre[idx];
-            wi[e] = fim[idx];
-        }
+        gather_rows2<H>(fre, fim, kk * 32 + g * 8 + 1, rstride, rbase, wr, wi);
         unpack8(wr, rh[kk], rl[kk]);
         unpack8(wi, ih[kk], il[kk]);
     }
@@ -503,20 +532,12 @@ __device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ 
     // (second half); slots past the last harmonic read row H (zero coefficients)
     bf16x8 bh[KD], bl[KD];
     if constexpr (KD == 1) {
-        const CS_LDS uint32_t* pl = g < 2 ? fre : fim;
         uint32_t w[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e)
-            w[e] = pl[(int)__umul24((unsigned)min(8 * (g & 1) + e + 1, H), (unsigned)rstride) + rbase];
+        gather_rows<H>(g < 2 ? fre : fim, 8 * (g & 1) + 1, rstride, rbase, w);
         unpack8(w, bh[0], bl[0]);
     } else {
         uint32_t wr[8], wi[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int idx = (int)__umul24((unsigned)min(8 * g + e + 1, H), (unsigned)rstride) + rbase;
-            wr[e] = fre[idx];
-            wi[e] = fim[idx];
-        }
+        gather_rows2<H>(fre, fim, 8 * g + 1, rstride, rbase, wr, wi);
         unpack8(wr, bh[0], bl[0]);
         unpack8(wi, bh[1], bl[1]);
     }
