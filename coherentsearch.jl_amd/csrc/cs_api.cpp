@@ -971,6 +971,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         md = c->mfma;
         if (total + 8 * md.S >= ((int64_t)1 << 32))
             return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
+        if (c->namps + 2 * kSpecPad >= ((int64_t)1 << 28))
+            return c->fail(CS_EINVAL, "fused MFMA mode: spectra are limited to 2^28 amplitudes");
         if ((s = ensure(c, c->m_rowmask, (size_t)nchunks * sizeof(uint64_t)))) return s;
         const int64_t spl_ld = c->namps + 2 * kSpecPad;
         if (!c->spl_valid) {

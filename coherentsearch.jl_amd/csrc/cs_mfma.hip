@@ -722,6 +722,12 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const int U = md.U;
     const CS_GLB uint32_t* plane = glb(md.spl) + (size_t)c * md.spl_ld + kSpecPad;
     const int64_t bmin = -kSpecPad, bmax = sp.namps + kSpecPad - 32 * KS;
+    // the pre-split planes as one buffer resource (2 spl_ld words, under 2^31
+    // bytes): a window is a 32-bit byte offset from this lane's plane column
+    const __amdgpu_buffer_rsrc_t srsc = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)md.spl, (short)0, (int)(2 * md.spl_ld * 4), 0x00020000);
+    const uint32_t lane_off = 4u * (uint32_t)((int64_t)c * md.spl_ld + kSpecPad + 8 * g4);
+    const int bmin32 = (int)bmin, bmax32 = (int)bmax;
     const uint32_t np = (uint32_t)g.Nprof;
 
     // Everything a tile reads from global memory before its compute: the
@@ -808,14 +814,17 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             }
         }
         // every window of this wave's harmonics (L2 / MALL / HBM)
+        // (32-bit: tile occurrences stay within the planes' range, whose
+        // product with an advance fits; the clamp keeps tail tiles inside)
+        const int occ32 = (int)occ;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            int64_t b0 = (int64_t)__builtin_amdgcn_readlane(dreg, 2 * s) +
-                         (int64_t)__builtin_amdgcn_readlane(dreg, 2 * s + 1) * occ;
-            b0 = b0 < bmin ? bmin : (b0 > bmax ? bmax : b0);
-            const CS_GLB uint32_t* p = plane + b0 + 8 * g4;
-            wa[s] = ldg4_a4((const void*)p);
-            wb[s] = ldg4_a4((const void*)(p + 4));
+            int b0 = __builtin_amdgcn_readlane(dreg, 2 * s) +
+                     __builtin_amdgcn_readlane(dreg, 2 * s + 1) * occ32;
+            b0 = b0 < bmin32 ? bmin32 : (b0 > bmax32 ? bmax32 : b0);
+            const uint32_t vo = lane_off + 4u * (uint32_t)b0;
+            wa[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, vo, 0, 0));
+            wb[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, vo + 16u, 0, 0));
         }
     };
     if (t_first < t_end) fetch(t_first);
