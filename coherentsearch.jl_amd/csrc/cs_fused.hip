@@ -163,6 +163,7 @@ __global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g,
 constexpr int kST = 512;
 constexpr int kSPer = 16;
 constexpr int kHist = 4096;
+static_assert(kST * kSPer == 8192, "the select's division by nbins is exact below 8192 samples");
 
 __device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* sh) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -257,14 +258,17 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
         // sample s is bin s % nbins of sampled column s / nbins; columns start at
         // multiples of nbins rounded up to salign
         const int nbs = (nbins + fu.salign - 1) / fu.salign * fu.salign;
+        // s / nbins as (s * M) >> 20, M = ceil(2^20 / nbins): exact for every
+        // s < 8192 and nbins <= 128 (checked exhaustively)
+        const uint32_t M = ((1u << 20) + (uint32_t)nbins - 1u) / (uint32_t)nbins;
         uint32_t keys[kSPer];
 #pragma unroll
         for (int r = 0; r < kSPer; ++r) {
             const int s = threadIdx.x + kST * r;
             int pos = s;
             if (nbs != nbins) {
-                const int cc = s / nbins;
-                pos = cc * nbs + (s - cc * nbins);
+                const int cc = (int)(((uint32_t)s * M) >> 20);
+                pos = s + cc * (nbs - nbins);
             }
             keys[r] = s < ns ? __float_as_uint(S[pos]) : 0xFFFFFFFFu;
         }
