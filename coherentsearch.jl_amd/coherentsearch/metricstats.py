@@ -9,16 +9,16 @@ normalisation model are host code, as in the reference.
 """
 import math
 from dataclasses import dataclass, field
-from typing import Dict, List, Optional, Tuple
+from typing import Dict, List, NamedTuple, Optional, Tuple
 
 import numpy as np
 
 MIN_WIN_TRIALS = 200   # _MIN_WIN_TRIALS, search.jl:1565
 
 
-@dataclass
-class BlockMetricStats:
-    """BlockMetricStats (search.jl:1321-1335); `block` is 1-based as there."""
+class BlockMetricStats(NamedTuple):
+    """BlockMetricStats (search.jl:1321-1335), immutable as there; `block`
+    is 1-based as there."""
     block: int
     k: int
     Hk: int
@@ -189,12 +189,9 @@ def fill_from_device(ms: MetricStats, blocks: np.ndarray, counts: np.ndarray,
     """Append the device's records (cs_stats_blocks / cs_stats_hists, folds in
     ascending k) to `ms`, as _search_region! does after its parallel region
     (search.jl:1872-1890)."""
-    for b in blocks:
-        ms.blocks.append(BlockMetricStats(int(b["block"]) + 1, int(b["k"]), int(b["Hk"]),
-                                          int(b["nbins"]), float(b["ngoodbins"]),
-                                          float(b["flo"]), float(b["fhi"]), int(b["n"]),
-                                          float(b["min"]), float(b["median"]),
-                                          float(b["mean"]), float(b["std"]), float(b["max"])))
+    cols = [blocks[f].tolist() for f in BlockMetricStats._fields]
+    cols[0] = [b + 1 for b in cols[0]]
+    ms.blocks.extend(map(BlockMetricStats._make, zip(*cols)))
     nf, nwin, nb = counts.shape
     invw = nb / (ms.hist_hi - ms.hist_lo)
     new_h, new_w = [], []

@@ -811,6 +811,48 @@ static int window_index(const double* e, int nwin, double f) {
 
 // After a statistics search: windows of every (fold, block), the block
 // records and the (fold, window) histograms, copied to the host.
+// Every valid trial's exact metric into st_metric through the plain pipeline
+// (FP32 interpolation, c2r, per-block sigma) and the per-trial exact scorer,
+// in batches of the plain path's size.
+static int stats_metrics_plain(cs_ctx* c, const FoldSet& fs0, const TabDev& tb, const SpecDev& sp,
+                               const GridDev& g, int64_t total, int64_t Nprof, int64_t nchunks) {
+    const int nf = fs0.nf, nh = c->nharms;
+    const int64_t Bp = std::max<int64_t>(1, std::min<int64_t>(nchunks, kBatchTrials / Nprof));
+    const int64_t BTp = Bp * Nprof;
+    int64_t rows = 0;
+    for (int f = 0; f < nf; ++f) rows += fs0.f[f].nbins;
+    int s;
+    if ((s = ensure(c, c->ftp, (size_t)nh * BTp * sizeof(float2)))) return s;
+    if ((s = ensure(c, c->prof, (size_t)rows * BTp * sizeof(float)))) return s;
+    FoldSet fs = fs0;
+    fs.ldp = BTp;
+    int64_t ro = 0;
+    for (int f = 0; f < nf; ++f) {
+        fs.prof[f] = (float*)c->prof.p + ro * BTp;
+        ro += fs.f[f].nbins;
+    }
+    for (int64_t c0 = 0; c0 < nchunks; c0 += Bp) {
+        const int64_t nb = std::min(Bp, nchunks - c0);
+        const int64_t t_base = c0 * Nprof, nt = std::min(nb * Nprof, total - t_base);
+        BatchDev b{};
+        b.t_base = t_base;
+        b.nt = nt;
+        b.c_first = c0;
+        b.nblocks = (int32_t)nb;
+        b.nvalid = (const int32_t*)c->nvalid.p;
+        b.sigma = (double*)c->sigma.p + c0 * nf;
+        HIPCHK(c, launch_interp(c->st, sp, tb, g, t_base, nt, false, c->ftp.p, BTp));
+        for (int f = 0; f < nf; ++f)
+            HIPCHK(c, launch_c2r_f32(c->st, (const float2*)c->ftp.p, BTp, fs.f[f].k, fs.f[f].nbins,
+                                     nt, fs.prof[f], BTp, fs.twid[f]));
+        HIPCHK(c, launch_sigma_f32(c->st, fs, g, b));
+        for (int f = 0; f < nf; ++f)
+            HIPCHK(c, launch_exact_all(c->st, fs.prof[f], BTp, fs.f[f].nbins, fs.bankmask[f], f, nf,
+                                       g, b, (double*)c->st_metric.p, total));
+    }
+    return CS_OK;
+}
+
 static int finish_stats(cs_ctx* c, const StatsReq& rq, const FoldSet& fs, const std::vector<int>& ks,
                         double r_lo, double r_hi, double lodr, int64_t total, int64_t Nprof,
                         int64_t nchunks) {
@@ -1108,12 +1150,20 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         cs_ctx* c;
         ~StatsGuard() { c->stats_on = false; }
     } sguard{c};
+    // The metrics come from FP32 profiles scored exactly per trial after the
+    // search (every fold of a supported length and default-bank widths), or
+    // else from the exact rescan of every trial with the gate disabled.
+    bool stats_fast = sreq != nullptr;
+    for (int f = 0; f < nf && stats_fast; ++f)
+        stats_fast = exact_all_supported(fs.f[f].nbins) && fs.bankmask[f] != 0;
     if (sreq) {
         c->sh.valid = false;
-        medcut = -INFINITY;
         if ((s = ensure(c, c->st_metric, (size_t)nf * total * sizeof(double)))) return s;
         HIPCHK(c, hipMemsetAsync(c->st_metric.p, 0xFF, (size_t)nf * total * sizeof(double), c->st));
-        c->stats_on = true;
+        if (!stats_fast) {
+            medcut = -INFINITY;
+            c->stats_on = true;
+        }
     }
 
     if (mfma) {
@@ -1303,6 +1353,9 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             c->timings[6] += tot;
         }
         c->timings[7] = (double)nbatch;
+        if (sreq && stats_fast &&
+            (s = stats_metrics_plain(c, fs, tb, sp, g, total, Nprof, nchunks)))
+            return s;
         if (sreq && (s = finish_stats(c, *sreq, fs, ks, r_lo, r_hi, lodr, total, Nprof, nchunks)))
             return s;
         *n_out = nc;

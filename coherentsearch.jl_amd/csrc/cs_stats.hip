@@ -11,6 +11,7 @@
 // normalised pass reads the same resident metrics.
 #include <hip/hip_runtime.h>
 
+#include "cs_device.hpp"
 #include "cs_stats.hpp"
 
 namespace cs {
@@ -197,7 +198,159 @@ __global__ __launch_bounds__(256) void k_normalize(NormDev nd, cs_cand_dev* __re
     }
 }
 
+constexpr int pow2ceil(int n) {
+    int p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+// per bank width (ascending, compile-time index A) the largest prefix-sum
+// difference times invsigma / sqrt(w); widths outside the fold's bank skipped
+template <int NB, int WX, int A>
+__device__ __forceinline__ void exact_widths(const float* ps, uint32_t bankmask, float invs,
+                                             float& best) {
+    if constexpr (A < bank_count(WX)) {
+        constexpr int w = bank_w(A);
+        if (bankmask & (1u << A)) {
+            float m = ps[w] - ps[0];
+#pragma unroll
+            for (int p = 1; p < NB; ++p) {
+                const float d = ps[p + w] - ps[p];
+                m = d > m ? d : m;
+            }
+            const float cand = m * (invs / __fsqrt_rn((float)w));
+            if (cand > best) best = cand;
+        }
+        exact_widths<NB, WX, A + 1>(ps, bankmask, invs, best);
+    }
+}
+
+// Exact metric of every valid trial of fold fidx from the batch's FP32
+// profiles (prof[bin][trial]), in the search's element type as the reference
+// scores it (_boxcar_exact / _boxcar_psum! / _boxcar_scan, search.jl:571-596,
+// 893-904): the exact median (a bitonic network over the column padded with
+// +inf), the circular prefix sums in order, per ascending width the largest
+// difference times invsigma / sqrt(w), strict >.
+template <int NB>
+__global__ __launch_bounds__(256) void k_exact_all(const float* __restrict__ prof, int64_t ldp,
+                                                   uint32_t bankmask, int fidx, int nf, GridDev g,
+                                                   BatchDev b, double* __restrict__ mall,
+                                                   int64_t ld) {
+    constexpr int NP = pow2ceil(NB);
+    constexpr int WX = gate_wmax(NB);
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= b.nt) return;
+    const int64_t t = b.t_base + j;
+    int64_t cg, first, last;
+    block_of(g, t, cg, first, last);
+    if (t - first >= b.nvalid[cg * nf + fidx]) return;
+    const double sig = b.sigma[(cg - b.c_first) * nf + fidx];
+    const float invs = sig > 0 ? 1.0f / (float)sig : 0.0f;
+    double metric = 0.0;
+    if (invs > 0.0f) {
+        float med;
+        if constexpr (NP > 64) {
+            // lower and upper middle order statistics by bisection on
+            // order-preserving keys (a 128-entry network would not fit in
+            // registers): upper = smallest key with #(keys <= it) > NB / 2
+            uint32_t kv[NB];
+#pragma unroll
+            for (int i = 0; i < NB; ++i) {
+                const uint32_t u = __float_as_uint(prof[(int64_t)i * ldp + j]);
+                kv[i] = (u >> 31) ? ~u : (u | 0x80000000u);
+            }
+            uint32_t lo = 0u, hi = 0xFFFFFFFFu;
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                int c = 0;
+#pragma unroll
+                for (int i = 0; i < NB; ++i) c += kv[i] <= mid ? 1 : 0;
+                if (c > NB / 2) hi = mid;
+                else lo = mid + 1;
+            }
+            const uint32_t up = lo;
+            auto fl = [](uint32_t k) {
+                return __uint_as_float((k >> 31) ? (k & 0x7FFFFFFFu) : ~k);
+            };
+            if (NB & 1) {
+                med = fl(up);
+            } else {
+                // lower middle: `up` if at least NB/2 keys lie at or below it
+                // besides one copy of itself, else the largest key below it
+                int below = 0;
+                uint32_t mx = 0u;
+#pragma unroll
+                for (int i = 0; i < NB; ++i) {
+                    const bool lt = kv[i] < up;
+                    below += lt ? 1 : 0;
+                    mx = lt && kv[i] > mx ? kv[i] : mx;
+                }
+                med = 0.5f * ((below >= NB / 2 ? fl(mx) : fl(up)) + fl(up));
+            }
+        } else {
+            float v[NP];
+#pragma unroll
+            for (int i = 0; i < NP; ++i) v[i] = i < NB ? prof[(int64_t)i * ldp + j] : INFINITY;
+#pragma unroll
+            for (int k = 2; k <= NP; k <<= 1)
+#pragma unroll
+                for (int q = k >> 1; q > 0; q >>= 1)
+#pragma unroll
+                    for (int i = 0; i < NP; ++i) {
+                        const int l = i ^ q;
+                        if (l > i) {
+                            const float a = v[i], c = v[l];
+                            const bool up = (i & k) == 0;
+                            v[i] = up ? fminf(a, c) : fmaxf(a, c);
+                            v[l] = up ? fmaxf(a, c) : fminf(a, c);
+                        }
+                    }
+            med = (NB & 1) ? v[NB / 2] : 0.5f * (v[NB / 2 - 1] + v[NB / 2]);
+        }
+        float ps[NB + WX + 1];
+        ps[0] = 0.0f;
+#pragma unroll
+        for (int i = 1; i <= NB + WX; ++i) {
+            const int idx = i > NB ? i - NB : i;
+            ps[i] = ps[i - 1] + (prof[(int64_t)(idx - 1) * ldp + j] - med);
+        }
+        float best = -INFINITY;
+        exact_widths<NB, WX, 0>(ps, bankmask, invs, best);
+        metric = (double)best;
+    }
+    mall[(int64_t)fidx * ld + t] = metric;
+}
+
 }  // namespace
+
+bool exact_all_supported(int nbins) {
+    switch (nbins) {
+        case 16: case 20: case 24: case 30: case 32: case 40: case 60: case 64: case 120:
+        case 128:
+            return true;
+        default:
+            return false;
+    }
+}
+
+hipError_t launch_exact_all(hipStream_t st, const float* prof, int64_t ldp, int nbins,
+                            uint32_t bankmask, int fidx, int nf, const GridDev& g,
+                            const BatchDev& b, double* mall, int64_t ld) {
+    if (b.nt <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((b.nt + 255) / 256));
+    switch (nbins) {
+#define X(S)                                                                                  \
+    case S:                                                                                   \
+        hipLaunchKernelGGL(k_exact_all<S>, grid, dim3(256), 0, st, prof, ldp, bankmask, fidx, \
+                           nf, g, b, mall, ld);                                               \
+        break;
+        X(16) X(20) X(24) X(30) X(32) X(40) X(60) X(64) X(120) X(128)
+#undef X
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_block_stats(hipStream_t st, const double* metrics, int64_t ld,
                               const int32_t* nvalid, int nf, int64_t Nprof, int64_t nblocks,

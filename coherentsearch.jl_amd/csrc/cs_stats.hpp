@@ -25,6 +25,15 @@ hipError_t launch_block_stats(hipStream_t st, const double* metrics, int64_t ld,
                               const int32_t* win, int nwin, double lo, double hi, int nb,
                               uint32_t* counts, BlockStatDev* out);
 
+// Every valid trial's exact metric of fold fidx from a contiguous batch's FP32
+// profiles (prof[bin][batch trial], stride ldp) into mall[fidx * ld + t]; the
+// fold's widths are the default-bank entries of bankmask (widths up to 0.3
+// nbins).
+bool exact_all_supported(int nbins);
+hipError_t launch_exact_all(hipStream_t st, const float* prof, int64_t ldp, int nbins,
+                            uint32_t bankmask, int fidx, int nf, const GridDev& g,
+                            const BatchDev& b, double* mall, int64_t ld);
+
 // Normalised detection over the resident metrics: for every valid (fold f,
 // trial t), r = k * (r_lo + i0 lodr + j lodr) (r_lo + ... alone for k = 1),
 // window = last edge <= r / T (clamped), z = (M - loc) / scale, and (t, k,
