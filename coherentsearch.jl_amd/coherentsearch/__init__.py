@@ -5,8 +5,9 @@ include/coherent_search.h."""
 from .candidate import (candidate_profile, measure_ducy, output_path, report_candidates,
                         rotate_to_peak, write_candidates)
 from .dedup import harmonically_related, remove_duplicates, remove_harmonics
-from .engine import (Engine, SearchCache, block_metrics, chunk_metrics, raw_to_candidates,
-                     search, search_block, search_files)
+from .engine import (Engine, SearchCache, block_metrics, chunk_metrics, chunk_ngoodbins,
+                     coherent_profiles, fourier_interp, raw_to_candidates, reference_profiles,
+                     search, search_block, search_files, snr_metrics)
 from .fileio import FFTFile, SimpleInf, freqs, write_fft
 from .metricstats import (BlockMetricStats, MetricHistogram, MetricNorm, MetricStats,
                           build_metricnorm, hist_quantile, metricstats_summary,
@@ -17,7 +18,8 @@ from .params import (Candidate, SearchParams, boxcar_widths, candidate_r, decima
 __all__ = [
     "BlockMetricStats", "Candidate", "Engine", "FFTFile", "MetricHistogram", "MetricNorm",
     "MetricStats", "SearchCache", "SearchParams", "SimpleInf", "build_metricnorm",
-    "hist_quantile", "metricstats_summary", "metricstats_windows",
+    "hist_quantile", "metricstats_summary", "metricstats_windows", "chunk_ngoodbins",
+    "coherent_profiles", "fourier_interp", "reference_profiles", "snr_metrics",
     "block_metrics", "boxcar_widths", "candidate_profile", "candidate_r", "chunk_metrics",
     "decimation_set", "measure_ducy", "output_path", "report_candidates", "rotate_to_peak",
     "write_candidates",

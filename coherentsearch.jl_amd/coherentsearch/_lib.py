@@ -123,6 +123,10 @@ def lib():
     L.cs_stats_metrics.argtypes = [vp, C.c_int32, dp, C.c_int64]
     L.cs_search_normalized.argtypes = [vp, C.c_double, dp, dp, C.POINTER(CsCand), C.c_int64,
                                        i64p]
+    L.cs_reference_profiles.argtypes = [vp, dp, C.c_int64, C.c_int32, dp]
+    L.cs_snr_metrics.argtypes = [vp, dp, C.c_int32, C.c_int64, C.c_double, C.c_double,
+                                 C.c_int64, C.c_int32, i32p, C.c_int32, dp]
+    L.cs_fourier_interp.argtypes = [vp, dp, C.c_int64, C.c_int32, dp]
     L.cs_candidate_profiles.argtypes = [vp, dp, i32p, C.c_int64, C.c_int32, dp, C.c_int64, i32p]
     L.cs_measure_ducy.argtypes = [vp, dp, i32p, C.c_int64, C.c_int32, dp]
     L.cs_decim_metrics.argtypes = [vp, C.c_double, C.c_int64, C.c_int32, C.c_int32, dp, i64p]

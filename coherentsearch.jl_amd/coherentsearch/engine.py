@@ -329,6 +329,53 @@ class Engine:
                                                  _dp(out)), "measure_ducy")
         return out
 
+    def reference_profiles(self, rfund: Sequence[float], kernel: str = "fft") -> np.ndarray:
+        """reference_profiles (src/search.jl:1011-1051): (2 nharms, L) FP64."""
+        if kernel not in ("fft", "direct"):
+            raise ValueError(f"kernel must be :fft or :direct, got :{kernel}")
+        r = np.ascontiguousarray(np.asarray(rfund, dtype=np.float64))
+        nb = 2 * self.params.nharms
+        out = np.zeros((len(r), nb))
+        if len(r):
+            self._chk(_lib.lib().cs_reference_profiles(self._h, _dp(r), len(r),
+                                                       1 if kernel == "direct" else 0, _dp(out)),
+                      "reference_profiles")
+        return out.T.copy()
+
+    def snr_metrics(self, profs, boxcar_fsp: float = 1.5, boxcar_maxfrac: float = 0.3,
+                    sigma_samples: Optional[int] = None, sigma_center: str = "zero",
+                    widths: Optional[Sequence[int]] = None) -> np.ndarray:
+        """snr_metrics (src/search.jl:933-952) of the columns of an (nbins, L)
+        profile matrix."""
+        if sigma_center not in ("zero", "median"):
+            raise ValueError(f"sigma_center must be :zero or :median, got :{sigma_center}")
+        P = np.asarray(profs, dtype=np.float64)
+        if P.ndim != 2:
+            raise ValueError("profs must be an (nbins, L) matrix")
+        nbins, L = P.shape
+        cols = np.ascontiguousarray(P.T)            # column j contiguous
+        out = np.zeros(L)
+        if widths is None:
+            wp, nw = None, 0
+        else:
+            w = np.ascontiguousarray(np.asarray(widths, dtype=np.int32))
+            wp, nw = _i32p(w), len(w)
+        ss = 0 if sigma_samples is None else int(sigma_samples)
+        self._chk(_lib.lib().cs_snr_metrics(self._h, _dp(cols), int(nbins), int(L),
+                                            float(boxcar_fsp), float(boxcar_maxfrac), ss,
+                                            1 if sigma_center == "median" else 0, wp, nw,
+                                            _dp(out)), "snr_metrics")
+        return out
+
+    def fourier_interp(self, r, m: int) -> np.ndarray:
+        """fourier_interp (src/fourierinterp.jl:66-74) at each r: complex128."""
+        rv = np.ascontiguousarray(np.atleast_1d(np.asarray(r, dtype=np.float64)))
+        out = np.zeros(2 * len(rv))
+        if len(rv):
+            self._chk(_lib.lib().cs_fourier_interp(self._h, _dp(rv), len(rv), int(m), _dp(out)),
+                      "fourier_interp")
+        return out.view(np.complex128)
+
     def interp_rows(self, r_lo: float, t0: int, n: int, weights_f64: bool = False) -> np.ndarray:
         nh = self.params.nharms
         out = np.zeros(2 * (nh + 1) * max(n, 0))
@@ -467,6 +514,42 @@ def search_block(ft: FFTFile, rfund, params: SearchParams, threshold: Optional[f
     m = block_metrics(ft, rfund, params, kernel=kernel, engine=engine)
     return [Candidate(float(r) / ft.T, float(v), float(r), params.nharms)
             for r, v in zip(rfund, m) if v > thr]
+
+
+def reference_profiles(ft: FFTFile, rfund, params: SearchParams, kernel: str = "fft",
+                       engine: Optional[Engine] = None) -> np.ndarray:
+    """reference_profiles (src/search.jl:1011-1051) on the device."""
+    eng = engine or Engine(params, 2048)
+    eng.set_spectrum(ft)
+    return eng.reference_profiles(rfund, kernel=kernel)
+
+
+def snr_metrics(profs, boxcar_fsp: float = 1.5, boxcar_maxfrac: float = 0.3,
+                sigma_samples: Optional[int] = None, sigma_center: str = "zero", widths=None,
+                engine: Optional[Engine] = None) -> np.ndarray:
+    """snr_metrics (src/search.jl:933-952) on the device."""
+    eng = engine or Engine(SearchParams(), 2048)
+    return eng.snr_metrics(profs, boxcar_fsp, boxcar_maxfrac, sigma_samples, sigma_center, widths)
+
+
+def fourier_interp(r, ft: FFTFile, m: int, engine: Optional[Engine] = None):
+    """fourier_interp (src/fourierinterp.jl:66-74) on the device; a scalar r
+    gives a complex, an array of r an array."""
+    eng = engine or Engine(SearchParams(), 2048)
+    eng.set_spectrum(ft)
+    out = eng.fourier_interp(r, m)
+    return complex(out[0]) if np.ndim(r) == 0 else out
+
+
+def coherent_profiles(ftprofs, nbins: int) -> np.ndarray:
+    """coherent_profiles (src/search.jl:964-965): the irfft of stacked harmonic
+    amplitudes along the first axis, as the reference's one-line wrapper."""
+    return np.fft.irfft(np.asarray(ftprofs), int(nbins), axis=0)
+
+
+def chunk_ngoodbins(ft: FFTFile, nharms: int, rmean: float) -> float:
+    """chunk_ngoodbins (src/search.jl:206-207)."""
+    return min(ft.N / 2 / rmean, float(nharms))
 
 
 def chunk_metrics(ft: FFTFile, params: SearchParams, rstart: float, n: int,

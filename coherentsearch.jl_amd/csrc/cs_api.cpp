@@ -621,17 +621,16 @@ int cs_decim_metrics(cs_ctx* c, double rstart, int64_t n, int32_t k, int32_t wei
     return bridge_metrics(c, k, n, nv, cap, ladder(c, nbins, k), 1.0, out);
 }
 
-int cs_block_metrics(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel,
-                     const int32_t* widths, int32_t nwidths, double* out) {
-    if (!c) return CS_EINVAL;
+// The stacked harmonic amplitudes of reference_profiles (search.jl:1011-1051)
+// for every fundamental of rfund into ftp[h-1][L]: per harmonic the row guard
+// on the band, then FP64 Eqn. 30 point by point (kernel 1, :direct) or at the
+// numbetween grid with the clamped linear interpolation (kernel 0, :fft).
+static int block_interp(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel) {
     if (kernel != 0 && kernel != 1) return c->fail(CS_EINVAL, "kernel must be :fft or :direct");
-    if (L < 1) return CS_OK;
-    if (widths && nwidths < 1) return c->fail(CS_EINVAL, "empty width bank");
     int s = check_spectrum(c);
     if (s) return s;
     HIPCHK(c, hipSetDevice(c->device));
-    const int nh = c->nharms, nbins = 2 * nh, m2 = c->m / 2;
-    // per-harmonic band and row guard (reference_profiles, search.jl:1027-1034)
+    const int nh = c->nharms, m2 = c->m / 2;
     std::vector<uint8_t> ok(nh, 0);
     std::vector<int64_t> lob(nh, 0), kgrid(nh, 1);
     const int64_t Nhalf = c->N / 2;
@@ -661,6 +660,18 @@ int cs_block_metrics(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel,
                                           c->numbetween, dl, dl + nh, (const uint8_t*)c->rowok.p,
                                           (double2*)c->ftp.p, L));
     }
+    return CS_OK;
+}
+
+int cs_block_metrics(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel,
+                     const int32_t* widths, int32_t nwidths, double* out) {
+    if (!c) return CS_EINVAL;
+    if (kernel != 0 && kernel != 1) return c->fail(CS_EINVAL, "kernel must be :fft or :direct");
+    if (L < 1) return CS_OK;
+    if (widths && nwidths < 1) return c->fail(CS_EINVAL, "empty width bank");
+    int s = block_interp(c, rfund, L, kernel);
+    if (s) return s;
+    const int nbins = 2 * c->nharms;
     std::vector<int> ws;
     if (widths) ws.assign(widths, widths + nwidths);
     else ws = ladder(c, nbins, 1);
@@ -669,6 +680,86 @@ int cs_block_metrics(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel,
     if (s) return s;
     for (int64_t j = 0; j < L; ++j)
         if (std::isnan(out[j])) return c->fail(CS_EDIM, "an interpolation window falls outside the amplitudes");
+    return CS_OK;
+}
+
+int cs_reference_profiles(cs_ctx* c, const double* rfund, int64_t L, int32_t kernel, double* out) {
+    if (!c || (L > 0 && (!rfund || !out))) return CS_EINVAL;
+    if (kernel != 0 && kernel != 1) return c->fail(CS_EINVAL, "kernel must be :fft or :direct");
+    if (L < 1) return CS_OK;
+    int s = block_interp(c, rfund, L, kernel);
+    if (s) return s;
+    const int nbins = 2 * c->nharms;
+    const double2* tw;
+    if ((s = get_twid(c, nbins, &tw))) return s;
+    if ((s = ensure(c, c->prof, (size_t)nbins * L * sizeof(double)))) return s;
+    HIPCHK(c, launch_c2r_f64(c->st, (const double2*)c->ftp.p, L, 1, nbins, L, (double*)c->prof.p, L,
+                             tw, 1.0 / (double)nbins));
+    std::vector<double> pr((size_t)nbins * L);
+    HIPCHK(c, hipMemcpyAsync(pr.data(), c->prof.p, pr.size() * sizeof(double), hipMemcpyDeviceToHost,
+                             c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int64_t j = 0; j < L; ++j)   // [bin][trial] -> column j of an nbins x L matrix
+        for (int i = 0; i < nbins; ++i) out[j * nbins + i] = pr[(size_t)i * L + j];
+    for (int64_t q = 0; q < (int64_t)nbins * L; ++q)
+        if (std::isnan(out[q])) return c->fail(CS_EDIM, "an interpolation window falls outside the amplitudes");
+    return CS_OK;
+}
+
+int cs_snr_metrics(cs_ctx* c, const double* profs, int32_t nbins, int64_t L, double fsp,
+                   double maxfrac, int64_t sigma_samples, int32_t sigma_center,
+                   const int32_t* widths, int32_t nwidths, double* out) {
+    if (!c || (L > 0 && (!profs || !out))) return CS_EINVAL;
+    if (nbins < 1 || nbins > 128) return c->fail(CS_EINVAL, "profiles of 1..128 bins");
+    if (sigma_center != 0 && sigma_center != 1)
+        return c->fail(CS_EINVAL, "sigma_center must be :zero or :median");
+    if (L < 1) return CS_OK;
+    std::vector<int> ws = widths ? std::vector<int>(widths, widths + nwidths)
+                                 : boxcar_widths(nbins, fsp, maxfrac);
+    if (ws.empty() || ws.size() > 64) return c->fail(CS_EINVAL, "boxcar bank must hold 1..64 widths");
+    for (int w : ws)
+        if (w < 1 || w > nbins) return c->fail(CS_EINVAL, "boxcar widths must lie in 1..nbins");
+    HIPCHK(c, hipSetDevice(c->device));
+    int s;
+    std::vector<double> pr((size_t)nbins * L);   // column j of the matrix -> [bin][trial]
+    for (int64_t j = 0; j < L; ++j)
+        for (int i = 0; i < nbins; ++i) pr[(size_t)i * L + j] = profs[j * nbins + i];
+    if ((s = upload(c, c->prof, pr))) return s;
+    const int64_t cap = sigma_samples > 0 ? sigma_samples : (int64_t)nbins * L;
+    const int64_t nsb = std::min<int64_t>((int64_t)nbins * L, cap);
+    if ((s = ensure(c, c->mraw, (size_t)std::max<int64_t>(nsb, 1) * sizeof(double)))) return s;
+    if ((s = ensure(c, c->sigma, sizeof(double)))) return s;
+    if ((s = ensure(c, c->mout, (size_t)L * sizeof(double)))) return s;
+    if ((s = upload(c, c->widths, std::vector<int32_t>(ws.begin(), ws.end())))) return s;
+    HIPCHK(c, launch_sigma_any(c->st, (const double*)c->prof.p, L, nbins, L, cap, sigma_center,
+                               (double*)c->mraw.p, (double*)c->sigma.p));
+    HIPCHK(c, launch_exact_f64(c->st, (const double*)c->prof.p, L, nbins, L,
+                               (const int32_t*)c->widths.p, (int)ws.size(),
+                               (const double*)c->sigma.p, (double*)c->mout.p));
+    HIPCHK(c, hipMemcpyAsync(out, c->mout.p, (size_t)L * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
+}
+
+int cs_fourier_interp(cs_ctx* c, const double* r, int64_t n, int32_t m, double* out) {
+    if (!c || (n > 0 && (!r || !out))) return CS_EINVAL;
+    if (m < 2 || m % 2 != 0) return c->fail(CS_EINVAL, "m must be even");
+    if (n < 1) return CS_OK;
+    for (int64_t i = 0; i < n; ++i)
+        if (!(r[i] >= 0.0)) return c->fail(CS_EINVAL, "r must be non-negative");
+    int s = check_spectrum(c);
+    if (s) return s;
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<double> rv(r, r + n);
+    if ((s = upload(c, c->rfund, rv))) return s;
+    if ((s = upload(c, c->rowok, std::vector<uint8_t>(1, 1)))) return s;
+    if ((s = ensure(c, c->ftp, (size_t)n * sizeof(double2)))) return s;
+    HIPCHK(c, launch_point_interp_f64(c->st, spec_of(c), (const double*)c->rfund.p, n, 1, m,
+                                      (const uint8_t*)c->rowok.p, (double2*)c->ftp.p, n));
+    HIPCHK(c, hipMemcpyAsync(out, c->ftp.p, (size_t)n * sizeof(double2), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    for (int64_t i = 0; i < 2 * n; ++i)
+        if (std::isnan(out[i])) return c->fail(CS_EDIM, "the interpolation window falls outside the amplitudes");
     return CS_OK;
 }
 
@@ -1150,9 +1241,11 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         cs_ctx* c;
         ~StatsGuard() { c->stats_on = false; }
     } sguard{c};
-    // The metrics come from FP32 profiles scored exactly per trial after the
-    // search (every fold of a supported length and default-bank widths), or
-    // else from the exact rescan of every trial with the gate disabled.
+    // The metrics come from FP32 profiles scored exactly per trial before the
+    // search (every fold of a supported length and default-bank widths), which
+    // then select the search's survivors; or else from the exact rescan of
+    // every trial with the gate disabled.  Either way the candidates are those
+    // of the exact scan everywhere, as when the reference collects statistics.
     bool stats_fast = sreq != nullptr;
     for (int f = 0; f < nf && stats_fast; ++f)
         stats_fast = exact_all_supported(fs.f[f].nbins) && fs.bankmask[f] != 0;
@@ -1226,6 +1319,13 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             o.cand_count = cand_count;
             o.mall = c->stats_on ? (double*)c->st_metric.p : nullptr;
             o.mall_ld = total;
+            // statistics fast path: the every-trial FP32 exact metrics select
+            // the survivors (everything within 0.01 of the threshold, far
+            // beyond their 1e-5 relative difference from this mode's FP64
+            // metrics), so the candidates are the exact-everywhere ones
+            o.gm = (sreq && stats_fast) ? (const double*)c->st_metric.p : nullptr;
+            o.gm_ld = total;
+            o.gmcut = threshold - 0.01;
             o.threshold = threshold;
             o.medcut = medcut;
             HIPCHK(c, rec(0));
@@ -1287,6 +1387,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         put(batches.data(), batches.size() * sizeof(batches[0]));
         put(tile_rng.data(), tile_rng.size() * sizeof(tile_rng[0]));
     }
+    if (sreq && stats_fast && (s = stats_metrics_plain(c, fs, tb, sp, g, total, Nprof, nchunks)))
+        return s;
     for (int attempt = 0; attempt < 2; ++attempt) {
         if ((s = ensure(c, c->cands, (size_t)cand_cap * sizeof(cs_cand_dev)))) return s;
         if (graph) {
@@ -1353,9 +1455,6 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             c->timings[6] += tot;
         }
         c->timings[7] = (double)nbatch;
-        if (sreq && stats_fast &&
-            (s = stats_metrics_plain(c, fs, tb, sp, g, total, Nprof, nchunks)))
-            return s;
         if (sreq && (s = finish_stats(c, *sreq, fs, ks, r_lo, r_hi, lodr, total, Nprof, nchunks)))
             return s;
         *n_out = nc;

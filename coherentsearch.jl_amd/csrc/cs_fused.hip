@@ -322,6 +322,7 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
                 pred = (double)m * invs >= o.medcut;
             }
         }
+        if (o.gm) pred = j < nv && o.gm[(int64_t)f * o.gm_ld + first + j] > o.gmcut;
         push_survivor(pred, first + j, f, o);
     }
 }

@@ -323,6 +323,7 @@ __global__ __launch_bounds__(256) void k_gate(const float* __restrict__ prof, in
             }
         }
     }
+    if (o.gm) pred = j < b.nt && o.gm[(int64_t)fidx * o.gm_ld + t] > o.gmcut;
     push_survivor(pred, t, fidx, o);
 }
 
@@ -370,6 +371,7 @@ __global__ __launch_bounds__(64) void k_gate_generic(const float* __restrict__ p
             }
         }
     }
+    if (o.gm) pred = j < b.nt && o.gm[(int64_t)fidx * o.gm_ld + t] > o.gmcut;
     push_survivor(pred, t, fidx, o);
 }
 

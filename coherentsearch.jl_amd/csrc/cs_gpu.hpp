@@ -94,6 +94,11 @@ struct OutDev {
     int64_t mall_ld;       //   mall[fold * mall_ld + t] (else null)
     double threshold;
     double medcut;
+    // statistics search: survivors are the trials whose metric in gm
+    // ([fold][trial], NaN where invalid) exceeds gmcut, instead of the bound
+    const double* gm;
+    int64_t gm_ld;
+    double gmcut;
 };
 
 struct cs_cand_dev {

@@ -321,7 +321,64 @@ __global__ __launch_bounds__(256) void k_exact_all(const float* __restrict__ pro
     mall[(int64_t)fidx * ld + t] = metric;
 }
 
+// sigma of a profile set (_block_sigma, search.jl:490-566) for any sample
+// count: the samples (every bin, or whole columns spread evenly) gathered to
+// buf, then 1.4826 x the exact median of |x| (center 0) or of |x - median(x)|
+// (center 1), medians by radix selection over buf.
+__global__ __launch_bounds__(kThreads) void k_sigma_any(const double* __restrict__ prof,
+                                                        int64_t ldp, int nbins, int64_t n,
+                                                        int64_t cap, int center_median,
+                                                        double* __restrict__ buf,
+                                                        double* __restrict__ sigma_out) {
+    __shared__ uint32_t sel[256];
+    __shared__ uint64_t selp[2];
+    const int64_t NN = (int64_t)nbins * n;
+    if (NN == 0) {
+        if (threadIdx.x == 0) *sigma_out = 0.0;
+        return;
+    }
+    const bool all = NN <= cap;
+    int64_t ncol, ns;
+    if (all) {
+        ncol = n;
+        ns = NN;
+    } else {
+        ncol = cap / nbins;
+        if (ncol < 1) ncol = 1;
+        if (ncol > n) ncol = n;
+        ns = ncol * nbins < cap ? ncol * nbins : cap;
+    }
+    for (int64_t s = threadIdx.x; s < ns; s += kThreads) {
+        const int64_t cc = s / nbins, i = s - cc * nbins;
+        const int64_t j = all ? cc : (cc * n) / ncol;
+        const double x = prof[i * ldp + j];
+        buf[s] = center_median ? x : fabs(x);
+    }
+    __threadfence_block();
+    __syncthreads();
+    auto median = [&]() {
+        const int64_t half = ns >> 1;
+        const double up = select_rank(buf, ns, half, sel, selp);
+        return (ns & 1) ? up : 0.5 * (select_rank(buf, ns, half - 1, sel, selp) + up);
+    };
+    if (center_median) {
+        const double m0 = median();
+        for (int64_t s = threadIdx.x; s < ns; s += kThreads) buf[s] = fabs(buf[s] - m0);
+        __threadfence_block();
+        __syncthreads();
+    }
+    const double med = median();
+    if (threadIdx.x == 0) *sigma_out = 1.4826 * med;
+}
+
 }  // namespace
+
+hipError_t launch_sigma_any(hipStream_t st, const double* prof, int64_t ldp, int nbins, int64_t n,
+                            int64_t cap, int center_median, double* buf, double* sigma_out) {
+    hipLaunchKernelGGL(k_sigma_any, dim3(1), dim3(kThreads), 0, st, prof, ldp, nbins, n, cap,
+                       center_median, buf, sigma_out);
+    return hipGetLastError();
+}
 
 bool exact_all_supported(int nbins) {
     switch (nbins) {

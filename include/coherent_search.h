@@ -205,6 +205,26 @@ int cs_block_metrics(cs_ctx *ctx, const double *rfund, int64_t L,
                      int32_t kernel, const int32_t *widths, int32_t nwidths,
                      double *out);
 
+/* reference_profiles(ft, rfund, params; kernel) (src/search.jl:1011-1051): the
+ * (2 nharms) x L FP64 profiles, column j of the matrix at out[j * 2 nharms]
+ * (the irfft of the stacked harmonics, DC zero; a harmonic whose band leaves
+ * the amplitudes or reaches N/2 stays zero). */
+int cs_reference_profiles(cs_ctx *ctx, const double *rfund, int64_t L, int32_t kernel,
+                          double *out);
+/* snr_metrics(profs; boxcar_fsp, boxcar_maxfrac, sigma_samples, sigma_center,
+ * widths) (src/search.jl:933-952) of an nbins x L FP64 profile matrix (column
+ * j at profs[j * nbins], nbins <= 128): one sigma for the set (every sample,
+ * or sigma_samples > 0 of them as whole columns; sigma_center 0 = :zero,
+ * 1 = :median), then each profile's exact median-baseline boxcar metric over
+ * widths (NULL: boxcar_widths(nbins; fsp, maxfrac)). */
+int cs_snr_metrics(cs_ctx *ctx, const double *profs, int32_t nbins, int64_t L, double fsp,
+                   double maxfrac, int64_t sigma_samples, int32_t sigma_center,
+                   const int32_t *widths, int32_t nwidths, double *out);
+/* fourier_interp(r, ft, m) (src/fourierinterp.jl:66-74) at n frequencies of
+ * the current spectrum: out[2i], out[2i+1] = re, im.  CS_EINVAL for r < 0 or
+ * odd m, CS_EDIM when a window leaves the amplitudes. */
+int cs_fourier_interp(cs_ctx *ctx, const double *r, int64_t n, int32_t m, double *out);
+
 /* candidate_profile(ft, r, nharm; m) (src/candidate.jl:37-61) for n
  * candidates: harmonics h = 1, 2, ... of r[i] by FP64 fourier_interp with an
  * m-bin kernel, stopping at the first whose window leaves the amplitudes or
