@@ -1771,6 +1771,14 @@ int cs_mlp_grad(cs_ctx* c, const float* params, const float* norm, const float* 
 static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, double threshold,
                              cs_cand* out, int64_t cap, int64_t* counts, int64_t* n_out) {
     *n_out = 0;
+    // files of different lengths change every launch argument: graph replay
+    // would capture once per file, so the files run eager
+    struct GraphOff {
+        cs_ctx* c;
+        bool was;
+        ~GraphOff() { c->use_graph = was; }
+    } goff{c, c->use_graph};
+    c->use_graph = false;
     if (nfiles <= 0) return CS_OK;
     if (cap < 0 || (cap > 0 && !out)) return c->fail(CS_EINVAL, "bad candidate buffer");
     int64_t maxn = 0;
