@@ -238,6 +238,13 @@ int main() {
     check_config(60, 32, 0.5, 0.1 * T2, {120, 60, 40, 30}, 15, {28, 13, 9, 9});
     check_config(32, 16, 0.5, 0.1 * 16777216 * 64e-6, {64, 32, 16}, 16, {19, 9, 6});
     check_config(12, 16, 0.5, 1000.0, {24}, 17);
+    // other trial-grid periods: hidr 0.25 (q = 240, S = 240), hidr 0.3
+    // (q = 200, S = 400), hidr 1/3 at nharms 30 (q = 90, S = 720), and the
+    // 128-bin even/odd fold (nharms 64)
+    check_config(60, 16, 0.25, 523.0, {120, 60, 40, 30, 24, 20}, 18);
+    check_config(60, 16, 0.3, 523.0, {120, 60, 40, 30, 24, 20}, 19);
+    check_config(30, 16, 1.0 / 3.0, 777.0, {60, 30, 20}, 20);
+    check_config(64, 16, 0.5, 911.0, {128, 64}, 21);
     printf(bad ? "FAIL %d\n" : "OK\n", bad);
     return bad ? 1 : 0;
 }
