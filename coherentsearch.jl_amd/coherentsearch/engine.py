@@ -266,6 +266,18 @@ class Engine:
         keys = ("interp", "c2r", "sigma", "gate", "rescan", "fused", "total", "batches")
         return dict(zip(keys, a.tolist()))
 
+    def last_sigmas(self) -> np.ndarray:
+        """(nblocks, nfolds) sigma of every (block, fold) of the last search
+        over a whole range (_block_sigma inside _search_region!, search.jl:
+        1817, and decim_pass!, :1688); folds: 1, then the decimations > 1."""
+        n = C.c_int64()
+        self._chk(_lib.lib().cs_last_sigmas(self._h, None, 0, C.byref(n)), "cs_last_sigmas")
+        a = np.zeros(n.value)
+        self._chk(_lib.lib().cs_last_sigmas(self._h, _dp(a), n.value, C.byref(n)),
+                  "cs_last_sigmas")
+        nf = 1 + sum(1 for k in self.params.decimations if k > 1)
+        return a.reshape(-1, nf)
+
     def chunk_metrics(self, rstart: float, n: int, lodr: Optional[float] = None,
                       weights_f64: bool = False) -> np.ndarray:
         out = np.zeros(max(n, 0))

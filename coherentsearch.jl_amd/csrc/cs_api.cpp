@@ -118,6 +118,11 @@ struct cs_ctx {
     int64_t nv_total = 0, nv_N = 0;
     std::vector<hipEvent_t> ev;
     double timings[8] = {0};
+    // block sigmas of the last search: [block][fold] in `sigma`, for a search
+    // over every block of its range
+    int64_t sig_blocks = 0;
+    int sig_nf = 0;
+    bool sig_sparse = false;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -505,6 +510,22 @@ int cs_autotune(cs_ctx* c, double r_lo, double r_hi, double threshold, const cha
 int cs_last_timings(const cs_ctx* c, double* ms8) {
     if (!c || !ms8) return CS_EINVAL;
     for (int i = 0; i < 8; ++i) ms8[i] = c->timings[i];
+    return CS_OK;
+}
+
+int cs_last_sigmas(cs_ctx* c, double* out, int64_t cap, int64_t* n_out) {
+    if (!c || !n_out) return CS_EINVAL;
+    *n_out = 0;
+    if (c->sig_blocks <= 0) return c->fail(CS_EINVAL, "no completed search on this context");
+    if (c->sig_sparse)
+        return c->fail(CS_EINVAL, "the last search covered selected blocks only");
+    const int64_t n = c->sig_blocks * c->sig_nf;
+    *n_out = n;
+    if (!out) return CS_OK;
+    if (cap < n) return c->fail(CS_ENOSPC, "sigma buffer too small");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(out, c->sigma.p, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     return CS_OK;
 }
 
@@ -1008,6 +1029,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     if (!c || !n_out) return CS_EINVAL;
     *n_out = 0;
     for (double& v : c->timings) v = 0.0;
+    c->sig_blocks = 0;
     int s = check_spectrum(c);
     if (s) return s;
     HIPCHK(c, hipSetDevice(c->device));
@@ -1457,6 +1479,9 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         c->timings[7] = (double)nbatch;
         if (sreq && (s = finish_stats(c, *sreq, fs, ks, r_lo, r_hi, lodr, total, Nprof, nchunks)))
             return s;
+        c->sig_blocks = nchunks;
+        c->sig_nf = nf;
+        c->sig_sparse = mask != nullptr;
         *n_out = nc;
         if (out) {
             int64_t w = std::min<int64_t>(nc, cap);

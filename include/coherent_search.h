@@ -286,6 +286,16 @@ int cs_autotune(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
  * [5] fused, [6] total device, [7] number of batches. */
 int cs_last_timings(const cs_ctx *ctx, double *ms8);
 
+/* The sigma estimate of every (block, fold) of the last cs_search (or
+ * statistics search) over a whole range: out[block * nfolds + fold], fold 0
+ * the undecimated one, then the decimations > 1 in parameter order, 0 where a
+ * fold has no valid trial in the block.  These are the sigmas each block's
+ * _block_sigma returns inside _search_region! (src/search.jl:490-566,
+ * :1817; decim_pass! :1688); the reference keeps them internal.
+ * *n_out = nblocks * nfolds; out may be NULL to query it.  CS_EINVAL if no
+ * search has completed or the last one covered selected blocks only. */
+int cs_last_sigmas(cs_ctx *ctx, double *out, int64_t cap, int64_t *n_out);
+
 /* Host-only (no device): the per-harmonic interpolation tables the library
  * builds for a grid anchored at r_lo (build_direct_plans,
  * src/directinterp.jl:345-405).  info[0..5] = q, s, base_adv, pnum, rfloor0, P;

@@ -1230,12 +1230,14 @@ static void cv_push(cand_vec *cv, or_cand c) {
 
 /* _search_region!, search.jl:1768-1894.  mall (optional, [fold][total], fold
  * order 1 then the decimations as listed) receives every scored metric: the
- * metricstats branch's raw metrics when called with medcut = -inf. */
+ * metricstats branch's raw metrics when called with medcut = -inf.  sigmas
+ * (optional, [chunk][fold], same fold order) receives each block's
+ * _block_sigma (search.jl:1817; decim_pass!, :1688). */
 int or_search_region(const or_params *p, const float *amps, int64_t namps,
                      int64_t N, double r_lo, int64_t total, int64_t Nprof,
                      int64_t c_begin, int64_t c_end, double threshold,
                      double medcut, int32_t nthreads, or_cand *out, int64_t cap,
-                     int64_t *n_out, double *mall) {
+                     int64_t *n_out, double *mall, double *sigmas) {
     if (p->m % 2 != 0 || Nprof < 1) return OR_EINVAL;
     int32_t nh = p->nharms, nbins = 2 * nh;
     double lodr = p->hidr / nh;
@@ -1286,6 +1288,7 @@ int or_search_region(const or_params *p, const float *amps, int64_t namps,
             int64_t capb = (int64_t)nbins * Nprof < BOXCAR_SIGMA_SAMPLES
                                ? (int64_t)nbins * Nprof : BOXCAR_SIGMA_SAMPLES;
             double sig = or_block_sigma(pr, n, 1, nbins, n, capb, 0);
+            if (sigmas) sigmas[c * (1 + nd)] = sig;
             double invs = sig > 0 ? 1.0 / sig : 0.0;
             or_boxcar_metrics(pr, n, n, wbase, nwbase, nbins, invs, medcut, mv);
             if (mall)
@@ -1303,6 +1306,7 @@ int or_search_region(const or_params *p, const float *amps, int64_t namps,
                 int64_t capk = (int64_t)nbk * Nprof < BOXCAR_SIGMA_SAMPLES
                                    ? (int64_t)nbk * Nprof : BOXCAR_SIGMA_SAMPLES;
                 double sk = or_block_sigma(pr, n, 1, nbk, nv, capk, 0);
+                if (sigmas) sigmas[c * (1 + nd) + 1 + d] = sk;
                 double ik = sk > 0 ? 1.0 / sk : 0.0;
                 or_boxcar_metrics(pr, n, nv, wdec[d], nwdec[d], nbk, ik, medcut, mv);
                 for (int64_t j = 0; j < n; j++) {

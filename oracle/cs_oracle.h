@@ -168,7 +168,7 @@ int or_search_region(const or_params *p, const float *amps, int64_t namps,
                      int64_t N, double r_lo, int64_t total, int64_t Nprof,
                      int64_t c_begin, int64_t c_end, double threshold,
                      double medcut, int32_t nthreads, or_cand *out, int64_t cap,
-                     int64_t *n_out, double *mall);
+                     int64_t *n_out, double *mall, double *sigmas);
 
 #ifdef __cplusplus
 }

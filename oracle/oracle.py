@@ -123,7 +123,7 @@ def lib():
         L.or_search_region.argtypes = [C.POINTER(OrParams), fp, C.c_int64, C.c_int64,
                                        C.c_double, C.c_int64, C.c_int64, C.c_int64,
                                        C.c_int64, C.c_double, C.c_double, C.c_int32,
-                                       C.POINTER(OrCand), C.c_int64, i64p, dp]
+                                       C.POINTER(OrCand), C.c_int64, i64p, dp, dp]
         _LIB = L
     return _LIB
 
@@ -447,10 +447,11 @@ def decim_metrics(params, amps, N, rstart, n, k, weights_f64=False):
 
 
 def search_region(params, amps, N, r_lo, total, Nprof=2048, c_begin=0, c_end=None,
-                  threshold=None, medcut=None, nthreads=0, metrics=None):
+                  threshold=None, medcut=None, nthreads=0, metrics=None, sigmas=None):
     """_search_region! (search.jl:1768): raw candidates as a CAND_DTYPE array sorted
     by (t, k).  `metrics`, a float64 array [1 + decimations > 1][total] (NaN
-    filled by the caller), receives every scored metric."""
+    filled by the caller), receives every scored metric; `sigmas`, a float64
+    array [chunk][1 + decimations > 1], each block's _block_sigma."""
     a, keep = _amps64(amps)
     thr = params.threshold if threshold is None else float(threshold)
     mc = math.nan if medcut is None else float(medcut)
@@ -460,13 +461,18 @@ def search_region(params, amps, N, r_lo, total, Nprof=2048, c_begin=0, c_end=Non
     if metrics is not None:
         assert metrics.dtype == np.float64 and metrics.flags.c_contiguous
         mp = _dp(metrics)
+    sp = None
+    if sigmas is not None:
+        assert sigmas.dtype == np.float64 and sigmas.flags.c_contiguous
+        assert sigmas.shape[0] >= ce
+        sp = _dp(sigmas)
     cap = 1 << 16
     while True:
         out = (OrCand * cap)()
         n = C.c_int64()
         st = lib().or_search_region(C.byref(params), _fp(a), len(keep), int(N), float(r_lo),
                                     int(total), int(Nprof), int(c_begin), ce, thr, mc,
-                                    int(nthreads), out, cap, C.byref(n), mp)
+                                    int(nthreads), out, cap, C.byref(n), mp, sp)
         if st == 3:
             cap = n.value + 16
             continue

@@ -302,6 +302,38 @@ def test_search_vs_oracle(cs, O, small_spectrum, nharms, maxdecim, blocksize, th
 
 
 @pytest.mark.parametrize("mode", ["plain", "fused", "fused_valu"])
+@pytest.mark.parametrize("blocksize,lo,hi", [(1000, 1.0, 60.0), (64, 10.0, 14.0)])
+def test_block_sigmas_vs_oracle(cs, O, small_spectrum, mode, blocksize, lo, hi):
+    """Every (block, fold) sigma of the search against _block_sigma of the
+    oracle's FP64 profiles (search.jl:490-566): 1000-trial blocks subsample
+    8192 of 20-120 bins x 1000 columns; 64-trial blocks of 120 bins take all
+    7680 samples (an even count: the mean of the two central values).  The
+    bar (1e-5 relative; 5e-5 for the split-BF16 profiles) is well inside the
+    ~3e-4 relative gap between neighbouring order statistics of 8192 samples
+    at their median, so a wrong rank or a dropped average fails it."""
+    amps, N, dt = small_spectrum
+    T = N * dt
+    dec = cs.decimation_set(60, 6)
+    _say(f"block sigmas mode={mode} blocksize={blocksize}")
+    eng, p = _engine(cs, nharms=60, m=16, decimations=dec, blocksize=blocksize)
+    eng.set_mode(mode)
+    eng.set_spectrum(_ft(cs, amps, N, dt))
+    r_lo, r_hi = lo * T, hi * T
+    total = math.floor((r_hi - r_lo) / (0.5 / 60)) + 1
+    nchunks = -(-total // blocksize)
+    eng.search_raw(r_lo, r_hi, 6.5)
+    got = eng.last_sigmas()
+    want = np.zeros((nchunks, len(dec)))
+    O.search_region(O.make_params(nharms=60, m=16, decimations=dec), amps, N, r_lo, total,
+                    blocksize, threshold=6.5, sigmas=want)
+    assert got.shape == want.shape and np.all(want > 0)
+    rel = np.abs(got - want) / want
+    tol = 5e-5 if mode == "fused" else 1e-5
+    _say(f"  {got.size} sigmas, worst rel {rel.max():.2e}")
+    assert rel.max() <= tol, (np.unravel_index(rel.argmax(), rel.shape), rel.max())
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused", "fused_valu"])
 def test_search_gate_equals_exact(cs, small_spectrum, mode):
     """boxcar_medmargin = Inf (exact scan everywhere) gives the identical list
     (test/test_search.jl:287-309)."""
