@@ -165,7 +165,10 @@ constexpr int kSPer = 16;
 constexpr int kHist = 4096;
 static_assert(kST * kSPer == 8192, "the select's division by nbins is exact below 8192 samples");
 
-__device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* sh) {
+// Exclusive prefix sum over the workgroup.  One barrier: every thread adds
+// the totals of the waves before its own.  `tot` (kST/64 words) is rewritten
+// by the next call only after the caller's next barrier.
+__device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* tot) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     uint32_t x = v;
 #pragma unroll
@@ -173,64 +176,63 @@ __device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* sh) {
         uint32_t y = __shfl_up(x, o, 64);
         if (lane >= o) x += y;
     }
-    if (lane == 63) sh[wid] = x;
+    if (lane == 63) tot[wid] = x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t acc = 0;
-        for (int w = 0; w < kST / 64; ++w) {
-            uint32_t tmp = sh[w];
-            sh[w] = acc;
-            acc += tmp;
-        }
+    uint32_t pre = 0;
+#pragma unroll
+    for (int w = 0; w < kST / 64; ++w) pre += w < wid ? tot[w] : 0u;
+    return pre + x - v;
+}
+
+// One radix pass: the histogram of digit (key >> SH) & (2^BITS - 1) over the
+// keys whose bits above SH + BITS equal `prefix` (under `mask`); returns the
+// digit that holds rank `kth` and leaves in `kth` the rank inside it.
+template <int SH, int BITS>
+__device__ __forceinline__ uint32_t radix_pass(const uint32_t (&keys)[kSPer], uint32_t prefix,
+                                               uint32_t mask, uint32_t& kth, uint32_t* hist,
+                                               uint32_t* sh) {
+    constexpr int NBIN = 1 << BITS, PQ = NBIN / kST;
+    static_assert(PQ >= 1 && NBIN <= kHist, "digit width");
+    constexpr uint32_t dmask = (uint32_t)NBIN - 1u;
+#pragma unroll
+    for (int q = 0; q < PQ; ++q) hist[threadIdx.x + kST * q] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSPer; ++r) {
+        const uint32_t key = keys[r];
+        if ((key & mask) == prefix) atomicAdd(&hist[(key >> SH) & dmask], 1u);
     }
     __syncthreads();
-    uint32_t r = x - v + sh[wid];
+    uint32_t a[PQ], loc = 0;
+#pragma unroll
+    for (int q = 0; q < PQ; ++q) { a[q] = hist[PQ * threadIdx.x + q]; loc += a[q]; }
+    const uint32_t ex = blk_excl_scan(loc, sh + 16);
+    if (ex <= kth && kth < ex + loc) {
+        uint32_t c = ex;
+        int d = 0;
+#pragma unroll
+        for (int q = 0; q < PQ; ++q) {
+            if (kth >= c && kth < c + a[q]) { d = q; break; }
+            c += a[q];
+        }
+        sh[0] = PQ * threadIdx.x + d;
+        sh[1] = kth - c;
+    }
     __syncthreads();
-    return r;
+    // sh[0..1] are next written after the next pass's two barriers, and hist
+    // was read before this pass's scan barrier: no trailing barrier needed
+    kth = sh[1];
+    return sh[0];
 }
 
 __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t kth, uint32_t* hist,
                                uint32_t* sh) {
-    uint32_t prefix = 0, mask = 0;
-    const int shifts[3] = {20, 10, 0};
-    const int bits[3] = {12, 10, 10};
-    constexpr int PQ = kHist / kST;   // bins per thread in the scan
-#pragma unroll
-    for (int pass = 0; pass < 3; ++pass) {
-        const int s_ = shifts[pass];
-        const uint32_t dmask = (1u << bits[pass]) - 1u;
-        for (int i = threadIdx.x; i < kHist; i += kST) hist[i] = 0;
-        __syncthreads();
-        // pass 0 counts every key; later passes only the selected bucket's
-#pragma unroll
-        for (int r = 0; r < kSPer; ++r) {
-            const uint32_t key = keys[r];
-            if ((key & mask) == prefix) atomicAdd(&hist[(key >> s_) & dmask], 1u);
-        }
-        __syncthreads();
-        uint32_t a[PQ], loc = 0;
-#pragma unroll
-        for (int q = 0; q < PQ; ++q) { a[q] = hist[PQ * threadIdx.x + q]; loc += a[q]; }
-        uint32_t ex = blk_excl_scan(loc, sh + 16);
-        if (ex <= kth && kth < ex + loc) {
-            uint32_t c = ex;
-            int d = 0;
-#pragma unroll
-            for (int q = 0; q < PQ; ++q) {
-                if (kth >= c && kth < c + a[q]) { d = q; break; }
-                c += a[q];
-            }
-            sh[0] = PQ * threadIdx.x + d;
-            sh[1] = kth - c;
-        }
-        __syncthreads();
-        uint32_t dsel = sh[0];
-        kth = sh[1];
-        prefix |= dsel << s_;
-        mask |= dmask << s_;
-        __syncthreads();
-    }
-    return prefix;
+    const uint32_t d0 = radix_pass<20, 12>(keys, 0u, 0u, kth, hist, sh);
+    const uint32_t p1 = d0 << 20;
+    const uint32_t d1 = radix_pass<10, 10>(keys, p1, 0xFFF00000u, kth, hist, sh);
+    const uint32_t p2 = p1 | (d1 << 10);
+    const uint32_t d2 = radix_pass<0, 10>(keys, p2, 0xFFFFFC00u, kth, hist, sh);
+    return p2 | d2;
 }
 
 __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev b, FusedDev fu,
@@ -262,15 +264,19 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
         // s < 8192 and nbins <= 128 (checked exhaustively)
         const uint32_t M = ((1u << 20) + (uint32_t)nbins - 1u) / (uint32_t)nbins;
         uint32_t keys[kSPer];
+        const int slast = (int)ns - 1;
 #pragma unroll
         for (int r = 0; r < kSPer; ++r) {
             const int s = threadIdx.x + kST * r;
-            int pos = s;
+            // unconditional load (past the samples: the last one's address)
+            const int sc = s < slast ? s : slast;
+            int pos = sc;
             if (nbs != nbins) {
-                const int cc = (int)(((uint32_t)s * M) >> 20);
-                pos = s + cc * (nbs - nbins);
+                const int cc = (int)(((uint32_t)sc * M) >> 20);
+                pos = sc + cc * (nbs - nbins);
             }
-            keys[r] = s < ns ? __float_as_uint(S[pos]) : 0xFFFFFFFFu;
+            const uint32_t v = __float_as_uint(S[pos]);
+            keys[r] = s < ns ? v : 0xFFFFFFFFu;
         }
         const uint32_t half = (uint32_t)(ns / 2);
         const uint32_t up = select_kth(keys, half, hist, sh);
