@@ -103,6 +103,19 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// A read-only descriptor block through a pointer the compiler cannot see
+// through, so loads through it are not hoisted out of a loop into registers
+// held for the whole kernel; the first-lane reads keep it uniform and the
+// constant address space makes its loads scalar (the kernel never writes it).
+template <class T>
+__device__ __forceinline__ const CS_CONST T* uniform_opaque(const T* p) {
+    uint64_t v = (uint64_t)(uintptr_t)p;
+    asm volatile("" : "+v"(v));
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+    return (const CS_CONST T*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
 // An LDS pointer the optimiser cannot see through: keeps a constant base
 // from being propagated into out-of-line callees, which would then fetch the
 // dynamic-LDS base from its offset table at every access.
@@ -921,6 +934,11 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             CS_LDS u32x4* segtab = opaque_lds((CS_LDS u32x4*)(smem + lay_seg) + wv * kMaxFolds);
             float bnd[kMaxFolds];
             int cursor = 0;
+            // the fold descriptors through pointers the compiler cannot follow
+            // across tiles: their fields are re-read (scalar cache) where used
+            // instead of being held in scalar registers for the whole kernel
+            const CS_CONST FoldSet* fsx = uniform_opaque(fsd);
+            const CS_CONST MfmaDev* mdx = uniform_opaque(mdd);
             // the trial's block in this batch (local block cl)
             const int clv = slotv - b.slot0;
             const bool inb = tin && (uint32_t)clv < (uint32_t)b.nblocks;
@@ -929,19 +947,19 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             for (int f = 0; f < kMaxFolds; ++f) {
                 bnd[f] = -INFINITY;
                 if (f >= nf) continue;
-                const int NB = fs.f[f].nbins;
+                const int NB = fsx->f[f].nbins;
                 const int64_t nv = nvf[f];
                 const bool act = inb && jj < nv;
                 // sampled column of this trial (sample_column for blocks that are
                 // not full, else the host table)
                 int sc = scf[f];
-                if (act && nv != np) sc = sample_column(jj, nv, NB, fs.cap[f]);
+                if (act && nv != np) sc = sample_column(jj, nv, NB, fsx->cap[f]);
                 CS_GLB float* gsamp =
                     (CS_GLB float*)(fu.samples + ((int64_t)cl * nf + f) * fu.sld);
-                const CS_LDS u32x4* th = opaque_lds((CS_LDS u32x4*)(dhi + (size_t)md.c_base[f] * 64));
-                const CS_LDS u32x4* tl = opaque_lds((CS_LDS u32x4*)(dlo + (size_t)md.c_base[f] * 64));
-                const int k = fs.f[f].k;
-                const uint32_t bm = fs.bankmask[f];
+                const CS_LDS u32x4* th = opaque_lds((CS_LDS u32x4*)(dhi + (size_t)mdx->c_base[f] * 64));
+                const CS_LDS u32x4* tl = opaque_lds((CS_LDS u32x4*)(dlo + (size_t)mdx->c_base[f] * 64));
+                const int k = fsx->f[f].k;
+                const uint32_t bm = fsx->bankmask[f];
                 const bool narrow = (bm >> 5) == 0;   // bank within widths 1, 2, 3, 4, 6
                 FoldOut fo{-INFINITY, cursor, 0ull};
                 switch (NB) {
