@@ -160,8 +160,8 @@ __global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g,
 // a max.  The 12-bit leading digit (sign, exponent, three mantissa bits)
 // spreads a wave's keys over enough bins that same-address atomics stay few.
 // ---------------------------------------------------------------------------
-constexpr int kST = 512;
-constexpr int kSPer = 16;
+constexpr int kST = 256;
+constexpr int kSPer = 32;
 constexpr int kHist = 4096;
 static_assert(kST * kSPer == 8192, "the select's division by nbins is exact below 8192 samples");
 
