@@ -93,3 +93,52 @@ def test_irrational_step_plain_vs_oracle(small_spectrum):
         e.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
         with pytest.raises(ValueError):
             e.search_raw(r_lo, r_hi, 6.0)
+
+
+@pytest.fixture(scope="module")
+def big_spectrum():
+    """2^28 samples (2^27 bins, 1 GiB): unit-power noise and a comb of 60
+    harmonics of r0 = 2e6 + 37.25 (amplitude 3 at the nearest bins)."""
+    N = 1 << 28
+    rng = np.random.default_rng(2028)
+    nb = N // 2
+    amps = np.empty(nb, dtype=np.complex64)
+    for i0 in range(0, nb, 1 << 24):   # in 16M-bin slabs: bounded host memory
+        n = min(1 << 24, nb - i0)
+        amps[i0:i0 + n] = ((rng.standard_normal(n) + 1j * rng.standard_normal(n))
+                           / np.sqrt(2)).astype(np.complex64)
+    amps[0] = 0
+    r0 = 2.0e6 + 37.25
+    for h in range(1, 61):
+        amps[int(round(h * r0))] += np.complex64(3.0 * np.exp(1j * 0.37 * h))
+    return amps, N, 64e-6, r0
+
+
+@pytest.mark.parametrize("mode", ["plain", "fused", "fused_valu"])
+def test_harmonic_bins_near_1e8_vs_oracle(big_spectrum, mode):
+    """Fundamentals near r = 2e6, so the 60 harmonics reach h r = 1.2e8 bins:
+    the integer harmonic-bin arithmetic (directinterp.jl:414-418, 470-496)
+    must stay exact there, and the comb at r0 is the strongest detection."""
+    import coherentsearch as cs
+    import oracle as O
+    amps, N, dt, r0 = big_spectrum
+    nharms = 60
+    dec = cs.decimation_set(nharms, 2)
+    p = cs.SearchParams(nharms=nharms, m=16, decimations=dec)
+    r_lo, r_hi = r0 - 40.0, r0 + 40.0
+    total = math.floor((r_hi - r_lo) / (0.5 / nharms)) + 1
+    _say(f"large r mode={mode} N=2^28 r0={r0} trials={total} max h r={nharms * r_hi:.4g}")
+    eng = cs.Engine(p, 2048)
+    eng.set_mode(mode)
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
+    got = eng.search_raw(r_lo, r_hi, 6.0)
+    want = O.search_region(O.make_params(nharms=nharms, m=16, decimations=dec), amps, N, r_lo,
+                           total, 2048, threshold=6.0)
+    assert len(want) > 0
+    worst, ncommon = _compare(got, want, 6.0)
+    best = want[np.argmax(want["metric"])]
+    rbest = r_lo + int(best["t"]) * (0.5 / nharms)
+    assert abs(rbest - r0) < 0.5 and int(best["k"]) == 1
+    _say(f"  {len(got)} vs {len(want)} candidates, common {ncommon}, worst rel {worst:.2e}, "
+         f"best r {rbest:.3f} S/N {float(best['metric']):.1f}")
+    eng.close()
