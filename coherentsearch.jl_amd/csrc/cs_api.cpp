@@ -82,6 +82,7 @@ struct cs_ctx {
     } sh;
     DBuf pyr, tfeat, mlp_w, mlp_x, mlp_p;   // block triage: power pyramid, features, MLP buffers
     DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol;
+    std::vector<uint8_t> m_desc_last;   // host copy of what m_desc holds
     // split-BF16 batches: identity list (u32 i at i, m_ident_n entries), and
     // for a block subset its block list, slot table and per-batch tile lists
     DBuf m_ident, m_sel, m_slot, m_tiles;
@@ -1302,7 +1303,12 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         std::vector<uint8_t> desc(sizeof(FoldSet) + sizeof(MfmaDev));
         std::memcpy(desc.data(), &fs, sizeof(FoldSet));
         std::memcpy(desc.data() + sizeof(FoldSet), &md, sizeof(MfmaDev));
-        if ((s = upload(c, c->m_desc, desc))) return s;
+        // the same descriptors as the last search need no upload (and no sync)
+        if (!c->m_desc.p || desc != c->m_desc_last) {
+            c->m_desc_last.clear();
+            if ((s = upload(c, c->m_desc, desc))) return s;
+            c->m_desc_last = desc;
+        }
     }
     // The launch sequence of one attempt: the row masks, the counters, then
     // every batch's kernels; stage events only when not captured into a graph.
