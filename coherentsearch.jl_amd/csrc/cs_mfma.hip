@@ -679,8 +679,7 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int nharms, int dft_tiles) {
     L.ftp = (size_t)2 * nharms * kFtpLd * 4;
     L.dft = (size_t)dft_tiles * 2 * 64 * 16;
     L.region = (size_t)(kThreads / 64) * kStageFloats * 4;   // per-wave sample staging
-    L.seg = (size_t)(kThreads / 64) * kMaxFolds * 16;        // per-wave {beg, end, dst} per fold
-    L.total = L.ftp + L.region + L.dft + L.seg;
+    L.total = L.ftp + L.region + L.dft;
     return L;
 }
 
@@ -705,7 +704,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     uint4* dhi = (uint4*)(smem + (size_t)2 * nh * kFtpLd * 4);
     uint4* dlo = dhi + (size_t)ntc * 64;
     const size_t lay_stage = (size_t)2 * nh * kFtpLd * 4 + (size_t)ntc * 2 * 64 * 16;
-    const size_t lay_seg = lay_stage + (size_t)(kThreads / 64) * kStageFloats * 4;
     for (int i = tid; i < ntc * 64; i += kThreads) {
         dhi[i] = ldg4(md.cA_hi + i);
         dlo[i] = ldg4(md.cA_lo + i);
@@ -929,9 +927,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const CS_LDS uint32_t* lre = opaque_lds((CS_LDS uint32_t*)fre);
             const CS_LDS uint32_t* lim = opaque_lds((CS_LDS uint32_t*)fim);
             CS_LDS float* lstg = opaque_lds((CS_LDS float*)(smem + lay_stage) + wv * kStageFloats);
-            // where each fold's staged columns sit and go: LDS, not registers,
-            // because they live across every fold call
-            CS_LDS u32x4* segtab = opaque_lds((CS_LDS u32x4*)(smem + lay_seg) + wv * kMaxFolds);
+            // where each fold's staged columns sit and go: lane f of four
+            // vector registers holds fold f's {begin, end, destination}
+            uint32_t seg_beg = 0, seg_end = 0, seg_dlo = 0, seg_dhi = 0;
             float bnd[kMaxFolds];
             int cursor = 0;
             // the fold descriptors through pointers the compiler cannot follow
@@ -975,9 +973,11 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 }
                 bnd[f] = act ? fo.bound : -INFINITY;
                 const int end = __builtin_amdgcn_readfirstlane(fo.stg_end);
-                if (lane == 0)
-                    segtab[f] = u32x4{(uint32_t)cursor, (uint32_t)end, (uint32_t)fo.dst,
-                                      (uint32_t)(fo.dst >> 32)};
+                const bool mine = lane == f;
+                seg_beg = mine ? (uint32_t)cursor : seg_beg;
+                seg_end = mine ? (uint32_t)end : seg_end;
+                seg_dlo = mine ? (uint32_t)fo.dst : seg_dlo;
+                seg_dhi = mine ? (uint32_t)(fo.dst >> 32) : seg_dhi;
                 cursor = end;
             }
             // next tile's loads go out before this tile's stores
@@ -991,13 +991,12 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                     if (f < nf) fu.mraw[(int64_t)f * fu.ldm + col_out] = bnd[f];
             }
             for (int f = 0; f < nf; ++f) {
-                const u32x4 e = segtab[f];
                 const uint64_t dsta =
-                    ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)e.w) << 32) |
-                    (uint32_t)__builtin_amdgcn_readfirstlane((int)e.z);
+                    ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)seg_dhi, f) << 32) |
+                    (uint32_t)__builtin_amdgcn_readlane((int)seg_dlo, f);
                 if (dsta == 0ull) continue;
-                const int beg = __builtin_amdgcn_readfirstlane((int)e.x);
-                const int nq = (__builtin_amdgcn_readfirstlane((int)e.y) - beg) / 4;
+                const int beg = __builtin_amdgcn_readlane((int)seg_beg, f);
+                const int nq = (__builtin_amdgcn_readlane((int)seg_end, f) - beg) / 4;
                 CS_GLB f32x4* dst = (CS_GLB f32x4*)(uintptr_t)dsta;
                 const CS_LDS f32x4* src = (const CS_LDS f32x4*)(lstg + beg);
                 for (int q = lane; q < nq; q += 64) dst[q] = src[q];

@@ -40,7 +40,7 @@ struct MfmaDev {
 };
 
 struct MfmaLds {
-    size_t ftp, region, dft, seg, total;   // seg: per-wave staged-segment table
+    size_t ftp, region, dft, total;
 };
 
 bool mfma_supported(const FoldSet& fs, int nharms);
