@@ -984,6 +984,11 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             // bound column: [local block][trial of the block]
             const int64_t col_out = (int64_t)cl * np + jj;
             if (tile + t_step < t_end) fetch(tile + t_step);
+            // every wave is past its folds: the next tile's interpolation may
+            // overwrite the stacked amplitudes.  The stores below touch only
+            // this wave's staging area and HBM, so they overlap the other
+            // waves' next interpolation instead of holding them at a barrier.
+            lds_barrier();
             // outputs of the tile: bounds, then the staged sigma samples
             if (inb && g4 == 0) {
 #pragma unroll
@@ -1002,7 +1007,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 for (int q = lane; q < nq; q += 64) dst[q] = src[q];
             }
         }
-        lds_barrier();
     }
 }
 
