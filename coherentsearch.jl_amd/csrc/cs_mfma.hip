@@ -353,12 +353,14 @@ __device__ __forceinline__ void stage_samples(FoldOut& out, bool active, int sco
     const bool smp = active && scol >= 0;
     const uint64_t bal = __ballot(smp && g == 0);   // one bit per sampled column
     if (bal == 0) return;
+    // the first sampled lane is wave-uniform: read its values with v_readlane
+    // rather than an LDS permute
     const int first = __builtin_ctzll(bal);
-    const int c_first = __shfl(scol, first, 64);
+    const int c_first = __builtin_amdgcn_readlane(scol, first);
     const uint64_t sa = (uint64_t)(uintptr_t)samp;
     const uint64_t s_first =
-        ((uint64_t)(uint32_t)__shfl((int)(uint32_t)(sa >> 32), first, 64) << 32) |
-        (uint32_t)__shfl((int)(uint32_t)sa, first, 64);
+        ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(sa >> 32), first) << 32) |
+        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)sa, first);
     const int rank = __builtin_popcountll(bal & ((1ull << col) - 1ull));
     const bool contig = !smp || (scol == c_first + rank && sa == s_first);
     const int need = __builtin_popcountll(bal) * NBS;
