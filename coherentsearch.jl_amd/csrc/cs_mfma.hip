@@ -93,6 +93,19 @@ __device__ __forceinline__ float pair_swap(float x) {
                                                               0xF, 0xF, true));
 }
 
+// Maximum over the four 16-lane rows, in every lane (the whole wave active):
+// v_permlane16_swap of x with itself yields rows (0,0,2,2) and (1,1,3,3),
+// v_permlane32_swap rows (0,1,0,1) and (2,3,2,3); no LDS round trip.
+__device__ __forceinline__ float max_over_rows(float x) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    const auto a = __builtin_amdgcn_permlane16_swap(u, u, false, false);
+    const float y = fmaxf(__builtin_bit_cast(float, (uint32_t)a[0]),
+                          __builtin_bit_cast(float, (uint32_t)a[1]));
+    const uint32_t v = __builtin_bit_cast(uint32_t, y);
+    const auto b = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return fmaxf(__builtin_bit_cast(float, (uint32_t)b[0]), __builtin_bit_cast(float, (uint32_t)b[1]));
+}
+
 // Workgroup barrier that orders LDS only.  No wave of the fused kernel reads
 // global memory another wave wrote, so the barrier need not wait for this
 // wave's outstanding global stores (bounds, sigma samples), which a full
@@ -488,8 +501,7 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
 #pragma unroll
     for (int a = 0; a < bank_count(WM <= 6 ? 6 : WM); ++a)
         if (bankmask & (1u << a)) res = fmaxf(res, m[a] * (1.0f / sqrtf((float)bank_w(a))));
-    res = fmaxf(res, __shfl_xor(res, 16, 64));
-    res = fmaxf(res, __shfl_xor(res, 32, 64));
+    res = max_over_rows(res);
     out.bound = res;
     return out;
 }
@@ -631,8 +643,7 @@ __device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ 
 #pragma unroll
     for (int a = 0; a < bank_count(WM <= 6 ? 6 : WM); ++a)
         if (bankmask & (1u << a)) res = fmaxf(res, m[a] * (1.0f / sqrtf((float)bank_w(a))));
-    res = fmaxf(res, __shfl_xor(res, 16, 64));
-    res = fmaxf(res, __shfl_xor(res, 32, 64));
+    res = max_over_rows(res);
     out.bound = res;
     return out;
 }
