@@ -165,17 +165,33 @@ constexpr int kSPer = 32;
 constexpr int kHist = 4096;
 static_assert(kST * kSPer == 8192, "the select's division by nbins is exact below 8192 samples");
 
+// Inclusive scan over the 64 lanes of a wave (whole wave active) with DPP
+// moves, no LDS: row_shr 1, 2, 4, 8 within each 16-lane row, then
+// row_bcast 15 into rows 1 and 3 and row_bcast 31 into rows 2 and 3.  Lanes
+// a move does not reach read 0, the identity of both ops used here.
+template <class Op>
+__device__ __forceinline__ uint32_t wave_scan(uint32_t v, Op op) {
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = op(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return v;
+}
+struct AddU32 {
+    __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a + b; }
+};
+struct MaxU32 {
+    __device__ uint32_t operator()(uint32_t a, uint32_t b) const { return a > b ? a : b; }
+};
+
 // Exclusive prefix sum over the workgroup.  One barrier: every thread adds
 // the totals of the waves before its own.  `tot` (kST/64 words) is rewritten
 // by the next call only after the caller's next barrier.
 __device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* tot) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= o) x += y;
-    }
+    const uint32_t x = wave_scan(v, AddU32{});
     if (lane == 63) tot[wid] = x;
     __syncthreads();
     uint32_t pre = 0;
@@ -292,14 +308,11 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
                 uint32_t key = keys[r];
                 if (key < up) { below++; mx = key > mx ? key : mx; }
             }
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) {
-                below += __shfl_xor(below, off, 64);
-                uint32_t om = __shfl_xor(mx, off, 64);
-                mx = om > mx ? om : mx;
-            }
+            // wave totals in lane 63
+            below = wave_scan(below, AddU32{});
+            mx = wave_scan(mx, MaxU32{});
             const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-            if (lane == 0) { hist[wid] = below; hist[16 + wid] = mx; }
+            if (lane == 63) { hist[wid] = below; hist[16 + wid] = mx; }
             __syncthreads();
             uint32_t tb_ = 0, tm = 0;
             for (int w = 0; w < kST / 64; ++w) {
