@@ -241,7 +241,9 @@ __device__ __forceinline__ uint32_t radix_pass(const uint32_t (&keys)[kSPer], ui
     return sh[0];
 }
 
-__device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t kth, uint32_t* hist,
+// The key of rank kth (0-based); kth is left as its rank among the keys equal
+// to it (the last digit's bin holds exactly those keys).
+__device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t& kth, uint32_t* hist,
                                uint32_t* sh) {
     const uint32_t d0 = radix_pass<20, 12>(keys, 0u, 0u, kth, hist, sh);
     const uint32_t p1 = d0 << 20;
@@ -295,32 +297,28 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
             keys[r] = s < ns ? v : 0xFFFFFFFFu;
         }
         const uint32_t half = (uint32_t)(ns / 2);
-        const uint32_t up = select_kth(keys, half, hist, sh);
+        uint32_t rk = half;   // becomes the rank of the selected key among its equals
+        const uint32_t up = select_kth(keys, rk, hist, sh);
         double med;
-        if (ns & 1) {
+        if ((ns & 1) || rk > 0) {
+            // odd count; or the key of rank half - 1 equals `up` as well
             med = (double)__uint_as_float(up);
         } else {
-            // lower central statistic: `up` itself if at least half keys are
-            // below-or-equal... i.e. if fewer than `half` keys are strictly below it
-            uint32_t below = 0, mx = 0;
+            // exactly `half` keys lie below `up`: the lower central value is
+            // the largest of them (rk is block-uniform, so is this branch)
+            uint32_t mx = 0;
 #pragma unroll
             for (int r = 0; r < kSPer; ++r) {
-                uint32_t key = keys[r];
-                if (key < up) { below++; mx = key > mx ? key : mx; }
+                const uint32_t key = keys[r];
+                mx = key < up && key > mx ? key : mx;
             }
-            // wave totals in lane 63
-            below = wave_scan(below, AddU32{});
-            mx = wave_scan(mx, MaxU32{});
+            mx = wave_scan(mx, MaxU32{});   // the wave's maximum in lane 63
             const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-            if (lane == 63) { hist[wid] = below; hist[16 + wid] = mx; }
+            if (lane == 63) hist[16 + wid] = mx;
             __syncthreads();
-            uint32_t tb_ = 0, tm = 0;
-            for (int w = 0; w < kST / 64; ++w) {
-                tb_ += hist[w];
-                tm = hist[16 + w] > tm ? hist[16 + w] : tm;
-            }
-            uint32_t lo = tb_ >= half ? tm : up;
-            med = 0.5 * ((double)__uint_as_float(lo) + (double)__uint_as_float(up));
+            uint32_t tm = 0;
+            for (int w = 0; w < kST / 64; ++w) tm = hist[16 + w] > tm ? hist[16 + w] : tm;
+            med = 0.5 * ((double)__uint_as_float(tm) + (double)__uint_as_float(up));
         }
         sigma = 1.4826 * med;
     }
