@@ -780,13 +780,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     int32_t nvf[kMaxFolds], scf[kMaxFolds];
     uint64_t rmv[4];
     uint4 wa[8], wb[8];
-    // window start (x) and advance (y) of this wave's harmonics, one value
-    // per lane (lane 2s + c holds component c of harmonic slot s; slots past
-    // nharms repeat the last harmonic), loaded one tile ahead of their use
+    // the harmonic descriptors of this wave's harmonics (window start x,
+    // advance y, weight tile z, conj(A) row w), one value per lane (lane
+    // 4s + c holds component c of harmonic slot s; slots past nharms repeat
+    // the last harmonic), loaded one tile ahead of their use
     auto load_desc = [&](int64_t tile) {
         const int uu = (int)((uint32_t)tile % (uint32_t)U);   // tile counts stay below 2^31
-        const int s = (lane >> 1) & 7, h = wv + 1 + 8 * s;
-        return glb((const int*)hdb)[((size_t)((h <= nh ? h : nh) - 1) * U + uu) * 4 + (lane & 1)];
+        const int s = (lane >> 2) & 7, h = wv + 1 + 8 * s;
+        return glb((const int*)hdb)[((size_t)((h <= nh ? h : nh) - 1) * U + uu) * 4 + (lane & 3)];
     };
     // tile number at list position k (clamped to the last one: past the
     // range a fetch reloads it), the same address in every lane
@@ -795,6 +796,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     // (in flight since the previous fetch)
     uint32_t idc = 0, idn = 0;
     int dnext = 0;
+    int dcur = 0;   // the descriptors of the tile last fetched (its interpolation reads them)
     if (t_first < t_end) {
         idc = __builtin_amdgcn_readfirstlane(tile_at(t_first));
         dnext = load_desc(idc);
@@ -806,6 +808,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         st = stu;
         u = (int)(tl - stu * (uint32_t)U);
         const int dreg = dnext;
+        dcur = dreg;
         // unconditional: a load under a branch leaves the merged wait state
         // pending at once
         idc = __builtin_amdgcn_readfirstlane(idn);
@@ -843,8 +846,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         const int occ32 = (int)occ;
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-            int b0 = __builtin_amdgcn_readlane(dreg, 2 * s) +
-                     __builtin_amdgcn_readlane(dreg, 2 * s + 1) * occ32;
+            int b0 = __builtin_amdgcn_readlane(dreg, 4 * s) +
+                     __builtin_amdgcn_readlane(dreg, 4 * s + 1) * occ32;
             b0 = b0 < bmin32 ? bmin32 : (b0 > bmax32 ? bmax32 : b0);
             const uint32_t vo = lane_off + 4u * (uint32_t)b0;
             wa[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, vo, 0, 0));
@@ -863,8 +866,13 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         };
         Op ring[3];
         auto issue = [&](int s, Op& x) {
-            const int h = wv + 1 + 8 * s;
-            const int4 d = ld_const4(md.hdesc, (size_t)((h <= nh ? h : nh) - 1) * U + u);
+            // this wave's descriptors of the tile, lane 4s + component, from
+            // the fetch (v_readlane: no scalar load, so no wait that would also
+            // drain this wave's LDS stores)
+            const int4 d = make_int4(__builtin_amdgcn_readlane(dcur, 4 * s),
+                                     __builtin_amdgcn_readlane(dcur, 4 * s + 1),
+                                     __builtin_amdgcn_readlane(dcur, 4 * s + 2),
+                                     __builtin_amdgcn_readlane(dcur, 4 * s + 3));
             x.d = d;
             x.ah = ldg4(md.wA_hi + (size_t)d.z * 64 + lane);
             x.al = ldg4(md.wA_lo + (size_t)d.z * 64 + lane);
