@@ -482,3 +482,35 @@ def test_search_detects_and_thread_invariant(small_spectrum):
     i0 = (t // 2048) * 2048
     r = k * (5 * T + i0 * (0.5 / 32) + (t - i0) * (0.5 / 32))
     assert abs(r / T - 13.7) < 1 / T
+
+
+def test_search_region_sigmas_layout():
+    """The sigmas output of the oracle's search ([chunk][fold], fold 1 then the
+    decimations > 1): every chunk's row equals that chunk searched alone, and
+    each entry is 1.4826 x numpy's median |x| of that chunk's fold profiles
+    built from the oracle's own interpolation rows and c2r."""
+    N = 1 << 16
+    amps = noise_amps(N // 2, 41)
+    p = O.make_params(nharms=8, m=16, numbetween=16, threshold=5.0, decimations=(1, 2, 3))
+    r_lo, Nprof = 900.0, 256
+    total = 5 * Nprof - 17                      # a ragged last chunk
+    s_all = np.zeros((5, 3))
+    O.search_region(p, amps, N, r_lo, total, Nprof, sigmas=s_all)
+    assert np.all(s_all > 0)
+    for c in range(5):
+        s_one = np.zeros((5, 3))
+        O.search_region(p, amps, N, r_lo, total, Nprof, c_begin=c, c_end=c + 1, sigmas=s_one)
+        assert np.array_equal(s_one[c], s_all[c]) and not s_one[np.arange(5) != c].any()
+    # every fold's profiles hold at most 16 x 256 = 4096 <= 8192 samples, so
+    # sigma is 1.4826 x numpy's median |x| over all of them (search.jl:490-566)
+    pl = O.Plans(8, 16, 0.5, r_lo)
+    lodr = 0.5 / 8
+    for c in range(5):
+        n = min(Nprof, total - c * Nprof)
+        X = pl.fill_chunk(amps, N, c * Nprof, n)
+        for d, k in enumerate((1, 2, 3)):
+            P = O.brfft_rows(X, 2 * (8 // k), k)
+            rstart = r_lo + c * Nprof * lodr
+            nv = sum(1 for j in range(n) if k * (rstart + j * lodr) < N / 2)
+            want = 1.4826 * float(np.median(np.abs(P[:, :nv])))
+            assert s_all[c, d] == pytest.approx(want, rel=1e-14), (c, k)
