@@ -106,6 +106,18 @@ __device__ __forceinline__ float max_over_rows(float x) {
     return fmaxf(__builtin_bit_cast(float, (uint32_t)b[0]), __builtin_bit_cast(float, (uint32_t)b[1]));
 }
 
+// Row SRC (16 lanes) of x in every row (whole wave active): rows (0,1,0,1)
+// or (2,3,2,3) by v_permlane32_swap, then the wanted one of each pair by
+// v_permlane16_swap.
+template <int SRC>
+__device__ __forceinline__ float row_broadcast(float x) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, x);
+    const auto h = __builtin_amdgcn_permlane32_swap(u, u, false, false);
+    const uint32_t v = SRC < 2 ? (uint32_t)h[0] : (uint32_t)h[1];
+    const auto q = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return __builtin_bit_cast(float, (SRC & 1) ? (uint32_t)q[1] : (uint32_t)q[0]);
+}
+
 // Workgroup barrier that orders LDS only.  No wave of the fused kernel reads
 // global memory another wave wrote, so the barrier need not wait for this
 // wave's outstanding global stores (bounds, sigma samples), which a full
@@ -469,7 +481,7 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
     // M[n] at H + n
     FoldOut out{-INFINITY, stg_off, 0ull};
     {
-        const float pH = __shfl(P[H % R], col + 16 * (H / R), 64);
+        const float pH = row_broadcast<(H / R) & 3>(P[H % R]);
         stage_samples<(NB + 15) / 16 * 16>(
             out, active, scol, g, col, samp, stg, stg_off,
             [&](CS_LDS float* d) { put_column<H, R, 3>(d, g, P, M, pH); },
