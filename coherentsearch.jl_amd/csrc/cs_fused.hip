@@ -205,8 +205,8 @@ __device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* tot) {
 // digit that holds rank `kth` and leaves in `kth` the rank inside it.
 template <int SH, int BITS>
 __device__ __forceinline__ uint32_t radix_pass(const uint32_t (&keys)[kSPer], uint32_t prefix,
-                                               uint32_t mask, uint32_t& kth, uint32_t* hist,
-                                               uint32_t* sh) {
+                                               uint32_t mask, uint32_t& kth, uint32_t& cnt,
+                                               uint32_t* hist, uint32_t* sh) {
     constexpr int NBIN = 1 << BITS, PQ = NBIN / kST;
     static_assert(PQ >= 1 && NBIN <= kHist, "digit width");
     constexpr uint32_t dmask = (uint32_t)NBIN - 1u;
@@ -224,20 +224,22 @@ __device__ __forceinline__ uint32_t radix_pass(const uint32_t (&keys)[kSPer], ui
     for (int q = 0; q < PQ; ++q) { a[q] = hist[PQ * threadIdx.x + q]; loc += a[q]; }
     const uint32_t ex = blk_excl_scan(loc, sh + 16);
     if (ex <= kth && kth < ex + loc) {
-        uint32_t c = ex;
+        uint32_t c = ex, n = 0;
         int d = 0;
 #pragma unroll
         for (int q = 0; q < PQ; ++q) {
-            if (kth >= c && kth < c + a[q]) { d = q; break; }
+            if (kth >= c && kth < c + a[q]) { d = q; n = a[q]; break; }
             c += a[q];
         }
         sh[0] = PQ * threadIdx.x + d;
         sh[1] = kth - c;
+        sh[2] = n;
     }
     __syncthreads();
-    // sh[0..1] are next written after the next pass's two barriers, and hist
+    // sh[0..2] are next written after the next pass's two barriers, and hist
     // was read before this pass's scan barrier: no trailing barrier needed
     kth = sh[1];
+    cnt = sh[2];
     return sh[0];
 }
 
@@ -245,11 +247,30 @@ __device__ __forceinline__ uint32_t radix_pass(const uint32_t (&keys)[kSPer], ui
 // to it (the last digit's bin holds exactly those keys).
 __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t& kth, uint32_t* hist,
                                uint32_t* sh) {
-    const uint32_t d0 = radix_pass<20, 12>(keys, 0u, 0u, kth, hist, sh);
+    uint32_t cnt;
+    const uint32_t d0 = radix_pass<20, 12>(keys, 0u, 0u, kth, cnt, hist, sh);
     const uint32_t p1 = d0 << 20;
-    const uint32_t d1 = radix_pass<10, 10>(keys, p1, 0xFFF00000u, kth, hist, sh);
+    const uint32_t d1 = radix_pass<10, 10>(keys, p1, 0xFFF00000u, kth, cnt, hist, sh);
     const uint32_t p2 = p1 | (d1 << 10);
-    const uint32_t d2 = radix_pass<0, 10>(keys, p2, 0xFFFFFC00u, kth, hist, sh);
+    if (cnt == 1) {
+        // one key has the selected 22-bit prefix: it is the answer (rank 0
+        // among its equals); the maximum over the keys with that prefix (the
+        // others count as 0) finds it without the third pass.  cnt is
+        // block-uniform, so is this branch.
+        uint32_t m = 0;
+#pragma unroll
+        for (int r = 0; r < kSPer; ++r)
+            m = (keys[r] & 0xFFFFFC00u) == p2 && keys[r] > m ? keys[r] : m;
+        m = wave_scan(m, MaxU32{});
+        if ((threadIdx.x & 63) == 63) sh[32 + (threadIdx.x >> 6)] = m;
+        __syncthreads();
+        uint32_t up = 0;
+#pragma unroll
+        for (int w = 0; w < kST / 64; ++w) up = sh[32 + w] > up ? sh[32 + w] : up;
+        kth = 0;
+        return up;
+    }
+    const uint32_t d2 = radix_pass<0, 10>(keys, p2, 0xFFFFFC00u, kth, cnt, hist, sh);
     return p2 | d2;
 }
 
