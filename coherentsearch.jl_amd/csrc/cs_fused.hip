@@ -154,10 +154,10 @@ __global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g,
 }
 
 // ---------------------------------------------------------------------------
-// Exact selection of the median |x| of one (block, fold)'s samples: 3-pass
-// radix select on the FP32 bit patterns (12, 10, 10 bits) with plain LDS
-// histogram atomics, then the lower central order statistic from a count and
-// a max.  The 12-bit leading digit (sign, exponent, three mantissa bits)
+// Exact selection of the median |x| of one (block, fold)'s samples: radix
+// select on the FP32 bit patterns (12, 10, 10 bits; the last pass is skipped
+// when the second leaves a single key) with plain LDS histogram atomics, then
+// for an even count the lower central value from the final rank and a max.  The 12-bit leading digit (sign, exponent, three mantissa bits)
 // spreads a wave's keys over enough bins that same-address atomics stay few.
 // ---------------------------------------------------------------------------
 constexpr int kST = 256;
