@@ -148,3 +148,17 @@ def config_c5():
 def make_config_spectrum(cfg: dict) -> np.ndarray:
     return make_spectrum(cfg["N"], cfg["dt"], cfg["seed"], cfg.get("pulsars", ()),
                          red_fc=cfg.get("red_fc"))
+
+
+def make_config_spectra(cfgs: Sequence[dict], workers: int = 8) -> List[np.ndarray]:
+    """make_config_spectrum of every config, in order, on `workers` processes
+    (spawned, so nothing the parent initialised is shared).  Each spectrum
+    depends on its own seed only, so the result equals the serial one."""
+    cfgs = list(cfgs)
+    if workers <= 1 or len(cfgs) <= 1:
+        return [make_config_spectrum(c) for c in cfgs]
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    with ProcessPoolExecutor(max_workers=min(workers, len(cfgs)),
+                             mp_context=mp.get_context("spawn")) as ex:
+        return list(ex.map(make_config_spectrum, cfgs))

@@ -1,0 +1,256 @@
+"""GPU parity at the named workloads, searched in full against the oracle.
+
+C1: 2^20 samples, nharms 8, 1-100 Hz, one 13.7 Hz pulsar; its .inf carries N
+    and dt only, and the file goes through the on-disk reader and the whole
+    host search (dedup included).
+C2: 2^25 samples, nharms 60, decimation_set(60, 6), 0.1-125 Hz, 3 pulsars:
+    32,186,485 trials in 15,717 blocks.
+C5: 2^27 samples with red noise below 1 Hz, 5 pulsars: 128,745,940 trials in
+    62,865 blocks (four split-BF16 batches).
+C3: 64 files of 2^22..2^25 samples, nharms 32, decimation_set(32, 4), through
+    one cs_search_files call; each file against the oracle on its first and
+    last blocks and on the blocks of its pulsar at every fold.
+
+Bars (the reference's search, src/search.jl:1768-1894, restated by the
+oracle): the (t, k) candidate sets are equal away from the threshold, nharm
+and block equal, metric (S/N) within 1e-4 relative; every (block, fold) sigma
+within 1e-5 relative (plain FP32 profiles) or 5e-5 (split-BF16 profiles); every
+injected pulsar whose fundamental lies in the band is detected.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MODES = ("fused", "plain")
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cs():
+    import coherentsearch
+    return coherentsearch
+
+
+@pytest.fixture(scope="module")
+def O():
+    import oracle
+    return oracle
+
+
+def _compare(got, want, threshold, tol=1e-4):
+    """(t, k) sets equal apart from metrics within tol of the threshold;
+    nharm, block and metric agree on the common ones.  Returns (worst relative
+    metric difference, common count, records only one side has)."""
+    g = {(int(c["t"]), int(c["k"])): c for c in got}
+    w = {(int(c["t"]), int(c["k"])): c for c in want}
+    odd = []
+    for key in set(g) ^ set(w):
+        rec = g.get(key, w.get(key))
+        odd.append(key)
+        assert abs(float(rec["metric"]) - threshold) <= tol * threshold, \
+            ("only on the " + ("device" if key in g else "oracle"), key, rec)
+    worst = 0.0
+    for key in set(g) & set(w):
+        assert int(g[key]["nharm"]) == int(w[key]["nharm"]), key
+        assert int(g[key]["block"]) == int(w[key]["block"]), key
+        worst = max(worst, abs(float(g[key]["metric"]) - float(w[key]["metric"]))
+                    / abs(float(w[key]["metric"])))
+    assert worst <= tol, worst
+    return worst, len(set(g) & set(w)), odd
+
+
+def _detected(cs, raw, r_lo, lodr, Nprof, T, f0):
+    """Best metric of a raw candidate within 3 bins of f0 (any fold)."""
+    best = 0.0
+    for rec in raw:
+        r = cs.candidate_r(r_lo, lodr, Nprof, int(rec["t"]), int(rec["k"]))
+        if abs(r / T - f0) < 3.0 / T:
+            best = max(best, float(rec["metric"]))
+    return best
+
+
+class Workload:
+    """A named single-spectrum workload with the oracle's full search of it."""
+
+    def __init__(self, cs, O, cfg):
+        from coherentsearch.synth import make_config_spectrum
+        self.cfg = cfg
+        self.N, self.dt = cfg["N"], cfg["dt"]
+        self.T = self.N * self.dt
+        _say(f"[configs] generating 2^{int(math.log2(self.N))} samples, seed {cfg['seed']}")
+        self.amps = make_config_spectrum(cfg)
+        self.params = cs.SearchParams(nharms=cfg["nharms"], m=16,
+                                      decimations=cs.decimation_set(cfg["nharms"],
+                                                                    cfg["maxdecim"]),
+                                      threshold=cfg["threshold"])
+        (self.r_lo, self.r_hi, self.lodr, self.total, self.Nprof,
+         self.nchunks) = cs.trial_grid(self.T, self.params, cfg["lofreq"], cfg["hifreq"], 100,
+                                       2048)
+        op = O.make_params(nharms=self.params.nharms, m=16, decimations=self.params.decimations,
+                           threshold=cfg["threshold"])
+        self.sigmas = np.zeros((self.nchunks, len(self.params.decimations)))
+        _say(f"[configs] oracle: {self.total} trials, {self.nchunks} blocks")
+        self.want = O.search_region(op, self.amps, self.N, self.r_lo, self.total, self.Nprof,
+                                    threshold=cfg["threshold"], sigmas=self.sigmas)
+        _say(f"[configs] oracle: {len(self.want)} raw candidates")
+
+    def in_band_pulsars(self):
+        return [p.freq for p in self.cfg["pulsars"]
+                if self.cfg["lofreq"] <= p.freq <= self.cfg["hifreq"]]
+
+    def check(self, cs, mode):
+        eng = cs.Engine(self.params, self.Nprof, device=0)
+        try:
+            eng.set_mode(mode)
+            eng.set_spectrum(cs.FFTFile.from_arrays(self.amps, self.N, self.dt))
+            got = eng.search_raw(self.r_lo, self.r_hi, self.cfg["threshold"])
+            sig = eng.last_sigmas()
+        finally:
+            eng.close()
+        worst, common, odd = _compare(got, self.want, self.cfg["threshold"])
+        assert sig.shape == self.sigmas.shape
+        pos = self.sigmas > 0
+        assert np.array_equal(sig > 0, pos)
+        rel = np.abs(sig[pos] - self.sigmas[pos]) / self.sigmas[pos]
+        tol = 5e-5 if mode == "fused" else 1e-5
+        _say(f"[configs] {mode}: {len(got)} vs {len(self.want)} raw candidates, common {common}, "
+             f"at threshold only {len(odd)}, worst S/N rel {worst:.2e}, "
+             f"{pos.sum()} sigmas worst rel {rel.max():.2e}")
+        assert rel.max() <= tol, rel.max()
+        for f0 in self.in_band_pulsars():
+            s = _detected(cs, got, self.r_lo, self.lodr, self.Nprof, self.T, f0)
+            _say(f"[configs] {mode}: pulsar {f0} Hz best S/N {s:.2f}")
+            assert s > self.cfg["threshold"], f"{f0} Hz not detected"
+        return got
+
+
+@pytest.fixture(scope="module")
+def c1(cs, O):
+    from coherentsearch.synth import config_c1
+    return Workload(cs, O, config_c1())
+
+
+@pytest.fixture(scope="module")
+def c2(cs, O):
+    from coherentsearch.synth import config_c2
+    return Workload(cs, O, config_c2())
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_c1_search_vs_oracle(cs, c1, mode):
+    _say(f"C1 mode={mode}")
+    c1.check(cs, mode)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_c1_from_disk_through_host_search(cs, O, c1, mode, tmp_path):
+    """C1 as files: a .fft of 2^19 complex64 values and an .inf holding only N
+    and dt (src/fileio.jl:19-78), read back by FFTFile, then search() with its
+    dedup (src/search.jl:2002-2111, 2148-2214) against the oracle's raw
+    candidates through the same reconstruction and dedup."""
+    from coherentsearch.dedup import remove_duplicates, remove_harmonics
+    from coherentsearch.fileio import write_fft
+    stem = str(tmp_path / "c1_DM0.00")
+    write_fft(stem, c1.amps, c1.N, c1.dt, full_inf=False)
+    inf_lines = open(stem + ".inf").read().splitlines()
+    assert len(inf_lines) == 2
+    ft = cs.FFTFile(stem + ".fft")
+    assert ft.N == c1.N and ft.inf.dt == c1.dt and len(ft.amps) == c1.N // 2
+    assert ft.inf.object is None and ft.inf.DM is None
+    assert np.array_equal(np.asarray(ft.amps), c1.amps)
+    cands = cs.search(ft, c1.params, lofreq=1.0, hifreq=100.0, threshold=8.0, mode=mode)
+    want = [cs.Candidate(cs.candidate_r(c1.r_lo, c1.lodr, c1.Nprof, int(c["t"]), int(c["k"]))
+                         / ft.T, float(c["metric"]),
+                         cs.candidate_r(c1.r_lo, c1.lodr, c1.Nprof, int(c["t"]), int(c["k"])),
+                         int(c["nharm"])) for c in c1.want]
+    want = remove_harmonics(remove_duplicates(want, 1.0), 16, 1.0)
+    _say(f"C1 {mode} from disk: {len(cands)} candidates after dedup, oracle {len(want)}")
+    assert [c.r for c in cands] == [c.r for c in want]
+    assert [c.nharm for c in cands] == [c.nharm for c in want]
+    got_m = np.array([c.metric for c in cands])
+    want_m = np.array([c.metric for c in want])
+    assert np.all(np.abs(got_m - want_m) <= 1e-4 * want_m)
+    best = max(cands, key=lambda c: c.metric)
+    assert abs(best.freq - 13.7) < 2.0 / ft.T
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("mode", MODES)
+def test_c2_search_vs_oracle(cs, c2, mode):
+    _say(f"C2 mode={mode}")
+    c2.check(cs, mode)
+
+
+@pytest.mark.timeout(900)
+def test_c5_search_vs_oracle(cs, O):
+    """C5 in full: 62,865 blocks (the split-BF16 mode runs four batches), with
+    red noise below 1 Hz; both modes against one oracle search."""
+    from coherentsearch.synth import config_c5
+    _say("C5")
+    w = Workload(cs, O, config_c5())
+    for mode in MODES:
+        w.check(cs, mode)
+
+
+@pytest.fixture(scope="module")
+def c3(cs):
+    from coherentsearch.synth import config_c3_file, make_config_spectra
+    _say("[configs] C3: generating 64 spectra")
+    cfgs = [config_c3_file(i) for i in range(64)]
+    amps = make_config_spectra(cfgs, workers=8)
+    params = cs.SearchParams(nharms=32, m=16, decimations=cs.decimation_set(32, 4), threshold=8.0)
+    fts = [cs.FFTFile.from_arrays(a, c["N"], c["dt"]) for a, c in zip(amps, cfgs)]
+    grids = [cs.trial_grid(ft.T, params, 0.1, 125.0, 100, 2048) for ft in fts]
+    return cfgs, amps, params, fts, grids
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("mode", MODES)
+def test_c3_sweep_one_call_vs_oracle(cs, O, c3, mode):
+    """The 64-file DM sweep through one cs_search_files call: each file's raw
+    candidates equal its single-file search, and on its first block, last
+    block and every block holding its pulsar's trial at each fold they match
+    the oracle; every in-band pulsar is detected."""
+    cfgs, amps, params, fts, grids = c3
+    _say(f"C3 mode={mode}")
+    eng = cs.Engine(params, 2048, device=0)
+    try:
+        eng.set_mode(mode)
+        multi = eng.search_files([(ft, g[0], g[1]) for ft, g in zip(fts, grids)], 8.0)
+        _say(f"C3 {mode}: {sum(len(r) for r in multi)} raw candidates over 64 files")
+        assert len(multi) == 64
+        for ft, g, got in zip(fts, grids, multi):
+            eng.set_spectrum(ft)
+            assert np.array_equal(got, eng.search_raw(g[0], g[1], 8.0))
+    finally:
+        eng.close()
+    op = O.make_params(nharms=32, m=16, decimations=params.decimations, threshold=8.0)
+    found = in_band = 0
+    checked = 0
+    for c, a, ft, g, got in zip(cfgs, amps, fts, grids, multi):
+        r_lo, r_hi, lodr, total, Nprof, nch = g
+        f0 = c["pulsars"][0].freq
+        blocks = {0, nch - 1}
+        for k in params.decimations:
+            t = math.floor((f0 * ft.T / k - r_lo) / lodr)
+            if 0 <= t < total:
+                blocks.add(t // Nprof)
+        for b in sorted(blocks):
+            want = O.search_region(op, a, c["N"], r_lo, total, Nprof, b, b + 1, threshold=8.0)
+            mine = got[got["block"] == b]
+            _compare(mine, want, 8.0)
+            checked += 1
+        if 0.1 <= f0 <= 125.0:
+            in_band += 1
+            s = _detected(cs, got, r_lo, lodr, Nprof, ft.T, f0)
+            found += int(s > 8.0)
+            assert s > 8.0, f"file seed {c['seed']}: {f0} Hz not detected"
+    _say(f"C3 {mode}: {checked} blocks equal to the oracle, pulsars {found}/{in_band} in band")
