@@ -35,6 +35,11 @@ for _p in (os.path.join(ROOT, "coherentsearch.jl_amd"), os.path.join(ROOT, "orac
 
 METRIC = "coherent trials/s @2^25 pts nharms=60: split-BF16+batched vs plain FP32 GPU"
 HBM_PEAK_GBS = 8000.0
+# the arithmetic of each mode: interpolation and c2r (the gate's profiles),
+# then the sigma and exact rescan that give every reported metric
+DTYPE = {"fused": "bf16x3/fp32-acc (interp, c2r, gate); f64 (sigma, rescan)",
+         "fused_valu": "f32 (interp, c2r, gate); f64 (sigma, rescan)",
+         "plain": "f32 (interp, c2r, gate); f64 (sigma, rescan)"}
 BF16_MFMA_PEAK_TFLOPS = 2500.0
 
 
@@ -42,36 +47,103 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(cfg, amps, params, r_lo, total, target_s=12.0):
-    """Oracle (C restatement, OpenMP over whole blocks round-robin) on a bounded
-    sample of the workload's blocks taken from the middle of the band."""
-    import oracle as O
+def cpu_model():
     try:
-        cores = len(os.sched_getaffinity(0))
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:  # pragma: no cover
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """Cores this process may run on (its affinity set), bounded by
+    OMP_NUM_THREADS when that is set."""
+    try:
+        n = len(os.sched_getaffinity(0))
     except AttributeError:  # pragma: no cover
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+        n = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(cfg, amps, params, r_lo, total, target_s=4.0, reps=3):
+    """The oracle (the C restatement, OpenMP over whole blocks round-robin) on
+    a bounded run of the workload's blocks from the middle of the band, on
+    every host core this process may use and on one thread; each the median of
+    `reps` timed runs.  Returns the figures and the oracle's raw candidates and
+    per-(block, fold) sigmas of the multi-thread sample, for the parity check."""
+    import numpy as np
+
+    import oracle as O
+    cores = host_cores()
     op = O.make_params(nharms=params.nharms, m=params.m, decimations=params.decimations,
                        threshold=cfg["threshold"])
     Nprof = 2048
     nchunks = -(-total // Nprof)
     mid = nchunks // 2
-    nc = cores                       # probe: one block per thread
-    t0 = time.perf_counter()
-    O.search_region(op, amps, cfg["N"], r_lo, total, Nprof, mid, mid + nc,
-                    threshold=cfg["threshold"], nthreads=cores)
-    dt = time.perf_counter() - t0
-    per_block = dt / nc
+    nf = len(params.decimations)
+
+    def run(c0, nc, nthreads, sig=None):
+        t0 = time.perf_counter()
+        cands = O.search_region(op, amps, cfg["N"], r_lo, total, Nprof, c0, c0 + nc,
+                                threshold=cfg["threshold"], nthreads=nthreads, sigmas=sig)
+        return time.perf_counter() - t0, cands
+
+    # size the samples from a probe: about target_s of work per timed run
+    dt, _ = run(mid, cores, cores)
+    per_block = dt / cores
     nc = int(max(cores, min(nchunks - mid, round(target_s / max(per_block, 1e-6)))))
     nc = (nc // cores) * cores or cores
-    t0 = time.perf_counter()
-    O.search_region(op, amps, cfg["N"], r_lo, total, Nprof, mid, mid + nc,
-                    threshold=cfg["threshold"], nthreads=cores)
-    dt = time.perf_counter() - t0
-    trials = nc * Nprof
-    return {"value": trials / dt, "unit": "trials/s", "cores": cores, "kind": "port",
-            "sample": f"{nc} blocks x {Nprof} trials (blocks {mid}..{mid + nc - 1} of {nchunks}) "
-                      f"of the same spectrum and parameters, {dt:.1f} s wall"}
+    sig = np.zeros((mid + nc, nf))
+    walls = []
+    cands = None
+    for _ in range(reps):
+        w, cands = run(mid, nc, cores, sig)
+        walls.append(w)
+    wall = sorted(walls)[len(walls) // 2]
+    dt1, _ = run(mid, 1, 1)
+    nc1 = int(max(1, min(nchunks - mid, round(target_s / max(dt1, 1e-6)))))
+    w1 = sorted(run(mid, nc1, 1)[0] for _ in range(reps))[reps // 2]
+    model = cpu_model()
+    out = {"value": nc * Nprof / wall, "unit": "trials/s", "cores": cores, "kind": "port",
+           "cpu": model,
+           "sample": f"{nc} blocks x {Nprof} trials (blocks {mid}..{mid + nc - 1} of {nchunks}) of "
+                     f"the same spectrum and parameters, median of {reps} runs of "
+                     f"{wall:.2f} s on {cores} threads ({model})",
+           "one_thread": {"value": round(nc1 * Nprof / w1, 1), "unit": "trials/s", "cores": 1,
+                          "sample": f"{nc1} blocks from block {mid}, median of {reps} runs of "
+                                    f"{w1:.2f} s"}}
+    return out, (mid, nc, cands, sig[mid:mid + nc])
+
+
+def oracle_parity(raw, sig_dev, sample, threshold):
+    """The device search against the oracle on the CPU sample's blocks: (t, k)
+    sets equal away from the threshold, S/N and every (block, fold) sigma
+    relative differences."""
+    import numpy as np
+    mid, nc, want, wsig = sample
+    got = raw[(raw["block"] >= mid) & (raw["block"] < mid + nc)]
+    g = {(int(c["t"]), int(c["k"])): float(c["metric"]) for c in got}
+    w = {(int(c["t"]), int(c["k"])): float(c["metric"]) for c in want}
+    odd = [g.get(k, w.get(k)) for k in set(g) ^ set(w)]
+    both = set(g) & set(w)
+    ds = sig_dev[mid:mid + nc]
+    pos = wsig > 0
+    return {"blocks": f"{mid}..{mid + nc - 1}", "oracle_candidates": len(w),
+            "device_candidates": len(g), "common": len(both),
+            "only_one_side": len(odd),
+            "only_one_side_max_dist_from_threshold": (
+                float(f"{max(abs(v - threshold) for v in odd):.3e}") if odd else None),
+            "max_rel_snr_diff": (float(f"{max(abs(g[k] - w[k]) / abs(w[k]) for k in both):.3e}")
+                                 if both else None),
+            "sigmas": int(pos.sum()),
+            "max_rel_sigma_diff": float(f"{float(np.max(np.abs(ds[pos] - wsig[pos]) / wsig[pos])):.3e}")
+            if pos.any() else None}
 
 
 def timed_search(eng, r_lo, r_hi, threshold, steps, warmup, label):
@@ -163,8 +235,8 @@ def run_c3(args, cs, nfiles):
     line = {
         "metric": METRIC, "value": round(total / wall, 1), "unit": "trials/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * wall, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": DTYPE[args.mode], "data": "synthetic",
         "config": {"workload": f"{nfiles}-file DM sweep: 2^22..2^25-sample .fft files (seeds "
                                f"3000+i, one von Mises pulsar each), nharms=32, m=16, "
                                f"decimation_set(32,4), 0.1-125 Hz, threshold 8, blocks of 2048; "
@@ -402,11 +474,7 @@ def run_stats(args, cs):
     # CPU: the oracle's every-trial pass over a bounded run of blocks
     op = O.make_params(nharms=60, m=16, numbetween=16, threshold=8.0,
                        decimations=list(params.decimations))
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:  # pragma: no cover
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores = host_cores()
     nb_cpu, tc = 0, 0.0
     while tc < min(args.cpu_seconds, 10.0) and nchunks // 2 + nb_cpu < nchunks:
         c0 = nchunks // 2 + nb_cpu
@@ -447,7 +515,8 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="searches timed in the plain FP32 mode for the ratio (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=4.0,
+                    help="CPU work per timed baseline run (median of 3)")
     ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5", "triage", "cands", "stats"])
     ap.add_argument("--c3-files", type=int, default=64)
     ap.add_argument("--autotune", action="store_true",
@@ -504,6 +573,7 @@ def main():
     log(f"[bench] {total} trials, {nchunks} blocks, mode {args.mode}; warmup {args.warmup}")
     raw, wall, stage = timed_search(eng, r_lo, r_hi, cfg["threshold"], args.steps, args.warmup,
                                     args.mode)
+    sig_dev = eng.last_sigmas()
     ms_per_step = 1e3 * wall / args.steps
     value = total / (wall / args.steps)
 
@@ -593,17 +663,20 @@ def main():
                                    if both else None)}}
         eng.set_mode(args.mode)
 
-    cpu = None
+    cpu = parity = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, OpenMP)")
-        cpu = cpu_baseline(cfg, amps, params, r_lo, total, args.cpu_seconds)
-        log(f"[bench] CPU baseline {cpu['value']:.3e} trials/s on {cpu['cores']} cores")
+        cpu, sample = cpu_baseline(cfg, amps, params, r_lo, total, args.cpu_seconds)
+        log(f"[bench] CPU baseline {cpu['value']:.3e} trials/s on {cpu['cores']} cores, "
+            f"{cpu['one_thread']['value']:.3e} on one")
+        parity = oracle_parity(raw, sig_dev, sample, cfg["threshold"])
+        log(f"[bench] oracle parity on the sample: {parity}")
 
     line = {
         "metric": METRIC, "value": round(value, 1), "unit": "trials/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": DTYPE[args.mode], "data": "synthetic",
         "config": {"workload": (
                        f"2^{int(math.log2(N))}-sample .fft (dt 64 us, seed {cfg['seed']}, "
                        f"{len(cfg['pulsars'])} von Mises pulsars"
@@ -623,6 +696,7 @@ def main():
         "plain_fp32_gpu": plain,
         "speedup_vs_plain_fp32_gpu": (round(value / plain["value"], 3) if plain else None),
         "cpu_baseline": cpu,
+        "oracle_parity": parity,
     }
     print(json.dumps(line), flush=True)
 

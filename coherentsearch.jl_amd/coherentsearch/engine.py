@@ -433,14 +433,18 @@ def search(ft: FFTFile, params: Optional[SearchParams] = None, lofreq: float = 0
            progress: str = "none", metricstats=None, normalize: bool = False,
            verbose: bool = False, wisdom: bool = True, wisdom_file=None,
            cache: Optional[SearchCache] = None, device: int = 0,
-           mode: str = "plain") -> List[Candidate]:
+           mode: str = "auto") -> List[Candidate]:
     """search(ft, params; ...) (src/search.jl:2002-2111) on the GPU.
 
     ``wisdom``/``wisdom_file`` are FFTW plan caches with no device counterpart
     and are accepted for signature parity.  ``metricstats`` (a MetricStats)
     collects the every-trial statistics; ``normalize`` runs the two-pass
     search (search.jl:2076-2095): statistics, the MetricNorm model, then the
-    detection over the normalised metrics (kept on the device from pass 1)."""
+    detection over the normalised metrics (kept on the device from pass 1).
+    ``mode`` "auto" runs the split-BF16 fused pipeline and, for parameters it
+    does not cover (profile lengths, boxcar banks, irrational trial steps),
+    the plain FP32 one; "fused", "fused_valu" and "plain" force a mode.  Under
+    the default precision :f64 every mode reports the same candidates."""
     from .metricstats import MetricStats, build_metricnorm
     params = SearchParams() if params is None else params
     if progress not in ("none", "text", "bar"):
@@ -450,18 +454,35 @@ def search(ft: FFTFile, params: Optional[SearchParams] = None, lofreq: float = 0
                                                          blocksize)
     if total == 0:
         return []
-    eng = (cache or SearchCache()).get(params, Nprof, device) if cache is not None else \
-        Engine(params, Nprof, device)
-    eng.set_mode(mode)
-    eng.set_spectrum(ft)
-    if normalize:
-        ms = metricstats if metricstats is not None else MetricStats()
-        eng.search_stats(r_lo, r_hi, math.inf, ms)
-        raw = eng.search_normalized(thr, build_metricnorm(ms))
-    elif metricstats is not None:
-        raw = eng.search_stats(r_lo, r_hi, thr, metricstats)
-    else:
-        raw = eng.search_raw(r_lo, r_hi, thr)
+    own = cache is None
+    eng = Engine(params, Nprof, device) if own else cache.get(params, Nprof, device)
+    try:
+        eng.set_spectrum(ft)
+
+        def run():
+            if normalize:
+                ms = metricstats if metricstats is not None else MetricStats()
+                eng.search_stats(r_lo, r_hi, math.inf, ms)
+                return eng.search_normalized(thr, build_metricnorm(ms))
+            if metricstats is not None:
+                return eng.search_stats(r_lo, r_hi, thr, metricstats)
+            return eng.search_raw(r_lo, r_hi, thr)
+
+        if mode == "auto":
+            eng.set_mode("fused")
+            try:
+                raw = run()
+            except ValueError as e:
+                if "fused" not in str(e):
+                    raise
+                eng.set_mode("plain")
+                raw = run()
+        else:
+            eng.set_mode(mode)
+            raw = run()
+    finally:
+        if own:
+            eng.close()
     cands = raw_to_candidates(raw, ft, r_lo, lodr, Nprof)
     if verbose:
         import sys
@@ -481,11 +502,11 @@ def search_files(fts: Sequence[FFTFile], params: Optional[SearchParams] = None,
                  blocksize: int = 2048, threshold: Optional[float] = None, remove: bool = True,
                  dr_tol: float = 1.0, harm_remove: bool = True, numharm: int = 16,
                  harm_tol: float = 1.0, device: int = 0,
-                 mode: str = "fused") -> List[List[Candidate]]:
+                 mode: str = "auto") -> List[List[Candidate]]:
     """search over several files with one device context, as the CLI's
     search_one loop shares one SearchCache (src/cli.jl:231-289); each file's
     list equals search(ft, params; ...) on that file.  The next file's upload
-    runs while a file is searched."""
+    runs while a file is searched.  ``mode`` as in search()."""
     params = SearchParams() if params is None else params
     thr = params.threshold if threshold is None else float(threshold)
     grids = [trial_grid(ft.T, params, lofreq, hifreq, lobin, blocksize) for ft in fts]
@@ -494,9 +515,20 @@ def search_files(fts: Sequence[FFTFile], params: Optional[SearchParams] = None,
     if not todo:
         return out
     eng = Engine(params, grids[todo[0]][4], device)
+    jobs = [(fts[i], grids[i][0], grids[i][1]) for i in todo]
     try:
-        eng.set_mode(mode)
-        raws = eng.search_files([(fts[i], grids[i][0], grids[i][1]) for i in todo], thr)
+        if mode == "auto":
+            eng.set_mode("fused")
+            try:
+                raws = eng.search_files(jobs, thr)
+            except ValueError as e:
+                if "fused" not in str(e):
+                    raise
+                eng.set_mode("plain")
+                raws = eng.search_files(jobs, thr)
+        else:
+            eng.set_mode(mode)
+            raws = eng.search_files(jobs, thr)
     finally:
         eng.close()
     for i, raw in zip(todo, raws):

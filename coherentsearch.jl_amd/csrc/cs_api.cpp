@@ -63,6 +63,7 @@ struct cs_ctx {
     std::map<int, DBuf> twid;
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
+    DBuf rflags;    // (block, fold) pairs of a batch whose sigma is refined in FP64
     DBuf bandtab;   // block_metrics(:fft): per-harmonic lobin, then fine-grid length
     DBuf cand_r, cand_h, cand_w, cand_prof;   // candidate profiles: inputs, widths, outputs
     // metric statistics: every metric of the last statistics search
@@ -1191,6 +1192,10 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     }
     if ((s = ensure(c, c->surv, (size_t)BT * nf * sizeof(uint64_t)))) return s;
     if ((s = ensure(c, c->counters, 4 * sizeof(uint32_t)))) return s;
+    // precision :f64 (the reference's default profile stage): the sigma that
+    // scales every reported metric comes from FP64 profiles
+    const bool refine = c->precision == 0;
+    if (refine && (s = ensure(c, c->rflags, (size_t)Bc * nf * sizeof(uint32_t)))) return s;
     int64_t rowo = 0;
     for (int f = 0; f < nf && !fused; ++f) {
         fs.prof[f] = (float*)c->prof.p + rowo * BT;
@@ -1386,6 +1391,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
                 HIPCHK(c, launch_gate_f32(c->st, fs, g, b, o));
                 HIPCHK(c, rec(4));
             }
+            if (refine)
+                HIPCHK(c, launch_sigma_refine(c->st, sp, tb, g, fs, b, o, (uint32_t*)c->rflags.p));
             HIPCHK(c, launch_rescan(c->st, sp, tb, g, fs, b, o, 8192));
             HIPCHK(c, rec(5));
         }
@@ -1400,7 +1407,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         };
         const void* ptrs[] = {c->surv.p, c->counters.p, c->nvalid.p, c->sigma.p, c->m_rowmask.p,
                               c->m_desc.p, c->m_ident.p, c->ftp.p, c->prof.p, c->rowok.p,
-                              c->mraw.p, c->samples.p};
+                              c->mraw.p, c->samples.p, refine ? c->rflags.p : nullptr};
         put(ptrs, sizeof(ptrs));
         put(&sp, sizeof(sp));
         put(&tb, sizeof(tb));

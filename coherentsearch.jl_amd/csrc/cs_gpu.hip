@@ -555,6 +555,243 @@ hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// FP64 block sigma of the (block, fold) pairs that hold gate survivors: the
+// reference's default profile stage is Float64 (SearchParams precision :f64,
+// search.jl:143,151-155), so its _block_sigma (search.jl:490-566) takes the
+// median of |x| over FP64 profiles.  The pass-1 sigma of this build comes from
+// FP32 (plain) or split-BF16 (fused) profiles; it only has to decide which
+// trials the gate keeps (the 2-sigma medmargin dwarfs its ~1e-5 relative
+// error).  The sigma that scales a reported metric is recomputed here from the
+// same whole-column subsample: each sampled column's stack with the FP32
+// weights (as the reference fills ftprofs), an FP64 real DFT, |x|, then the
+// exact median by a bitonic sort of up to 8192 doubles in LDS.
+// ---------------------------------------------------------------------------
+__global__ void k_reset_u32(uint32_t* p, int n);
+
+__global__ void k_refine_mark(const OutDev o, GridDev g, BatchDev b, int nf,
+                              uint32_t* __restrict__ flags) {
+    uint32_t nsurv = *o.surv_count;
+    if (nsurv > o.surv_cap) nsurv = o.surv_cap;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nsurv;
+         i += gridDim.x * blockDim.x) {
+        uint64_t pk = o.surv[i];
+        int64_t t = (int64_t)(pk >> 4);
+        int f = (int)(pk & 15u);
+        int64_t cg, first, last;
+        block_of(g, t, cg, first, last);
+        int cl = batch_slot(b, cg);
+        if (cl >= 0) flags[cl * nf + f] = 1u;
+    }
+}
+
+constexpr int kRefineThreads = 1024;
+constexpr int kRefinePer = kSigmaSamples / kRefineThreads;   // sample keys per thread
+constexpr int kRefineX = kRefineThreads + 64;                // stack entries of one round
+
+// Exclusive prefix sum over the workgroup; `tot` holds kRefineThreads/64 words,
+// rewritten by the next call only after the caller's next two barriers.
+__device__ __forceinline__ uint32_t refine_excl_scan(uint32_t v, uint32_t* tot) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) tot[wid] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int w = 0; w < wid; ++w) pre += tot[w];
+    return pre + x - v;
+}
+
+// Largest key of the workgroup among those for which `take` holds (others 0).
+__device__ __forceinline__ uint64_t refine_max(uint64_t m, uint64_t* red) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const uint64_t u = __shfl_xor(m, o, 64);
+        m = u > m ? u : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    uint64_t r = 0;
+    for (int w = 0; w < kRefineThreads / 64; ++w) r = red[w] > r ? red[w] : r;
+    __syncthreads();
+    return r;
+}
+
+// The key of rank kth (0-based) among the workgroup's keys (bit patterns of
+// doubles >= 0, whose unsigned order is the numeric order): radix digits of
+// 11 bits from the top; kth is left as the key's rank among its equals.  A
+// digit whose bin holds one key ends the search: that key is the maximum of
+// the keys with the prefix so far.
+__device__ uint64_t refine_select(const uint64_t (&keys)[kRefinePer], uint32_t& kth,
+                                  uint32_t* hist, uint32_t* shv, uint64_t* red) {
+    uint64_t prefix = 0, mask = 0;
+    for (int pass = 0; pass < 6; ++pass) {
+        const int sh = pass < 5 ? 53 - 11 * pass : 0;
+        const int bits = pass < 5 ? 11 : 9;
+        const uint64_t dmask = ((uint64_t)1 << bits) - 1u;
+        hist[2 * threadIdx.x] = 0;
+        hist[2 * threadIdx.x + 1] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < kRefinePer; ++r)
+            if ((keys[r] & mask) == prefix) atomicAdd(&hist[(keys[r] >> sh) & dmask], 1u);
+        __syncthreads();
+        const uint32_t a0 = hist[2 * threadIdx.x], a1 = hist[2 * threadIdx.x + 1];
+        const uint32_t ex = refine_excl_scan(a0 + a1, shv + 16);
+        if (ex <= kth && kth < ex + a0 + a1) {
+            const bool second = kth >= ex + a0;
+            shv[0] = 2 * threadIdx.x + (second ? 1u : 0u);
+            shv[1] = kth - ex - (second ? a0 : 0u);
+            shv[2] = second ? a1 : a0;
+        }
+        __syncthreads();
+        prefix |= (uint64_t)shv[0] << sh;
+        mask |= dmask << sh;
+        kth = shv[1];
+        const uint32_t cnt = shv[2];
+        __syncthreads();
+        if (cnt == 1u && pass < 5) {    // block-uniform
+            uint64_t m = 0;
+#pragma unroll
+            for (int r = 0; r < kRefinePer; ++r)
+                m = (keys[r] & mask) == prefix && keys[r] > m ? keys[r] : m;
+            kth = 0;
+            return refine_max(m, red);
+        }
+    }
+    return prefix;
+}
+
+__global__ __launch_bounds__(kRefineThreads) void k_sigma_refine(
+    SpecDev sp, TabDev tb, GridDev g, FoldSet fs, BatchDev b, const uint32_t* __restrict__ flags) {
+    __shared__ double v[kSigmaSamples];
+    __shared__ double2 X[kRefineX];
+    __shared__ double2 tw[kMaxNB];
+    __shared__ uint32_t hist[2 * kRefineThreads];
+    __shared__ uint32_t shv[32];
+    __shared__ uint64_t red[kRefineThreads / 64];
+    const int tid = threadIdx.x;
+    const int nf = fs.nf;
+    const int npair = b.nblocks * nf;
+    for (int pair = blockIdx.x; pair < npair; pair += gridDim.x) {
+        if (flags[pair] == 0u) continue;                // uniform over the workgroup
+        const int cl = pair / nf, f = pair - cl * nf;
+        const int64_t cg = batch_block(b, cl);
+        const int64_t first = g.origin + cg * g.Nprof;
+        const int64_t last = min(first + g.Nprof, g.total) - 1;
+        const FoldDev& fd = fs.f[f];
+        const int k = fd.k, Hk = fd.Hk, NB = fd.nbins;
+        const int64_t nv = b.nvalid[cg * nf + f];
+        const int64_t NN = (int64_t)NB * nv;
+        const int64_t cap = fs.cap[f];
+        if (NN == 0) {
+            if (tid == 0) b.sigma[pair] = 0.0;
+            continue;
+        }
+        const bool all = NN <= cap;
+        int64_t ncol, ns;
+        if (all) { ncol = nv; ns = NN; }
+        else {
+            ncol = cap / NB;
+            if (ncol < 1) ncol = 1;
+            if (ncol > nv) ncol = nv;
+            ns = ncol * NB < cap ? ncol * NB : cap;
+        }
+        for (int e = tid; e < NB; e += kRefineThreads) tw[e] = fs.twid[f][e];
+        // rounds of CR columns: every (column, harmonic) stack entry, then every
+        // (column, bin pair) of the FP64 real DFT, spread over the workgroup
+        const int W = Hk + 1;
+        const int CR = kRefineThreads / W;
+        for (int64_t c0 = 0; c0 < ncol; c0 += CR) {
+            const int nc = (int)min((int64_t)CR, ncol - c0);
+            for (int task = tid; task < nc * W; task += kRefineThreads) {
+                const int cr = task / W, l = task - cr * W;
+                const int64_t cc = c0 + cr;
+                const int64_t t = first + (all ? cc : (cc * nv) / ncol);
+                double2 x = make_double2(0.0, 0.0);
+                if (l >= 1) {
+                    if (tb.m == 16 && !tb.weights_f64) x = interp_one<16>(sp, tb, l * k, t, first, last, false, false);
+                    else if (tb.m == 32 && !tb.weights_f64) x = interp_one<32>(sp, tb, l * k, t, first, last, false, false);
+                    else x = interp_one<0>(sp, tb, l * k, t, first, last, false, false);
+                }
+                X[task] = x;
+            }
+            __syncthreads();
+            const int H = Hk;
+            for (int task = tid; task < nc * (H + 1); task += kRefineThreads) {
+                const int cr = task / (H + 1), n = task - cr * (H + 1);
+                const double2* Xc = X + cr * W;
+                const int64_t s0 = (c0 + cr) * NB;
+                // the column gives min(NB, ns - s0) samples: its bins 0 .. that - 1
+                const int64_t take = ns - s0 < NB ? ns - s0 : NB;
+                double E = 0.0, O = 0.0;
+                int idx = 0;                                 // (h n) mod NB
+                for (int h = 1; h <= H; ++h) {
+                    idx += n;
+                    if (idx >= NB) idx -= NB;
+                    double c, sn;
+                    if (h == H) { c = (n & 1) ? -1.0 : 1.0; sn = 0.0; }
+                    else {
+                        const double2 w = tw[idx];
+                        c = 2.0 * w.x;
+                        sn = -(2.0 * w.y);
+                    }
+                    E = fma(c, Xc[h].x, E);
+                    O = fma(sn, Xc[h].y, O);
+                }
+                if (n == 0 || n == H) {
+                    if (n < take) v[s0 + n] = fabs(E);
+                } else {
+                    if (n < take) v[s0 + n] = fabs(E + O);
+                    if (NB - n < take) v[s0 + NB - n] = fabs(E - O);
+                }
+            }
+            __syncthreads();
+        }
+        uint64_t keys[kRefinePer];
+#pragma unroll
+        for (int r = 0; r < kRefinePer; ++r) {
+            const int64_t s = tid + (int64_t)kRefineThreads * r;
+            keys[r] = s < ns ? (uint64_t)__double_as_longlong(v[s]) : ~(uint64_t)0;
+        }
+        const uint32_t half = (uint32_t)(ns / 2);
+        uint32_t rk = half;
+        const uint64_t up = refine_select(keys, rk, hist, shv, red);
+        double med;
+        if ((ns & 1) || rk > 0) {
+            med = __longlong_as_double((long long)up);
+        } else {
+            // exactly `half` keys lie below `up`: the lower central value is the
+            // largest of them
+            uint64_t m = 0;
+#pragma unroll
+            for (int r = 0; r < kRefinePer; ++r) m = keys[r] < up && keys[r] > m ? keys[r] : m;
+            const uint64_t lo = refine_max(m, red);
+            med = 0.5 * (__longlong_as_double((long long)lo) + __longlong_as_double((long long)up));
+        }
+        if (tid == 0) b.sigma[pair] = 1.4826 * med;
+        __syncthreads();
+    }
+}
+
+hipError_t launch_sigma_refine(hipStream_t st, const SpecDev& sp, const TabDev& tb,
+                               const GridDev& g, const FoldSet& fs, const BatchDev& b,
+                               const OutDev& o, uint32_t* flags) {
+    const int npair = b.nblocks * fs.nf;
+    if (npair <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reset_u32, dim3((unsigned)((npair + 63) / 64)), dim3(64), 0, st, flags,
+                       npair);
+    hipLaunchKernelGGL(k_refine_mark, dim3(64), dim3(256), 0, st, o, g, b, fs.nf, flags);
+    const int grid = npair < 1024 ? npair : 1024;
+    hipLaunchKernelGGL(k_sigma_refine, dim3((unsigned)grid), dim3(kRefineThreads), 0, st, sp, tb, g,
+                       fs, b, (const uint32_t*)flags);
+    return hipGetLastError();
+}
+
 __global__ void k_reset_u32(uint32_t* p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -667,7 +904,8 @@ hipError_t launch_sigma_f64(hipStream_t st, const double* prof, int64_t ldp, int
 }
 
 __global__ __launch_bounds__(64) void k_exact_f64(const double* __restrict__ profs, int64_t ldp,
-                                                  int NB, int64_t n, const int32_t* widths, int nw,
+                                                  int64_t ldt, int NB, int64_t n,
+                                                  const int32_t* widths, int nw,
                                                   const double* sigma, double* out) {
     __shared__ double prof[kMaxNB];
     __shared__ double srt[kMaxNB];
@@ -680,7 +918,7 @@ __global__ __launch_bounds__(64) void k_exact_f64(const double* __restrict__ pro
     for (int64_t j = blockIdx.x; j < n; j += gridDim.x) {
         double metric = 0.0;
         if (invs > 0) {
-            for (int i = threadIdx.x; i < NB; i += 64) prof[i] = profs[(int64_t)i * ldp + j];
+            for (int i = threadIdx.x; i < NB; i += 64) prof[i] = profs[(int64_t)i * ldp + j * ldt];
             __syncthreads();
             metric = group_exact_metric(prof, NB, wsh, nw, invs, srt, ps);
         }
@@ -690,11 +928,12 @@ __global__ __launch_bounds__(64) void k_exact_f64(const double* __restrict__ pro
 }
 
 hipError_t launch_exact_f64(hipStream_t st, const double* prof, int64_t ldp, int nbins, int64_t n,
-                            const int32_t* widths, int nw, const double* sigma, double* out) {
+                            const int32_t* widths, int nw, const double* sigma, double* out,
+                            int64_t ldt) {
     if (n <= 0) return hipSuccess;
     int64_t gsz = n < 65536 ? n : 65536;
-    hipLaunchKernelGGL(k_exact_f64, dim3((unsigned)gsz), dim3(64), 0, st, prof, ldp, nbins, n, widths,
-                       nw, sigma, out);
+    hipLaunchKernelGGL(k_exact_f64, dim3((unsigned)gsz), dim3(64), 0, st, prof, ldp, ldt, nbins, n,
+                       widths, nw, sigma, out);
     return hipGetLastError();
 }
 

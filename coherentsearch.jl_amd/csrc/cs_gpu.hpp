@@ -124,6 +124,11 @@ hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
                          const GridDev& g, const FoldSet& fs, const BatchDev& b,
                          const OutDev& o, int nblocks_grid);
 hipError_t launch_reset_u32(hipStream_t st, uint32_t* p, int n);
+// FP64 sigma of the batch's (block, fold) pairs that hold gate survivors
+// (flags: scratch of nblocks * nf words, cleared by the launch itself)
+hipError_t launch_sigma_refine(hipStream_t st, const SpecDev& sp, const TabDev& tb,
+                               const GridDev& g, const FoldSet& fs, const BatchDev& b,
+                               const OutDev& o, uint32_t* flags);
 
 // FP64 bridge path (chunk_metrics / decim / block_metrics).
 hipError_t launch_c2r_f64(hipStream_t st, const double2* ftp, int64_t ldf, int k,
@@ -131,9 +136,10 @@ hipError_t launch_c2r_f64(hipStream_t st, const double2* ftp, int64_t ldf, int k
                           const double2* twid, double scale);
 hipError_t launch_sigma_f64(hipStream_t st, const double* prof, int64_t ldp,
                             int nbins, int64_t n, int64_t cap, double* sigma_out);
+// profile j's bin i at prof[i * ldp + j * ldt]
 hipError_t launch_exact_f64(hipStream_t st, const double* prof, int64_t ldp, int nbins,
                             int64_t n, const int32_t* widths, int nw,
-                            const double* sigma, double* out);
+                            const double* sigma, double* out, int64_t ldt = 1);
 hipError_t launch_point_interp_f64(hipStream_t st, const SpecDev& sp, const double* rfund,
                                    int64_t L, int nharms, int m, const uint8_t* rowok,
                                    double2* out, int64_t ld);

@@ -326,7 +326,8 @@ __global__ __launch_bounds__(256) void k_exact_all(const float* __restrict__ pro
 // buf, then 1.4826 x the exact median of |x| (center 0) or of |x - median(x)|
 // (center 1), medians by radix selection over buf.
 __global__ __launch_bounds__(kThreads) void k_sigma_any(const double* __restrict__ prof,
-                                                        int64_t ldp, int nbins, int64_t n,
+                                                        int64_t ldp, int64_t ldt, int nbins,
+                                                        int64_t n,
                                                         int64_t cap, int center_median,
                                                         double* __restrict__ buf,
                                                         double* __restrict__ sigma_out) {
@@ -351,7 +352,7 @@ __global__ __launch_bounds__(kThreads) void k_sigma_any(const double* __restrict
     for (int64_t s = threadIdx.x; s < ns; s += kThreads) {
         const int64_t cc = s / nbins, i = s - cc * nbins;
         const int64_t j = all ? cc : (cc * n) / ncol;
-        const double x = prof[i * ldp + j];
+        const double x = prof[i * ldp + j * ldt];
         buf[s] = center_median ? x : fabs(x);
     }
     __threadfence_block();
@@ -374,8 +375,9 @@ __global__ __launch_bounds__(kThreads) void k_sigma_any(const double* __restrict
 }  // namespace
 
 hipError_t launch_sigma_any(hipStream_t st, const double* prof, int64_t ldp, int nbins, int64_t n,
-                            int64_t cap, int center_median, double* buf, double* sigma_out) {
-    hipLaunchKernelGGL(k_sigma_any, dim3(1), dim3(kThreads), 0, st, prof, ldp, nbins, n, cap,
+                            int64_t cap, int center_median, double* buf, double* sigma_out,
+                            int64_t ldt) {
+    hipLaunchKernelGGL(k_sigma_any, dim3(1), dim3(kThreads), 0, st, prof, ldp, ldt, nbins, n, cap,
                        center_median, buf, sigma_out);
     return hipGetLastError();
 }

@@ -34,11 +34,12 @@ hipError_t launch_exact_all(hipStream_t st, const float* prof, int64_t ldp, int 
                             uint32_t bankmask, int fidx, int nf, const GridDev& g,
                             const BatchDev& b, double* mall, int64_t ld);
 
-// sigma of the profile set prof[bin][trial] (n trials, stride ldp) by the
-// search's subsampling rule with cap samples, about zero (center_median 0) or
-// about the sample median (1); buf holds min(nbins n, cap) doubles.
+// sigma of a set of n profiles (bin i of profile j at prof[i * ldp + j * ldt])
+// by the search's subsampling rule with cap samples, about zero (center_median
+// 0) or about the sample median (1); buf holds min(nbins n, cap) doubles.
 hipError_t launch_sigma_any(hipStream_t st, const double* prof, int64_t ldp, int nbins, int64_t n,
-                            int64_t cap, int center_median, double* buf, double* sigma_out);
+                            int64_t cap, int center_median, double* buf, double* sigma_out,
+                            int64_t ldt = 1);
 
 // Normalised detection over the resident metrics: for every valid (fold f,
 // trial t), r = k * (r_lo + i0 lodr + j lodr) (r_lo + ... alone for k = 1),

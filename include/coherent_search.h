@@ -41,9 +41,15 @@ extern "C" {
                                    timers then report only the total); default 0 */
 
 /* SearchParams (src/search.jl:131-144).  `decimations` is pointer + length
- * (the reference's Vector{Int}); precision 0 = :f64, 1 = :f32 (the profile
- * stage of the reference; the GPU search computes its profile stage in FP32
- * and its exact rescan in FP64 either way). */
+ * (the reference's Vector{Int}).  precision is the reference's profile-stage
+ * type (src/search.jl:143,151-155): 0 = :f64 (the default), 1 = :f32.  Every
+ * mode runs its gate on FP32 (plain) or split-BF16 (fused) profiles and its
+ * exact rescan in FP64.  With :f64 the block sigma that scales each reported
+ * metric is recomputed from FP64 profiles of the reference's sampled columns
+ * (stack with FP32 weights, FP64 real DFT) for every (block, fold) holding a
+ * gate survivor, so all modes report the same metrics, those of a Float64
+ * profile stage; with :f32 it is the pass's own sigma (FP32 or split-BF16
+ * samples, ~1e-7 or ~2e-5 relative from the FP64 one). */
 typedef struct cs_params {
     int32_t nharms;
     int32_t m;
@@ -282,8 +288,9 @@ int cs_autotune(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
                 int32_t *best_workgroups);
 
 /* Per-stage device time of the last cs_search, milliseconds, HIP events on the
- * context's stream: [0] interp, [1] c2r, [2] sigma, [3] gate, [4] rescan,
- * [5] fused, [6] total device, [7] number of batches. */
+ * context's stream: [0] interp, [1] c2r, [2] sigma, [3] gate, [4] rescan (with
+ * the FP64 sigma of the survivors' blocks under precision :f64), [5] fused,
+ * [6] total device, [7] number of batches. */
 int cs_last_timings(const cs_ctx *ctx, double *ms8);
 
 /* The sigma estimate of every (block, fold) of the last cs_search (or
