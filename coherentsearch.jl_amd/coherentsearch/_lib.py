@@ -126,6 +126,8 @@ def lib():
     L.cs_reference_profiles.argtypes = [vp, dp, C.c_int64, C.c_int32, dp]
     L.cs_snr_metrics.argtypes = [vp, dp, C.c_int32, C.c_int64, C.c_double, C.c_double,
                                  C.c_int64, C.c_int32, i32p, C.c_int32, dp]
+    L.cs_snr_metrics_noalloc.argtypes = [vp, vp, C.c_int32, C.c_int64, C.c_double, C.c_double,
+                                         C.c_int64, C.c_int32, i32p, C.c_int32, vp]
     L.cs_fourier_interp.argtypes = [vp, dp, C.c_int64, C.c_int32, dp]
     L.cs_candidate_profiles.argtypes = [vp, dp, i32p, C.c_int64, C.c_int32, dp, C.c_int64, i32p]
     L.cs_measure_ducy.argtypes = [vp, dp, i32p, C.c_int64, C.c_int32, dp]

@@ -379,6 +379,26 @@ class Engine:
                                             _dp(out)), "snr_metrics")
         return out
 
+    def snr_metrics_device(self, d_profs: int, nbins: int, L: int, d_out: int,
+                           boxcar_fsp: float = 1.5, boxcar_maxfrac: float = 0.3,
+                           sigma_samples: Optional[int] = None, sigma_center: str = "zero",
+                           widths: Optional[Sequence[int]] = None) -> None:
+        """snr_metrics on device memory with no allocation (cs_snr_metrics_noalloc):
+        d_profs is the address of an nbins x L FP64 matrix, column j at
+        d_profs + 8 j nbins; the L metrics are written at d_out."""
+        if sigma_center not in ("zero", "median"):
+            raise ValueError(f"sigma_center must be :zero or :median, got :{sigma_center}")
+        if widths is None:
+            wp, nw = None, 0
+        else:
+            w = np.ascontiguousarray(np.asarray(widths, dtype=np.int32))
+            wp, nw = _i32p(w), len(w)
+        ss = 0 if sigma_samples is None else int(sigma_samples)
+        self._chk(_lib.lib().cs_snr_metrics_noalloc(
+            self._h, C.c_void_p(int(d_profs)), int(nbins), int(L), float(boxcar_fsp),
+            float(boxcar_maxfrac), ss, 1 if sigma_center == "median" else 0, wp, nw,
+            C.c_void_p(int(d_out))), "snr_metrics_noalloc")
+
     def fourier_interp(self, r, m: int) -> np.ndarray:
         """fourier_interp (src/fourierinterp.jl:66-74) at each r: complex128."""
         rv = np.ascontiguousarray(np.atleast_1d(np.asarray(r, dtype=np.float64)))

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <future>
 #include <map>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -125,6 +126,10 @@ struct cs_ctx {
     int64_t sig_blocks = 0;
     int sig_nf = 0;
     bool sig_sparse = false;
+    // this device's copy of the allocation-free entry point's scratch array
+    // (its address from hipGetSymbolAddress at context creation)
+    void* scratch_dev = nullptr;
+    hipError_t scratch_err = hipSuccess;
 
     int fail(int code, const std::string& msg) {
         err = msg;
@@ -286,6 +291,52 @@ int check_spectrum(cs_ctx* c) {
 
 }  // namespace
 
+// ---- scratch of cs_snr_metrics_noalloc ----------------------------------------
+// One file-scope device array: every GPU the library runs on holds its own
+// copy until the process ends, at an address of its own, looked up once per
+// device (hipGetSymbolAddress with that device current) and kept per device.
+// Host code never passes `scratch` itself to a kernel: on the host that name
+// is a stand-in, not a device address.  Layout: [0, 8) the sigma, [256, 512)
+// the width bank (int32), [kScratchHead, ...) up to kScratchSamples sigma
+// samples (double).
+constexpr int64_t kScratchSamples = (int64_t)1 << 20;
+constexpr size_t kScratchHead = 4096;
+constexpr size_t kScratchBytes = kScratchHead + (size_t)kScratchSamples * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_at[kScratchDevices] = {};
+
+// this device's address of `scratch` (the caller has made `device` current)
+hipError_t scratch_address(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_at[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_at[device] = ptr;
+    }
+    *out = g_scratch_at[device];
+    return hipSuccess;
+}
+
+// a boxcar width bank passed by value (no host-to-device copy)
+struct WidthBank {
+    int32_t n;
+    int32_t w[64];
+};
+
+__global__ void k_put_widths(WidthBank wb, int32_t* dst) {
+    if ((int)threadIdx.x < wb.n) dst[threadIdx.x] = wb.w[threadIdx.x];
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* cs_version(void) { return "coherentsearch-mi355x 0.1 (gfx950)"; }
@@ -326,6 +377,9 @@ int cs_ctx_create(const cs_params* p, int32_t blocksize, int32_t device, cs_ctx*
         delete c;
         return CS_EDEVICE;
     }
+    // the allocation-free entry point's scratch on this device; a failure is
+    // reported by that entry point only
+    c->scratch_err = scratch_address(device, &c->scratch_dev);
     *out = c;
     return CS_OK;
 }
@@ -760,6 +814,55 @@ int cs_snr_metrics(cs_ctx* c, const double* profs, int32_t nbins, int64_t L, dou
                                (const int32_t*)c->widths.p, (int)ws.size(),
                                (const double*)c->sigma.p, (double*)c->mout.p));
     HIPCHK(c, hipMemcpyAsync(out, c->mout.p, (size_t)L * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
+}
+
+int cs_snr_metrics_noalloc(cs_ctx* c, const double* d_profs, int32_t nbins, int64_t L,
+                           double fsp, double maxfrac, int64_t sigma_samples,
+                           int32_t sigma_center, const int32_t* widths, int32_t nwidths,
+                           double* d_out) {
+    if (!c || (L > 0 && (!d_profs || !d_out))) return CS_EINVAL;
+    if (nbins < 1 || nbins > 128) return c->fail(CS_EINVAL, "profiles of 1..128 bins");
+    if (sigma_center != 0 && sigma_center != 1)
+        return c->fail(CS_EINVAL, "sigma_center must be :zero or :median");
+    if (L < 1) return CS_OK;
+    // the width bank, on the stack (boxcar_widths, search.jl:387-397, when none is given)
+    WidthBank wb{};
+    if (widths) {
+        if (nwidths < 1 || nwidths > 64) return c->fail(CS_EINVAL, "boxcar bank must hold 1..64 widths");
+        for (int a = 0; a < nwidths; ++a) wb.w[a] = widths[a];
+        wb.n = nwidths;
+    } else {
+        const int wmax = std::max(1, (int)std::floor(maxfrac * nbins));
+        for (int w = 1; w <= wmax; w = std::max((int)std::floor(fsp * w), w + 1)) {
+            if (wb.n == 64) return c->fail(CS_EINVAL, "boxcar bank must hold 1..64 widths");
+            wb.w[wb.n++] = w;
+        }
+    }
+    for (int a = 0; a < wb.n; ++a)
+        if (wb.w[a] < 1 || wb.w[a] > nbins)
+            return c->fail(CS_EINVAL, "boxcar widths must lie in 1..nbins");
+    const int64_t NN = (int64_t)nbins * L;
+    const int64_t cap = sigma_samples > 0 ? sigma_samples : NN;
+    if (std::min(NN, cap) > kScratchSamples)
+        return c->fail(CS_EINVAL, "cs_snr_metrics_noalloc takes at most 2^20 sigma samples "
+                                  "(min(nbins * L, sigma_samples)); cs_snr_metrics has no limit");
+    if (!c->scratch_dev)
+        return c->fail(CS_EDEVICE, std::string("no scratch address on this device: ") +
+                                       hipGetErrorString(c->scratch_err));
+    HIPCHK(c, hipSetDevice(c->device));
+    unsigned char* base = (unsigned char*)c->scratch_dev;
+    double* d_sigma = (double*)base;
+    int32_t* d_w = (int32_t*)(base + 256);
+    double* d_buf = (double*)(base + kScratchHead);
+    // nothing in the scratch needs clearing: each kernel writes what the next reads
+    hipLaunchKernelGGL(k_put_widths, dim3(1), dim3(64), 0, c->st, wb, d_w);
+    HIPCHK(c, hipGetLastError());
+    // profile j's bin i at d_profs[j * nbins + i]
+    HIPCHK(c, launch_sigma_any(c->st, d_profs, 1, nbins, L, cap, sigma_center, d_buf, d_sigma,
+                               nbins));
+    HIPCHK(c, launch_exact_f64(c->st, d_profs, 1, nbins, L, d_w, wb.n, d_sigma, d_out, nbins));
     HIPCHK(c, hipStreamSynchronize(c->st));
     return CS_OK;
 }

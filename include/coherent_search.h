@@ -226,6 +226,22 @@ int cs_reference_profiles(cs_ctx *ctx, const double *rfund, int64_t L, int32_t k
 int cs_snr_metrics(cs_ctx *ctx, const double *profs, int32_t nbins, int64_t L, double fsp,
                    double maxfrac, int64_t sigma_samples, int32_t sigma_center,
                    const int32_t *widths, int32_t nwidths, double *out);
+/* cs_snr_metrics on device buffers, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * d_profs: the nbins x L FP64 profile matrix in device memory, column j at
+ * d_profs[j * nbins] (the layout cs_snr_metrics takes on the host); d_out: L
+ * doubles in device memory; widths: a host array (or NULL), as there.  The
+ * sigma, its samples and the width bank live in a fixed scratch array of
+ * 8 MiB + 4 KiB per GPU, held until the process ends: at most 2^20 sigma
+ * samples (min(nbins * L, sigma_samples), CS_EINVAL beyond).  Every call on
+ * one GPU uses that GPU's one scratch array, so calls that overlap on a GPU
+ * (from several contexts or threads) race.  Runs on the context's stream and
+ * returns after it has drained; no hipMalloc, hipFree or host-to-device copy. */
+int cs_snr_metrics_noalloc(cs_ctx *ctx, const double *d_profs, int32_t nbins, int64_t L,
+                           double fsp, double maxfrac, int64_t sigma_samples,
+                           int32_t sigma_center, const int32_t *widths, int32_t nwidths,
+                           double *d_out);
 /* fourier_interp(r, ft, m) (src/fourierinterp.jl:66-74) at n frequencies of
  * the current spectrum: out[2i], out[2i+1] = re, im.  CS_EINVAL for r < 0 or
  * odd m, CS_EDIM when a window leaves the amplitudes. */
