@@ -139,6 +139,7 @@ def lib():
                               i64p, C.POINTER(C.c_int32)]
     L.cs_last_timings.argtypes = [vp, dp]
     L.cs_last_sigmas.argtypes = [vp, dp, C.c_int64, i64p]
+    L.cs_probe_split_bf16.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_int64, fp, fp]
     L.cs_plan_tables.argtypes = [C.POINTER(CsParams), C.c_double, C.c_int32, i64p, i32p,
                                  C.POINTER(C.c_uint8), fp, fp]
     _LIB = L

@@ -278,6 +278,19 @@ class Engine:
         nf = 1 + sum(1 for k in self.params.decimations if k > 1)
         return a.reshape(-1, nf)
 
+    def probe_split_bf16(self, r_lo: float, r_hi: float, t0: int, n: int):
+        """The split-BF16 pass's stacked amplitudes ((nharms, n) complex64, as its
+        DFT reads them) and per-fold bounds before sigma ((folds, n) float32) of
+        the band's trials t0 .. t0 + n - 1 (cs_probe_split_bf16); fused mode."""
+        nh = self.params.nharms
+        nf = 1 + sum(1 for k in self.params.decimations if k > 1)
+        amps = np.zeros(2 * nh * n, dtype=np.float32)
+        bnd = np.zeros(nf * n, dtype=np.float32)
+        self._chk(_lib.lib().cs_probe_split_bf16(self._h, float(r_lo), float(r_hi), int(t0),
+                                                 int(n), _fp(amps), _fp(bnd)),
+                  "probe_split_bf16")
+        return amps.view(np.complex64).reshape(nh, n), bnd.reshape(nf, n)
+
     def chunk_metrics(self, rstart: float, n: int, lodr: Optional[float] = None,
                       weights_f64: bool = False) -> np.ndarray:
         out = np.zeros(max(n, 0))

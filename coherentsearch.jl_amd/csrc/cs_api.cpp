@@ -65,6 +65,7 @@ struct cs_ctx {
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
     DBuf rflags;    // (block, fold) pairs of a batch whose sigma is refined in FP64
+    DBuf probe_buf; // cs_probe_split_bf16: the copied-out stacked amplitudes
     DBuf bandtab;   // block_metrics(:fft): per-harmonic lobin, then fine-grid length
     DBuf cand_r, cand_h, cand_w, cand_prof;   // candidate profiles: inputs, widths, outputs
     // metric statistics: every metric of the last statistics search
@@ -1001,6 +1002,14 @@ int cs_plan_tables(const cs_params* p, double r_lo, int32_t h, int64_t* info, in
     return CS_OK;
 }
 
+// cs_probe_split_bf16: the trial range whose stacked amplitudes the split-BF16
+// kernel copies out, and where the amplitudes and bounds go on the host
+struct ProbeReq {
+    int64_t t0, n;
+    float* amps;
+    float* bounds;
+};
+
 // cs_search, or with a block mask (one byte per block, nonzero = search it)
 // only the selected blocks: the batches are the runs of selected blocks
 // Statistics request of cs_search_stats: histogram range and bins, windows.
@@ -1130,7 +1139,7 @@ static int finish_stats(cs_ctx* c, const StatsReq& rq, const FoldSet& fs, const 
 
 static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, const uint8_t* mask,
                        int64_t nmask, cs_cand* out, int64_t cap, int64_t* n_out,
-                       const StatsReq* sreq = nullptr) {
+                       const StatsReq* sreq = nullptr, const ProbeReq* probe = nullptr) {
     if (!c || !n_out) return CS_EINVAL;
     *n_out = 0;
     for (double& v : c->timings) v = 0.0;
@@ -1287,11 +1296,39 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         if ((s = ensure(c, c->rowok, (size_t)Bc * nh))) return s;
         fu.mraw = (float*)c->mraw.p;
         fu.ldm = BT;
+        if (probe) {
+            if (!mfma) return c->fail(CS_EINVAL, "the split-BF16 probe needs CS_MODE_FUSED");
+            if (Bc < nchunks)
+                return c->fail(CS_EINVAL, "the split-BF16 probe covers one batch (at most " +
+                                              std::to_string(Bc) + " blocks)");
+            if (probe->t0 < 0 || probe->n < 1 || probe->t0 + probe->n > total)
+                return c->fail(CS_EINVAL, "probe trials outside the band's " +
+                                              std::to_string(total) + " trials");
+            const size_t db = (size_t)nh * probe->n * sizeof(float2);
+            if ((s = ensure(c, c->probe_buf, db))) return s;
+            HIPCHK(c, hipMemsetAsync(c->probe_buf.p, 0, db, c->st));
+            fu.dump = (float2*)c->probe_buf.p;
+            fu.dump_t0 = probe->t0;
+            fu.dump_n = probe->n;
+        }
         fu.samples = (float*)c->samples.p;
         fu.rowok = (const uint8_t*)c->rowok.p;
     } else {
         if ((s = ensure(c, c->ftp, (size_t)nh * BT * sizeof(float2)))) return s;
         if ((s = ensure(c, c->prof, (size_t)rows_total * BT * sizeof(float)))) return s;
+    }
+    // plain pipeline: the c2r writes the sampled columns and the radix select
+    // of the fused modes takes each block's median (generated c2r lengths);
+    // otherwise the per-block select reads the profiles in place
+    bool plain_samples = !fused;
+    for (int f = 0; f < nf && plain_samples; ++f)
+        plain_samples = c2r_samples_supported(fs.f[f].nbins);
+    FusedDev fp{};
+    if (plain_samples) {
+        if ((s = ensure(c, c->samples, (size_t)Bc * nf * kSigmaSamples * sizeof(float)))) return s;
+        fp.samples = (float*)c->samples.p;
+        fp.sld = kSigmaSamples;
+        fp.salign = 1;
     }
     if ((s = ensure(c, c->surv, (size_t)BT * nf * sizeof(uint64_t)))) return s;
     if ((s = ensure(c, c->counters, 4 * sizeof(uint32_t)))) return s;
@@ -1484,11 +1521,21 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             } else {
                 HIPCHK(c, launch_interp(c->st, sp, tb, g, t_base, nt, false, c->ftp.p, BT));
                 HIPCHK(c, rec(1));
-                for (int f = 0; f < nf; ++f)
-                    HIPCHK(c, launch_c2r_f32(c->st, (const float2*)c->ftp.p, BT, fs.f[f].k,
-                                             fs.f[f].nbins, nt, fs.prof[f], BT, fs.twid[f]));
+                for (int f = 0; f < nf; ++f) {
+                    if (plain_samples) {
+                        SampleSink sk{fp.samples, fp.sld, b.nvalid, g, t_base, c0, f, nf,
+                                      fs.cap[f], 0};
+                        HIPCHK(c, launch_c2r_samples(c->st, (const float2*)c->ftp.p, BT,
+                                                     fs.f[f].k, fs.f[f].nbins, nt, fs.prof[f], BT,
+                                                     sk));
+                    } else {
+                        HIPCHK(c, launch_c2r_f32(c->st, (const float2*)c->ftp.p, BT, fs.f[f].k,
+                                                 fs.f[f].nbins, nt, fs.prof[f], BT, fs.twid[f]));
+                    }
+                }
                 HIPCHK(c, rec(2));
-                HIPCHK(c, launch_sigma_f32(c->st, fs, g, b));
+                if (plain_samples) HIPCHK(c, launch_select(c->st, fs, g, b, fp, o));
+                else HIPCHK(c, launch_sigma_f32(c->st, fs, g, b));
                 HIPCHK(c, rec(3));
                 HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
                 HIPCHK(c, launch_gate_f32(c->st, fs, g, b, o));
@@ -1501,7 +1548,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         }
         return CS_OK;
     };
-    const bool graph = c->use_graph && !sreq && !mask;
+    const bool graph = c->use_graph && !sreq && !mask && !probe;
     std::vector<uint8_t> gkey;
     if (graph) {   // everything the captured launches bake in
         auto put = [&](const void* p, size_t n) {
@@ -1598,6 +1645,15 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         c->sig_blocks = nchunks;
         c->sig_nf = nf;
         c->sig_sparse = mask != nullptr;
+        if (probe) {   // one batch: mraw column = global trial
+            if (probe->amps)
+                HIPCHK(c, hipMemcpy(probe->amps, c->probe_buf.p,
+                                    (size_t)nh * probe->n * sizeof(float2), hipMemcpyDeviceToHost));
+            for (int f = 0; f < nf && probe->bounds; ++f)
+                HIPCHK(c, hipMemcpy(probe->bounds + (size_t)f * probe->n,
+                                    (const float*)c->mraw.p + (size_t)f * BT + probe->t0,
+                                    (size_t)probe->n * sizeof(float), hipMemcpyDeviceToHost));
+        }
         *n_out = nc;
         if (out) {
             int64_t w = std::min<int64_t>(nc, cap);
@@ -1613,6 +1669,15 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         return CS_OK;
     }
     return c->fail(CS_ENOSPC, "candidate buffer could not be sized");
+}
+
+int cs_probe_split_bf16(cs_ctx* c, double r_lo, double r_hi, int64_t t0, int64_t n, float* amps,
+                        float* bounds) {
+    if (!c) return CS_EINVAL;
+    if (c->mode != CS_MODE_FUSED) return c->fail(CS_EINVAL, "the split-BF16 probe needs CS_MODE_FUSED");
+    ProbeReq pr{t0, n, amps, bounds};
+    int64_t nout = 0;
+    return search_impl(c, r_lo, r_hi, INFINITY, nullptr, 0, nullptr, 0, &nout, nullptr, &pr);
 }
 
 int cs_search(cs_ctx* c, double r_lo, double r_hi, double threshold, cs_cand* out, int64_t cap,

@@ -131,6 +131,19 @@ __device__ __forceinline__ double2 interp_one(const SpecDev& sp, const TabDev& t
     }
 }
 
+// Sample slot column of block trial j (_block_sigma's whole-column subsample,
+// search.jl:504-538): j itself when all nv columns fit in cap samples, else
+// c when j = floor(c * nv / ncol) for some c < ncol = clamp(cap / nbins, 1, nv);
+// -1 if the column is not sampled.
+__host__ __device__ inline int sample_column(int64_t j, int64_t nv, int nbins, int64_t cap) {
+    if ((int64_t)nbins * nv <= cap) return (int)j;
+    int64_t ncol = cap / nbins;
+    if (ncol < 1) ncol = 1;
+    if (ncol > nv) ncol = nv;
+    const int64_t c = (j * ncol + nv - 1) / nv;
+    return (c < ncol && (c * nv) / ncol == j) ? (int)c : -1;
+}
+
 __device__ __forceinline__ void block_of(const GridDev& g, int64_t t, int64_t& cg,
                                          int64_t& first, int64_t& last) {
     cg = (t - g.origin) / g.Nprof;

@@ -344,6 +344,7 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
         sigma = 1.4826 * med;
     }
     if (threadIdx.x == 0) b.sigma[cl * nf + f] = sigma;
+    if (!fu.mraw) return;   // sigma only (the plain pipeline gates from its profiles)
     // threshold the block's bounds: m0 = bound * invsigma against medcut
     const double invs = sigma > 0 ? 1.0 / sigma : 0.0;
     const bool all = !(o.medcut > -INFINITY) || invs <= 0;

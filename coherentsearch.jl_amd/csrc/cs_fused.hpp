@@ -12,6 +12,12 @@ struct FusedDev {
     int32_t salign;         // sampled columns start at multiples of round_up(nbins, salign)
     int32_t pad;
     const uint8_t* rowok;   // [block_local * nharms + h - 1]: harmonic row kept for the block
+    // split-BF16 kernel, diagnostic instantiation only: the stacked amplitudes
+    // (hi + lo, as the DFT reads them) of trials dump_t0 .. dump_t0 + dump_n - 1
+    // at dump[(h - 1) * dump_n + trial - dump_t0]; null otherwise
+    float2* dump;
+    int64_t dump_t0;
+    int64_t dump_n;
 };
 
 bool fused_supported(const FoldSet& fs, int nharms);

@@ -108,6 +108,72 @@ __global__ __launch_bounds__(256) void k_c2r_generic(const float2* __restrict__ 
     }
 }
 
+// The same c2r for the search's batches, which also writes |x| of every
+// sampled column (_block_sigma's whole-column subsample, search.jl:504-538) to
+// its block's sample region for the median select: column slot sc of (local
+// block cl, fold f) at samples[(cl * nf + f) * sld + sc * NB].
+template <int NB>
+__global__ __launch_bounds__(256) void k_c2r_samples(const float2* __restrict__ ftp, int64_t ldf,
+                                                     int k, int64_t nt, float* __restrict__ prof,
+                                                     int64_t ldp, SampleSink sk) {
+    int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nt) return;
+    constexpr int H = NB / 2;
+    float xr[H + 1], xi[H + 1];
+    xr[0] = 0.0f;
+    xi[0] = 0.0f;
+#pragma unroll
+    for (int h = 1; h <= H; ++h) {
+        float2 v = ftp[(int64_t)(h * k - 1) * ldf + j];
+        xr[h] = v.x;
+        xi[h] = v.y;
+    }
+    float out[NB];
+    cs_c2r::Fixed<NB>::run(xr, xi, out);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) prof[(int64_t)i * ldp + j] = out[i];
+    const int64_t t = sk.t_base + j;
+    int64_t cg, first, last;
+    block_of(sk.g, t, cg, first, last);
+    const int64_t jj = t - first;
+    const int64_t nv = sk.nvalid[cg * sk.nf + sk.f];
+    if (jj >= nv) return;
+    const int sc = sample_column(jj, nv, NB, sk.cap);
+    if (sc < 0) return;
+    float* d = sk.samples + ((cg - sk.c_first) * sk.nf + sk.f) * sk.sld + (int64_t)sc * NB;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) d[i] = fabsf(out[i]);
+}
+
+bool c2r_samples_supported(int nbins) {
+    switch (nbins) {
+#define X(S) case S:
+        CS_C2R_SIZES(X)
+#undef X
+        return true;
+        default:
+            return false;
+    }
+}
+
+hipError_t launch_c2r_samples(hipStream_t st, const float2* ftp, int64_t ldf, int k, int nbins,
+                              int64_t nt, float* prof, int64_t ldp, const SampleSink& sk) {
+    if (nt <= 0) return hipSuccess;
+    dim3 grid((unsigned)((nt + 255) / 256));
+    switch (nbins) {
+#define X(S)                                                                                      \
+    case S:                                                                                       \
+        hipLaunchKernelGGL(k_c2r_samples<S>, grid, dim3(256), 0, st, ftp, ldf, k, nt, prof, ldp,   \
+                           sk);                                                                   \
+        break;
+        CS_C2R_SIZES(X)
+#undef X
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_c2r_f32(hipStream_t st, const float2* ftp, int64_t ldf, int k,
                           int nbins, int64_t nt, float* prof, int64_t ldp,
                           const double2* twid) {

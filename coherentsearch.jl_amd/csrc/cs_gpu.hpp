@@ -118,6 +118,22 @@ hipError_t launch_c2r_f32(hipStream_t st, const float2* ftp, int64_t ldf, int k,
                           const double2* twid);
 hipError_t launch_sigma_f32(hipStream_t st, const FoldSet& fs, const GridDev& g,
                             const BatchDev& b);
+// where the search's c2r puts the sampled columns of fold f (a run-of-blocks
+// batch starting at block c_first, trial t_base)
+struct SampleSink {
+    float* samples;
+    int64_t sld;
+    const int32_t* nvalid;   // [global block * nf + f]
+    GridDev g;
+    int64_t t_base;
+    int64_t c_first;
+    int32_t f, nf;
+    int32_t cap;
+    int32_t pad;
+};
+bool c2r_samples_supported(int nbins);
+hipError_t launch_c2r_samples(hipStream_t st, const float2* ftp, int64_t ldf, int k, int nbins,
+                              int64_t nt, float* prof, int64_t ldp, const SampleSink& sk);
 hipError_t launch_gate_f32(hipStream_t st, const FoldSet& fs, const GridDev& g,
                            const BatchDev& b, const OutDev& o);
 hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,

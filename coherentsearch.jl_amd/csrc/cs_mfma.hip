@@ -302,19 +302,6 @@ __device__ __forceinline__ void bank_max(const float (&Xs)[G::X], int g, float (
     }
 }
 
-// Sample slot column of block trial j (_block_sigma's whole-column subsample,
-// search.jl:504-538): j itself when all nv columns fit in cap samples, else
-// c when j = floor(c * nv / ncol) for some c < ncol = clamp(cap / nbins, 1, nv);
-// -1 if the column is not sampled.
-__host__ __device__ inline int sample_column(int64_t j, int64_t nv, int nbins, int64_t cap) {
-    if ((int64_t)nbins * nv <= cap) return (int)j;
-    int64_t ncol = cap / nbins;
-    if (ncol < 1) ncol = 1;
-    if (ncol > nv) ncol = nv;
-    const int64_t c = (j * ncol + nv - 1) / nv;
-    return (c < ncol && (c * nv) / ncol == j) ? (int)c : -1;
-}
-
 // One sampled column's share of this lane group (runs n = g*R .. g*R + R - 1
 // of P and M) into a column slot block: P[n] at n, M[n] at H + n, P[H] in
 // place of M[0]; whole CH-word chunks.
@@ -710,7 +697,7 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int nharms, int dft_tiles) {
 
 // KS: interpolation K steps (1 or 2); WIDE: some fold's bank has widths > 6
 // (kept apart so the narrow kernel does not reserve the wide bounds' registers)
-template <int KS, bool WIDE>
+template <int KS, bool WIDE, bool DUMP>
 __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev tb, GridDev g,
                                                             const FoldSet* __restrict__ fsd,
                                                             BatchDev b, FusedDev fu,
@@ -952,6 +939,21 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         if (__all(((rmv[0] & rmv[1] & rmv[2] & rmv[3]) & need) == need)) interp(BoolC<false>{});
         else interp(BoolC<true>{});
         lds_barrier();
+        if constexpr (DUMP) {
+            // diagnostic copy of the stacked amplitudes this wave's DFT reads:
+            // occurrence wv, trial lane & 15 of the group, each word hi + lo
+            const int i = lane & 15;
+            const int64_t rel = tbase + (int64_t)wv * S + i - fu.dump_t0;
+            if (rel >= 0 && rel < fu.dump_n && tbase + (int64_t)wv * S + i < g.total) {
+                for (int h = 1 + (lane >> 4); h <= nh; h += 4) {
+                    const uint32_t wr = fre[(h - 1) * kFtpLd + wv * 16 + i];
+                    const uint32_t wi = fim[(h - 1) * kFtpLd + wv * 16 + i];
+                    const float re = __uint_as_float(wr << 16) + __uint_as_float(wr & 0xFFFF0000u);
+                    const float im = __uint_as_float(wi << 16) + __uint_as_float(wi & 0xFFFF0000u);
+                    fu.dump[(int64_t)(h - 1) * fu.dump_n + rel] = make_float2(re, im);
+                }
+            }
+        }
 
         // ---------------- c2r + bound, fold by fold, wave-local --------------------------
         {
@@ -1159,8 +1161,13 @@ hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb
                            fu, md_dev);
         return hipGetLastError();
     };
-    if (md.ksmax == 2) return wide ? go(k_fused_mfma<2, true>) : go(k_fused_mfma<2, false>);
-    return wide ? go(k_fused_mfma<1, true>) : go(k_fused_mfma<1, false>);
+    if (fu.dump) {
+        if (md.ksmax == 2)
+            return wide ? go(k_fused_mfma<2, true, true>) : go(k_fused_mfma<2, false, true>);
+        return wide ? go(k_fused_mfma<1, true, true>) : go(k_fused_mfma<1, false, true>);
+    }
+    if (md.ksmax == 2) return wide ? go(k_fused_mfma<2, true, false>) : go(k_fused_mfma<2, false, false>);
+    return wide ? go(k_fused_mfma<1, true, false>) : go(k_fused_mfma<1, false, false>);
 }
 
 }  // namespace cs

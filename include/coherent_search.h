@@ -319,6 +319,27 @@ int cs_last_timings(const cs_ctx *ctx, double *ms8);
  * search has completed or the last one covered selected blocks only. */
 int cs_last_sigmas(cs_ctx *ctx, double *out, int64_t cap, int64_t *n_out);
 
+/* Diagnostic of the split-BF16 mode, for its per-result error statement (the
+ * reference states the error of its own Float32-weight interpolation,
+ * src/directinterp.jl:310-319): runs the CS_MODE_FUSED pass of cs_search over
+ * r_lo .. r_hi (in one batch: at most 16384 blocks) with no candidates, and
+ * returns for the band's trials t0 .. t0 + n - 1
+ *  - amps[2 ((h - 1) n + j)] (re, then im at + 1), h = 1 .. nharms: the stacked
+ *    amplitude of harmonic h (fill_harmonic_row_direct!, directinterp.jl:470-619)
+ *    as the split-BF16 DFT reads it, the hi + lo bf16 pair of the interpolation's
+ *    FP32-accumulated split-precision product (zero where the block's row
+ *    guard drops the row);
+ *  - bounds[f n + j]: fold f's zero-baseline boxcar bound before sigma (the
+ *    quantity _boxcar_gate! tests, search.jl:676-807: the maximum over the
+ *    fold's widths w and circular window starts of the window sum of the
+ *    unnormalised profile over sqrt(w)), f = 0 the undecimated fold, then the
+ *    decimations > 1 in parameter order; -inf where the trial is past the
+ *    fold's valid prefix.
+ * Either output may be NULL.  CS_EINVAL unless the context is in
+ * CS_MODE_FUSED and the trials lie in the band. */
+int cs_probe_split_bf16(cs_ctx *ctx, double r_lo, double r_hi, int64_t t0, int64_t n,
+                        float *amps, float *bounds);
+
 /* Host-only (no device): the per-harmonic interpolation tables the library
  * builds for a grid anchored at r_lo (build_direct_plans,
  * src/directinterp.jl:345-405).  info[0..5] = q, s, base_adv, pnum, rfloor0, P;

@@ -254,3 +254,100 @@ def test_c3_sweep_one_call_vs_oracle(cs, O, c3, mode):
             found += int(s > 8.0)
             assert s > 8.0, f"file seed {c['seed']}: {f0} Hz not detected"
     _say(f"C3 {mode}: {checked} blocks equal to the oracle, pulsars {found}/{in_band} in band")
+
+
+def _nvalid(k, rstart, lodr, n, nyq):
+    """decim_pass!'s valid prefix (src/search.jl:1681-1688): trials with
+    k (rstart + j lodr) < N/2."""
+    if k == 1:
+        return n
+    j = 0
+    while j < n and k * (rstart + j * lodr) < nyq:
+        j += 1
+    return j
+
+
+@pytest.mark.timeout(600)
+def test_c2_split_bf16_error_per_result(cs, O, c2):
+    """The split-BF16 interpolation's error, result by result, at the 2^25-point
+    workload: the stacked amplitudes the MFMA DFT reads (hi + lo of the
+    FP32-accumulated split-precision products) and every fold's zero-baseline
+    bound, against FP64 Eqn. 30 (the FP64-weight direct form, checked against
+    fourier_interp itself on a sample), on whole blocks: the first, the last
+    (partial), one at the 61.3 Hz pulsar and one mid-band.  The reference
+    states its own Float32-weight error as 2.0e-7 relative
+    (src/directinterp.jl:310-319); the same figures are given for it here.
+    Bars, about twice the worst case measured here (split-BF16 products keep
+    ~16 significand bits; the error grows with the amplitude): relative error
+    within 1e-4 for amplitudes above half the noise rms, absolute within 2e-4 in
+    noise-sigma units (the bins have unit mean power) and rms within 1e-5; every
+    bound within 1e-4 of the exact bound in units of the block sigma, against the
+    gate's 2-sigma margin (boxcar_medmargin).  The reported S/N carries none of
+    this error under the default precision :f64 (FP64 sigma and rescan)."""
+    w = c2
+    p = w.params
+    eng = cs.Engine(p, w.Nprof, device=0)
+    eng.set_mode("fused")
+    eng.set_spectrum(cs.FFTFile.from_arrays(w.amps, w.N, w.dt))
+    ex = O.Plans(p.nharms, p.m, p.hidr, w.r_lo, weights_f64=True)
+    f32 = O.Plans(p.nharms, p.m, p.hidr, w.r_lo, weights_f64=False)
+    op = O.make_params(nharms=p.nharms, m=16, decimations=p.decimations)
+    ks = [1] + [k for k in p.decimations if k > 1]
+    psr_b = math.floor((61.3 * w.T - w.r_lo) / w.lodr) // w.Nprof
+    blocks = [0, w.nchunks // 2, psr_b, w.nchunks - 1]
+    amp_err, amp_rel, a32_err, a32_rel, bnd_err, b32_err = [], [], [], [], [], []
+    try:
+        for b in blocks:
+            t0 = b * w.Nprof
+            n = min(w.Nprof, w.total - t0)
+            got_a, got_b = eng.probe_split_bf16(w.r_lo, w.r_hi, t0, n)
+            X = ex.fill_chunk(w.amps, w.N, t0, n)           # (nharms + 1, n), row 0 = DC
+            X32 = f32.fill_chunk(w.amps, w.N, t0, n)
+            if b == psr_b:
+                # the FP64-weight direct form is Eqn. 30 itself, up to the FP64
+                # rounding of h r (~1e-9 bins at h r ~ 8e6, moving the amplitude
+                # by ~1e-8), far below the split-BF16 error measured here
+                r0 = w.r_lo + (t0 + 7) * w.lodr
+                for h in (1, 17, 60):
+                    e30 = O.fourier_interp(h * r0, w.amps, p.m)
+                    assert abs(X[h, 7] - e30) <= 1e-7 * max(1.0, abs(e30))
+            live = np.any(X[1:] != 0, axis=1)
+            assert np.array_equal(np.any(got_a != 0, axis=1), live), f"block {b}: row guards"
+            d = np.abs(got_a[live] - X[1:][live])
+            d32 = np.abs(X32[1:][live] - X[1:][live])
+            mag = np.abs(X[1:][live])
+            amp_err.append(d.ravel())
+            a32_err.append(d32.ravel())
+            big = mag > 0.5
+            amp_rel.append((d[big] / mag[big]).ravel())
+            a32_rel.append((d32[big] / mag[big]).ravel())
+            rstart = w.r_lo + t0 * w.lodr
+            for f, k in enumerate(ks):
+                nb = 2 * (p.nharms // k)
+                nv = _nvalid(k, rstart, w.lodr, n, w.N / 2)
+                assert np.all(np.isneginf(got_b[f, nv:]))
+                if nv == 0:
+                    continue
+                widths = O.ladder_boxcar_widths(nb, k, op)
+                P = O.brfft_rows(X, nb, k)[:, :nv]
+                P32 = O.brfft_rows(X32, nb, k)[:, :nv]
+                exact = np.array([O.boxcar_scan(P[:, j], widths, 1.0, 0.0) for j in range(nv)])
+                b32 = np.array([O.boxcar_scan(P32[:, j], widths, 1.0, 0.0) for j in range(nv)])
+                sig = w.sigmas[b, f]
+                bnd_err.append(np.abs(got_b[f, :nv] - exact) / sig)
+                b32_err.append(np.abs(b32 - exact) / sig)
+    finally:
+        eng.close()
+    amp_err, amp_rel = np.concatenate(amp_err), np.concatenate(amp_rel)
+    a32_err, a32_rel = np.concatenate(a32_err), np.concatenate(a32_rel)
+    bnd_err, b32_err = np.concatenate(bnd_err), np.concatenate(b32_err)
+    rms = float(np.sqrt(np.mean(amp_err ** 2)))
+    _say(f"[n1] {amp_err.size} amplitudes (noise-sigma units): split-BF16 worst {amp_err.max():.2e} "
+         f"rms {rms:.2e}, relative (|a| > 0.5) worst {amp_rel.max():.2e} rms "
+         f"{np.sqrt(np.mean(amp_rel ** 2)):.2e}; FP32 weights worst {a32_err.max():.2e} rms "
+         f"{np.sqrt(np.mean(a32_err ** 2)):.2e}, relative worst {a32_rel.max():.2e}")
+    _say(f"[n1] {bnd_err.size} fold bounds (block-sigma units): split-BF16 worst {bnd_err.max():.2e} "
+         f"rms {np.sqrt(np.mean(bnd_err ** 2)):.2e}; FP32 weights worst {b32_err.max():.2e}; "
+         f"gate margin 2")
+    assert amp_rel.max() <= 1e-4 and amp_err.max() <= 2e-4 and rms <= 1e-5
+    assert bnd_err.max() <= 1e-4
