@@ -243,6 +243,7 @@ def run_c3(args, cs, nfiles):
                                f"host amplitudes, uploads inside the timed region",
                    "name": "c3", "mode": args.mode, "files": nfiles, "trials": total},
         "device_ms_per_step": round(dev, 3),
+        "stages_ms": {k: round(v, 3) for k, v in eng.timings().items()},
         "device_only_rate": round(total / (dev / 1e3), 1) if dev > 0 else None,
         "sequential_upload_then_search_ms": round(1e3 * seq_wall, 3),
         "overlap_gain": round(seq_wall / wall, 3),
@@ -366,6 +367,142 @@ def run_triage(args, cs):
         "features_ms": round(1e3 * t_feat, 3), "fit_s": round(fit_s, 2),
         "train_loss": [round(hist[0], 4), round(hist[-1], 4)],
         "full_raw_candidates": len(full), "curve": curve,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_c4(args, cs):
+    """Block triage at the training-set scale: train the scorer (both feature
+    kinds) on the device on `--c4-files` 2^24-sample files (seeds 4000 + i,
+    eight von Mises pulsars each, f log-uniform on 0.1-750 Hz, duty 2-20 %,
+    S/N 6-20), labelled by their full plain search; then on `--c4-held`
+    held-out files compare, per block fraction q, the triaged search (features,
+    scorer, the top-q blocks) with the same search mode given the same time on
+    the blocks in band order: recall of the full plain search's raw candidates
+    and of the injected pulsars (fundamentals 0.1-750 Hz, which the folds
+    k <= 6 of the 0.1-125 Hz band reach)."""
+    import numpy as np
+
+    from coherentsearch.synth import config_c4_file, make_config_spectra
+    from coherentsearch.triage import BlockTriage, block_labels
+
+    ntr, nho = args.c4_files, args.c4_held
+    params = cs.SearchParams(nharms=60, m=16, decimations=cs.decimation_set(60, 6), threshold=8.0)
+    eng = cs.Engine(params, 2048, device=0)
+    kinds = (0, 1)
+    feats = {k: [] for k in kinds}
+    labels = []
+    held = []          # (amps, cfg, grid, full raw)
+    t0 = time.perf_counter()
+    chunk = 16
+    for c0 in range(0, ntr + nho, chunk):
+        idx = list(range(c0, min(ntr + nho, c0 + chunk)))
+        cfgs = [config_c4_file(i) for i in idx]
+        amps = make_config_spectra(cfgs, workers=16)
+        for i, a, c in zip(idx, amps, cfgs):
+            ft = cs.FFTFile.from_arrays(a, c["N"], c["dt"])
+            eng.set_mode("plain")
+            eng.set_spectrum(ft)
+            g = cs.trial_grid(ft.T, params, 0.1, 125.0, 100, 2048)
+            raw = eng.search_raw(g[0], g[1], 8.0)
+            if i < ntr:
+                labels.append(block_labels(raw, g[5]))
+                for k in kinds:
+                    feats[k].append(eng.triage_features(g[0], g[1], kind=k))
+            else:
+                held.append((a, c, g, raw))
+        del amps
+        log(f"[bench] c4: {min(ntr + nho, c0 + chunk)}/{ntr + nho} files labelled "
+            f"({time.perf_counter() - t0:.0f} s)")
+    y = np.concatenate(labels)
+    log(f"[bench] c4: {len(y)} training blocks, {int(y.sum())} positive")
+    models, fit = {}, {}
+    for k in kinds:
+        X = np.concatenate(feats[k])
+        m = BlockTriage(seed=0, kind=k)
+        t1 = time.perf_counter()
+        hist = m.fit(eng, X, y, epochs=args.c4_epochs, lr=3e-3)
+        fit[k] = {"seconds": round(time.perf_counter() - t1, 2),
+                  "loss": [round(hist[0], 4), round(hist[-1], 4)]}
+        models[k] = m
+        log(f"[bench] c4: kind {k} trained, loss {hist[0]:.3f} -> {hist[-1]:.3f} "
+            f"in {fit[k]['seconds']} s")
+
+    def psr_found(raw, c, g, T):
+        r_lo, lodr, Nprof = g[0], g[2], g[4]
+        rs = np.array([cs.candidate_r(r_lo, lodr, Nprof, int(x["t"]), int(x["k"])) for x in raw])
+        n = 0
+        for ps in c["pulsars"]:
+            n += int(len(rs) > 0 and np.min(np.abs(rs / T - ps.freq)) < 3.0 / T)
+        return n
+
+    fracs = (0.01, 0.02, 0.05, 0.1, 0.2)
+    acc = {}
+    tot_c = tot_p = judge_p = 0
+    for a, c, g, full in held:
+        ft = cs.FFTFile.from_arrays(a, c["N"], c["dt"])
+        eng.set_spectrum(ft)
+        r_lo, r_hi, nch = g[0], g[1], g[5]
+        T = ft.T
+        fullset = {(int(x["t"]), int(x["k"])) for x in full}
+        tot_c += len(fullset)
+        tot_p += len(c["pulsars"])
+        judge_p += psr_found(full, c, g, T)
+        for mode in ("plain", "fused"):
+            eng.set_mode(mode)
+            eng.search_raw(r_lo, r_hi, 8.0)
+            t1 = time.perf_counter()
+            eng.search_raw(r_lo, r_hi, 8.0)
+            t_full = time.perf_counter() - t1
+            for k in kinds:
+                for q in fracs:
+                    ts = []
+                    for _ in range(3):
+                        t1 = time.perf_counter()
+                        X = models[k].features(eng, r_lo, r_hi)
+                        mask = models[k].select(models[k].logits(eng, X), q)
+                        raw = eng.search_blocks(r_lo, r_hi, 8.0, mask)
+                        ts.append(time.perf_counter() - t1)
+                    tq = sorted(ts)[1]
+                    kb = max(1, min(nch, int(nch * tq / t_full)))
+                    band = np.zeros(nch, np.uint8)
+                    band[:kb] = 1
+                    braw = eng.search_blocks(r_lo, r_hi, 8.0, band)
+                    key = (mode, k, q)
+                    d = acc.setdefault(key, [0, 0, 0, 0, 0.0, 0.0])
+                    d[0] += len(fullset & {(int(x["t"]), int(x["k"])) for x in raw})
+                    d[1] += len(fullset & {(int(x["t"]), int(x["k"])) for x in braw})
+                    d[2] += psr_found(raw, c, g, T)
+                    d[3] += psr_found(braw, c, g, T)
+                    d[4] += tq
+                    d[5] += t_full
+    curves = []
+    for (mode, k, q), d in sorted(acc.items()):
+        curves.append({"mode": mode, "features": k, "blocks_fraction": q,
+                       "triage_ms_per_file": round(1e3 * d[4] / len(held), 3),
+                       "full_search_ms_per_file": round(1e3 * d[5] / len(held), 3),
+                       "triage_candidate_recall": round(d[0] / max(1, tot_c), 4),
+                       "band_order_same_time_candidate_recall": round(d[1] / max(1, tot_c), 4),
+                       "triage_pulsars": f"{d[2]}/{judge_p}",
+                       "band_order_same_time_pulsars": f"{d[3]}/{judge_p}"})
+        log(f"[bench] c4: {curves[-1]}")
+    line = {
+        "metric": "block-triage recall at equal time, held-out 2^24-point files",
+        "value": next(cv["triage_candidate_recall"] for cv in curves
+                      if cv["mode"] == "plain" and cv["features"] == 1 and
+                      cv["blocks_fraction"] == 0.05),
+        "unit": "recall of the full plain search's raw candidates with 5% of the blocks "
+                "(plain mode, harmonic-sum features)",
+        "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+        "config": {"workload": f"train: {ntr} files of 2^24 samples (seeds 4000..{4000 + ntr - 1}, "
+                               f"8 pulsars each), labels from the full plain search; evaluate: "
+                               f"{nho} held-out files; nharms 60, folds 1..6, 0.1-125 Hz",
+                   "name": "c4", "train_blocks": int(len(y)),
+                   "train_positive_blocks": int(y.sum()), "held_out_files": nho,
+                   "held_out_candidates": tot_c,
+                   "injected_pulsars": tot_p, "pulsars_found_by_full_plain_search": judge_p},
+        "fit": {str(k): v for k, v in fit.items()}, "epochs": args.c4_epochs,
+        "curves": curves,
     }
     print(json.dumps(line), flush=True)
 
@@ -517,7 +654,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=4.0,
                     help="CPU work per timed baseline run (median of 3)")
-    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c3", "c5", "triage", "cands", "stats"])
+    ap.add_argument("--config", default="c2",
+                    choices=["c1", "c2", "c3", "c4", "c5", "triage", "cands", "stats"])
+    ap.add_argument("--c4-files", type=int, default=128, help="c4: training files")
+    ap.add_argument("--c4-held", type=int, default=16, help="c4: held-out files")
+    ap.add_argument("--c4-epochs", type=int, default=400)
     ap.add_argument("--c3-files", type=int, default=64)
     ap.add_argument("--autotune", action="store_true",
                     help="run cs_autotune before timing (the default launch shape is the one "
@@ -539,6 +680,9 @@ def main():
         return
     if args.config == "triage":
         run_triage(args, cs)
+        return
+    if args.config == "c4":
+        run_c4(args, cs)
         return
     if args.config == "cands":
         run_cands(args, cs)

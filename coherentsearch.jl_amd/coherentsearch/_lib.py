@@ -112,6 +112,9 @@ def lib():
     L.cs_triage_features.argtypes = [vp, C.c_double, C.c_double, fp, C.c_int64, i64p]
     L.cs_mlp_forward.argtypes = [vp, fp, fp, fp, C.c_int64, fp]
     L.cs_mlp_grad.argtypes = [vp, fp, fp, fp, fp, C.c_int64, fp]
+    L.cs_mlp_train.argtypes = [vp, fp, fp, fp, fp, C.c_int64, C.c_int32, C.c_float, fp]
+    L.cs_triage_features_kind.argtypes = [vp, C.c_int32, C.c_double, C.c_double, fp, C.c_int64,
+                                          i64p]
     L.cs_search_blocks.argtypes = [vp, C.c_double, C.c_double, C.c_double,
                                    C.POINTER(C.c_uint8), C.c_int64, C.POINTER(CsCand), C.c_int64,
                                    i64p]

@@ -216,16 +216,21 @@ class Engine:
             return [allc[offs[i]:offs[i + 1]] for i in range(len(files))]
 
     # -- block triage ------------------------------------------------------------
-    def triage_features(self, r_lo: float, r_hi: float) -> np.ndarray:
-        """[block][32] features of the band's search blocks (cs_triage_features)."""
+    def triage_features(self, r_lo: float, r_hi: float, kind: int = 0) -> np.ndarray:
+        """[block][32] features of the band's search blocks (cs_triage_features_kind):
+        kind 0 from power-pyramid cell maxima per integer fundamental, kind 1
+        from the nearest-bin power at the harmonics of every trial."""
+        L = _lib.lib()
         nb = C.c_int64()
-        st = _lib.lib().cs_triage_features(self._h, float(r_lo), float(r_hi), None, 0, C.byref(nb))
+        st = L.cs_triage_features_kind(self._h, int(kind), float(r_lo), float(r_hi), None, 0,
+                                       C.byref(nb))
         if st not in (_lib.CS_OK, _lib.CS_ENOSPC):
             self._chk(st, "cs_triage_features")
         out = np.zeros((nb.value, _lib.CS_TRIAGE_FEATURES), dtype=np.float32)
         if nb.value:
-            self._chk(_lib.lib().cs_triage_features(self._h, float(r_lo), float(r_hi), _fp(out),
-                                                    nb.value, C.byref(nb)), "cs_triage_features")
+            self._chk(L.cs_triage_features_kind(self._h, int(kind), float(r_lo), float(r_hi),
+                                                _fp(out), nb.value, C.byref(nb)),
+                      "cs_triage_features")
         return out
 
     def mlp_forward(self, params: np.ndarray, norm: np.ndarray, X: np.ndarray) -> np.ndarray:
@@ -242,6 +247,15 @@ class Engine:
         self._chk(_lib.lib().cs_mlp_grad(self._h, _fp(p), _fp(nm), _fp(x), _fp(yy), len(x),
                                          _fp(g)), "cs_mlp_grad")
         return g[:-1], float(g[-1])
+
+    def mlp_train(self, params: np.ndarray, norm: np.ndarray, X: np.ndarray, y: np.ndarray,
+                  epochs: int, lr: float):
+        """(trained parameters, per-step losses): cs_mlp_train, every step on the device"""
+        p, nm, x, yy = _f32(params).copy(), _f32(norm), _f32(X), _f32(y)
+        losses = np.zeros(int(epochs), dtype=np.float32)
+        self._chk(_lib.lib().cs_mlp_train(self._h, _fp(p), _fp(nm), _fp(x), _fp(yy), len(x),
+                                          int(epochs), float(lr), _fp(losses)), "cs_mlp_train")
+        return p, losses
 
     def search_blocks(self, r_lo: float, r_hi: float, threshold: float,
                       mask: np.ndarray) -> np.ndarray:

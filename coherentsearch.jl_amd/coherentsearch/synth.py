@@ -136,6 +136,23 @@ def config_c3_file(i: int):
                 nharms=32, lofreq=0.1, hifreq=125.0, threshold=8.0, maxdecim=4)
 
 
+def config_c4_file(i: int, npulsars: int = 8):
+    """File i of the block-triage training and evaluation set: 2^24 samples,
+    seed 4000 + i, npulsars von Mises pulsars with f log-uniform on
+    [0.1, 750] Hz, duty uniform on [2, 20] %, S/N uniform on [6, 20]; searched
+    like the 2^25-sample workload (nharms 60, folds 1..6, 0.1-125 Hz).  The
+    pulsars are drawn from the substream (4000 + i, 1) of numpy's SeedSequence,
+    apart from the noise stream of seed 4000 + i."""
+    rng = np.random.default_rng([4000 + i, 1])
+    psrs = []
+    for _ in range(npulsars):
+        f = float(math.exp(rng.uniform(math.log(0.1), math.log(750.0))))
+        psrs.append(Pulsar(f, float(rng.uniform(0.02, 0.20)), float(rng.uniform(6.0, 20.0)),
+                           float(rng.uniform(0.0, 1.0))))
+    return dict(N=1 << 24, dt=DT, seed=4000 + i, pulsars=psrs,
+                nharms=60, lofreq=0.1, hifreq=125.0, threshold=8.0, maxdecim=6)
+
+
 def config_c5():
     """2^27 samples with red noise below 1 Hz and five pulsars (incl. 716 Hz)."""
     return dict(N=1 << 27, dt=DT, seed=5001, red_fc=1.0,

@@ -3,10 +3,13 @@ only the best ones (no counterpart in the reference search; for callers that
 trade recall for time).
 
 Features: 32 incoherent-power statistics per search block, computed on the
-device (cs_triage_features).  Scorer: an MLP 32 -> 64 (ReLU) -> 1 logit whose
-forward and backward passes run on the device in bf16 with FP32 accumulation
-(cs_mlp_forward / cs_mlp_grad); the Adam update of its 2177 FP32 parameters
-runs here.  Labels: whether a block holds a raw candidate of the full search.
+device (cs_triage_features_kind; kind 1 sums the power at the nearest bins of
+the harmonics of every trial, kind 0 the power-pyramid maxima over the bins
+each harmonic of an integer fundamental can fall in).  Scorer: an MLP
+32 -> 64 (ReLU) -> 1 logit whose forward and backward passes run on the device
+in bf16 with FP32 accumulation; training runs every Adam step on the device as
+well (cs_mlp_train).  Labels: whether a block holds a raw candidate of the full
+plain search.
 """
 from __future__ import annotations
 
@@ -31,10 +34,12 @@ def block_labels(raw: np.ndarray, nblocks: int) -> np.ndarray:
 
 
 class BlockTriage:
-    """The scorer's parameters (W1 [64][32], b1, w2, b2) and the feature
-    standardisation (mean, inverse std) fitted on the training features."""
+    """The scorer's parameters (W1 [64][32], b1, w2, b2), the feature
+    standardisation (mean, inverse std) fitted on the training features and
+    the feature kind it was trained on."""
 
-    def __init__(self, seed: int = 0):
+    def __init__(self, seed: int = 0, kind: int = 0):
+        self.kind = int(kind)
         rng = np.random.default_rng(seed)
         W1 = rng.standard_normal((NH, NF)) * math.sqrt(2.0 / NF)
         w2 = rng.standard_normal(NH) * math.sqrt(1.0 / NH)
@@ -42,9 +47,11 @@ class BlockTriage:
         self.norm = np.concatenate([np.zeros(NF), np.ones(NF)]).astype(np.float32)
 
     def fit(self, engine, X: np.ndarray, y: np.ndarray, epochs: int = 400, lr: float = 3e-3,
-            pos_fraction: float = 0.25, seed: int = 0) -> List[float]:
+            pos_fraction: float = 0.25, seed: int = 0, on_device: bool = True) -> List[float]:
         """Full-batch Adam on the mean binary cross-entropy; the rare positive
-        blocks are repeated until they make up pos_fraction of the rows."""
+        blocks are repeated until they make up pos_fraction of the rows.  With
+        on_device every step runs on the GPU (cs_mlp_train); otherwise the
+        device computes the gradients and the FP64 Adam step runs here."""
         X = np.asarray(X, dtype=np.float32)
         y = np.asarray(y, dtype=np.float32)
         mean = X.mean(axis=0)
@@ -56,6 +63,10 @@ class BlockTriage:
             reps = int(math.ceil(pos_fraction * len(y) / len(pos)))
             rows = np.concatenate([rows] + [pos] * (reps - 1))
         Xr, yr = X[rows], y[rows]
+        if on_device:
+            p, losses = engine.mlp_train(self.params, self.norm, Xr, yr, epochs, lr)
+            self.params = p.astype(np.float32)
+            return [float(v) for v in losses]
         p = self.params.astype(np.float64)
         m1 = np.zeros_like(p)
         m2 = np.zeros_like(p)
@@ -83,13 +94,16 @@ class BlockTriage:
         mask[np.argsort(-logits, kind="stable")[:k]] = 1
         return mask
 
+    def features(self, engine, r_lo: float, r_hi: float) -> np.ndarray:
+        return engine.triage_features(r_lo, r_hi, self.kind)
+
     def save(self, path: str):
-        np.savez(path, params=self.params, norm=self.norm)
+        np.savez(path, params=self.params, norm=self.norm, kind=np.int32(self.kind))
 
     @classmethod
     def load(cls, path: str) -> "BlockTriage":
         d = np.load(path)
-        t = cls()
+        t = cls(kind=int(d["kind"]) if "kind" in d else 0)
         t.params = d["params"].astype(np.float32)
         t.norm = d["norm"].astype(np.float32)
         return t
@@ -98,6 +112,6 @@ class BlockTriage:
 def triage_search(engine, model: BlockTriage, r_lo: float, r_hi: float, threshold: float,
                   fraction: float) -> np.ndarray:
     """Raw candidates of the top `fraction` of the band's blocks by score."""
-    X = engine.triage_features(r_lo, r_hi)
+    X = model.features(engine, r_lo, r_hi)
     mask = model.select(model.logits(engine, X), fraction)
     return engine.search_blocks(r_lo, r_hi, threshold, mask)

@@ -1895,7 +1895,13 @@ static_assert(CS_TRIAGE_FEATURES == kTriageFeatures && CS_TRIAGE_PARAMS == kTria
               "triage sizes of the header and the kernels differ");
 int cs_triage_features(cs_ctx* c, double r_lo, double r_hi, float* out, int64_t cap_blocks,
                        int64_t* nblocks) {
+    return cs_triage_features_kind(c, 0, r_lo, r_hi, out, cap_blocks, nblocks);
+}
+
+int cs_triage_features_kind(cs_ctx* c, int32_t kind, double r_lo, double r_hi, float* out,
+                            int64_t cap_blocks, int64_t* nblocks) {
     if (!c || !nblocks) return CS_EINVAL;
+    if (kind != 0 && kind != 1) return c->fail(CS_EINVAL, "feature kind must be 0 or 1");
     *nblocks = 0;
     int s = check_spectrum(c);
     if (s) return s;
@@ -1912,9 +1918,11 @@ int cs_triage_features(cs_ctx* c, double r_lo, double r_hi, float* out, int64_t 
     for (int d : c->decims)
         if (d >= 2 && d <= 6) fm |= 1u << d;
     const SpecDev sp = spec_of(c);
-    if ((s = ensure(c, c->pyr, (size_t)triage_pyramid_floats(sp.namps) * sizeof(float)))) return s;
+    if (kind == 0 &&
+        (s = ensure(c, c->pyr, (size_t)triage_pyramid_floats(sp.namps) * sizeof(float))))
+        return s;
     if ((s = ensure(c, c->tfeat, (size_t)nch * kTriageFeatures * sizeof(float)))) return s;
-    HIPCHK(c, launch_triage_features(c->st, sp.amps, sp.namps, (float*)c->pyr.p, r_lo, lodr,
+    HIPCHK(c, launch_triage_features(c->st, kind, sp.amps, sp.namps, (float*)c->pyr.p, r_lo, lodr,
                                      c->Nprof, total, c->nharms, fm, nch, (float*)c->tfeat.p));
     HIPCHK(c, hipMemcpyAsync(out, c->tfeat.p, (size_t)nch * kTriageFeatures * sizeof(float),
                              hipMemcpyDeviceToHost, c->st));
@@ -1972,6 +1980,51 @@ int cs_mlp_grad(cs_ctx* c, const float* params, const float* norm, const float* 
                 int64_t n, float* grad) {
     if (!grad || (n > 0 && !y)) return CS_EINVAL;
     return mlp_call(c, params, norm, X, y, n, nullptr, grad);
+}
+
+int cs_mlp_train(cs_ctx* c, float* params, const float* norm, const float* X, const float* y,
+                 int64_t n, int32_t epochs, float lr, float* losses) {
+    if (!c || !params || !norm) return CS_EINVAL;
+    if (n < 1 || !X || !y) return c->fail(CS_EINVAL, "training needs n >= 1 rows and labels");
+    if (epochs < 1 || epochs > 100000) return c->fail(CS_EINVAL, "epochs must be in 1..100000");
+    if (!(lr > 0.0f)) return c->fail(CS_EINVAL, "lr must be positive");
+    HIPCHK(c, hipSetDevice(c->device));
+    int s;
+    const size_t np = kTriageParams, nx = (size_t)n * kTriageFeatures;
+    // parameters, norm, Adam moments and the loss history
+    const size_t nw = np + 2 * kTriageFeatures + 2 * np + (size_t)epochs;
+    if ((s = ensure(c, c->mlp_w, nw * sizeof(float)))) return s;
+    if ((s = ensure(c, c->mlp_x, (nx + 2 * (size_t)n) * sizeof(float)))) return s;
+    const size_t rows = (size_t)mlp_partial_rows(n);
+    if ((s = ensure(c, c->mlp_p, (rows + 1) * (np + 1) * sizeof(float)))) return s;
+    float* dW = (float*)c->mlp_w.p;
+    float* dN = dW + np;
+    float* dM1 = dN + 2 * kTriageFeatures;
+    float* dM2 = dM1 + np;
+    float* dL = dM2 + np;
+    float* dX = (float*)c->mlp_x.p;
+    float* dY = dX + nx;
+    float* dP = (float*)c->mlp_p.p;
+    float* dG = dP + rows * (np + 1);
+    HIPCHK(c, hipMemcpyAsync(dW, params, np * sizeof(float), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(dN, norm, 2 * kTriageFeatures * sizeof(float), hipMemcpyHostToDevice,
+                             c->st));
+    HIPCHK(c, hipMemcpyAsync(dX, X, nx * sizeof(float), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemcpyAsync(dY, y, (size_t)n * sizeof(float), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipMemsetAsync(dM1, 0, 2 * np * sizeof(float), c->st));
+    // every epoch on the device: forward and backward over all rows (fixed-order
+    // reduction), then the Adam step; the host waits once, at the end
+    for (int32_t it = 0; it < epochs; ++it) {
+        HIPCHK(c, launch_mlp(c->st, dW, dN, dX, dY, n, nullptr, dP, dG));
+        HIPCHK(c, launch_adam(c->st, dW, dM1, dM2, dG, (int)np, lr, 0.9f, 0.999f, 1e-8f, it + 1, dL,
+                              it));
+    }
+    HIPCHK(c, hipMemcpyAsync(params, dW, np * sizeof(float), hipMemcpyDeviceToHost, c->st));
+    if (losses)
+        HIPCHK(c, hipMemcpyAsync(losses, dL, (size_t)epochs * sizeof(float), hipMemcpyDeviceToHost,
+                                 c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return CS_OK;
 }
 
 static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, double threshold,

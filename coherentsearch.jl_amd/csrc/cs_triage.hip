@@ -21,7 +21,16 @@
 // The medians are the lower medians over the sampled fundamentals
 // f = floor(rs) + i * stride, stride = ceil(nfund / 512) (every fundamental
 // when a block has at most 512).
+//
+// Kind 1 keeps that layout with incoherent summed power at the harmonics of
+// every trial of the block instead: the units are the block's trials
+// r_t = (r_lo + i0 lodr) + j lodr (j = 0 .. n - 1, FP64, the search's own
+// expression order) and Q(g, t) = P(nint(g r_t)), nint(x) = floor(x + 1/2) in
+// FP64, so S_H(t) = sum_{h=1..H} P(nint(h r_t)); P(f) becomes P(nint(r_t)),
+// and the medians sample every stride-th trial.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
 
 #include "cs_triage.hpp"
 
@@ -126,7 +135,9 @@ __device__ float wave_lower_median(const float* v, int n, int lane) {
     return __uint_as_float(lo);
 }
 
-// one wave per search block; lanes over its integer fundamentals
+// one wave per search block; lanes over its integer fundamentals (KIND 0) or
+// its trials (KIND 1)
+template <int KIND>
 __global__ __launch_bounds__(256) void k_triage_features(PyrDev p, double r_lo, double lodr,
                                                          int64_t Nprof, int64_t total, int nh,
                                                          uint32_t foldmask, int64_t nblocks,
@@ -140,7 +151,7 @@ __global__ __launch_bounds__(256) void k_triage_features(PyrDev p, double r_lo, 
     const double rs = r_lo + (double)i0 * lodr;
     const double re = rs + (double)(n - 1) * lodr;
     const int64_t f0 = (int64_t)floor(rs), f1 = (int64_t)floor(re);
-    const int64_t nfund = f1 - f0 + 1;
+    const int64_t nfund = KIND == 0 ? f1 - f0 + 1 : n;   // units: fundamentals or trials
     const int64_t stride = (nfund + kMedSamples - 1) / kMedSamples;
     const int nsamp = (int)((nfund + stride - 1) / stride);
     int Hs[7] = {1, 2, 4, 8, 16, 32, nh};
@@ -153,12 +164,15 @@ __global__ __launch_bounds__(256) void k_triage_features(PyrDev p, double r_lo, 
         const int64_t idx = base + lane;
         const bool on = idx < nfund;
         const int64_t f = f0 + (on ? idx : 0);
+        const double rt = rs + (double)(on ? idx : 0) * lodr;
         float S[7], Sk[5];
         for (int j = 0; j < 5; ++j) Sk[j] = 0.0f;
-        float acc = 0.0f, qlast = 0.0f;
+        float acc = 0.0f, qlast = 0.0f, q1 = 0.0f;
         int hj = 0;
         for (int g = 1; g <= nh; ++g) {
-            const float q = q_of(p, g, f);
+            const float q = KIND == 0 ? q_of(p, g, f)
+                                      : power_at(p, (int64_t)floor((double)g * rt + 0.5));
+            if (g == 1) q1 = q;
             acc += q;
             qlast = q;
             for (int k = 2; k <= 6; ++k)
@@ -174,7 +188,7 @@ __global__ __launch_bounds__(256) void k_triage_features(PyrDev p, double r_lo, 
             }
             for (int k = 2; k <= 6; ++k)
                 if ((foldmask >> k) & 1u) kmax[k - 2] = fmaxf(kmax[k - 2], Sk[k - 2] / (float)(nh / k));
-            const float p1 = power_at(p, f);
+            const float p1 = q1;   // P(f), or P(nint(r_t))
             pmean += p1;
             qtop += qlast;
             over += (S[6] / (float)Hs[6] > 1.5f) ? 1.0f : 0.0f;
@@ -373,12 +387,19 @@ int64_t triage_pyramid_floats(int64_t namps) {
     return tot;
 }
 
-hipError_t launch_triage_features(hipStream_t st, const float2* amps, int64_t namps, float* pyr,
-                                  double r_lo, double lodr, int64_t Nprof, int64_t total, int nh,
-                                  uint32_t foldmask, int64_t nblocks, float* out) {
+hipError_t launch_triage_features(hipStream_t st, int kind, const float2* amps, int64_t namps,
+                                  float* pyr, double r_lo, double lodr, int64_t Nprof,
+                                  int64_t total, int nh, uint32_t foldmask, int64_t nblocks,
+                                  float* out) {
     PyrDev p{};
     p.amps = amps;
     p.namps = namps;
+    if (kind == 1) {   // harmonic sums at nint(h r_t): no pyramid
+        if (nblocks > 0)
+            hipLaunchKernelGGL(k_triage_features<1>, dim3((unsigned)((nblocks + 3) / 4)), dim3(256),
+                               0, st, p, r_lo, lodr, Nprof, total, nh, foldmask, nblocks, out);
+        return hipGetLastError();
+    }
     int64_t len = namps, off = 0;
     for (int s = 1; s < kPyrLevels; ++s) {
         const int64_t nl = (len + 1) / 2;
@@ -394,8 +415,8 @@ hipError_t launch_triage_features(hipStream_t st, const float2* amps, int64_t na
         len = nl;
     }
     if (nblocks > 0)
-        hipLaunchKernelGGL(k_triage_features, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0, st,
-                           p, r_lo, lodr, Nprof, total, nh, foldmask, nblocks, out);
+        hipLaunchKernelGGL(k_triage_features<0>, dim3((unsigned)((nblocks + 3) / 4)), dim3(256), 0,
+                           st, p, r_lo, lodr, Nprof, total, nh, foldmask, nblocks, out);
     return hipGetLastError();
 }
 
@@ -410,6 +431,32 @@ hipError_t launch_mlp(hipStream_t st, const float* params, const float* norm, co
     if (partial && grad)
         hipLaunchKernelGGL(k_mlp_reduce, dim3((kTriageParams + 1 + 255) / 256), dim3(256), 0, st,
                            partial, mlp_partial_rows(n), grad);
+    return hipGetLastError();
+}
+
+// One Adam step (FP32) of the scorer's parameters from the gradient the MLP
+// pass left in grad[0 .. n) (grad[n] = that pass's loss, kept in losses[it]).
+// c1, c2 = 1 - beta1^t, 1 - beta2^t of step t.
+__global__ void k_adam(float* __restrict__ p, float* __restrict__ m1, float* __restrict__ m2,
+                       const float* __restrict__ grad, int n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, float* __restrict__ losses, int it) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && losses) losses[it] = grad[n];
+    if (i >= n) return;
+    const float g = grad[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * g;
+    const float v = b2 * m2[i] + (1.0f - b2) * (g * g);
+    m1[i] = a;
+    m2[i] = v;
+    p[i] = p[i] - lr * (a / c1) / (sqrtf(v / c2) + eps);
+}
+
+hipError_t launch_adam(hipStream_t st, float* p, float* m1, float* m2, const float* grad, int n,
+                       float lr, float b1, float b2, float eps, int step, float* losses, int it) {
+    const float c1 = (float)(1.0 - std::pow((double)b1, step));
+    const float c2 = (float)(1.0 - std::pow((double)b2, step));
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, m1, m2, grad,
+                       n, lr, b1, b2, eps, c1, c2, losses, it);
     return hipGetLastError();
 }
 

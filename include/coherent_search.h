@@ -117,6 +117,16 @@ int cs_search(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
  * cap_blocks is smaller). */
 int cs_triage_features(cs_ctx *ctx, double r_lo, double r_hi, float *out, int64_t cap_blocks,
                        int64_t *nblocks);
+/* The same features computed two ways: kind 0 (cs_triage_features) takes, for
+ * each integer fundamental f of the block and harmonic g, the power-pyramid
+ * maximum over the bins g f .. g f + g - 1 the harmonic can fall in; kind 1
+ * takes the incoherent summed power at the harmonics of every trial of the
+ * block, P(nint(h r_t)) with P = |X|^2, nint(x) = floor(x + 1/2) in FP64.
+ * Both fill the same 32 slots (max, mean and std over the block of the sums at
+ * 1, 2, 4, ..., 32 and nharms harmonics, the fold sums, local power levels and
+ * medians). */
+int cs_triage_features_kind(cs_ctx *ctx, int32_t kind, double r_lo, double r_hi, float *out,
+                            int64_t cap_blocks, int64_t *nblocks);
 /* logits of n feature rows */
 int cs_mlp_forward(cs_ctx *ctx, const float *params, const float *norm, const float *X,
                    int64_t n, float *logits);
@@ -125,6 +135,13 @@ int cs_mlp_forward(cs_ctx *ctx, const float *params, const float *norm, const fl
  * the loss.  Reductions run in a fixed order (deterministic). */
 int cs_mlp_grad(cs_ctx *ctx, const float *params, const float *norm, const float *X,
                 const float *y, int64_t n, float *grad);
+/* Training on the device: `epochs` full-batch steps, each the forward and
+ * backward pass of cs_mlp_grad over the n rows followed by an FP32 Adam step
+ * (beta1 0.9, beta2 0.999, eps 1e-8, step size lr) on the device; params is
+ * read as the start and receives the result, losses[epochs] (optional) each
+ * step's mean binary cross-entropy before its update. */
+int cs_mlp_train(cs_ctx *ctx, float *params, const float *norm, const float *X, const float *y,
+                 int64_t n, int32_t epochs, float lr, float *losses);
 /* cs_search over the blocks with mask[block] != 0 only (nmask = the band's
  * block count, CS_EDIM otherwise); each selected block's candidates equal
  * those cs_search reports for it. */

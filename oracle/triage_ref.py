@@ -10,6 +10,9 @@ scorer are this project's own, so this module restates their definitions
   [g f, g f + g - 1] of each integer fundamental f, evaluated here as a
   plain max over the bins of those cells; features 30 and 31 are lower
   medians over every stride-th fundamental, stride = ceil(nfund / 512);
+* block_features_trials: kind 1, the same features from the nearest-bin
+  power at the harmonics of every trial of a block;
+* adam_steps: FP32 Adam, the device's training update;
 * mlp_forward / mlp_grad: 32 -> 64 (ReLU) -> 1 logit and the gradient of
   the mean binary cross-entropy, in FP64, optionally with the operands
   rounded to bf16 where the device rounds them (inputs after
@@ -95,6 +98,62 @@ def block_features(amps, r_lo, lodr, Nprof, total, nharms, decimations):
         F[30] = lower_median(p1[::stride])
         F[31] = F[6] / max(lower_median(S[::stride, 6]), 1e-30)
     return out
+
+
+def block_features_trials(amps, r_lo, lodr, Nprof, total, nharms, decimations):
+    """Kind 1: the same 32 features over every trial r_t = (r_lo + i0 lodr) +
+    j lodr of a block, with the harmonic power P(nint(h r_t)), nint(x) =
+    floor(x + 1/2) in FP64, in place of the pyramid maxima; medians over
+    every stride-th trial, stride = ceil(n / 512)."""
+    a = np.asarray(amps, dtype=np.complex64)
+    P = (a.real.astype(np.float32) ** 2 + a.imag.astype(np.float32) ** 2).astype(np.float64)
+    nb = len(P)
+    nblocks = -(-total // Nprof)
+    Hs = [min(h, nharms) for h in (1, 2, 4, 8, 16, 32, nharms)]
+    folds = [k for k in range(2, 7) if k in decimations]
+    h = np.arange(1, nharms + 1, dtype=np.float64)
+    out = np.zeros((nblocks, NF))
+    for c in range(nblocks):
+        i0 = c * Nprof
+        n = min(Nprof, total - i0)
+        rs = r_lo + i0 * lodr
+        rt = rs + np.arange(n, dtype=np.float64) * lodr
+        b = np.floor(h[None, :] * rt[:, None] + 0.5).astype(np.int64)   # [trial][h]
+        q = np.where((b >= 0) & (b < nb), P[np.clip(b, 0, nb - 1)], 0.0)
+        cs = np.cumsum(q, axis=1)
+        S = np.stack([cs[:, H - 1] / H for H in Hs], axis=1)
+        Sk = np.zeros((n, 5))
+        for k in folds:
+            Hk = nharms // k
+            Sk[:, k - 2] = q[:, np.arange(1, Hk + 1) * k - 1].sum(axis=1) / Hk
+        F = out[c]
+        F[0:7] = S.max(axis=0)
+        F[7:14] = S.mean(axis=0)
+        F[14:21] = S.std(axis=0)
+        F[21:26] = Sk.max(axis=0)
+        F[26] = q[:, 0].mean()
+        F[27] = q[:, nharms - 1].mean()
+        F[28] = math.log2(max(rs, 1.0)) / 20.0
+        F[29] = np.mean(S[:, 6] > 1.5)
+        stride = -(-n // MED_SAMPLES)
+        F[30] = lower_median(q[::stride, 0])
+        F[31] = F[6] / max(lower_median(S[::stride, 6]), 1e-30)
+    return out
+
+
+def adam_steps(params, grads, lr, b1=0.9, b2=0.999, eps=1e-8):
+    """FP32 Adam, as the device applies it, over a sequence of gradients."""
+    p = np.asarray(params, dtype=np.float32).copy()
+    m1 = np.zeros_like(p)
+    m2 = np.zeros_like(p)
+    f = np.float32
+    for t, g in enumerate(grads, start=1):
+        g = np.asarray(g, dtype=np.float32)
+        c1, c2 = f(1.0 - b1 ** t), f(1.0 - b2 ** t)
+        m1 = f(b1) * m1 + f(1.0 - b1) * g
+        m2 = f(b2) * m2 + f(1.0 - b2) * (g * g)
+        p = p - f(lr) * (m1 / c1) / (np.sqrt(m2 / c2) + f(eps))
+    return p
 
 
 def lower_median(v):

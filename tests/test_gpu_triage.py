@@ -130,3 +130,108 @@ def test_triage_training_separates_blocks(cs):
     assert hist[-1] < 0.7 * hist[0]
     z = model.logits(eng, X)
     assert z[y > 0].mean() > z[y == 0].mean() + 1.0
+
+
+@pytest.mark.parametrize("nharms,decims,nprof", [(32, [1, 2, 3, 4], 256),
+                                                 (60, [1, 2, 3, 4, 5, 6], 512), (8, [1], 2048)])
+def test_trial_features_vs_reference(cs, T, nharms, decims, nprof):
+    """Feature kind 1 (nearest-bin power at the harmonics of every trial)
+    against the FP64 reference: FP32 sums against FP64 ones."""
+    _say(f"triage trial features nharms={nharms} nprof={nprof}")
+    from coherentsearch.synth import Pulsar
+    N = 1 << 18
+    amps = _spectrum(cs, N, 6, [Pulsar(31.3, 0.05, 30.0)])
+    p = cs.SearchParams(nharms=nharms, m=16, decimations=decims)
+    eng = cs.Engine(p, nprof)
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, 64e-6))
+    lodr = 0.5 / nharms
+    T_ = N * 64e-6
+    r_lo = 31.3 * T_ - 2.5 * nprof * lodr
+    r_hi = r_lo + nprof * 5.5 * lodr
+    X = eng.triage_features(r_lo, r_hi, kind=1)
+    total = int(math.floor((r_hi - r_lo) / lodr)) + 1
+    want = T.block_features_trials(amps, r_lo, lodr, nprof, total, nharms, decims)
+    assert X.shape == want.shape == (6, 32)
+    np.testing.assert_allclose(X, want, rtol=2e-5, atol=2e-5)
+    assert np.argmax(X[:, 6]) == 2          # the block holding 31.3 Hz
+
+
+def test_device_training_equals_fp32_adam(cs, T):
+    """cs_mlp_train (forward, backward and the Adam step on the device) equals
+    FP32 Adam applied on the host to the device's own gradients, step by step,
+    and its reported losses are those gradients' losses."""
+    _say("device training")
+    from coherentsearch.triage import BlockTriage
+    rng = np.random.default_rng(8)
+    X = rng.standard_normal((3000, 32)).astype(np.float32)
+    y = (X[:, 3] + 0.5 * X[:, 7] > 0.8).astype(np.float32)
+    m = BlockTriage(seed=2)
+    m.norm = np.concatenate([X.mean(0), 1.0 / X.std(0)]).astype(np.float32)
+    eng = cs.Engine(cs.SearchParams(), 2048)
+    steps, lr = 6, 5e-3
+    got, losses = eng.mlp_train(m.params, m.norm, X, y, steps, lr)
+    p = m.params.copy()
+    grads, want_losses = [], []
+    for _ in range(steps):
+        g, loss = eng.mlp_grad(p, m.norm, X, y)
+        grads.append(g)
+        want_losses.append(loss)
+        p = T.adam_steps(m.params, grads, lr)
+    assert np.allclose(losses, want_losses, rtol=1e-6, atol=0)
+    assert np.max(np.abs(got - p)) <= 1e-6 * max(1.0, np.max(np.abs(p)))
+    # and it learns: many steps separate the classes
+    m.params, hist = eng.mlp_train(m.params, m.norm, X, y, 300, 1e-2)
+    z = eng.mlp_forward(m.params, m.norm, X)
+    assert hist[-1] < 0.5 * hist[0]
+    assert z[y > 0].mean() > z[y == 0].mean() + 2.0
+
+
+@pytest.mark.timeout(600)
+def test_c4_triage_on_held_out_files(cs):
+    """A reduced run of the triage workload: six 2^24-sample files of eight
+    injected pulsars each (seeds 4000..4005), labels from the full plain
+    search, both feature kinds trained on the device on four files; on the two
+    held-out files the top-scored 10 % of the blocks hold more of the full
+    search's candidates than the first 10 % in band order (the plain search
+    under the same block budget)."""
+    from coherentsearch.synth import config_c4_file, make_config_spectra
+    from coherentsearch.triage import BlockTriage, block_labels
+    cfgs = [config_c4_file(i) for i in range(6)]
+    _say("C4 reduced: generating 6 files")
+    amps = make_config_spectra(cfgs, workers=6)
+    p = cs.SearchParams(nharms=60, m=16, decimations=cs.decimation_set(60, 6), threshold=8.0)
+    eng = cs.Engine(p, 2048)
+    eng.set_mode("plain")
+    feats = {0: [], 1: []}
+    labels, raws, grids = [], [], []
+    for a, c in zip(amps, cfgs):
+        ft = cs.FFTFile.from_arrays(a, c["N"], c["dt"])
+        eng.set_spectrum(ft)
+        g = cs.trial_grid(ft.T, p, 0.1, 125.0, 100, 2048)
+        raw = eng.search_raw(g[0], g[1], 8.0)
+        for k in (0, 1):
+            feats[k].append(eng.triage_features(g[0], g[1], kind=k))
+        labels.append(block_labels(raw, g[5]))
+        raws.append(raw)
+        grids.append(g)
+    ytr = np.concatenate(labels[:4])
+    _say(f"C4 reduced: {len(ytr)} training blocks, {int(ytr.sum())} positive")
+    for k in (0, 1):
+        model = BlockTriage(seed=0, kind=k)
+        hist = model.fit(eng, np.concatenate(feats[k][:4]), ytr, epochs=300, lr=3e-3)
+        assert hist[-1] < hist[0]
+        rec_t = rec_b = 0
+        tot = 0
+        for X, raw, g in zip(feats[k][4:], raws[4:], grids[4:]):
+            mask = model.select(model.logits(eng, X), 0.10)
+            nb = g[5]
+            band = np.zeros(nb, np.uint8)
+            band[:int(mask.sum())] = 1
+            blk = raw["block"]
+            rec_t += int(np.sum(mask[blk] == 1))
+            rec_b += int(np.sum(band[blk] == 1))
+            tot += len(raw)
+        _say(f"C4 reduced kind {k}: loss {hist[0]:.3f} -> {hist[-1]:.3f}; held-out candidates in "
+             f"the top 10 % of blocks {rec_t}/{tot}, in the first 10 % in band order {rec_b}/{tot}")
+        assert tot > 0 and rec_t > rec_b
+    eng.close()

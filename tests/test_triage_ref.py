@@ -100,3 +100,70 @@ def test_lower_median_and_sampling():
     fs = np.arange(40, 40 + 1024)
     assert F[0, 30] == np.sort(P[fs[::2]])[255]
     assert F[0, 30] != np.sort(P[fs])[511]   # the full median differs on this seed
+
+
+def test_trial_features_against_brute_force():
+    """Kind 1 against plain Python loops over trials and harmonics."""
+    rng = np.random.default_rng(3)
+    nb = 3000
+    amps = (rng.standard_normal(nb) + 1j * rng.standard_normal(nb)).astype(np.complex64)
+    amps[0] = 0
+    P = [float(np.float32(a.real) * np.float32(a.real) + np.float32(a.imag) * np.float32(a.imag))
+         for a in amps]
+    r_lo, lodr, Nprof, total, nh = 40.3, 0.5 / 8, 64, 150, 8
+    got = T.block_features_trials(amps, r_lo, lodr, Nprof, total, nh, [1, 2, 3, 4])
+    for c in range(-(-total // Nprof)):
+        i0 = c * Nprof
+        n = min(Nprof, total - i0)
+        rs = r_lo + i0 * lodr
+        E = []
+        for j in range(n):
+            r = rs + j * lodr
+            E.append([P[int(math.floor(h * r + 0.5))] if math.floor(h * r + 0.5) < nb else 0.0
+                      for h in range(1, nh + 1)])
+        E = np.array(E)
+        s8 = E.sum(axis=1) / 8
+        s2 = E[:, :2].sum(axis=1) / 2
+        k2 = E[:, [1, 3, 5, 7]].sum(axis=1) / 4
+        assert math.isclose(got[c, 6], s8.max(), rel_tol=1e-12)
+        assert math.isclose(got[c, 8], s2.mean(), rel_tol=1e-12)
+        assert math.isclose(got[c, 21], k2.max(), rel_tol=1e-12)
+        assert math.isclose(got[c, 26], E[:, 0].mean(), rel_tol=1e-12)
+        assert math.isclose(got[c, 30], sorted(E[:, 0])[(n - 1) // 2], rel_tol=1e-12)
+
+
+def test_trial_features_find_a_comb():
+    """A comb of power at nint(h r0) lights up the block holding r0 and its
+    harmonic-sum features grow with the harmonics summed."""
+    nb = 20000
+    amps = np.zeros(nb, dtype=np.complex64) + 0.1
+    r0 = 1234.56
+    for h in range(1, 9):
+        amps[int(math.floor(h * r0 + 0.5))] = 4.0
+    lodr = 0.5 / 8
+    r_lo = 1200.0
+    F = T.block_features_trials(amps, r_lo, lodr, 256, 1200, 8, [1])
+    b = int((r0 - r_lo) / lodr) // 256
+    assert np.argmax(F[:, 6]) == b
+    assert math.isclose(F[b, 6], 16.0, rel_tol=1e-6)        # every harmonic on the comb
+    assert F[b, 0] == pytest.approx(16.0)
+
+
+def test_adam_steps_closed_form_and_fp64():
+    """The device's FP32 Adam: the first step moves each parameter by lr
+    sign(g) |g| / (|g| + eps); later steps follow FP64 Adam to FP32 rounding."""
+    rng = np.random.default_rng(4)
+    p0 = rng.standard_normal(50).astype(np.float32)
+    gs = [rng.standard_normal(50).astype(np.float32) for _ in range(5)]
+    p1 = T.adam_steps(p0, gs[:1], 1e-2)
+    want1 = p0.astype(np.float64) - 1e-2 * np.sign(gs[0]) * np.abs(gs[0]) / (np.abs(gs[0]) + 1e-8)
+    assert np.allclose(p1, want1, rtol=0, atol=2e-7)
+    p = p0.astype(np.float64)
+    m1 = np.zeros(50)
+    m2 = np.zeros(50)
+    for t, g in enumerate(gs, start=1):
+        g = g.astype(np.float64)
+        m1 = 0.9 * m1 + 0.1 * g
+        m2 = 0.999 * m2 + 0.001 * g * g
+        p = p - 1e-2 * (m1 / (1 - 0.9 ** t)) / (np.sqrt(m2 / (1 - 0.999 ** t)) + 1e-8)
+    assert np.allclose(T.adam_steps(p0, gs, 1e-2), p, rtol=0, atol=1e-6)
