@@ -94,14 +94,15 @@ struct cs_ctx {
     bool spl_valid = false;    // spl matches the current spectrum
     int ncu = 0;               // compute units of the device
     int64_t tune_batch = 0;    // CS_TUNE_BATCH_BLOCKS (0: default)
-    // multi-file search: spectrum view (a device slot instead of `amps`), two
-    // device slots, two pinned staging buffers and the upload stream
+    // multi-file search: spectrum view (a file's resident copy instead of
+    // `amps`), two pinned staging buffers and the upload stream
     const void* amps_view = nullptr;
-    DBuf slot[2];
+    DBuf sweep_amps, sweep_cands, sweep_ctr;   // cs_search_files: resident spectra, results
+    std::vector<hipEvent_t> sweep_up;          // per file: its upload has completed
+    hipEvent_t sweep_ev[2] = {nullptr, nullptr};
     void* pin[2] = {nullptr, nullptr};
     size_t pin_bytes = 0;
     hipStream_t cst = nullptr;
-    hipEvent_t up[2] = {nullptr, nullptr};
     int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
     bool use_graph = false;    // CS_TUNE_GRAPHS: replay a captured launch sequence
     hipGraph_t g_graph = nullptr;
@@ -397,11 +398,14 @@ int cs_ctx_destroy(cs_ctx* c) {
                     &c->pyr, &c->tfeat, &c->mlp_w, &c->mlp_x, &c->mlp_p,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
                     &c->m_desc, &c->m_rowmask, &c->m_scol, &c->m_ident, &c->m_sel, &c->m_slot,
-                    &c->m_tiles, &c->spl, &c->slot[0], &c->slot[1]};
+                    &c->m_tiles, &c->spl, &c->rflags, &c->probe_buf, &c->sweep_amps,
+                    &c->sweep_cands, &c->sweep_ctr};
     for (DBuf* b : bufs) release(*b);
     for (void* p : c->pin)
         if (p) (void)hipHostFree(p);
-    for (hipEvent_t e : c->up)
+    for (hipEvent_t e : c->sweep_up)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : c->sweep_ev)
         if (e) (void)hipEventDestroy(e);
     if (c->cst) (void)hipStreamDestroy(c->cst);
     for (auto& kv : c->twid) release(kv.second);
@@ -1002,6 +1006,17 @@ int cs_plan_tables(const cs_params* p, double r_lo, int32_t h, int64_t* info, in
     return CS_OK;
 }
 
+// cs_search_files: every file's candidates go to one device buffer with a
+// running count, the count after file `file` to ends[file]; the search
+// neither resets that count nor waits for its stream
+struct SweepSink {
+    cs_cand_dev* cands;
+    uint32_t* count;
+    uint32_t cap;
+    uint32_t* ends;
+    int32_t file;
+};
+
 // cs_probe_split_bf16: the trial range whose stacked amplitudes the split-BF16
 // kernel copies out, and where the amplitudes and bounds go on the host
 struct ProbeReq {
@@ -1139,7 +1154,8 @@ static int finish_stats(cs_ctx* c, const StatsReq& rq, const FoldSet& fs, const 
 
 static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, const uint8_t* mask,
                        int64_t nmask, cs_cand* out, int64_t cap, int64_t* n_out,
-                       const StatsReq* sreq = nullptr, const ProbeReq* probe = nullptr) {
+                       const StatsReq* sreq = nullptr, const ProbeReq* probe = nullptr,
+                       const SweepSink* sweep = nullptr) {
     if (!c || !n_out) return CS_EINVAL;
     *n_out = 0;
     for (double& v : c->timings) v = 0.0;
@@ -1380,6 +1396,12 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             }
             tile_rng.emplace_back(off, (int64_t)tiles.size() - off);
         }
+    } else if (mask && !fused) {
+        // plain pipeline over selected blocks: lists of up to Bc blocks
+        for (int64_t cg = 0; cg < nchunks; ++cg)
+            if (mask[cg]) selv.push_back((int32_t)cg);
+        for (int64_t p0 = 0; p0 < (int64_t)selv.size(); p0 += Bc)
+            batches.emplace_back(p0, std::min<int64_t>(Bc, (int64_t)selv.size() - p0));
     } else {
         for (int64_t c0 = 0; c0 < nchunks;) {
             if (mask && !mask[c0]) {
@@ -1427,6 +1449,13 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         }
     }
 
+    const bool plain_list = mask && !fused;
+    if (plain_list) {
+        if ((s = upload(c, c->m_sel, selv))) return s;
+        if ((s = ensure(c, c->m_slot, (size_t)nchunks * sizeof(int32_t)))) return s;
+        HIPCHK(c, launch_slot_table(c->st, (const int32_t*)c->m_sel.p, (int64_t)selv.size(),
+                                    nchunks, (int32_t*)c->m_slot.p));
+    }
     if (mfma) {
         // the fold and table descriptors, once per search (the batches only
         // differ in their by-value BatchDev)
@@ -1459,7 +1488,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     // every batch's kernels; stage events only when not captured into a graph.
     auto emit = [&](bool ev) -> int {
         if (mfma) HIPCHK(c, launch_rowmask(c->st, sp, tb, g, nchunks, (uint64_t*)c->m_rowmask.p));
-        HIPCHK(c, launch_reset_u32(c->st, surv_count, 2));
+        HIPCHK(c, launch_reset_u32(c->st, surv_count, sweep ? 1 : 2));
         for (int64_t bi = 0; bi < nbatch; ++bi) {
             hipEvent_t* E = &c->ev[bi * nev];
             auto rec = [&](int q) { return ev ? hipEventRecord(E[q], c->st) : hipSuccess; };
@@ -1474,6 +1503,13 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             b.nblocks = (int32_t)nb;
             b.nvalid = (const int32_t*)c->nvalid.p;
             b.sigma = (double*)c->sigma.p + c0 * nf;
+            if (plain_list) {   // c0 is a selection position; columns block by block
+                b.t_base = b.c_first = 0;
+                b.nt = nb * Nprof;
+                b.slot0 = (int32_t)c0;
+                b.sel = (const int32_t*)c->m_sel.p;
+                b.slot = (const int32_t*)c->m_slot.p;
+            }
             if (mfma) {   // c0 is a selection position
                 b.t_base = b.nt = b.c_first = 0;
                 b.slot0 = (int32_t)c0;
@@ -1487,9 +1523,9 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             o.surv = (uint64_t*)c->surv.p;
             o.surv_count = surv_count;
             o.surv_cap = (uint32_t)(BT * nf);
-            o.cand_cap = cand_cap;
-            o.cands = (cs_cand_dev*)c->cands.p;
-            o.cand_count = cand_count;
+            o.cand_cap = sweep ? sweep->cap : cand_cap;
+            o.cands = sweep ? sweep->cands : (cs_cand_dev*)c->cands.p;
+            o.cand_count = sweep ? sweep->count : cand_count;
             o.mall = c->stats_on ? (double*)c->st_metric.p : nullptr;
             o.mall_ld = total;
             // statistics fast path: the every-trial FP32 exact metrics select
@@ -1519,18 +1555,19 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
                 HIPCHK(c, rec(3));
                 HIPCHK(c, rec(4));
             } else {
-                HIPCHK(c, launch_interp(c->st, sp, tb, g, t_base, nt, false, c->ftp.p, BT));
+                HIPCHK(c, launch_interp(c->st, sp, tb, g, b.t_base, b.nt, false, c->ftp.p, BT, b.sel,
+                                        b.slot0));
                 HIPCHK(c, rec(1));
                 for (int f = 0; f < nf; ++f) {
                     if (plain_samples) {
-                        SampleSink sk{fp.samples, fp.sld, b.nvalid, g, t_base, c0, f, nf,
-                                      fs.cap[f], 0};
+                        SampleSink sk{fp.samples, fp.sld, b.nvalid, g, b.t_base, b.c_first, f, nf,
+                                      fs.cap[f], b.slot0, b.sel};
                         HIPCHK(c, launch_c2r_samples(c->st, (const float2*)c->ftp.p, BT,
-                                                     fs.f[f].k, fs.f[f].nbins, nt, fs.prof[f], BT,
-                                                     sk));
+                                                     fs.f[f].k, fs.f[f].nbins, b.nt, fs.prof[f],
+                                                     BT, sk));
                     } else {
                         HIPCHK(c, launch_c2r_f32(c->st, (const float2*)c->ftp.p, BT, fs.f[f].k,
-                                                 fs.f[f].nbins, nt, fs.prof[f], BT, fs.twid[f]));
+                                                 fs.f[f].nbins, b.nt, fs.prof[f], BT, fs.twid[f]));
                     }
                 }
                 HIPCHK(c, rec(2));
@@ -1548,7 +1585,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         }
         return CS_OK;
     };
-    const bool graph = c->use_graph && !sreq && !mask && !probe;
+    const bool graph = c->use_graph && !sreq && !mask && !probe && !sweep;
     std::vector<uint8_t> gkey;
     if (graph) {   // everything the captured launches bake in
         auto put = [&](const void* p, size_t n) {
@@ -1574,6 +1611,12 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     }
     if (sreq && stats_fast && (s = stats_metrics_plain(c, fs, tb, sp, g, total, Nprof, nchunks)))
         return s;
+    if (sweep) {   // the caller reads every file's results after its last search
+        c->sig_blocks = 0;
+        if ((s = emit(false))) return s;
+        HIPCHK(c, launch_copy_u32(c->st, sweep->count, sweep->ends + sweep->file));
+        return CS_OK;
+    }
     for (int attempt = 0; attempt < 2; ++attempt) {
         if ((s = ensure(c, c->cands, (size_t)cand_cap * sizeof(cs_cand_dev)))) return s;
         if (graph) {
@@ -2030,8 +2073,7 @@ int cs_mlp_train(cs_ctx* c, float* params, const float* norm, const float* X, co
 static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, double threshold,
                              cs_cand* out, int64_t cap, int64_t* counts, int64_t* n_out) {
     *n_out = 0;
-    // files of different lengths change every launch argument: graph replay
-    // would capture once per file, so the files run eager
+    // the sweep is one launch sequence: graph replay (keyed per search) stays off
     struct GraphOff {
         cs_ctx* c;
         bool was;
@@ -2040,19 +2082,23 @@ static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, do
     c->use_graph = false;
     if (nfiles <= 0) return CS_OK;
     if (cap < 0 || (cap > 0 && !out)) return c->fail(CS_EINVAL, "bad candidate buffer");
-    int64_t maxn = 0;
+    int64_t maxn = 0, namps_all = 0;
+    std::vector<int64_t> off(nfiles);
     for (int32_t i = 0; i < nfiles; ++i) {
         const cs_file& f = files[i];
         if (!f.amps || f.namps <= 0 || f.N <= 0 || !(f.T > 0))
             return c->fail(CS_EINVAL, "file " + std::to_string(i) + ": invalid spectrum");
         maxn = std::max(maxn, f.namps);
+        off[i] = namps_all;
+        namps_all += f.namps;
         counts[i] = 0;
     }
     HIPCHK(c, hipSetDevice(c->device));
-    const size_t maxb = (size_t)maxn * sizeof(float2);
     int s;
-    for (int k = 0; k < 2; ++k)
-        if ((s = ensure(c, c->slot[k], maxb))) return s;
+    // every spectrum stays resident for the sweep (3.9 GB for 64 files of
+    // 2^22..2^25 samples): an upload never waits for a search to release its slot
+    if ((s = ensure(c, c->sweep_amps, (size_t)namps_all * sizeof(float2)))) return s;
+    const size_t maxb = (size_t)maxn * sizeof(float2);
     if (c->pin_bytes < maxb) {
         for (void*& p : c->pin) {
             if (p) (void)hipHostFree(p);
@@ -2065,12 +2111,18 @@ static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, do
         c->pin_bytes = maxb;
     }
     if (!c->cst) HIPCHK(c, hipStreamCreateWithFlags(&c->cst, hipStreamNonBlocking));
-    for (hipEvent_t& e : c->up)
-        if (!e) HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    while ((int32_t)c->sweep_up.size() < nfiles) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->sweep_up.push_back(e);
+    }
+    for (hipEvent_t& e : c->sweep_ev)
+        if (!e) HIPCHK(c, hipEventCreate(&e));
+    if ((s = ensure(c, c->sweep_ctr, (size_t)(nfiles + 1) * sizeof(uint32_t)))) return s;
 
     // files of equal geometry run back to back, so the plan and table caches
-    // (keyed on r_lo and the trial count) are reused; results do not depend on
-    // the order
+    // (keyed on r_lo and the trial count) are rebuilt only between groups;
+    // results do not depend on the order
     std::vector<int32_t> order(nfiles);
     for (int32_t i = 0; i < nfiles; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
@@ -2079,12 +2131,16 @@ static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, do
         if (x.r_lo != y.r_lo) return x.r_lo < y.r_lo;
         return x.r_hi < y.r_hi;
     });
-    // staging of file k (worker thread): a multi-threaded copy into pinned
-    // slot k & 1, then its upload on the copy stream.  Slot k & 1 was last read
-    // by the search of file k - 2, which has completed.
-    auto stage = [c, files, &order](int k) -> hipError_t {
-        const cs_file& f = files[order[k]];
+    // staging of file k (worker thread): once the upload that last used pinned
+    // buffer k & 1 (file k - 2) has completed, a multi-threaded copy into it,
+    // then its upload to the file's resident slot on the copy stream
+    auto stage = [c, files, &order, &off](int k) -> hipError_t {
+        const int32_t fi = order[k];
+        const cs_file& f = files[fi];
         const int sl = k & 1;
+        hipError_t e = hipSetDevice(c->device);
+        if (e == hipSuccess && k >= 2) e = hipEventSynchronize(c->sweep_up[order[k - 2]]);
+        if (e != hipSuccess) return e;
         const size_t bytes = (size_t)f.namps * sizeof(float2);
         const unsigned hw = std::thread::hardware_concurrency();
         const int nt = (int)std::max<size_t>(1, std::min<size_t>({(size_t)(hw ? hw : 4), 8,
@@ -2098,59 +2154,97 @@ static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, do
             });
         }
         for (auto& th : pool) th.join();
-        hipError_t e = hipSetDevice(c->device);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(c->slot[sl].p, c->pin[sl], bytes, hipMemcpyHostToDevice, c->cst);
-        if (e == hipSuccess) e = hipEventRecord(c->up[sl], c->cst);
+        e = hipMemcpyAsync((float2*)c->sweep_amps.p + off[fi], c->pin[sl], bytes,
+                           hipMemcpyHostToDevice, c->cst);
+        if (e == hipSuccess) e = hipEventRecord(c->sweep_up[fi], c->cst);
         return e;
     };
-    std::vector<std::vector<cs_cand>> res(nfiles);
-    std::vector<double> tsum(8, 0.0);
-    HIPCHK(c, stage(0));
+    uint32_t ccap = (uint32_t)std::max<int64_t>(1 << 18, std::min<int64_t>(cap, (int64_t)1 << 26));
     int rc = CS_OK;
-    for (int32_t k = 0; k < nfiles && rc == CS_OK; ++k) {
+    uint32_t nc_all = 0;
+    std::vector<uint32_t> ends(nfiles);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if ((s = ensure(c, c->sweep_cands, (size_t)ccap * sizeof(cs_cand_dev)))) return s;
+        SweepSink sk{(cs_cand_dev*)c->sweep_cands.p, (uint32_t*)c->sweep_ctr.p, ccap,
+                     (uint32_t*)c->sweep_ctr.p + 1, 0};
+        HIPCHK(c, launch_reset_u32(c->st, sk.count, 1));
+        HIPCHK(c, hipEventRecord(c->sweep_ev[0], c->st));
         std::future<hipError_t> next;
-        if (k + 1 < nfiles) next = std::async(std::launch::async, stage, k + 1);
-        const cs_file& f = files[order[k]];
-        hipError_t e = hipStreamWaitEvent(c->st, c->up[k & 1], 0);
-        if (e == hipSuccess) {
-            c->amps_view = c->slot[k & 1].p;
+        if (attempt == 0) next = std::async(std::launch::async, stage, 0);
+        for (int32_t k = 0; k < nfiles && rc == CS_OK; ++k) {
+            if (next.valid()) {
+                const hipError_t e = next.get();
+                if (e != hipSuccess) {
+                    rc = c->fail(CS_EDEVICE, std::string("upload: ") + hipGetErrorString(e));
+                    break;
+                }
+            }
+            if (attempt == 0 && k + 1 < nfiles) next = std::async(std::launch::async, stage, k + 1);
+            const int32_t fi = order[k];
+            const cs_file& f = files[fi];
+            hipError_t e = hipStreamWaitEvent(c->st, c->sweep_up[fi], 0);
+            if (e != hipSuccess) {
+                rc = c->fail(CS_EDEVICE, std::string("upload wait: ") + hipGetErrorString(e));
+                break;
+            }
+            c->amps_view = (const float2*)c->sweep_amps.p + off[fi];
             c->namps = f.namps;
             c->N = f.N;
             c->T = f.T;
             c->spl_valid = false;
-            std::vector<cs_cand> buf(1024);
+            sk.file = fi;
             int64_t n = 0;
-            int st = cs_search(c, f.r_lo, f.r_hi, threshold, buf.data(), (int64_t)buf.size(), &n);
-            if (st == CS_ENOSPC && n > (int64_t)buf.size()) {
-                buf.resize((size_t)n);
-                st = cs_search(c, f.r_lo, f.r_hi, threshold, buf.data(), (int64_t)buf.size(), &n);
-            }
+            const int st = search_impl(c, f.r_lo, f.r_hi, threshold, nullptr, 0, nullptr, 0, &n,
+                                       nullptr, nullptr, &sk);
             if (st != CS_OK) rc = st;
-            buf.resize((size_t)std::max<int64_t>(0, std::min<int64_t>(n, (int64_t)buf.size())));
-            res[order[k]] = std::move(buf);
-            for (int i = 0; i < 7; ++i) tsum[i] += c->timings[i];
-            tsum[7] += c->timings[7];
-        } else {
-            rc = c->fail(CS_EDEVICE, std::string("upload wait: ") + hipGetErrorString(e));
         }
-        if (next.valid()) {
-            const hipError_t ne = next.get();
-            if (ne != hipSuccess && rc == CS_OK)
-                rc = c->fail(CS_EDEVICE, std::string("upload: ") + hipGetErrorString(ne));
+        if (next.valid()) (void)next.get();
+        HIPCHK(c, hipEventRecord(c->sweep_ev[1], c->st));
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        if (rc != CS_OK) break;
+        HIPCHK(c, hipMemcpy(&nc_all, sk.count, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        if (nc_all <= ccap) {
+            HIPCHK(c, hipMemcpy(ends.data(), sk.ends, (size_t)nfiles * sizeof(uint32_t),
+                                hipMemcpyDeviceToHost));
+            break;
         }
+        ccap = nc_all + 1024;   // the spectra are still resident: search again
     }
     (void)hipStreamSynchronize(c->cst);
     c->amps_view = nullptr;
     c->namps = 0;
     c->spl_valid = false;
-    for (int i = 0; i < 8; ++i) c->timings[i] = tsum[i];
     if (rc != CS_OK) return rc;
+    for (double& v : c->timings) v = 0.0;
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->sweep_ev[0], c->sweep_ev[1]));
+    c->timings[6] = ms;
+    std::vector<cs_cand_dev> hc((size_t)nc_all);
+    if (nc_all)
+        HIPCHK(c, hipMemcpy(hc.data(), c->sweep_cands.p, (size_t)nc_all * sizeof(cs_cand_dev),
+                            hipMemcpyDeviceToHost));
+    // file order[k] holds records ends[order[k - 1]] .. ends[order[k]] - 1
+    std::vector<std::pair<uint32_t, uint32_t>> rng(nfiles);
+    uint32_t prev = 0;
+    for (int32_t k = 0; k < nfiles; ++k) {
+        rng[order[k]] = {prev, ends[order[k]]};
+        prev = ends[order[k]];
+    }
     int64_t total = 0;
     for (int32_t i = 0; i < nfiles; ++i) {
-        counts[i] = (int64_t)res[i].size();
-        for (const cs_cand& cd : res[i]) {
-            if (total < cap) out[total] = cd;
+        auto b0 = hc.begin() + rng[i].first, b1 = hc.begin() + rng[i].second;
+        std::sort(b0, b1, [](const cs_cand_dev& a, const cs_cand_dev& b) {
+            return a.t != b.t ? a.t < b.t : a.k < b.k;
+        });
+        counts[i] = (int64_t)(rng[i].second - rng[i].first);
+        for (auto it = b0; it != b1; ++it) {
+            if (total < cap) {
+                out[total].t = it->t;
+                out[total].k = it->k;
+                out[total].nharm = it->nharm;
+                out[total].block = it->block;
+                out[total].metric = it->metric;
+            }
             ++total;
         }
     }

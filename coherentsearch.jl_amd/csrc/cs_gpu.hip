@@ -18,14 +18,22 @@ namespace cs {
 template <int MT, bool OUT64>
 __global__ __launch_bounds__(256) void k_interp(SpecDev sp, TabDev tb, GridDev g,
                                                 int64_t t_base, int64_t nt, void* out,
-                                                int64_t ld) {
+                                                int64_t ld, const int32_t* __restrict__ sel,
+                                                int32_t slot0) {
     const int h = blockIdx.y + 1;
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nt) return;
     int64_t t = t_base + j;
-    int64_t cg, first, last;
-    block_of(g, t, cg, first, last);
-    double2 v = interp_one<MT>(sp, tb, h, t, first, last, false, false);
+    if (sel) {
+        const int64_t cl = j / g.Nprof;
+        t = g.origin + (int64_t)sel[slot0 + cl] * g.Nprof + (j - cl * g.Nprof);
+    }
+    double2 v = make_double2(0.0, 0.0);
+    if (t < g.total) {
+        int64_t cg, first, last;
+        block_of(g, t, cg, first, last);
+        v = interp_one<MT>(sp, tb, h, t, first, last, false, false);
+    }
     if (OUT64) {
         ((double2*)out)[(int64_t)(h - 1) * ld + j] = v;
     } else {
@@ -35,19 +43,22 @@ __global__ __launch_bounds__(256) void k_interp(SpecDev sp, TabDev tb, GridDev g
 
 hipError_t launch_interp(hipStream_t st, const SpecDev& sp, const TabDev& tb,
                          const GridDev& g, int64_t t_base, int64_t nt, bool out64,
-                         void* out, int64_t ld) {
+                         void* out, int64_t ld, const int32_t* sel, int32_t slot0) {
     if (nt <= 0) return hipSuccess;
     dim3 grid((unsigned)((nt + 255) / 256), (unsigned)tb.nharms);
     bool f32 = !tb.weights_f64;
+#define CS_INTERP(MT, O64) \
+    hipLaunchKernelGGL((k_interp<MT, O64>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld, sel, slot0)
     if (out64) {
-        if (f32 && tb.m == 16) hipLaunchKernelGGL((k_interp<16, true>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
-        else if (f32 && tb.m == 32) hipLaunchKernelGGL((k_interp<32, true>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
-        else hipLaunchKernelGGL((k_interp<0, true>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
+        if (f32 && tb.m == 16) CS_INTERP(16, true);
+        else if (f32 && tb.m == 32) CS_INTERP(32, true);
+        else CS_INTERP(0, true);
     } else {
-        if (f32 && tb.m == 16) hipLaunchKernelGGL((k_interp<16, false>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
-        else if (f32 && tb.m == 32) hipLaunchKernelGGL((k_interp<32, false>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
-        else hipLaunchKernelGGL((k_interp<0, false>), grid, dim3(256), 0, st, sp, tb, g, t_base, nt, out, ld);
+        if (f32 && tb.m == 16) CS_INTERP(16, false);
+        else if (f32 && tb.m == 32) CS_INTERP(32, false);
+        else CS_INTERP(0, false);
     }
+#undef CS_INTERP
     return hipGetLastError();
 }
 
@@ -132,15 +143,21 @@ __global__ __launch_bounds__(256) void k_c2r_samples(const float2* __restrict__ 
     cs_c2r::Fixed<NB>::run(xr, xi, out);
 #pragma unroll
     for (int i = 0; i < NB; ++i) prof[(int64_t)i * ldp + j] = out[i];
-    const int64_t t = sk.t_base + j;
+    int64_t t = sk.t_base + j, cl;
+    if (sk.sel) {
+        cl = j / sk.g.Nprof;
+        t = sk.g.origin + (int64_t)sk.sel[sk.slot0 + cl] * sk.g.Nprof + (j - cl * sk.g.Nprof);
+        if (t >= sk.g.total) return;
+    }
     int64_t cg, first, last;
     block_of(sk.g, t, cg, first, last);
+    if (!sk.sel) cl = cg - sk.c_first;
     const int64_t jj = t - first;
     const int64_t nv = sk.nvalid[cg * sk.nf + sk.f];
     if (jj >= nv) return;
     const int sc = sample_column(jj, nv, NB, sk.cap);
     if (sc < 0) return;
-    float* d = sk.samples + ((cg - sk.c_first) * sk.nf + sk.f) * sk.sld + (int64_t)sc * NB;
+    float* d = sk.samples + (cl * sk.nf + sk.f) * sk.sld + (int64_t)sc * NB;
 #pragma unroll
     for (int i = 0; i < NB; ++i) d[i] = fabsf(out[i]);
 }
@@ -277,7 +294,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sigma32(FoldSet fs, GridDev g, 
     __shared__ uint32_t hist[2048];
     __shared__ uint32_t sh[64];
     const int cl = blockIdx.x, f = blockIdx.y, nf = fs.nf;
-    const int64_t cg = b.c_first + cl;
+    const int64_t cg = batch_block(b, cl);
     const int64_t first = g.origin + cg * g.Nprof;
     const int64_t nv = b.nvalid[cg * nf + f];
     const int nbins = fs.f[f].nbins;
@@ -297,7 +314,7 @@ __global__ __launch_bounds__(kSelThreads) void k_sigma32(FoldSet fs, GridDev g, 
         ns = ncol * nbins < cap ? ncol * nbins : cap;
     }
     const float* P = fs.prof[f];
-    const int64_t col0 = first - b.t_base;
+    const int64_t col0 = b.sel ? (int64_t)cl * g.Nprof : first - b.t_base;
     uint32_t keys[kSelPer];
 #pragma unroll
     for (int r = 0; r < kSelPer; ++r) {
@@ -365,11 +382,11 @@ __global__ __launch_bounds__(256) void k_gate(const float* __restrict__ prof, in
                                               BatchDev b, int nf, OutDev o) {
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool pred = false;
-    int64_t t = b.t_base + j;
-    if (j < b.nt) {
+    const int64_t t = j < b.nt ? batch_trial(b, g, j) : -1;
+    if (t >= 0) {
         int64_t cg, first, last;
         block_of(g, t, cg, first, last);
-        int cl = (int)(cg - b.c_first);
+        int cl = batch_slot(b, cg);
         if (t - first < b.nvalid[cg * nf + fidx]) {
             double sig = b.sigma[cl * nf + fidx];
             double invs = sig > 0 ? 1.0 / sig : 0.0;
@@ -389,7 +406,7 @@ __global__ __launch_bounds__(256) void k_gate(const float* __restrict__ prof, in
             }
         }
     }
-    if (o.gm) pred = j < b.nt && o.gm[(int64_t)fidx * o.gm_ld + t] > o.gmcut;
+    if (o.gm) pred = t >= 0 && o.gm[(int64_t)fidx * o.gm_ld + t] > o.gmcut;
     push_survivor(pred, t, fidx, o);
 }
 
@@ -403,12 +420,12 @@ __global__ __launch_bounds__(64) void k_gate_generic(const float* __restrict__ p
     const int wmax = fd.widths[fd.nw - 1];
     int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     bool pred = false;
-    int64_t t = b.t_base + j;
+    const int64_t t = j < b.nt ? batch_trial(b, g, j) : -1;
     const int L = threadIdx.x;
-    if (j < b.nt) {
+    if (t >= 0) {
         int64_t cg, first, last;
         block_of(g, t, cg, first, last);
-        int cl = (int)(cg - b.c_first);
+        int cl = batch_slot(b, cg);
         if (t - first < b.nvalid[cg * nf + fidx]) {
             double sig = b.sigma[cl * nf + fidx];
             double invs = sig > 0 ? 1.0 / sig : 0.0;
@@ -437,7 +454,7 @@ __global__ __launch_bounds__(64) void k_gate_generic(const float* __restrict__ p
             }
         }
     }
-    if (o.gm) pred = j < b.nt && o.gm[(int64_t)fidx * o.gm_ld + t] > o.gmcut;
+    if (o.gm) pred = t >= 0 && o.gm[(int64_t)fidx * o.gm_ld + t] > o.gmcut;
     push_survivor(pred, t, fidx, o);
 }
 
@@ -861,6 +878,15 @@ hipError_t launch_sigma_refine(hipStream_t st, const SpecDev& sp, const TabDev& 
 __global__ void k_reset_u32(uint32_t* p, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+__global__ void k_copy_u32(const uint32_t* src, uint32_t* dst) {
+    if (threadIdx.x == 0) *dst = *src;
+}
+
+hipError_t launch_copy_u32(hipStream_t st, const uint32_t* src, uint32_t* dst) {
+    hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(64), 0, st, src, dst);
+    return hipGetLastError();
 }
 
 hipError_t launch_reset_u32(hipStream_t st, uint32_t* p, int n) {

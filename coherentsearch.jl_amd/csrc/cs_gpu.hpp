@@ -83,6 +83,16 @@ __host__ __device__ inline int batch_slot(const BatchDev& b, int64_t cg) {
     return (cl >= 0 && cl < b.nblocks) ? (int)cl : -1;
 }
 
+// Trial of batch column j: a run of blocks maps column j to t_base + j; a
+// block list (plain pipeline over selected blocks) maps it to trial j % Nprof
+// of block sel[slot0 + j / Nprof], and a column past the band's last trial to -1.
+__host__ __device__ inline int64_t batch_trial(const BatchDev& b, const GridDev& g, int64_t j) {
+    if (!b.sel) return b.t_base + j;
+    const int64_t cl = j / g.Nprof;
+    const int64_t t = g.origin + (int64_t)b.sel[b.slot0 + cl] * g.Nprof + (j - cl * g.Nprof);
+    return t < g.total ? t : -1;
+}
+
 struct OutDev {
     uint64_t* surv;        // packed (t << 4) | fold
     uint32_t* surv_count;
@@ -110,9 +120,11 @@ struct cs_cand_dev {
 };
 
 // ---- launchers (all asynchronous on `st`) ------------------------------------
+// columns j < nt: trial t_base + j, or with `sel` trial j % Nprof of block
+// sel[slot0 + j / Nprof] (columns past the band's end are zero)
 hipError_t launch_interp(hipStream_t st, const SpecDev& sp, const TabDev& tb,
                          const GridDev& g, int64_t t_base, int64_t nt, bool out64,
-                         void* out, int64_t ld);
+                         void* out, int64_t ld, const int32_t* sel = nullptr, int32_t slot0 = 0);
 hipError_t launch_c2r_f32(hipStream_t st, const float2* ftp, int64_t ldf, int k,
                           int nbins, int64_t nt, float* prof, int64_t ldp,
                           const double2* twid);
@@ -129,7 +141,8 @@ struct SampleSink {
     int64_t c_first;
     int32_t f, nf;
     int32_t cap;
-    int32_t pad;
+    int32_t slot0;           // with sel: the batch's first selection position
+    const int32_t* sel;      // block list of the batch (see batch_trial), or null
 };
 bool c2r_samples_supported(int nbins);
 hipError_t launch_c2r_samples(hipStream_t st, const float2* ftp, int64_t ldf, int k, int nbins,
@@ -140,6 +153,7 @@ hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
                          const GridDev& g, const FoldSet& fs, const BatchDev& b,
                          const OutDev& o, int nblocks_grid);
 hipError_t launch_reset_u32(hipStream_t st, uint32_t* p, int n);
+hipError_t launch_copy_u32(hipStream_t st, const uint32_t* src, uint32_t* dst);
 // FP64 sigma of the batch's (block, fold) pairs that hold gate survivors
 // (flags: scratch of nblocks * nf words, cleared by the launch itself)
 hipError_t launch_sigma_refine(hipStream_t st, const SpecDev& sp, const TabDev& tb,
