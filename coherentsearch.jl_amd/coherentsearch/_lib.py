@@ -17,7 +17,7 @@ HEADER = os.path.join(_HERE, "..", "..", "include", "coherent_search.h")
 
 CS_OK, CS_EINVAL, CS_EDIM, CS_ENOSPC, CS_EDEVICE, CS_ENOMEM = range(6)
 CS_MODE_PLAIN, CS_MODE_FUSED, CS_MODE_FUSED_VALU = 0, 1, 2
-CS_TUNE_BATCH_BLOCKS, CS_TUNE_WORKGROUPS, CS_TUNE_GRAPHS = 1, 2, 3
+CS_TUNE_BATCH_BLOCKS, CS_TUNE_WORKGROUPS, CS_TUNE_GRAPHS, CS_TUNE_TILE_ORDER = 1, 2, 3, 4
 CS_TRIAGE_FEATURES, CS_TRIAGE_PARAMS = 32, 2177
 
 
@@ -138,6 +138,7 @@ def lib():
     L.cs_interp_rows.argtypes = [vp, C.c_double, C.c_int64, C.c_int64, C.c_int32, dp]
     L.cs_set_mode.argtypes = [vp, C.c_int32]
     L.cs_set_tuning.argtypes = [vp, C.c_int32, C.c_int64]
+    L.cs_get_tuning.argtypes = [vp, C.c_int32, i64p]
     L.cs_autotune.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_int32,
                               i64p, C.POINTER(C.c_int32)]
     L.cs_last_timings.argtypes = [vp, dp]

@@ -95,10 +95,13 @@ class Engine:
         self._chk(_lib.lib().cs_set_mode(self._h, m), "cs_set_mode")
 
     def set_tuning(self, batch_blocks: Optional[int] = None, workgroups: Optional[int] = None,
-                   graphs: Optional[bool] = None):
-        """Launch shape of the split-BF16 search (0 or None leaves the default)
-        and graph replay of the search's launches; results do not depend on
-        them."""
+                   graphs: Optional[bool] = None, tile_order: Optional[int] = None):
+        """Launch shape of the search (0 leaves the default, None leaves the
+        knob as it is) and graph replay of the search's launches; results do
+        not depend on them."""
+        if tile_order is not None:
+            self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_TILE_ORDER, int(tile_order)),
+                      "cs_set_tuning")
         if graphs is not None:
             self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_GRAPHS, int(bool(graphs))),
                       "cs_set_tuning")
@@ -108,6 +111,17 @@ class Engine:
         if workgroups is not None:
             self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_WORKGROUPS, int(workgroups)),
                       "cs_set_tuning")
+
+    def tuning(self) -> dict:
+        """Every CS_TUNE_* knob's current value (cs_get_tuning)."""
+        out = {}
+        for name, knob in (("batch_blocks", _lib.CS_TUNE_BATCH_BLOCKS),
+                           ("workgroups", _lib.CS_TUNE_WORKGROUPS),
+                           ("graphs", _lib.CS_TUNE_GRAPHS), ("tile_order", _lib.CS_TUNE_TILE_ORDER)):
+            v = C.c_int64()
+            self._chk(_lib.lib().cs_get_tuning(self._h, knob, C.byref(v)), "cs_get_tuning")
+            out[name] = int(v.value)
+        return out
 
     def autotune(self, r_lo: float, r_hi: float, threshold: float,
                  cache_path: Optional[str] = None, reps: int = 2) -> Tuple[int, int]:

@@ -38,7 +38,7 @@ struct DBuf {
 constexpr int kBankW[12] = {1, 2, 3, 4, 6, 9, 13, 19, 28, 42, 63, 94};
 constexpr int64_t kBatchTrials = (int64_t)1 << 21;   // plain path: trials per batch (HBM scratch bound)
 constexpr int64_t kBatchTrialsFused = (int64_t)1 << 25;   // fused path: ~1 GB of bounds + samples per batch
-// split-BF16 path: blocks per batch (6 folds: 6.4 GB of sigma samples and
+// split-BF16 path: blocks per batch (6 folds: 3.5 GB of sigma samples and
 // 0.8 GB of bounds at 2048-trial blocks), one batch at 2^25 points
 constexpr int64_t kBatchBlocksMfma = 16384;
 constexpr int kMaxHarm = 64;                         // device path: profiles of <= 128 bins
@@ -104,6 +104,7 @@ struct cs_ctx {
     size_t pin_bytes = 0;
     hipStream_t cst = nullptr;
     int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
+    int tune_order = 0;        // CS_TUNE_TILE_ORDER (0: contiguous tile ranges per XCD)
     bool use_graph = false;    // CS_TUNE_GRAPHS: replay a captured launch sequence
     hipGraph_t g_graph = nullptr;
     hipGraphExec_t g_exec = nullptr;
@@ -436,6 +437,9 @@ int cs_set_tuning(cs_ctx* c, int32_t knob, int64_t value) {
     } else if (knob == CS_TUNE_GRAPHS) {
         if (value > 1) return c->fail(CS_EINVAL, "CS_TUNE_GRAPHS takes 0 or 1");
         c->use_graph = value == 1;
+    } else if (knob == CS_TUNE_TILE_ORDER) {
+        if (value > 1) return c->fail(CS_EINVAL, "CS_TUNE_TILE_ORDER takes 0 or 1");
+        c->tune_order = (int)value;
     } else {
         return c->fail(CS_EINVAL, "unknown tuning knob");
     }
@@ -444,6 +448,16 @@ int cs_set_tuning(cs_ctx* c, int32_t knob, int64_t value) {
 
 // cs_autotune: the cache line of a search geometry (parameters, trial range,
 // spectrum length, device architecture)
+int cs_get_tuning(const cs_ctx* c, int32_t knob, int64_t* value) {
+    if (!c || !value) return CS_EINVAL;
+    if (knob == CS_TUNE_BATCH_BLOCKS) *value = c->tune_batch;
+    else if (knob == CS_TUNE_WORKGROUPS) *value = c->tune_wg;
+    else if (knob == CS_TUNE_GRAPHS) *value = c->use_graph ? 1 : 0;
+    else if (knob == CS_TUNE_TILE_ORDER) *value = c->tune_order;
+    else return CS_EINVAL;
+    return CS_OK;
+}
+
 static std::string tune_key(cs_ctx* c, double r_lo, double r_hi) {
     hipDeviceProp_t prop;
     const char* arch = hipGetDeviceProperties(&prop, c->device) == hipSuccess ? prop.gcnArchName : "?";
@@ -486,11 +500,6 @@ int cs_autotune(cs_ctx* c, double r_lo, double r_hi, double threshold, const cha
         if (out_wg) *out_wg = c->tune_wg;
         return CS_OK;
     };
-    if (c->mode != CS_MODE_FUSED) {   // the knobs only shape the split-BF16 search
-        c->tune_batch = 0;
-        c->tune_wg = 0;
-        return report();
-    }
     const std::string key = tune_key(c, r_lo, r_hi);
     std::vector<std::string> lines;
     if (cache_path && *cache_path) {
@@ -502,12 +511,14 @@ int cs_autotune(cs_ctx* c, double r_lo, double r_hi, double threshold, const cha
                 if (l.empty()) continue;
                 const size_t tab = l.find('\t');
                 if (tab != std::string::npos && l.compare(0, tab, key) == 0) {
-                    long long b = 0, w = 0;
-                    if (std::sscanf(l.c_str() + tab + 1, "%lld %lld", &b, &w) == 2 && b >= 0 &&
-                        w >= 0 && w <= 65536) {
+                    long long b = 0, w = 0, o = 0;
+                    double msv = 0.0;
+                    if (std::sscanf(l.c_str() + tab + 1, "%lld %lld %lf %lld", &b, &w, &msv, &o) >= 2 &&
+                        b >= 0 && w >= 0 && w <= 65536 && (o == 0 || o == 1)) {
                         std::fclose(fh);
                         c->tune_batch = b;
                         c->tune_wg = (int)w;
+                        c->tune_order = (int)o;
                         return report();
                     }
                     continue;   // malformed entry: measured again and replaced
@@ -517,41 +528,59 @@ int cs_autotune(cs_ctx* c, double r_lo, double r_hi, double threshold, const cha
             std::fclose(fh);
         }
     }
-    // candidates: persistent grid sizes around one workgroup per CU, batch
-    // sizes dividing the search into 1, 2 or 4 batches
+    // candidates.  Split-BF16 mode: persistent grid sizes around one
+    // workgroup per CU, batch sizes dividing the search into 1, 2 or 4
+    // batches, and the tile order (contiguous ranges per XCD or round-robin).
+    // Plain and VALU-fused modes: blocks per batch (the default, half and
+    // twice).  Every candidate returns the same results (the knobs only
+    // schedule the trials).
     const double lodr = c->hidr / c->nharms;
     const double span = std::floor((r_hi - r_lo) / lodr);
     const int64_t total = span >= 0 ? (int64_t)span + 1 : 0;
     const int64_t nchunks = (total + c->Nprof - 1) / c->Nprof;
     const int ncu = c->ncu > 0 ? c->ncu : 256;
-    std::vector<int> wgs{0, std::max(8, ncu * 3 / 4 / 8 * 8), std::max(8, ncu / 2 / 8 * 8)};
+    std::vector<int> wgs{0}, orders{0};
     std::vector<int64_t> bats{0};
-    for (int64_t d : {2, 4})
-        if (nchunks >= d) bats.push_back((nchunks + d - 1) / d);
+    if (c->mode == CS_MODE_FUSED) {
+        wgs = {0, std::max(8, ncu * 3 / 4 / 8 * 8), std::max(8, ncu / 2 / 8 * 8)};
+        orders = {0, 1};
+        for (int64_t d : {2, 4})
+            if (nchunks >= d) bats.push_back((nchunks + d - 1) / d);
+    } else {
+        const int64_t def = std::max<int64_t>(
+            1, (c->mode == CS_MODE_FUSED_VALU ? kBatchTrialsFused : kBatchTrials) / c->Nprof);
+        for (int64_t bt : {def / 2, def * 2})
+            if (bt >= 1 && bt < nchunks * 2) bats.push_back(bt);
+    }
     double best = INFINITY;
     int64_t bb = 0;
-    int bw = 0;
-    for (int w : wgs)
-        for (int64_t bt : bats) {
-            c->tune_wg = w;
-            c->tune_batch = bt;
-            double ms = 0.0;
-            if ((s = tune_time(c, r_lo, r_hi, threshold, reps, &ms))) {
-                c->tune_wg = 0;
-                c->tune_batch = 0;
-                return s;
+    int bw = 0, bo = 0;
+    for (int o : orders)
+        for (int w : wgs)
+            for (int64_t bt : bats) {
+                c->tune_wg = w;
+                c->tune_batch = bt;
+                c->tune_order = o;
+                double ms = 0.0;
+                if ((s = tune_time(c, r_lo, r_hi, threshold, reps, &ms))) {
+                    c->tune_wg = 0;
+                    c->tune_batch = 0;
+                    c->tune_order = 0;
+                    return s;
+                }
+                if (ms < best) {
+                    best = ms;
+                    bb = bt;
+                    bw = w;
+                    bo = o;
+                }
             }
-            if (ms < best) {
-                best = ms;
-                bb = bt;
-                bw = w;
-            }
-        }
     c->tune_batch = bb;
     c->tune_wg = bw;
+    c->tune_order = bo;
     if (cache_path && *cache_path) {   // temp file + rename, as the FFTW wisdom export
-        char val[96];
-        std::snprintf(val, sizeof(val), "\t%lld %d %.4f", (long long)bb, bw, best);
+        char val[128];
+        std::snprintf(val, sizeof(val), "\t%lld %d %.4f %d", (long long)bb, bw, best, bo);
         lines.push_back(key + val);
         const std::string tmp = std::string(cache_path) + "." + std::to_string((long long)getpid()) + ".tmp";
         bool ok = false;
@@ -1269,6 +1298,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         md.rowmask = (const uint64_t*)c->m_rowmask.p;
         md.spl_ld = spl_ld;
         md.ntiles = (total + 8 * md.S - 1) / (8 * md.S) * md.U;
+        md.tile_order = c->tune_order;
         const MfmaLds lay = mfma_lds_plan(nh, fs, md);
         if (lay.total > 160 * 1024)
             return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");
@@ -1279,7 +1309,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     const int64_t Bc =
         mfma ? std::min<int64_t>(nchunks, c->tune_batch > 0 ? c->tune_batch : kBatchBlocksMfma)
              : std::max<int64_t>(1, std::min<int64_t>(
-                   nchunks, (fused ? kBatchTrialsFused : kBatchTrials) / Nprof));
+                   nchunks, c->tune_batch > 0 ? c->tune_batch
+                                              : (fused ? kBatchTrialsFused : kBatchTrials) / Nprof));
     const int64_t BT = Bc * Nprof;
     fs.ldp = BT;
 
@@ -1308,7 +1339,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         const int64_t sld = mfma ? kSigmaSamplesPadded : kSigmaSamples;
         if ((s = ensure(c, c->samples, (size_t)Bc * nf * sld * sizeof(float)))) return s;
         fu.sld = sld;
-        fu.salign = mfma ? 16 : 1;
+        fu.salign = mfma ? kSigmaSampleAlign : 1;
         if ((s = ensure(c, c->rowok, (size_t)Bc * nh))) return s;
         fu.mraw = (float*)c->mraw.p;
         fu.ldm = BT;

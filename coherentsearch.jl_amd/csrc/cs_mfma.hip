@@ -469,7 +469,7 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
     FoldOut out{-INFINITY, stg_off, 0ull};
     {
         const float pH = row_broadcast<(H / R) & 3>(P[H % R]);
-        stage_samples<(NB + 15) / 16 * 16>(
+        stage_samples<(NB + 3) / 4 * 4>(
             out, active, scol, g, col, samp, stg, stg_off,
             [&](CS_LDS float* d) { put_column<H, R, 3>(d, g, P, M, pH); },
             [&](CS_GLB float* d) { put_column<H, R, 1>(d, g, P, M, pH); });
@@ -591,7 +591,7 @@ __device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ 
     }
 
     FoldOut out{-INFINITY, stg_off, 0ull};
-    stage_samples<(NB + 15) / 16 * 16>(
+    stage_samples<(NB + 3) / 4 * 4>(
         out, active, scol, g, col, samp, stg, stg_off,
         [&](CS_LDS float* d) { put_run<RD, SH, 3>(d, g, Xs); },
         [&](CS_GLB float* d) { put_run<RD, SH, 1>(d, g, Xs); });
@@ -727,7 +727,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     // share that XCD's L2.  `tile` below is a list position.
     int64_t t_first, t_end, t_step;
     {
-        const int G = gridDim.x, nx = (G % 8 == 0) ? 8 : 1;
+        const int G = gridDim.x, nx = (G % 8 == 0 && md.tile_order == 0) ? 8 : 1;
         if (nx > 1) {
             const int x = blockIdx.x % nx, l = blockIdx.x / nx;
             t_first = b.ntile * x / nx + l;

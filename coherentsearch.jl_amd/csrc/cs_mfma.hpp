@@ -10,8 +10,10 @@ namespace cs {
 // inside the allocation.
 constexpr int kSpecPad = 64;
 // Sample region per (block, fold) of the MFMA mode: at most cap / nbins columns
-// of round_up(nbins, 16) slots, <= 8192 * 32 / 20 < 16384.
-constexpr int kSigmaSamplesPadded = 16384;
+// of round_up(nbins, 4) slots (whole 16-byte stores); the largest is 273
+// columns of 32 slots at 30 bins (8736), 8960 keeps regions 256-byte aligned.
+constexpr int kSigmaSampleAlign = 4;
+constexpr int kSigmaSamplesPadded = 8960;
 
 struct MfmaDev {
     const uint4* wA_hi;       // interpolation weight tiles, [tile][64 lanes] x 8 bf16
@@ -37,6 +39,8 @@ struct MfmaDev {
     int32_t ksmax;            // largest interpolation K steps (1 or 2)
     int32_t nbp;              // profile row stride in LDS (floats)
     int32_t dft_tiles_max;    // c2r DFT tiles of all folds (hi or lo)
+    int32_t tile_order;       // 0: contiguous tile-list ranges per XCD; 1: round-robin
+    int32_t pad;
 };
 
 struct MfmaLds {

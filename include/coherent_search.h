@@ -34,11 +34,15 @@ extern "C" {
 #define CS_MODE_FUSED_VALU 2  /* fused pipeline with FP32 VALU interpolation and c2r */
 
 /* Tuning knobs of cs_set_tuning (0 restores the default). */
-#define CS_TUNE_BATCH_BLOCKS 1  /* split-BF16 mode: blocks per batch (default 16384) */
+#define CS_TUNE_BATCH_BLOCKS 1  /* blocks per batch (split-BF16 default 16384; plain and
+                                   VALU-fused: 2^21 and 2^25 trials' worth) */
 #define CS_TUNE_WORKGROUPS 2    /* split-BF16 mode: persistent workgroups (default: one per CU) */
 #define CS_TUNE_GRAPHS 3        /* 1: cs_search replays its launch sequence as a HIP graph,
                                    captured again whenever an argument changes (stage
                                    timers then report only the total); default 0 */
+#define CS_TUNE_TILE_ORDER 4    /* split-BF16 mode: 0 deals the tile list to the 8 XCDs in
+                                   contiguous ranges (neighbouring windows share an L2), 1
+                                   round-robin; default 0 */
 
 /* SearchParams (src/search.jl:131-144).  `decimations` is pointer + length
  * (the reference's Vector{Int}).  precision is the reference's profile-stage
@@ -308,14 +312,21 @@ int cs_set_mode(cs_ctx *ctx, int32_t mode);
  * the search keeps (test/test_search.jl:234-276): every knob changes only how
  * the trials are scheduled, never a result bit. */
 int cs_set_tuning(cs_ctx *ctx, int32_t knob, int64_t value);
+/* The current value of a CS_TUNE_* knob. */
+int cs_get_tuning(const cs_ctx *ctx, int32_t knob, int64_t *value);
 
-/* Autotuner of the split-BF16 search's launch shape for one search geometry:
- * times cs_search (device time, median of reps after one warm search) over
- * a few persistent-grid and batch sizes and keeps the fastest, then reports
- * it (either output may be NULL).  With cache_path, a line per geometry
- * (parameters, band, spectrum length, device) is looked up first and written
- * after measuring, by temp file + rename as the reference's FFTW wisdom
- * export does (src/wisdom.jl:46-61).  Other modes have no knobs (0, 0). */
+/* Autotuner of the search's launch shape for one search geometry (so per
+ * spectrum length): times cs_search (device time, median of reps after one
+ * warm search) over the knobs of the context's mode and keeps the fastest,
+ * then reports blocks per batch and workgroups (either output may be NULL;
+ * cs_get_tuning reads every knob).  Split-BF16 mode: persistent grid sizes,
+ * batch sizes (1, 2 or 4 batches) and the tile order; plain and VALU-fused
+ * modes: blocks per batch.  With cache_path, a line per geometry (parameters,
+ * band, spectrum length, mode, device) is looked up first and written after
+ * measuring, by temp file + rename as the reference's FFTW wisdom export does
+ * (src/wisdom.jl:46-61).  The tile shape (16 trials x 8 occurrences, the
+ * v_mfma_f32_16x16x32_bf16 output) and the block size Nprof (the reference's
+ * blocksize, which the results depend on) are not knobs. */
 int cs_autotune(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
                 const char *cache_path, int32_t reps, int64_t *best_batch,
                 int32_t *best_workgroups);

@@ -178,8 +178,8 @@ def test_search_block_default_fft(cs, O, small_spectrum):
     assert np.allclose(got, want[want > 6.0], rtol=1e-9, atol=0)
 
 
-@pytest.mark.parametrize("batch,wg", [(3, 0), (1, 37), (5, 8), (2, 1)])
-def test_fused_launch_shape_invariance(cs, small_spectrum, batch, wg):
+@pytest.mark.parametrize("batch,wg,order", [(3, 0, 0), (1, 37, 0), (5, 8, 1), (2, 1, 1), (0, 0, 1)])
+def test_fused_launch_shape_invariance(cs, small_spectrum, batch, wg, order):
     """The split-BF16 search returns bit-identical raw candidates whatever the
     batch size and persistent grid: batches of 500-trial blocks cut 512-trial
     super-tiles, whose trials outside the batch are left to the neighbouring
@@ -191,13 +191,21 @@ def test_fused_launch_shape_invariance(cs, small_spectrum, batch, wg):
     eng.set_spectrum(_ft(cs, amps, N, dt))
     eng.set_mode("fused")
     ref = eng.search_raw(5.0 * T, 30.0 * T, 6.0)
-    eng.set_tuning(batch_blocks=batch, workgroups=wg)
+    eng.set_tuning(batch_blocks=batch, workgroups=wg, tile_order=order)
     got = eng.search_raw(5.0 * T, 30.0 * T, 6.0)
     nb = eng.timings()["batches"]
-    eng.set_tuning(batch_blocks=0, workgroups=0)
+    eng.set_tuning(batch_blocks=0, workgroups=0, tile_order=0)
     assert len(ref) > 0
-    assert nb > 1
+    assert nb > 1 or batch == 0
     assert np.array_equal(ref, got)
+    # the plain mode's batches (runs of blocks) give the same list too
+    if batch:
+        eng.set_mode("plain")
+        p0 = eng.search_raw(5.0 * T, 30.0 * T, 6.0)
+        eng.set_tuning(batch_blocks=batch)
+        p1 = eng.search_raw(5.0 * T, 30.0 * T, 6.0)
+        eng.set_tuning(batch_blocks=0)
+        assert np.array_equal(p0, p1)
 
 
 def test_autotune_cache_and_invariance(cs, small_spectrum, tmp_path):
@@ -222,8 +230,15 @@ def test_autotune_cache_and_invariance(cs, small_spectrum, tmp_path):
     assert open(path).read().splitlines() == lines          # read, not re-measured
     eng.autotune(5.0 * T, 20.0 * T, 6.0, cache_path=path, reps=1)
     assert len(open(path).read().splitlines()) == 2
+    assert eng.tuning()["tile_order"] in (0, 1)
+    # the plain mode tunes its blocks per batch, with the same results
     eng.set_mode("plain")
-    assert eng.autotune(5.0 * T, 30.0 * T, 6.0) == (0, 0)
+    eng.set_tuning(batch_blocks=0, workgroups=0, tile_order=0)
+    pref = eng.search_raw(5.0 * T, 30.0 * T, 6.0)
+    b, w = eng.autotune(5.0 * T, 30.0 * T, 6.0, cache_path=path, reps=1)
+    assert w == 0 and b >= 0
+    assert np.array_equal(eng.search_raw(5.0 * T, 30.0 * T, 6.0), pref)
+    assert len(open(path).read().splitlines()) == 3
 
 
 @pytest.mark.parametrize("mode", ["fused", "plain"])
