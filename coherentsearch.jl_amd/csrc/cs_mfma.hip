@@ -25,6 +25,8 @@
 // phase bookkeeping stay exact integer arithmetic.
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "cs_device.hpp"
 #include "cs_geo.hpp"
 #include "cs_mfma.hpp"
@@ -302,6 +304,25 @@ __device__ __forceinline__ void bank_max(const float (&Xs)[G::X], int g, float (
     }
 }
 
+// max over the fold's bank of (largest window sum of width w) / sqrt(w).  The
+// bank mask is wave-uniform: read into a scalar register, its bits choose the
+// widths by scalar branches, and the usual bank (widths 1, 2, 3, 4, 6 all
+// present) takes one straight max chain with no per-width select.
+template <int NA>
+__device__ __forceinline__ float bank_bound(const float (&m)[12], uint32_t bankmask) {
+    const uint32_t bm = __builtin_amdgcn_readfirstlane(bankmask);
+    if (NA == 5 && bm == 0x1Fu) {
+        const float r0 = fmaxf(m[0], m[1] * (1.0f / sqrtf(2.0f)));
+        const float r1 = fmaxf(m[2] * (1.0f / sqrtf(3.0f)), m[3] * 0.5f);
+        return fmaxf(fmaxf(r0, r1), m[4] * (1.0f / sqrtf(6.0f)));
+    }
+    float res = -INFINITY;
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+        if (bm & (1u << a)) res = fmaxf(res, m[a] * (1.0f / sqrtf((float)bank_w(a))));
+    return res;
+}
+
 // One sampled column's share of this lane group (runs n = g*R .. g*R + R - 1
 // of P and M) into a column slot block: P[n] at n, M[n] at H + n, P[H] in
 // place of M[0]; whole CH-word chunks.
@@ -423,12 +444,13 @@ __device__ __forceinline__ void gather_rows2(const CS_LDS uint32_t* re, const CS
 }
 
 template <int NB, int WM>
-__device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fre,
-                                       const CS_LDS uint32_t* __restrict__ fim,
-                                       const CS_LDS u32x4* __restrict__ thi,
-                                       const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
-                                       uint32_t bankmask, bool active, int scol,
-                                       CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+__device__ __forceinline__ FoldOut fold_pass_body(const CS_LDS uint32_t* __restrict__ fre,
+                                                  const CS_LDS uint32_t* __restrict__ fim,
+                                                  const CS_LDS u32x4* __restrict__ thi,
+                                                  const CS_LDS u32x4* __restrict__ tlo, int k,
+                                                  int tcol, uint32_t bankmask, bool active,
+                                                  int scol, CS_GLB float* samp, CS_LDS float* stg,
+                                                  int stg_off) {
     constexpr int EXT = WM <= 6 ? 5 : WM - 1;
     using G = geo::Geo<NB, EXT>;
     constexpr int H = G::H, MT = G::MT, R = G::R, KS = G::KS, X = G::X;
@@ -496,12 +518,7 @@ __device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fr
             bank_max<G, 1, WM>(XM, g, m);
         }
     }
-    float res = -INFINITY;
-#pragma unroll
-    for (int a = 0; a < bank_count(WM <= 6 ? 6 : WM); ++a)
-        if (bankmask & (1u << a)) res = fmaxf(res, m[a] * (1.0f / sqrtf((float)bank_w(a))));
-    res = max_over_rows(res);
-    out.bound = res;
+    out.bound = max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
     return out;
 }
 
@@ -540,12 +557,13 @@ __device__ __forceinline__ void put_run(__attribute__((address_space(ASN))) floa
 // read from the next lane group; sigma samples; the bound
 // max_w max_p boxsum / sqrt(w) over the fold's bank (widths <= WM).
 template <int NB, int WM>
-__device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ fre,
-                                            const CS_LDS uint32_t* __restrict__ fim,
-                                            const CS_LDS u32x4* __restrict__ thi,
-                                            const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
-                                            uint32_t bankmask, bool active, int scol,
-                                            CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+__device__ __forceinline__ FoldOut fold_direct_body(const CS_LDS uint32_t* __restrict__ fre,
+                                                    const CS_LDS uint32_t* __restrict__ fim,
+                                                    const CS_LDS u32x4* __restrict__ thi,
+                                                    const CS_LDS u32x4* __restrict__ tlo, int k,
+                                                    int tcol, uint32_t bankmask, bool active,
+                                                    int scol, CS_GLB float* samp,
+                                                    CS_LDS float* stg, int stg_off) {
     using D = geo::DGeo<NB, WM>;
     constexpr int H = D::H, RD = D::RD, SH = D::SH, MT = D::MT, NR = D::NR, X = D::X,
                   NF = D::NF, KD = D::KD;
@@ -638,45 +656,96 @@ __device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ 
             }
         }
     }
-    float res = -INFINITY;
-#pragma unroll
-    for (int a = 0; a < bank_count(WM <= 6 ? 6 : WM); ++a)
-        if (bankmask & (1u << a)) res = fmaxf(res, m[a] * (1.0f / sqrtf((float)bank_w(a))));
-    res = max_over_rows(res);
-    out.bound = res;
+    out.bound = max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
     return out;
+}
+
+// Out-of-line copies of the fold passes, for the kernel that dispatches on
+// the profile length at run time (one call site per length and fold).
+template <int NB, int WM>
+__device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fre,
+                                          const CS_LDS uint32_t* __restrict__ fim,
+                                          const CS_LDS u32x4* __restrict__ thi,
+                                          const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
+                                          uint32_t bankmask, bool active, int scol,
+                                          CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+    return fold_pass_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
+                                  stg_off);
+}
+template <int NB, int WM>
+__device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ fre,
+                                            const CS_LDS uint32_t* __restrict__ fim,
+                                            const CS_LDS u32x4* __restrict__ thi,
+                                            const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
+                                            uint32_t bankmask, bool active, int scol,
+                                            CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+    return fold_direct_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                    stg, stg_off);
 }
 
 // The pass for a fold of NB bins: the direct layout up to 64 bins, the
 // even/odd layout above; widths up to 6 unless the fold's bank is wider.
-template <int NB, bool WIDE>
+// INL: the pass body inlined into the caller (the kernels specialised on a
+// fold set) rather than called.
+template <int NB, int WM, bool INL>
+__device__ __forceinline__ FoldOut fold_of(const CS_LDS uint32_t* fre, const CS_LDS uint32_t* fim,
+                                           const CS_LDS u32x4* thi, const CS_LDS u32x4* tlo, int k,
+                                           int tcol, uint32_t bankmask, bool active, int scol,
+                                           CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+    if constexpr (geo::c2r_direct(NB)) {
+        if constexpr (INL)
+            return fold_direct_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol,
+                                            samp, stg, stg_off);
+        else
+            return fold_direct<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                       stg, stg_off);
+    } else {
+        if constexpr (INL)
+            return fold_pass_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol,
+                                          samp, stg, stg_off);
+        else
+            return fold_pass<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                     stg, stg_off);
+    }
+}
+
+template <int NB, bool WIDE, bool INL = false>
 __device__ __forceinline__ FoldOut run_fold(bool narrow, const CS_LDS uint32_t* fre,
                                             const CS_LDS uint32_t* fim, const CS_LDS u32x4* thi,
                                             const CS_LDS u32x4* tlo, int k, int tcol,
                                             uint32_t bankmask, bool active, int scol,
                                             CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
     constexpr int WW = gate_wmax(NB);
-    if constexpr (geo::c2r_direct(NB)) {
-        if constexpr (WW <= 6 || !WIDE)
-            return fold_direct<NB, 6>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                      stg, stg_off);
-        else if (narrow)
-            return fold_direct<NB, 6>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                      stg, stg_off);
-        else
-            return fold_direct<NB, WW>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                       stg, stg_off);
-    } else {
-        if constexpr (WW <= 6 || !WIDE)
-            return fold_pass<NB, 6>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
-                                    stg_off);
-        else if (narrow)
-            return fold_pass<NB, 6>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
-                                    stg_off);
-        else
-            return fold_pass<NB, WW>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                     stg, stg_off);
-    }
+    if constexpr (WW <= 6 || !WIDE)
+        return fold_of<NB, 6, INL>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
+                                   stg_off);
+    else if (narrow)
+        return fold_of<NB, 6, INL>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
+                                   stg_off);
+    else
+        return fold_of<NB, WW, INL>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                    stg, stg_off);
+}
+
+// Fold sets with a kernel of their own (profile length of fold f, in order):
+// set 1 is decimation_set(60, 6), nharms 60 folded by 1..6.  Set 0 is the
+// kernel that takes any supported lengths.
+constexpr int kFoldSets = 2;
+constexpr int kFoldSetNB[kFoldSets][kMaxFolds] = {{0}, {120, 60, 40, 30, 24, 20}};
+constexpr int fold_set_size(int set) {
+    int n = 0;
+    for (int f = 0; f < kMaxFolds; ++f) n += kFoldSetNB[set][f] > 0 ? 1 : 0;
+    return n;
+}
+
+template <class Fn, int... I>
+__device__ __forceinline__ void static_for_impl(Fn& fn, std::integer_sequence<int, I...>) {
+    (fn(std::integral_constant<int, I>{}), ...);
+}
+// fn(std::integral_constant<int, i>) for i = 0 .. N-1, in order
+template <int N, class Fn>
+__device__ __forceinline__ void static_for(Fn fn) {
+    static_for_impl(fn, std::make_integer_sequence<int, N>{});
 }
 
 #define CS_MFMA_SIZES(X) X(16) X(20) X(24) X(30) X(32) X(40) X(60) X(64) X(120) X(128)
@@ -696,8 +765,12 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int nharms, int dft_tiles) {
 }
 
 // KS: interpolation K steps (1 or 2); WIDE: some fold's bank has widths > 6
-// (kept apart so the narrow kernel does not reserve the wide bounds' registers)
-template <int KS, bool WIDE, bool DUMP>
+// (kept apart so the narrow kernel does not reserve the wide bounds' registers);
+// SET: 0 dispatches each fold on its profile length at run time (an
+// out-of-line pass per length), a fold set above 0 has its lengths fixed at
+// compile time and every pass inlined, so no call boundary separates one
+// fold's cross-lane and LDS latency from the next fold's work.
+template <int KS, bool WIDE, bool DUMP, int SET>
 __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev tb, GridDev g,
                                                             const FoldSet* __restrict__ fsd,
                                                             BatchDev b, FusedDev fu,
@@ -976,11 +1049,11 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const int clv = slotv - b.slot0;
             const bool inb = tin && (uint32_t)clv < (uint32_t)b.nblocks;
             const int cl = inb ? clv : 0;
-#pragma unroll
-            for (int f = 0; f < kMaxFolds; ++f) {
-                bnd[f] = -INFINITY;
-                if (f >= nf) continue;
-                const int NB = fsx->f[f].nbins;
+            // fold f of NB bins (NBC: the length as a compile-time constant,
+            // or 0 to dispatch on the descriptor's length at run time)
+            auto fold_step = [&](const int f, auto nbc) {
+                constexpr int NBC = decltype(nbc)::value;
+                const int NB = NBC > 0 ? NBC : fsx->f[f].nbins;
                 const int64_t nv = nvf[f];
                 const bool act = inb && jj < nv;
                 // sampled column of this trial (sample_column for blocks that are
@@ -995,16 +1068,21 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 const uint32_t bm = fsx->bankmask[f];
                 const bool narrow = (bm >> 5) == 0;   // bank within widths 1, 2, 3, 4, 6
                 FoldOut fo{-INFINITY, cursor, 0ull};
-                switch (NB) {
+                if constexpr (NBC > 0) {
+                    fo = run_fold<NBC, WIDE, true>(narrow, lre, lim, th, tl, k, tcol, bm, act, sc,
+                                                   gsamp, lstg, cursor);
+                } else {
+                    switch (NB) {
 #define X(SZ)                                                                                 \
     case SZ:                                                                                  \
         fo = run_fold<SZ, WIDE>(narrow, lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, lstg, \
                                 cursor);                                                      \
         break;
-                    CS_MFMA_SIZES(X)
+                        CS_MFMA_SIZES(X)
 #undef X
-                    default:
-                        break;
+                        default:
+                            break;
+                    }
                 }
                 bnd[f] = act ? fo.bound : -INFINITY;
                 const int end = __builtin_amdgcn_readfirstlane(fo.stg_end);
@@ -1014,6 +1092,19 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 seg_dlo = mine ? (uint32_t)fo.dst : seg_dlo;
                 seg_dhi = mine ? (uint32_t)(fo.dst >> 32) : seg_dhi;
                 cursor = end;
+            };
+#pragma unroll
+            for (int f = 0; f < kMaxFolds; ++f) bnd[f] = -INFINITY;
+            if constexpr (SET == 0) {
+#pragma unroll
+                for (int f = 0; f < kMaxFolds; ++f)
+                    if (f < nf) fold_step(f, std::integral_constant<int, 0>{});
+            } else {
+                // the launcher checked that the search's folds are exactly this set
+                static_for<fold_set_size(SET)>([&](auto fc) {
+                    constexpr int F = decltype(fc)::value;
+                    fold_step(F, std::integral_constant<int, kFoldSetNB[SET][F]>{});
+                });
             }
             // next tile's loads go out before this tile's stores
             // bound column: [local block][trial of the block]
@@ -1145,6 +1236,16 @@ hipError_t launch_rowmask(hipStream_t st, const SpecDev& sp, const TabDev& tb, c
     return hipGetLastError();
 }
 
+// the fold set (kFoldSetNB) whose kernel runs this search's folds, 0 if none
+static int fold_set_of(const FoldSet& fs) {
+    for (int set = 1; set < kFoldSets; ++set) {
+        bool same = fs.nf == fold_set_size(set);
+        for (int f = 0; f < fs.nf && same; ++f) same = fs.f[f].nbins == kFoldSetNB[set][f];
+        if (same) return set;
+    }
+    return 0;
+}
+
 hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb, const GridDev& g,
                              const FoldSet& fs, const BatchDev& b, const FusedDev& fu,
                              const MfmaDev& md, const FoldSet* fs_dev, const MfmaDev* md_dev,
@@ -1163,11 +1264,14 @@ hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb
     };
     if (fu.dump) {
         if (md.ksmax == 2)
-            return wide ? go(k_fused_mfma<2, true, true>) : go(k_fused_mfma<2, false, true>);
-        return wide ? go(k_fused_mfma<1, true, true>) : go(k_fused_mfma<1, false, true>);
+            return wide ? go(k_fused_mfma<2, true, true, 0>) : go(k_fused_mfma<2, false, true, 0>);
+        if (!wide && fold_set_of(fs) == 1) return go(k_fused_mfma<1, false, true, 1>);
+        return wide ? go(k_fused_mfma<1, true, true, 0>) : go(k_fused_mfma<1, false, true, 0>);
     }
-    if (md.ksmax == 2) return wide ? go(k_fused_mfma<2, true, false>) : go(k_fused_mfma<2, false, false>);
-    return wide ? go(k_fused_mfma<1, true, false>) : go(k_fused_mfma<1, false, false>);
+    if (md.ksmax == 2)
+        return wide ? go(k_fused_mfma<2, true, false, 0>) : go(k_fused_mfma<2, false, false, 0>);
+    if (!wide && fold_set_of(fs) == 1) return go(k_fused_mfma<1, false, false, 1>);
+    return wide ? go(k_fused_mfma<1, true, false, 0>) : go(k_fused_mfma<1, false, false, 0>);
 }
 
 }  // namespace cs
