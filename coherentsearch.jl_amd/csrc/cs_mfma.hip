@@ -728,10 +728,12 @@ __device__ __forceinline__ FoldOut run_fold(bool narrow, const CS_LDS uint32_t* 
 }
 
 // Fold sets with a kernel of their own (profile length of fold f, in order):
-// set 1 is decimation_set(60, 6), nharms 60 folded by 1..6.  Set 0 is the
-// kernel that takes any supported lengths.
-constexpr int kFoldSets = 2;
-constexpr int kFoldSetNB[kFoldSets][kMaxFolds] = {{0}, {120, 60, 40, 30, 24, 20}};
+// set 1 is decimation_set(60, 6), nharms 60 folded by 1..6; set 2 is
+// decimation_set(32, 4); set 3 is nharms 8 unfolded.  Set 0 is the kernel that
+// takes any supported lengths.
+constexpr int kFoldSets = 4;
+constexpr int kFoldSetNB[kFoldSets][kMaxFolds] = {
+    {0}, {120, 60, 40, 30, 24, 20}, {64, 32, 20, 16}, {16}};
 constexpr int fold_set_size(int set) {
     int n = 0;
     for (int f = 0; f < kMaxFolds; ++f) n += kFoldSetNB[set][f] > 0 ? 1 : 0;
@@ -1270,7 +1272,14 @@ hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb
     }
     if (md.ksmax == 2)
         return wide ? go(k_fused_mfma<2, true, false, 0>) : go(k_fused_mfma<2, false, false, 0>);
-    if (!wide && fold_set_of(fs) == 1) return go(k_fused_mfma<1, false, false, 1>);
+    if (!wide) {
+        switch (fold_set_of(fs)) {
+            case 1: return go(k_fused_mfma<1, false, false, 1>);
+            case 2: return go(k_fused_mfma<1, false, false, 2>);
+            case 3: return go(k_fused_mfma<1, false, false, 3>);
+            default: break;
+        }
+    }
     return wide ? go(k_fused_mfma<1, true, false, 0>) : go(k_fused_mfma<1, false, false, 0>);
 }
 
