@@ -443,7 +443,31 @@ __device__ __forceinline__ void gather_rows2(const CS_LDS uint32_t* re, const CS
     }
 }
 
-template <int NB, int WM>
+// The same gathers at a compile-time fold factor KF (row stride KF rows), with
+// no clamp: a row past the fold's last harmonic meets c2r coefficients that
+// are exactly zero, so any finite word serves, and the kernels that use these
+// (fold sets checked by set_reads_in_tables) keep every such row inside the
+// stacked planes and the DFT tiles.  The row offsets become immediate LDS
+// offsets.  Stack row h*KF sits at plane row h*KF - 1.
+template <int KF>
+__device__ __forceinline__ void gather_rows_k(const CS_LDS uint32_t* pl, int h0, int tcol,
+                                              uint32_t* w) {
+    const CS_LDS uint32_t* p = pl + ((int)__umul24((unsigned)h0, KF) - 1) * kFtpLd + tcol;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) w[e] = p[e * KF * kFtpLd];
+}
+template <int KF>
+__device__ __forceinline__ void gather_rows2_k(const CS_LDS uint32_t* re, const CS_LDS uint32_t* im,
+                                               int h0, int tcol, uint32_t* wr, uint32_t* wi) {
+    const int off = ((int)__umul24((unsigned)h0, KF) - 1) * kFtpLd + tcol;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        wr[e] = re[off + e * KF * kFtpLd];
+        wi[e] = im[off + e * KF * kFtpLd];
+    }
+}
+
+template <int NB, int WM, int KF = 0>
 __device__ __forceinline__ FoldOut fold_pass_body(const CS_LDS uint32_t* __restrict__ fre,
                                                   const CS_LDS uint32_t* __restrict__ fim,
                                                   const CS_LDS u32x4* __restrict__ thi,
@@ -463,7 +487,8 @@ __device__ __forceinline__ FoldOut fold_pass_body(const CS_LDS uint32_t* __restr
 #pragma unroll
     for (int kk = 0; kk < KS; ++kk) {
         uint32_t wr[8], wi[8];
-        gather_rows2<H>(fre, fim, kk * 32 + g * 8 + 1, rstride, rbase, wr, wi);
+        if constexpr (KF > 0) gather_rows2_k<KF>(fre, fim, kk * 32 + g * 8 + 1, tcol, wr, wi);
+        else gather_rows2<H>(fre, fim, kk * 32 + g * 8 + 1, rstride, rbase, wr, wi);
         unpack8(wr, rh[kk], rl[kk]);
         unpack8(wi, ih[kk], il[kk]);
     }
@@ -556,7 +581,7 @@ __device__ __forceinline__ void put_run(__attribute__((address_space(ASN))) floa
 // bins g*RD .. g*RD + 4*MT - 1 (mod NB), the rest of its window extension
 // read from the next lane group; sigma samples; the bound
 // max_w max_p boxsum / sqrt(w) over the fold's bank (widths <= WM).
-template <int NB, int WM>
+template <int NB, int WM, int KF = 0>
 __device__ __forceinline__ FoldOut fold_direct_body(const CS_LDS uint32_t* __restrict__ fre,
                                                     const CS_LDS uint32_t* __restrict__ fim,
                                                     const CS_LDS u32x4* __restrict__ thi,
@@ -577,11 +602,13 @@ __device__ __forceinline__ FoldOut fold_direct_body(const CS_LDS uint32_t* __res
     bf16x8 bh[KD], bl[KD];
     if constexpr (KD == 1) {
         uint32_t w[8];
-        gather_rows<H>(g < 2 ? fre : fim, 8 * (g & 1) + 1, rstride, rbase, w);
+        if constexpr (KF > 0) gather_rows_k<KF>(g < 2 ? fre : fim, 8 * (g & 1) + 1, tcol, w);
+        else gather_rows<H>(g < 2 ? fre : fim, 8 * (g & 1) + 1, rstride, rbase, w);
         unpack8(w, bh[0], bl[0]);
     } else {
         uint32_t wr[8], wi[8];
-        gather_rows2<H>(fre, fim, 8 * g + 1, rstride, rbase, wr, wi);
+        if constexpr (KF > 0) gather_rows2_k<KF>(fre, fim, 8 * g + 1, tcol, wr, wi);
+        else gather_rows2<H>(fre, fim, 8 * g + 1, rstride, rbase, wr, wi);
         unpack8(wr, bh[0], bl[0]);
         unpack8(wi, bh[1], bl[1]);
     }
@@ -687,29 +714,29 @@ __device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ 
 // even/odd layout above; widths up to 6 unless the fold's bank is wider.
 // INL: the pass body inlined into the caller (the kernels specialised on a
 // fold set) rather than called.
-template <int NB, int WM, bool INL>
+template <int NB, int WM, bool INL, int KF>
 __device__ __forceinline__ FoldOut fold_of(const CS_LDS uint32_t* fre, const CS_LDS uint32_t* fim,
                                            const CS_LDS u32x4* thi, const CS_LDS u32x4* tlo, int k,
                                            int tcol, uint32_t bankmask, bool active, int scol,
                                            CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
     if constexpr (geo::c2r_direct(NB)) {
         if constexpr (INL)
-            return fold_direct_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol,
-                                            samp, stg, stg_off);
+            return fold_direct_body<NB, WM, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active,
+                                                scol, samp, stg, stg_off);
         else
             return fold_direct<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
                                        stg, stg_off);
     } else {
         if constexpr (INL)
-            return fold_pass_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol,
-                                          samp, stg, stg_off);
+            return fold_pass_body<NB, WM, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol,
+                                              samp, stg, stg_off);
         else
             return fold_pass<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
                                      stg, stg_off);
     }
 }
 
-template <int NB, bool WIDE, bool INL = false>
+template <int NB, bool WIDE, bool INL = false, int KF = 0>
 __device__ __forceinline__ FoldOut run_fold(bool narrow, const CS_LDS uint32_t* fre,
                                             const CS_LDS uint32_t* fim, const CS_LDS u32x4* thi,
                                             const CS_LDS u32x4* tlo, int k, int tcol,
@@ -717,14 +744,14 @@ __device__ __forceinline__ FoldOut run_fold(bool narrow, const CS_LDS uint32_t* 
                                             CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
     constexpr int WW = gate_wmax(NB);
     if constexpr (WW <= 6 || !WIDE)
-        return fold_of<NB, 6, INL>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
-                                   stg_off);
+        return fold_of<NB, 6, INL, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                       stg, stg_off);
     else if (narrow)
-        return fold_of<NB, 6, INL>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
-                                   stg_off);
+        return fold_of<NB, 6, INL, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                       stg, stg_off);
     else
-        return fold_of<NB, WW, INL>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                    stg, stg_off);
+        return fold_of<NB, WW, INL, 0>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
+                                       stg, stg_off);
 }
 
 // Fold sets with a kernel of their own (profile length of fold f, in order):
@@ -739,6 +766,43 @@ constexpr int fold_set_size(int set) {
     for (int f = 0; f < kMaxFolds; ++f) n += kFoldSetNB[set][f] > 0 ? 1 : 0;
     return n;
 }
+// c2r DFT tiles (hi or lo) of a fold of NB bins with a bank within widths 1..6
+// (the tables cs_plan.cpp builds for it)
+constexpr int fold_tiles(int NB) {
+    const int H = NB / 2;
+    if (geo::c2r_direct(NB)) {
+        const int RD = (NB + 3) / 4, SH = 4 * RD - NB, ext = 5;
+        const int mt = (RD + 3) / 4 > (ext + SH + 3) / 4 ? (RD + 3) / 4 : (ext + SH + 3) / 4;
+        return mt * (H <= 16 ? 1 : H <= 32 ? 2 : 4);
+    }
+    return 2 * ((H + 1 + 15) / 16) * ((H + 31) / 32);
+}
+constexpr int set_tiles(int set) {
+    int n = 0;
+    for (int f = 0; f < fold_set_size(set); ++f) n += fold_tiles(kFoldSetNB[set][f]);
+    return n;
+}
+// highest stack row index (in units of the fold factor) a fold pass gathers:
+// 32 per K step of the even/odd layout, 16 per K step of the direct one
+constexpr int fold_hmax(int NB) {
+    const int H = NB / 2;
+    if (geo::c2r_direct(NB)) return 16 * (H <= 16 ? 1 : H <= 32 ? 2 : 4);
+    return 32 * ((H + 31) / 32);
+}
+// Whether the unclamped gathers of every fold of the set (fold f has factor
+// f + 1) stay below the sample staging area, i.e. read only the two stacked
+// planes and the DFT tiles behind them, all finite words.
+constexpr bool set_reads_in_tables(int set) {
+    const int nh = kFoldSetNB[set][0] / 2;
+    const int stage = 4 * (2 * nh * kFtpLd) + set_tiles(set) * 2 * 64 * 16;
+    for (int f = 0; f < fold_set_size(set); ++f) {
+        const int last = 4 * (nh * kFtpLd + (fold_hmax(kFoldSetNB[set][f]) * (f + 1) - 1) * kFtpLd +
+                              (kThreads / 64) * 16 - 1);
+        if (last + 4 > stage) return false;
+    }
+    return true;
+}
+static_assert(set_reads_in_tables(1) && set_reads_in_tables(2), "fold set reads past the tables");
 
 template <class Fn, int... I>
 __device__ __forceinline__ void static_for_impl(Fn& fn, std::integer_sequence<int, I...>) {
@@ -1053,8 +1117,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const int cl = inb ? clv : 0;
             // fold f of NB bins (NBC: the length as a compile-time constant,
             // or 0 to dispatch on the descriptor's length at run time)
-            auto fold_step = [&](const int f, auto nbc) {
+            auto fold_step = [&](const int f, auto nbc, auto kfc) {
                 constexpr int NBC = decltype(nbc)::value;
+                constexpr int KF = decltype(kfc)::value;   // fold factor, 0: run time
                 const int NB = NBC > 0 ? NBC : fsx->f[f].nbins;
                 const int64_t nv = nvf[f];
                 const bool act = inb && jj < nv;
@@ -1071,8 +1136,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 const bool narrow = (bm >> 5) == 0;   // bank within widths 1, 2, 3, 4, 6
                 FoldOut fo{-INFINITY, cursor, 0ull};
                 if constexpr (NBC > 0) {
-                    fo = run_fold<NBC, WIDE, true>(narrow, lre, lim, th, tl, k, tcol, bm, act, sc,
-                                                   gsamp, lstg, cursor);
+                    fo = run_fold<NBC, WIDE, true, KF>(narrow, lre, lim, th, tl, k, tcol, bm, act,
+                                                       sc, gsamp, lstg, cursor);
                 } else {
                     switch (NB) {
 #define X(SZ)                                                                                 \
@@ -1100,12 +1165,14 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             if constexpr (SET == 0) {
 #pragma unroll
                 for (int f = 0; f < kMaxFolds; ++f)
-                    if (f < nf) fold_step(f, std::integral_constant<int, 0>{});
+                    if (f < nf)
+                        fold_step(f, std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{});
             } else {
                 // the launcher checked that the search's folds are exactly this set
                 static_for<fold_set_size(SET)>([&](auto fc) {
                     constexpr int F = decltype(fc)::value;
-                    fold_step(F, std::integral_constant<int, kFoldSetNB[SET][F]>{});
+                    fold_step(F, std::integral_constant<int, kFoldSetNB[SET][F]>{},
+                              std::integral_constant<int, set_reads_in_tables(SET) ? F + 1 : 0>{});
                 });
             }
             // next tile's loads go out before this tile's stores
@@ -1238,11 +1305,15 @@ hipError_t launch_rowmask(hipStream_t st, const SpecDev& sp, const TabDev& tb, c
     return hipGetLastError();
 }
 
-// the fold set (kFoldSetNB) whose kernel runs this search's folds, 0 if none
-static int fold_set_of(const FoldSet& fs) {
+// the fold set (kFoldSetNB) whose kernel runs this search's folds, 0 if none:
+// the same profile lengths with fold factors 1, 2, ..., the stacked planes of
+// the set's nharms and its DFT tile count (the LDS layout its gathers assume)
+static int fold_set_of(const FoldSet& fs, const MfmaDev& md, int nharms) {
     for (int set = 1; set < kFoldSets; ++set) {
-        bool same = fs.nf == fold_set_size(set);
-        for (int f = 0; f < fs.nf && same; ++f) same = fs.f[f].nbins == kFoldSetNB[set][f];
+        bool same = fs.nf == fold_set_size(set) && nharms == kFoldSetNB[set][0] / 2 &&
+                    md.dft_tiles_max == set_tiles(set);
+        for (int f = 0; f < fs.nf && same; ++f)
+            same = fs.f[f].nbins == kFoldSetNB[set][f] && fs.f[f].k == f + 1;
         if (same) return set;
     }
     return 0;
@@ -1267,13 +1338,13 @@ hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb
     if (fu.dump) {
         if (md.ksmax == 2)
             return wide ? go(k_fused_mfma<2, true, true, 0>) : go(k_fused_mfma<2, false, true, 0>);
-        if (!wide && fold_set_of(fs) == 1) return go(k_fused_mfma<1, false, true, 1>);
+        if (!wide && fold_set_of(fs, md, tb.nharms) == 1) return go(k_fused_mfma<1, false, true, 1>);
         return wide ? go(k_fused_mfma<1, true, true, 0>) : go(k_fused_mfma<1, false, true, 0>);
     }
     if (md.ksmax == 2)
         return wide ? go(k_fused_mfma<2, true, false, 0>) : go(k_fused_mfma<2, false, false, 0>);
     if (!wide) {
-        switch (fold_set_of(fs)) {
+        switch (fold_set_of(fs, md, tb.nharms)) {
             case 1: return go(k_fused_mfma<1, false, false, 1>);
             case 2: return go(k_fused_mfma<1, false, false, 2>);
             case 3: return go(k_fused_mfma<1, false, false, 3>);
