@@ -18,8 +18,10 @@
 // consecutive sequence entries.  The entries past the lane's own run (and the
 // ones that cross the P/M junction at bin H or the wrap at bin NB) come from
 // other lanes of the same column; this header gives, at compile time, where
-// every entry lives.  Entries no valid window reads are left as the lane's
-// own (finite) registers.
+// every entry lives.  Every entry is a real bin, so the window at every
+// position i < R is a real boxcar: a position whose start another lane owns
+// repeats that lane's window (same values, same sums), which leaves each
+// width's maximum unchanged, so the bound evaluates every position unmasked.
 //
 // Profile lengths up to 64 use the direct layout instead (DGeo below): one
 // real c2r with Re and Im stacked along K, bins in ascending order per lane.
@@ -67,7 +69,8 @@ namespace geo {
 // A window of width w <= WM at start r < RD reads X[r .. r + w - 1]; entries
 // r >= 4*MT come from the next lane group, register r - RD (group 3 reads
 // group 0's register r - RD + SH, SH = 4*RD - NB, because of the wrap).
-// Group 3 owns only RD - SH starts.
+// Group 3 owns only RD - SH starts (the sample writes); its windows at the
+// other SH positions start at bins 0 .. SH-1 again, repeats of group 0's.
 // ---------------------------------------------------------------------------
 CS_HD constexpr bool c2r_direct(int nb) { return nb <= 64; }
 
@@ -131,7 +134,7 @@ struct Geo {
 
     // ---- P run: starts at bins g*R + i <= H ----
     static constexpr bool p_start(int g, int i) { return i < R && g * R + i <= H; }
-    static constexpr bool p_need(int g, int i) { return g * R <= H && i <= H - g * R + EXT; }
+    static constexpr bool p_need(int g, int i) { return i < X; }
     static constexpr Src p_src(int g, int i) {
         return p_need(g, i) ? chain(g * R + i) : Src{g, 0, i < R ? i : 0};
     }
@@ -152,9 +155,7 @@ struct Geo {
             if (m_start(g, i)) l = i;
         return l;
     }
-    static constexpr bool m_need(int g, int i) {
-        return m_last(g) >= 0 && i >= m_first(g) && i <= m_last(g) + EXT;
-    }
+    static constexpr bool m_need(int g, int i) { return i < X; }
     static constexpr Src m_src(int g, int i) {
         return m_need(g, i) ? chain(NB - g * R - R + 1 + i) : Src{g, 1, i < R ? i : 0};
     }

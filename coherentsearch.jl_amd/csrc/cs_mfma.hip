@@ -244,42 +244,30 @@ struct Fill {
     }
 };
 
-// per-width maxima (widths 1, 2, 3, 4, 6) of the windows starting in this
-// lane's run: running pair sums
+// per-width maxima (widths 1, 2, 3, 4, 6) of the windows at every position of
+// this lane's sequence (each a real boxcar, cs_geo.hpp): running pair sums
 template <class G, int W, int I>
 struct WinMax {
-    static __device__ __forceinline__ void run(const float (&Xs)[G::X], int g, float (&m)[12]) {
+    static __device__ __forceinline__ void run(const float (&Xs)[G::X], float (&m)[12]) {
         if constexpr (I < G::R) {
-            constexpr bool a0 = G::start(W, 0, I), a1 = G::start(W, 1, I), a2 = G::start(W, 2, I),
-                           a3 = G::start(W, 3, I);
-            if constexpr (a0 || a1 || a2 || a3) {
-                float s1 = Xs[I];
-                float s2 = Xs[I] + Xs[I + 1];
-                float s3 = s2 + Xs[I + 2];
-                float s4 = s2 + (Xs[I + 2] + Xs[I + 3]);
-                float s6 = s4 + (Xs[I + 4] + Xs[I + 5]);
-                if constexpr (!(a0 && a1 && a2 && a3)) {
-                    const bool ok = g == 0 ? a0 : g == 1 ? a1 : g == 2 ? a2 : a3;
-                    s1 = ok ? s1 : -INFINITY;
-                    s2 = ok ? s2 : -INFINITY;
-                    s3 = ok ? s3 : -INFINITY;
-                    s4 = ok ? s4 : -INFINITY;
-                    s6 = ok ? s6 : -INFINITY;
-                }
-                m[0] = fmaxf(m[0], s1);
-                m[1] = fmaxf(m[1], s2);
-                m[2] = fmaxf(m[2], s3);
-                m[3] = fmaxf(m[3], s4);
-                m[4] = fmaxf(m[4], s6);
-            }
-            WinMax<G, W, I + 1>::run(Xs, g, m);
+            const float s1 = Xs[I];
+            const float s2 = Xs[I] + Xs[I + 1];
+            const float s3 = s2 + Xs[I + 2];
+            const float s4 = s2 + (Xs[I + 2] + Xs[I + 3]);
+            const float s6 = s4 + (Xs[I + 4] + Xs[I + 5]);
+            m[0] = fmaxf(m[0], s1);
+            m[1] = fmaxf(m[1], s2);
+            m[2] = fmaxf(m[2], s3);
+            m[3] = fmaxf(m[3], s4);
+            m[4] = fmaxf(m[4], s6);
+            WinMax<G, W, I + 1>::run(Xs, m);
         }
     }
 };
 
 // per-width maxima over the bank widths <= WM: prefix-sum differences
 template <class G, int W, int WM>
-__device__ __forceinline__ void bank_max(const float (&Xs)[G::X], int g, float (&m)[12]) {
+__device__ __forceinline__ void bank_max(const float (&Xs)[G::X], float (&m)[12]) {
     constexpr int R = G::R, X = G::X;
     float ps[X + 1];
     ps[0] = 0.0f;
@@ -289,18 +277,7 @@ __device__ __forceinline__ void bank_max(const float (&Xs)[G::X], int g, float (
     for (int a = 0; a < bank_count(WM); ++a) {
         const int w = bank_w(a);
 #pragma unroll
-        for (int i = 0; i < R; ++i) {
-            const bool a0 = G::start(W, 0, i), a1 = G::start(W, 1, i), a2 = G::start(W, 2, i),
-                       a3 = G::start(W, 3, i);
-            if (a0 || a1 || a2 || a3) {
-                float d = ps[i + w] - ps[i];
-                if (!(a0 && a1 && a2 && a3)) {
-                    const bool ok = g == 0 ? a0 : g == 1 ? a1 : g == 2 ? a2 : a3;
-                    d = ok ? d : -INFINITY;
-                }
-                m[a] = fmaxf(m[a], d);
-            }
-        }
+        for (int i = 0; i < R; ++i) m[a] = fmaxf(m[a], ps[i + w] - ps[i]);
     }
 }
 
@@ -529,18 +506,18 @@ __device__ __forceinline__ FoldOut fold_pass_body(const CS_LDS uint32_t* __restr
         float XP[X], XM[X];
         Fill<G, 0, 0>::run(XP, P, M, g, col);
         Fill<G, 1, 0>::run(XM, P, M, g, col);
-        WinMax<G, 0, 0>::run(XP, g, m);
-        WinMax<G, 1, 0>::run(XM, g, m);
+        WinMax<G, 0, 0>::run(XP, m);
+        WinMax<G, 1, 0>::run(XM, m);
     } else {
         {
             float XP[X];
             Fill<G, 0, 0>::run(XP, P, M, g, col);
-            bank_max<G, 0, WM>(XP, g, m);
+            bank_max<G, 0, WM>(XP, m);
         }
         {
             float XM[X];
             Fill<G, 1, 0>::run(XM, P, M, g, col);
-            bank_max<G, 1, WM>(XM, g, m);
+            bank_max<G, 1, WM>(XM, m);
         }
     }
     out.bound = max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
@@ -641,26 +618,19 @@ __device__ __forceinline__ FoldOut fold_direct_body(const CS_LDS uint32_t* __res
         [&](CS_LDS float* d) { put_run<RD, SH, 3>(d, g, Xs); },
         [&](CS_GLB float* d) { put_run<RD, SH, 1>(d, g, Xs); });
 
-    // boxcar bound over the starts this lane group owns
+    // boxcar bound over the windows at the lane group's RD positions (group 3's
+    // last SH repeat group 0's first windows: no mask needed)
     float m[12];
 #pragma unroll
     for (int a = 0; a < 12; ++a) m[a] = -INFINITY;
-    const bool last = g == 3;
     if constexpr (WM <= 6) {
 #pragma unroll
         for (int r = 0; r < RD; ++r) {
-            float s1 = Xs[r];
-            float s2 = Xs[r] + Xs[r + 1];
-            float s3 = s2 + Xs[r + 2];
-            float s4 = s2 + (Xs[r + 2] + Xs[r + 3]);
-            float s6 = s4 + (Xs[r + 4] + Xs[r + 5]);
-            if (SH != 0 && r >= RD - SH) {
-                s1 = last ? -INFINITY : s1;
-                s2 = last ? -INFINITY : s2;
-                s3 = last ? -INFINITY : s3;
-                s4 = last ? -INFINITY : s4;
-                s6 = last ? -INFINITY : s6;
-            }
+            const float s1 = Xs[r];
+            const float s2 = Xs[r] + Xs[r + 1];
+            const float s3 = s2 + Xs[r + 2];
+            const float s4 = s2 + (Xs[r + 2] + Xs[r + 3]);
+            const float s6 = s4 + (Xs[r + 4] + Xs[r + 5]);
             m[0] = fmaxf(m[0], s1);
             m[1] = fmaxf(m[1], s2);
             m[2] = fmaxf(m[2], s3);
@@ -676,11 +646,7 @@ __device__ __forceinline__ FoldOut fold_direct_body(const CS_LDS uint32_t* __res
         for (int a = 0; a < bank_count(WM); ++a) {
             const int w = bank_w(a);
 #pragma unroll
-            for (int r = 0; r < RD; ++r) {
-                float d = ps[r + w] - ps[r];
-                if (SH != 0 && r >= RD - SH) d = last ? -INFINITY : d;
-                m[a] = fmaxf(m[a], d);
-            }
+            for (int r = 0; r < RD; ++r) m[a] = fmaxf(m[a], ps[r + w] - ps[r]);
         }
     }
     out.bound = max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));

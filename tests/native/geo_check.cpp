@@ -55,10 +55,11 @@ static int check_nb(unsigned seed) {
                     Src s = G::src(w, g, i);
                     S[i] = s.kind ? M[s.lane][s.reg] : P[s.lane][s.reg];
                 }
+                // every position's window is a real boxcar (a position another
+                // lane owns repeats that lane's window); ownership is exact
                 for (int i = 0; i < R; ++i) {
-                    if (!G::start(w, g, i)) continue;
-                    int bin = w == 0 ? g * R + i : NB - (g * R + R - 1 - i);
-                    starts[bin] += 1;
+                    int bin = (w == 0 ? g * R + i : NB - (g * R + R - 1 - i)) % NB;
+                    if (G::start(w, g, i)) starts[bin] += 1;
                     for (int a = 0; a < nwid; ++a) {
                         if (kW[a] > NB) continue;
                         long s = 0, ref = 0;
@@ -127,11 +128,14 @@ static int check_direct(unsigned seed) {
         std::vector<int> starts(NB, 0), slots(NB, 0);
         for (int g = 0; g < 4; ++g)
             for (int r = 0; r < RD; ++r) {
-                if (!D::owns(g, r)) continue;
-                const int bin = g * RD + r;
-                starts[bin] += 1;
-                slots[bin] += 1;
-                if (reg[g][r] != x[bin]) ++bad;   // sample slot value
+                // every position's window (group 3's last SH repeat group 0's
+                // first); starts and sample slots only where the group owns it
+                const int bin = (g * RD + r) % NB;
+                if (D::owns(g, r)) {
+                    starts[bin] += 1;
+                    slots[bin] += 1;
+                    if (reg[g][r] != x[bin]) ++bad;   // sample slot value
+                }
                 for (int a = 0; a < nwid; ++a) {
                     long s = 0, ref = 0;
                     for (int j = 0; j < kW[a]; ++j) {
