@@ -85,11 +85,13 @@ __device__ inline double2 interp_slow(const HarmDev& d, const TabDev& tb, const 
 // FP32 weights: an m-term FP32 FMA chain, then the complex FP32 product with
 // conj(A) evaluated without contraction, exactly as the reference stores
 // `gA[k, g] * complex(ar[k], ai[k])`.
+// interp_with takes harmonic h's plan d from the caller (who may hold it in
+// LDS); interp_one reads it from the plan table.
 template <int MT>
-__device__ __forceinline__ double2 interp_one(const SpecDev& sp, const TabDev& tb, int h,
-                                              int64_t t, int64_t first, int64_t last,
-                                              bool guard_known, bool guard_ok) {
-    const HarmDev d = tb.harm[h - 1];
+__device__ __forceinline__ double2 interp_with(const HarmDev& d, const SpecDev& sp,
+                                               const TabDev& tb, int h, int64_t t,
+                                               int64_t first, int64_t last, bool guard_known,
+                                               bool guard_ok) {
     if (d.P == 0) return interp_slow(d, tb, sp, h, t, first, last);
     bool ok = guard_known ? guard_ok : row_guard(d, tb, sp, first, last);
     if (!ok) return make_double2(0.0, 0.0);
@@ -129,6 +131,13 @@ __device__ __forceinline__ double2 interp_one(const SpecDev& sp, const TabDev& t
         double2 A = tb.A64[d.tab_off + p];
         return make_double2(A.x * ar - A.y * ai, A.x * ai + A.y * ar);
     }
+}
+template <int MT>
+__device__ __forceinline__ double2 interp_one(const SpecDev& sp, const TabDev& tb, int h,
+                                              int64_t t, int64_t first, int64_t last,
+                                              bool guard_known, bool guard_ok) {
+    const HarmDev d = tb.harm[h - 1];
+    return interp_with<MT>(d, sp, tb, h, t, first, last, guard_known, guard_ok);
 }
 
 // Sample slot column of block trial j (_block_sigma's whole-column subsample,

@@ -648,7 +648,7 @@ hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
 // error).  The sigma that scales a reported metric is recomputed here from the
 // same whole-column subsample: each sampled column's stack with the FP32
 // weights (as the reference fills ftprofs), an FP64 real DFT, |x|, then the
-// exact median by a bitonic sort of up to 8192 doubles in LDS.
+// exact median by radix selection over the sample keys held in registers.
 // ---------------------------------------------------------------------------
 __global__ void k_reset_u32(uint32_t* p, int n);
 
@@ -757,6 +757,8 @@ __global__ __launch_bounds__(kRefineThreads) void k_sigma_refine(
     __shared__ uint32_t hist[2 * kRefineThreads];
     __shared__ uint32_t shv[32];
     __shared__ uint64_t red[kRefineThreads / 64];
+    __shared__ HarmDev hd[kMaxNB / 2];   // the pair's harmonics l*k, l = 1..Hk
+    __shared__ uint8_t hok[kMaxNB / 2];  // their row guards on the block
     const int tid = threadIdx.x;
     const int nf = fs.nf;
     const int npair = b.nblocks * nf;
@@ -785,6 +787,13 @@ __global__ __launch_bounds__(kRefineThreads) void k_sigma_refine(
             ns = ncol * NB < cap ? ncol * NB : cap;
         }
         for (int e = tid; e < NB; e += kRefineThreads) tw[e] = fs.twid[f][e];
+        // plans and row guards of the pair's harmonics, once for all its columns
+        for (int l = tid; l < Hk; l += kRefineThreads) {
+            const HarmDev d = tb.harm[(l + 1) * k - 1];
+            hd[l] = d;
+            hok[l] = d.P == 0 || row_guard(d, tb, sp, first, last) ? 1 : 0;
+        }
+        __syncthreads();
         // rounds of CR columns: every (column, harmonic) stack entry, then every
         // (column, bin pair) of the FP64 real DFT, spread over the workgroup
         const int W = Hk + 1;
@@ -796,10 +805,11 @@ __global__ __launch_bounds__(kRefineThreads) void k_sigma_refine(
                 const int64_t cc = c0 + cr;
                 const int64_t t = first + (all ? cc : (cc * nv) / ncol);
                 double2 x = make_double2(0.0, 0.0);
-                if (l >= 1) {
-                    if (tb.m == 16 && !tb.weights_f64) x = interp_one<16>(sp, tb, l * k, t, first, last, false, false);
-                    else if (tb.m == 32 && !tb.weights_f64) x = interp_one<32>(sp, tb, l * k, t, first, last, false, false);
-                    else x = interp_one<0>(sp, tb, l * k, t, first, last, false, false);
+                if (l >= 1 && hok[l - 1]) {
+                    const HarmDev& d = hd[l - 1];
+                    if (tb.m == 16 && !tb.weights_f64) x = interp_with<16>(d, sp, tb, l * k, t, first, last, true, true);
+                    else if (tb.m == 32 && !tb.weights_f64) x = interp_with<32>(d, sp, tb, l * k, t, first, last, true, true);
+                    else x = interp_with<0>(d, sp, tb, l * k, t, first, last, true, true);
                 }
                 X[task] = x;
             }
