@@ -220,6 +220,36 @@ def run_c3(args, cs, nfiles):
         same = same and bool(np.array_equal(got, want))
         seq.append(len(want))
     seq_wall = time.perf_counter() - t1
+    # the autotuner per length class: cs_autotune on one file of each length
+    # into a cache, then the sweep with the cached shape of every class
+    tuned = None
+    if args.autotune:
+        import tempfile
+        cache = os.path.join(tempfile.mkdtemp(prefix="cs_tune_"), "tuning.txt")
+        per_len = {}
+        t1 = time.perf_counter()
+        for ft, r_lo, r_hi in jobs:
+            if ft.N in per_len:
+                continue
+            eng.set_spectrum(ft)
+            eng.autotune(r_lo, r_hi, 8.0, cache_path=cache, reps=2)
+            per_len[ft.N] = eng.tuning()
+            eng.set_tuning(batch_blocks=0, workgroups=0, tile_order=0)
+        tune_s = time.perf_counter() - t1
+        eng.set_tuning_cache(cache)
+        twalls = []
+        for i in range(args.steps):
+            t1 = time.perf_counter()
+            tmulti = eng.search_files(jobs, 8.0)
+            twalls.append(time.perf_counter() - t1)
+        tdev = eng.timings()["total"]
+        eng.set_tuning_cache(None)
+        twall = sum(twalls) / len(twalls)
+        tuned = {"per_length": {str(int(n)): v for n, v in sorted(per_len.items())},
+                 "tuning_s": round(tune_s, 2), "ms_per_step": round(1e3 * twall, 3),
+                 "trials_per_s": round(total / twall, 1), "device_ms": round(tdev, 3),
+                 "results_equal": all(bool(np.array_equal(a, b)) for a, b in zip(tmulti, multi))}
+        log(f"[bench] c3 autotuned per length: {tuned['ms_per_step']} ms ({tuned['per_length']})")
     found = 0
     for c, ft, g, raw in zip(cfgs, fts, grids, multi):
         f0 = c["pulsars"][0].freq
@@ -248,6 +278,7 @@ def run_c3(args, cs, nfiles):
         "sequential_upload_then_search_ms": round(1e3 * seq_wall, 3),
         "overlap_gain": round(seq_wall / wall, 3),
         "per_file_equal_to_single_runs": same,
+        "autotuned": tuned,
         "raw_candidates": int(sum(len(r) for r in multi)),
         "pulsars_detected": f"{found}/{in_band} in band",
     }

@@ -141,6 +141,7 @@ def lib():
     L.cs_get_tuning.argtypes = [vp, C.c_int32, i64p]
     L.cs_autotune.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_char_p, C.c_int32,
                               i64p, C.POINTER(C.c_int32)]
+    L.cs_set_tuning_cache.argtypes = [vp, C.c_char_p]
     L.cs_last_timings.argtypes = [vp, dp]
     L.cs_last_sigmas.argtypes = [vp, dp, C.c_int64, i64p]
     L.cs_probe_split_bf16.argtypes = [vp, C.c_double, C.c_double, C.c_int64, C.c_int64, fp, fp]

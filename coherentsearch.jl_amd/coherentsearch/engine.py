@@ -134,6 +134,13 @@ class Engine:
                                          C.byref(b), C.byref(w)), "cs_autotune")
         return int(b.value), int(w.value)
 
+    def set_tuning_cache(self, path: Optional[str]):
+        """The tuning cache search_files reads: each length class of a sweep
+        runs with the launch shape autotune stored for its geometry there (no
+        timing during the sweep); None clears it (cs_set_tuning_cache)."""
+        self._chk(_lib.lib().cs_set_tuning_cache(self._h, path.encode() if path else None),
+                  "cs_set_tuning_cache")
+
     # -- entry points ------------------------------------------------------------
     def search_raw(self, r_lo: float, r_hi: float, threshold: float) -> np.ndarray:
         cap = 1 << 16

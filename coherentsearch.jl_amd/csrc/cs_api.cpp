@@ -106,6 +106,7 @@ struct cs_ctx {
     int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
     int tune_order = 0;        // CS_TUNE_TILE_ORDER (0: contiguous tile ranges per XCD)
     bool use_graph = false;    // CS_TUNE_GRAPHS: replay a captured launch sequence
+    std::string tune_cache;    // cs_set_tuning_cache: cs_autotune's cache for cs_search_files
     hipGraph_t g_graph = nullptr;
     hipGraphExec_t g_exec = nullptr;
     std::vector<uint8_t> g_key;   // every launch argument the graph was captured with
@@ -471,6 +472,40 @@ static std::string tune_key(cs_ctx* c, double r_lo, double r_hi) {
     return k;
 }
 
+// The launch shape of a cache line "key<TAB>batch workgroups ms order" whose
+// key is `key` (false for another key or a malformed entry).
+static bool tune_parse(const std::string& l, const std::string& key, int64_t& b, int& w, int& o) {
+    const size_t tab = l.find('\t');
+    if (tab == std::string::npos || l.compare(0, tab, key) != 0 || tab != key.size()) return false;
+    long long bb = 0, ww = 0, oo = 0;
+    double msv = 0.0;
+    if (std::sscanf(l.c_str() + tab + 1, "%lld %lld %lf %lld", &bb, &ww, &msv, &oo) < 2 || bb < 0 ||
+        ww < 0 || ww > 65536 || (oo != 0 && oo != 1))
+        return false;
+    b = bb;
+    w = (int)ww;
+    o = (int)oo;
+    return true;
+}
+
+// The cached launch shape of the context's geometry over band [r_lo, r_hi]
+// in the cache file at `path`, if it holds one.
+static bool tune_lookup(cs_ctx* c, const std::string& path, double r_lo, double r_hi, int64_t& b,
+                        int& w, int& o) {
+    FILE* fh = std::fopen(path.c_str(), "r");
+    if (!fh) return false;
+    const std::string key = tune_key(c, r_lo, r_hi);
+    char ln[1024];
+    bool found = false;
+    while (!found && std::fgets(ln, sizeof(ln), fh)) {
+        std::string l(ln);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+        found = tune_parse(l, key, b, w, o);
+    }
+    std::fclose(fh);
+    return found;
+}
+
 static int tune_time(cs_ctx* c, double r_lo, double r_hi, double thr, int reps, double* ms) {
     std::vector<cs_cand> buf(1 << 12);
     std::vector<double> t;
@@ -486,6 +521,12 @@ static int tune_time(cs_ctx* c, double r_lo, double r_hi, double thr, int reps, 
     }
     std::sort(t.begin(), t.end());
     *ms = t[t.size() / 2];
+    return CS_OK;
+}
+
+int cs_set_tuning_cache(cs_ctx* c, const char* path) {
+    if (!c) return CS_EINVAL;
+    c->tune_cache = path ? path : "";
     return CS_OK;
 }
 
@@ -510,15 +551,14 @@ int cs_autotune(cs_ctx* c, double r_lo, double r_hi, double threshold, const cha
                 while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
                 if (l.empty()) continue;
                 const size_t tab = l.find('\t');
-                if (tab != std::string::npos && l.compare(0, tab, key) == 0) {
-                    long long b = 0, w = 0, o = 0;
-                    double msv = 0.0;
-                    if (std::sscanf(l.c_str() + tab + 1, "%lld %lld %lf %lld", &b, &w, &msv, &o) >= 2 &&
-                        b >= 0 && w >= 0 && w <= 65536 && (o == 0 || o == 1)) {
+                if (tab != std::string::npos && l.compare(0, tab, key) == 0 && tab == key.size()) {
+                    int64_t b = 0;
+                    int w = 0, o = 0;
+                    if (tune_parse(l, key, b, w, o)) {
                         std::fclose(fh);
                         c->tune_batch = b;
-                        c->tune_wg = (int)w;
-                        c->tune_order = (int)o;
+                        c->tune_wg = w;
+                        c->tune_order = o;
                         return report();
                     }
                     continue;   // malformed entry: measured again and replaced
@@ -2111,6 +2151,19 @@ static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, do
         ~GraphOff() { c->use_graph = was; }
     } goff{c, c->use_graph};
     c->use_graph = false;
+    // the caller's launch shape, restored after the sweep; a length class with
+    // a line in the tuning cache runs with the shape cs_autotune stored for it
+    struct TuneKeep {
+        cs_ctx* c;
+        int64_t b;
+        int w, o;
+        void restore() const {
+            c->tune_batch = b;
+            c->tune_wg = w;
+            c->tune_order = o;
+        }
+        ~TuneKeep() { restore(); }
+    } tkeep{c, c->tune_batch, c->tune_wg, c->tune_order};
     if (nfiles <= 0) return CS_OK;
     if (cap < 0 || (cap > 0 && !out)) return c->fail(CS_EINVAL, "bad candidate buffer");
     int64_t maxn = 0, namps_all = 0;
@@ -2223,6 +2276,18 @@ static int search_files_impl(cs_ctx* c, int32_t nfiles, const cs_file* files, do
             c->N = f.N;
             c->T = f.T;
             c->spl_valid = false;
+            if (!c->tune_cache.empty() &&
+                (k == 0 || files[order[k - 1]].N != f.N || files[order[k - 1]].r_lo != f.r_lo ||
+                 files[order[k - 1]].r_hi != f.r_hi)) {
+                int64_t b = 0;
+                int w = 0, o = 0;
+                tkeep.restore();
+                if (tune_lookup(c, c->tune_cache, f.r_lo, f.r_hi, b, w, o)) {
+                    c->tune_batch = b;
+                    c->tune_wg = w;
+                    c->tune_order = o;
+                }
+            }
             sk.file = fi;
             int64_t n = 0;
             const int st = search_impl(c, f.r_lo, f.r_hi, threshold, nullptr, 0, nullptr, 0, &n,

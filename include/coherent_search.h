@@ -331,6 +331,16 @@ int cs_autotune(cs_ctx *ctx, double r_lo, double r_hi, double threshold,
                 const char *cache_path, int32_t reps, int64_t *best_batch,
                 int32_t *best_workgroups);
 
+/* The tuning cache cs_search_files reads (NULL or "" for none): each length
+ * class of a sweep (files of one spectrum length and band) runs with the
+ * launch shape cs_autotune stored for its geometry in that file, looked up
+ * without timing; a class with no line runs with the context's knobs, which
+ * the sweep leaves as it found them.  Like cs_autotune, it has no counterpart
+ * in the reference beyond the FFTW wisdom file it mirrors (src/wisdom.jl:18-61,
+ * the plan cache one SearchCache carries across the CLI's files,
+ * src/cli.jl:231-289). */
+int cs_set_tuning_cache(cs_ctx *ctx, const char *path);
+
 /* Per-stage device time of the last cs_search, milliseconds, HIP events on the
  * context's stream: [0] interp, [1] c2r, [2] sigma, [3] gate, [4] rescan (with
  * the FP64 sigma of the survivors' blocks under precision :f64), [5] fused,

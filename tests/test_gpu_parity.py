@@ -276,6 +276,47 @@ def test_search_files_equals_single_file_runs(cs, mode):
     assert any(abs(c.freq - 21.3) < 2.0 / fts[1].T for c in lists[1])
 
 
+def test_search_files_with_tuning_cache(cs, tmp_path):
+    """A sweep whose length classes take their launch shapes from cs_autotune's
+    cache (cs_set_tuning_cache) returns exactly the raw candidates of the sweep
+    with default shapes; a cache holding non-default shapes for every class
+    (one block per batch, a quarter of the grid, round-robin tiles) does too;
+    and the context's own knobs are as before the sweep."""
+    _say("search_files with a tuning cache")
+    from coherentsearch.synth import Pulsar, make_spectrum
+    dt = 64e-6
+    specs = [(1 << 17, 51), (1 << 18, 52), (1 << 17, 53), (1 << 18, 54)]
+    fts = [_ft(cs, make_spectrum(N, dt, seed=sd,
+                                 pulsars=[Pulsar(17.9, 0.05, 18.0)] if sd == 52 else []), N, dt)
+           for N, sd in specs]
+    p = cs.SearchParams(nharms=32, m=16, decimations=cs.decimation_set(32, 4))
+    eng = cs.Engine(p, 1024)
+    eng.set_mode("fused")
+    jobs = [(ft, 5.0 * ft.T, 40.0 * ft.T) for ft in fts]
+    want = eng.search_files(jobs, 6.5)
+    assert len(want[1]) > 0
+    cache = str(tmp_path / "tuning.txt")
+    for ft, r_lo, r_hi in jobs[:2]:           # one file of each length
+        eng.set_spectrum(ft)
+        eng.autotune(r_lo, r_hi, 6.5, cache_path=cache, reps=1)
+    lines = open(cache).read().splitlines()
+    assert len(lines) == 2
+    eng.set_tuning(batch_blocks=0, workgroups=0, tile_order=0)
+    eng.set_tuning_cache(cache)
+    got = eng.search_files(jobs, 6.5)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    # every class forced to an unusual shape through the cache
+    odd = [l.split("\t")[0] + "\t1 64 1.0 1" for l in lines]
+    open(cache, "w").write("\n".join(odd) + "\n")
+    eng.set_tuning(batch_blocks=3, workgroups=0, tile_order=0)
+    before = eng.tuning()
+    got = eng.search_files(jobs, 6.5)
+    assert all(np.array_equal(a, b) for a, b in zip(got, want))
+    assert eng.tuning() == before
+    eng.set_tuning_cache(None)
+    eng.close()
+
+
 def _compare_search(cands, want, threshold, tol=1e-4):
     """(t, k) sets equal apart from trials within tol of threshold; metrics agree."""
     g = {(int(c["t"]), int(c["k"])): c for c in cands}
