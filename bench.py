@@ -538,6 +538,84 @@ def run_c4(args, cs):
     print(json.dumps(line), flush=True)
 
 
+def scheduled_triage(args, cs, eng, params, cfg, r_lo, r_hi, nchunks, raw_full, t_full):
+    """The 2^27-point workload with MLP-ordered block scheduling: the scorer
+    (pyramid features) trained on the device on `--triage-files` 2^24-sample
+    training files (seeds 4000 + i, labels from their full plain search), then
+    the band's blocks visited most-likely first under wall-clock budgets of
+    10, 25 and 50 % of the full search (scoring time included), against the
+    same search mode given the time the schedule took on the blocks in band
+    order: recall of the full search's raw candidates and the injected pulsars
+    each finds (every candidate is the coherent search's own in its block; the
+    full search stays the judge)."""
+    import numpy as np
+
+    from coherentsearch.synth import config_c4_file, make_config_spectra
+    from coherentsearch.triage import BlockTriage, block_labels, scheduled_search, triage_order
+
+    t0 = time.perf_counter()
+    lab = cs.Engine(params, 2048, device=0)
+    lab.set_mode("plain")
+    feats, labels = [], []
+    n = args.triage_files
+    for c0 in range(0, n, 16):
+        cfgs = [config_c4_file(i) for i in range(c0, min(n, c0 + 16))]
+        for a, c in zip(make_config_spectra(cfgs, workers=16), cfgs):
+            ft = cs.FFTFile.from_arrays(a, c["N"], c["dt"])
+            lab.set_spectrum(ft)
+            g = cs.trial_grid(ft.T, params, 0.1, 125.0, 100, 2048)
+            labels.append(block_labels(lab.search_raw(g[0], g[1], 8.0), g[5]))
+            feats.append(lab.triage_features(g[0], g[1], kind=0))
+    y = np.concatenate(labels)
+    model = BlockTriage(seed=0, kind=0)
+    hist = model.fit(lab, np.concatenate(feats), y, epochs=args.c4_epochs, lr=3e-3)
+    lab.close()
+    train_s = time.perf_counter() - t0
+    log(f"[bench] triage: trained on {n} files ({len(y)} blocks) in {train_s:.0f} s")
+    T = cfg["N"] * cfg["dt"]
+    lodr = params.hidr / params.nharms
+
+    def pulsars(raw):
+        rs = np.array([cs.candidate_r(r_lo, lodr, 2048, int(x["t"]), int(x["k"])) for x in raw])
+        return sorted(p.freq for p in cfg["pulsars"]
+                      if len(rs) and np.min(np.abs(rs / T - p.freq)) < 3.0 / T)
+
+    fullset = {(int(x["t"]), int(x["k"])) for x in raw_full}
+    judge = pulsars(raw_full)
+    model.features(eng, r_lo, r_hi)          # warm
+    rows = []
+
+    def summary(raw, secs, done):
+        got = {(int(x["t"]), int(x["k"])) for x in raw}
+        return {"seconds": round(secs, 4), "blocks_fraction": round(done / nchunks, 4),
+                "candidate_recall": round(len(got & fullset) / max(1, len(fullset)), 4),
+                "pulsars": pulsars(raw)}
+
+    for b in (0.1, 0.25, 0.5):
+        budget = b * t_full
+        # scheduled: scoring, then rounds most-likely first within what is left
+        t1 = time.perf_counter()
+        order = triage_order(eng, model, r_lo, r_hi)
+        used = time.perf_counter() - t1
+        raw, done, ts = scheduled_search(eng, r_lo, r_hi, 8.0, order,
+                                         budget_s=max(0.0, budget - used))
+        mlp = summary(raw, used + ts, done)
+        # the same search mode given the time the schedule took, blocks in band
+        # order (as many as the full search covers in that time)
+        kb = max(1, min(nchunks, int(round(nchunks * (used + ts) / t_full))))
+        band = np.zeros(nchunks, np.uint8)
+        band[:kb] = 1
+        t1 = time.perf_counter()
+        braw = eng.search_blocks(r_lo, r_hi, 8.0, band)
+        rows.append({"budget_fraction_of_full_search": b, "mlp": mlp,
+                     "band_order_same_time": summary(braw, time.perf_counter() - t1, kb)})
+        log(f"[bench] triage budget {b:.0%}: {rows[-1]}")
+    return {"train_files": n, "train_blocks": int(len(y)), "train_positive_blocks": int(y.sum()),
+            "train_s": round(train_s, 1), "loss": [round(hist[0], 4), round(hist[-1], 4)],
+            "full_search_s": round(t_full, 4), "full_raw_candidates": len(fullset),
+            "pulsars_found_by_full_search": judge, "budgets": rows}
+
+
 def run_cands(args, cs):
     """Post-detection throughput: measure_ducy (candidate_profile at m = 64 plus
     the best boxcar width) of 4096 candidates, fundamentals uniform over the
@@ -691,6 +769,9 @@ def main():
     ap.add_argument("--c4-held", type=int, default=16, help="c4: held-out files")
     ap.add_argument("--c4-epochs", type=int, default=400)
     ap.add_argument("--c3-files", type=int, default=64)
+    ap.add_argument("--triage-files", type=int, default=0,
+                    help="c2/c5: train the block scorer on this many 2^24-sample files and "
+                         "report MLP-ordered scheduling under time budgets (0 = skip)")
     ap.add_argument("--autotune", action="store_true",
                     help="run cs_autotune before timing (the default launch shape is the one "
                          "the committed rocprof summaries measure)")
@@ -838,6 +919,11 @@ def main():
                                    if both else None)}}
         eng.set_mode(args.mode)
 
+    sched = None
+    if args.triage_files > 0:
+        sched = scheduled_triage(args, cs, eng, params, cfg, r_lo, r_hi, nchunks, raw,
+                                 wall / args.steps)
+
     cpu = parity = None
     if not args.no_cpu_baseline:
         log("[bench] CPU baseline (oracle, OpenMP)")
@@ -872,6 +958,7 @@ def main():
         "speedup_vs_plain_fp32_gpu": (round(value / plain["value"], 3) if plain else None),
         "cpu_baseline": cpu,
         "oracle_parity": parity,
+        "mlp_scheduled": sched,
     }
     print(json.dumps(line), flush=True)
 

@@ -9,11 +9,13 @@ each harmonic of an integer fundamental can fall in).  Scorer: an MLP
 32 -> 64 (ReLU) -> 1 logit whose forward and backward passes run on the device
 in bf16 with FP32 accumulation; training runs every Adam step on the device as
 well (cs_mlp_train).  Labels: whether a block holds a raw candidate of the full
-plain search.
+plain search.  scheduled_search visits blocks most-likely first under a
+wall-clock budget (every block when it allows, which is the full search).
 """
 from __future__ import annotations
 
 import math
+import time
 from typing import List, Optional
 
 import numpy as np
@@ -107,6 +109,52 @@ class BlockTriage:
         t.params = d["params"].astype(np.float32)
         t.norm = d["norm"].astype(np.float32)
         return t
+
+
+def scheduled_search(engine, r_lo: float, r_hi: float, threshold: float, order: np.ndarray,
+                     budget_s: Optional[float] = None, first_fraction: float = 0.01,
+                     growth: float = 2.0):
+    """Visit the band's blocks in the given order (block indices, most likely
+    first) in rounds of first_fraction, then growth times more, ... of the
+    blocks, until the wall-clock budget (seconds, from the call; None: every
+    block) is spent.  The first round always runs; after it, each round is cut
+    to the blocks the remaining budget affords at the rate the rounds so far
+    took (per-call overheads included), and none starts when not one block
+    fits.  Each round is the coherent search of its blocks (cs_search_blocks),
+    so every candidate is the full search's own in its block, and visiting
+    every block returns the full search's raw candidates.  Returns (raw
+    candidates sorted by (t, k), blocks searched, seconds)."""
+    t0 = time.perf_counter()
+    order = np.asarray(order, dtype=np.int64)
+    n = len(order)
+    parts = []
+    done = 0
+    spent = 0.0           # seconds in rounds
+    step = max(1, int(math.ceil(first_fraction * n)))
+    while done < n:
+        if budget_s is not None and done > 0:
+            left = budget_s - (time.perf_counter() - t0)
+            fit = int(left / (spent / done)) if left > 0 else 0
+            if fit < 1:
+                break
+            step = min(step, fit)
+        end = min(n, done + step)
+        mask = np.zeros(n, dtype=np.uint8)
+        mask[order[done:end]] = 1
+        t1 = time.perf_counter()
+        parts.append(engine.search_blocks(r_lo, r_hi, threshold, mask))
+        spent += time.perf_counter() - t1
+        done = end
+        step = max(1, int(math.ceil(step * growth)))
+    raw = np.concatenate(parts) if parts else np.zeros(0, dtype=_lib.CAND_DTYPE)
+    raw = raw[np.lexsort((raw["k"], raw["t"]))] if len(raw) else raw
+    return raw, done, time.perf_counter() - t0
+
+
+def triage_order(engine, model: BlockTriage, r_lo: float, r_hi: float) -> np.ndarray:
+    """The band's blocks from the highest score down (stable among ties)."""
+    logits = model.logits(engine, model.features(engine, r_lo, r_hi))
+    return np.argsort(-logits, kind="stable")
 
 
 def triage_search(engine, model: BlockTriage, r_lo: float, r_hi: float, threshold: float,

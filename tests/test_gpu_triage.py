@@ -104,6 +104,35 @@ def test_search_blocks_equals_filtered_search(cs, small_spectrum, mode):
         eng.search_blocks(r_lo, r_hi, 6.0, np.ones(nb + 1, np.uint8))
 
 
+@pytest.mark.parametrize("mode", ["fused", "plain"])
+def test_scheduled_search_rounds(cs, small_spectrum, mode):
+    """The budgeted scheduler: with no budget it visits every block and returns
+    exactly the full search's raw candidates (in any block order); with a zero
+    budget it stops after its first round, whose candidates are the full
+    search's in exactly the first ceil(1 %) of the given order."""
+    _say(f"scheduled search mode={mode}")
+    from coherentsearch.triage import scheduled_search
+    amps, N, dt = small_spectrum
+    T_ = N * dt
+    p = cs.SearchParams(nharms=32, m=16, decimations=cs.decimation_set(32, 4))
+    eng = cs.Engine(p, 500)
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
+    eng.set_mode(mode)
+    r_lo, r_hi = 5.0 * T_, 30.0 * T_
+    full = eng.search_raw(r_lo, r_hi, 6.0)
+    nb = len(eng.triage_features(r_lo, r_hi))
+    order = np.random.default_rng(3).permutation(nb)
+    got, done, _ = scheduled_search(eng, r_lo, r_hi, 6.0, order)
+    assert done == nb and np.array_equal(got, full)
+    # put a block holding candidates first, then stop after one round
+    order = np.concatenate([[full["block"][0]], np.setdiff1d(order, [full["block"][0]])])
+    got, done, _ = scheduled_search(eng, r_lo, r_hi, 6.0, order, budget_s=0.0, first_fraction=0.01)
+    first = max(1, int(np.ceil(0.01 * nb)))
+    assert done == first
+    assert len(got) > 0 and np.array_equal(got, full[np.isin(full["block"], order[:first])])
+    eng.close()
+
+
 def test_triage_training_separates_blocks(cs):
     """Full-batch training on one spectrum lowers the loss and ranks the blocks
     holding candidates above the others."""
