@@ -105,6 +105,7 @@ struct cs_ctx {
     hipStream_t cst = nullptr;
     int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
     int tune_order = 0;        // CS_TUNE_TILE_ORDER (0: contiguous tile ranges per XCD)
+    int tune_generic = 0;      // CS_TUNE_GENERIC (1: no fold-set kernel)
     bool use_graph = false;    // CS_TUNE_GRAPHS: replay a captured launch sequence
     std::string tune_cache;    // cs_set_tuning_cache: cs_autotune's cache for cs_search_files
     hipGraph_t g_graph = nullptr;
@@ -441,6 +442,9 @@ int cs_set_tuning(cs_ctx* c, int32_t knob, int64_t value) {
     } else if (knob == CS_TUNE_TILE_ORDER) {
         if (value > 1) return c->fail(CS_EINVAL, "CS_TUNE_TILE_ORDER takes 0 or 1");
         c->tune_order = (int)value;
+    } else if (knob == CS_TUNE_GENERIC) {
+        if (value > 1) return c->fail(CS_EINVAL, "CS_TUNE_GENERIC takes 0 or 1");
+        c->tune_generic = (int)value;
     } else {
         return c->fail(CS_EINVAL, "unknown tuning knob");
     }
@@ -455,6 +459,7 @@ int cs_get_tuning(const cs_ctx* c, int32_t knob, int64_t* value) {
     else if (knob == CS_TUNE_WORKGROUPS) *value = c->tune_wg;
     else if (knob == CS_TUNE_GRAPHS) *value = c->use_graph ? 1 : 0;
     else if (knob == CS_TUNE_TILE_ORDER) *value = c->tune_order;
+    else if (knob == CS_TUNE_GENERIC) *value = c->tune_generic;
     else return CS_EINVAL;
     return CS_OK;
 }
@@ -1339,6 +1344,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         md.spl_ld = spl_ld;
         md.ntiles = (total + 8 * md.S - 1) / (8 * md.S) * md.U;
         md.tile_order = c->tune_order;
+        md.generic = c->tune_generic;
         const MfmaLds lay = mfma_lds_plan(nh, fs, md);
         if (lay.total > 160 * 1024)
             return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");

@@ -1275,6 +1275,7 @@ hipError_t launch_rowmask(hipStream_t st, const SpecDev& sp, const TabDev& tb, c
 // the same profile lengths with fold factors 1, 2, ..., the stacked planes of
 // the set's nharms and its DFT tile count (the LDS layout its gathers assume)
 static int fold_set_of(const FoldSet& fs, const MfmaDev& md, int nharms) {
+    if (md.generic) return 0;
     for (int set = 1; set < kFoldSets; ++set) {
         bool same = fs.nf == fold_set_size(set) && nharms == kFoldSetNB[set][0] / 2 &&
                     md.dft_tiles_max == set_tiles(set);

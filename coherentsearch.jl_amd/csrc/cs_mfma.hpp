@@ -40,7 +40,7 @@ struct MfmaDev {
     int32_t nbp;              // profile row stride in LDS (floats)
     int32_t dft_tiles_max;    // c2r DFT tiles of all folds (hi or lo)
     int32_t tile_order;       // 0: contiguous tile-list ranges per XCD; 1: round-robin
-    int32_t pad;
+    int32_t generic;          // 1: the run-time-dispatch kernel even for a known fold set
 };
 
 struct MfmaLds {

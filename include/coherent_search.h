@@ -43,6 +43,10 @@ extern "C" {
 #define CS_TUNE_TILE_ORDER 4    /* split-BF16 mode: 0 deals the tile list to the 8 XCDs in
                                    contiguous ranges (neighbouring windows share an L2), 1
                                    round-robin; default 0 */
+#define CS_TUNE_GENERIC 5       /* split-BF16 mode: 1 runs the kernel that dispatches each
+                                   fold on its profile length at run time even where a kernel
+                                   specialised on the search's fold set exists (same results);
+                                   default 0 */
 
 /* SearchParams (src/search.jl:131-144).  `decimations` is pointer + length
  * (the reference's Vector{Int}).  precision is the reference's profile-stage

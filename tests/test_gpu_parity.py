@@ -276,6 +276,37 @@ def test_search_files_equals_single_file_runs(cs, mode):
     assert any(abs(c.freq - 21.3) < 2.0 / fts[1].T for c in lists[1])
 
 
+@pytest.mark.parametrize("nharms,maxdecim,blocksize", [(60, 6, 2048), (32, 4, 512), (8, 1, 2048)])
+def test_fold_set_kernel_equals_generic(cs, small_spectrum, nharms, maxdecim, blocksize):
+    """The split-BF16 kernels specialised on a fold set (every pass inlined,
+    unclamped constant-stride row gathers) compute exactly what the kernel
+    that dispatches each fold on its length does: raw candidates and every
+    (block, fold) pass sigma (precision :f32, so the sigmas are the pass's own,
+    from the kernel's sample columns) bit for bit; the split-BF16 probe's
+    stacked amplitudes and per-fold bounds too."""
+    _say(f"fold-set kernel vs generic nharms={nharms}")
+    amps, N, dt = small_spectrum
+    T_ = N * dt
+    p = cs.SearchParams(nharms=nharms, m=16, decimations=cs.decimation_set(nharms, maxdecim),
+                        precision="f32")
+    eng = cs.Engine(p, blocksize)
+    eng.set_mode("fused")
+    eng.set_spectrum(_ft(cs, amps, N, dt))
+    r_lo, r_hi = 1.0 * T_, 60.0 * T_
+    out = {}
+    for generic in (False, True):
+        eng.set_tuning(generic=generic)
+        assert eng.tuning()["generic"] == int(generic)
+        raw = eng.search_raw(r_lo, r_hi, 6.0)
+        out[generic] = (raw, eng.last_sigmas(), eng.probe_split_bf16(r_lo, r_hi, 1000, 256))
+    assert len(out[False][0]) > 0
+    assert np.array_equal(out[False][0], out[True][0])
+    assert np.array_equal(out[False][1], out[True][1])
+    for a, b in zip(out[False][2], out[True][2]):
+        assert np.array_equal(a, b)
+    eng.close()
+
+
 def test_search_files_with_tuning_cache(cs, tmp_path):
     """A sweep whose length classes take their launch shapes from cs_autotune's
     cache (cs_set_tuning_cache) returns exactly the raw candidates of the sweep
