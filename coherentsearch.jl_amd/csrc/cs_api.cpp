@@ -64,7 +64,9 @@ struct cs_ctx {
     std::map<int, DBuf> twid;
     // scratch
     DBuf ftp, prof, nvalid, sigma, surv, counters, cands, widths, rfund, rowok, mout, mraw, samples;
-    DBuf rflags;    // (block, fold) pairs of a batch whose sigma is refined in FP64
+    DBuf rflags;    // (block, fold) pairs of a batch whose sigma is refined in FP64, + count
+    DBuf rlist;     // the flagged pairs, each once
+    DBuf rsamp;     // FP64 samples of the first kRefineCap listed pairs
     DBuf probe_buf; // cs_probe_split_bf16: the copied-out stacked amplitudes
     DBuf bandtab;   // block_metrics(:fft): per-harmonic lobin, then fine-grid length
     DBuf cand_r, cand_h, cand_w, cand_prof;   // candidate profiles: inputs, widths, outputs
@@ -401,7 +403,7 @@ int cs_ctx_destroy(cs_ctx* c) {
                     &c->pyr, &c->tfeat, &c->mlp_w, &c->mlp_x, &c->mlp_p,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
                     &c->m_desc, &c->m_rowmask, &c->m_scol, &c->m_ident, &c->m_sel, &c->m_slot,
-                    &c->m_tiles, &c->spl, &c->rflags, &c->probe_buf, &c->sweep_amps,
+                    &c->m_tiles, &c->spl, &c->rflags, &c->rlist, &c->rsamp, &c->probe_buf, &c->sweep_amps,
                     &c->sweep_cands, &c->sweep_ctr};
     for (DBuf* b : bufs) release(*b);
     for (void* p : c->pin)
@@ -1428,7 +1430,17 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     // precision :f64 (the reference's default profile stage): the sigma that
     // scales every reported metric comes from FP64 profiles
     const bool refine = c->precision == 0;
-    if (refine && (s = ensure(c, c->rflags, (size_t)Bc * nf * sizeof(uint32_t)))) return s;
+    RefineBufs rbufs{};
+    if (refine) {
+        const size_t npmax = (size_t)Bc * nf;
+        if ((s = ensure(c, c->rflags, (npmax + 1) * sizeof(uint32_t)))) return s;
+        if ((s = ensure(c, c->rlist, npmax * sizeof(uint32_t)))) return s;
+        rbufs.cap = (uint32_t)std::min<size_t>(kRefineCap, npmax);
+        if ((s = ensure(c, c->rsamp, (size_t)rbufs.cap * kSigmaSamples * sizeof(double)))) return s;
+        rbufs.flags = (uint32_t*)c->rflags.p;
+        rbufs.list = (uint32_t*)c->rlist.p;
+        rbufs.samples = (double*)c->rsamp.p;
+    }
     int64_t rowo = 0;
     for (int f = 0; f < nf && !fused; ++f) {
         fs.prof[f] = (float*)c->prof.p + rowo * BT;
@@ -1656,7 +1668,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
                 HIPCHK(c, rec(4));
             }
             if (refine)
-                HIPCHK(c, launch_sigma_refine(c->st, sp, tb, g, fs, b, o, (uint32_t*)c->rflags.p));
+                HIPCHK(c, launch_sigma_refine(c->st, sp, tb, g, fs, b, o, rbufs));
             HIPCHK(c, launch_rescan(c->st, sp, tb, g, fs, b, o, 8192));
             HIPCHK(c, rec(5));
         }
@@ -1671,7 +1683,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         };
         const void* ptrs[] = {c->surv.p, c->counters.p, c->nvalid.p, c->sigma.p, c->m_rowmask.p,
                               c->m_desc.p, c->m_ident.p, c->ftp.p, c->prof.p, c->rowok.p,
-                              c->mraw.p, c->samples.p, refine ? c->rflags.p : nullptr};
+                              c->mraw.p, c->samples.p, refine ? c->rflags.p : nullptr,
+                              refine ? c->rlist.p : nullptr, refine ? c->rsamp.p : nullptr};
         put(ptrs, sizeof(ptrs));
         put(&sp, sizeof(sp));
         put(&tb, sizeof(tb));

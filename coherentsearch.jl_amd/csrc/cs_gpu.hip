@@ -649,11 +649,20 @@ hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
 // same whole-column subsample: each sampled column's stack with the FP32
 // weights (as the reference fills ftprofs), an FP64 real DFT, |x|, then the
 // exact median by radix selection over the sample keys held in registers.
+//
+// The flagged pairs are listed once each; eight workgroups per listed pair
+// compute its columns into a scratch of FP64 samples (so a search with few
+// flagged pairs still spreads over the GPU), then one workgroup per pair
+// selects its median.  Pairs past the scratch's capacity are done whole by
+// one workgroup each, columns in LDS.
 // ---------------------------------------------------------------------------
 __global__ void k_reset_u32(uint32_t* p, int n);
 
+// Flag every (block, fold) pair holding a survivor; the flag's first setter
+// appends the pair to the list (count: the word after the flags).
 __global__ void k_refine_mark(const OutDev o, GridDev g, BatchDev b, int nf,
-                              uint32_t* __restrict__ flags) {
+                              uint32_t* __restrict__ flags, uint32_t* __restrict__ list,
+                              uint32_t* __restrict__ count) {
     uint32_t nsurv = *o.surv_count;
     if (nsurv > o.surv_cap) nsurv = o.surv_cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nsurv;
@@ -664,13 +673,129 @@ __global__ void k_refine_mark(const OutDev o, GridDev g, BatchDev b, int nf,
         int64_t cg, first, last;
         block_of(g, t, cg, first, last);
         int cl = batch_slot(b, cg);
-        if (cl >= 0) flags[cl * nf + f] = 1u;
+        if (cl >= 0) {
+            const uint32_t pair = (uint32_t)(cl * nf + f);
+            if (atomicCAS(&flags[pair], 0u, 1u) == 0u) list[atomicAdd(count, 1u)] = pair;
+        }
     }
 }
 
 constexpr int kRefineThreads = 1024;
 constexpr int kRefinePer = kSigmaSamples / kRefineThreads;   // sample keys per thread
 constexpr int kRefineX = kRefineThreads + 64;                // stack entries of one round
+constexpr int kRefineColThreads = 256;                       // a column chunk's workgroup
+constexpr int kRefineColX = 1024 + 64;                       // its stack entries per round
+
+// One refined pair: its block, fold and sampled columns (_block_sigma's
+// whole-column subsample, search.jl:504-538).  False when the fold has no
+// valid trial in the block (sigma 0).
+struct RefinePair {
+    int f, k, Hk, NB;
+    int64_t first, last, nv, ncol, ns;
+    bool all;
+};
+__device__ __forceinline__ bool refine_pair(const GridDev& g, const FoldSet& fs,
+                                            const BatchDev& b, uint32_t pair, RefinePair& r) {
+    const int nf = fs.nf;
+    const int cl = (int)pair / nf;
+    r.f = (int)pair - cl * nf;
+    const int64_t cg = batch_block(b, cl);
+    r.first = g.origin + cg * g.Nprof;
+    r.last = min(r.first + g.Nprof, g.total) - 1;
+    const FoldDev& fd = fs.f[r.f];
+    r.k = fd.k;
+    r.Hk = fd.Hk;
+    r.NB = fd.nbins;
+    r.nv = b.nvalid[cg * nf + r.f];
+    const int64_t NN = (int64_t)r.NB * r.nv;
+    const int64_t cap = fs.cap[r.f];
+    r.all = NN <= cap;
+    if (r.all) {
+        r.ncol = r.nv;
+        r.ns = NN;
+    } else {
+        r.ncol = cap / r.NB;
+        if (r.ncol < 1) r.ncol = 1;
+        if (r.ncol > r.nv) r.ncol = r.nv;
+        r.ns = r.ncol * r.NB < cap ? r.ncol * r.NB : cap;
+    }
+    return NN > 0;
+}
+
+// The pair's twiddles, and the plans and row guards of its harmonics l*k
+// (l = 1..Hk), once for all its columns.
+template <int NT>
+__device__ __forceinline__ void refine_setup(const SpecDev& sp, const TabDev& tb,
+                                             const FoldSet& fs, const RefinePair& r, double2* tw,
+                                             HarmDev* hd, uint8_t* hok) {
+    for (int e = threadIdx.x; e < r.NB; e += NT) tw[e] = fs.twid[r.f][e];
+    for (int l = threadIdx.x; l < r.Hk; l += NT) {
+        const HarmDev d = tb.harm[(l + 1) * r.k - 1];
+        hd[l] = d;
+        hok[l] = d.P == 0 || row_guard(d, tb, sp, r.first, r.last) ? 1 : 0;
+    }
+    __syncthreads();
+}
+
+// Sampled columns c_lo .. c_hi - 1 of pair r, in rounds of NX stack entries:
+// every (column, harmonic) stack entry, then every (column, bin pair) of the
+// FP64 real DFT; put(s, |x|) receives sample s = column * NB + bin.  Called
+// by the whole workgroup of NT threads; ends at a barrier.
+template <int NT, int NX, class Put>
+__device__ __forceinline__ void refine_columns(const SpecDev& sp, const TabDev& tb,
+                                               const RefinePair& r, int64_t c_lo, int64_t c_hi,
+                                               double2* X, const double2* tw, const HarmDev* hd,
+                                               const uint8_t* hok, Put put) {
+    const int tid = threadIdx.x;
+    const int H = r.Hk, W = H + 1, NB = r.NB;
+    const int CR = NX / W;
+    for (int64_t c0 = c_lo; c0 < c_hi; c0 += CR) {
+        const int nc = (int)min((int64_t)CR, c_hi - c0);
+        for (int task = tid; task < nc * W; task += NT) {
+            const int cr = task / W, l = task - cr * W;
+            const int64_t cc = c0 + cr;
+            const int64_t t = r.first + (r.all ? cc : (cc * r.nv) / r.ncol);
+            double2 x = make_double2(0.0, 0.0);
+            if (l >= 1 && hok[l - 1]) {
+                const HarmDev& d = hd[l - 1];
+                if (tb.m == 16 && !tb.weights_f64) x = interp_with<16>(d, sp, tb, l * r.k, t, r.first, r.last, true, true);
+                else if (tb.m == 32 && !tb.weights_f64) x = interp_with<32>(d, sp, tb, l * r.k, t, r.first, r.last, true, true);
+                else x = interp_with<0>(d, sp, tb, l * r.k, t, r.first, r.last, true, true);
+            }
+            X[task] = x;
+        }
+        __syncthreads();
+        for (int task = tid; task < nc * (H + 1); task += NT) {
+            const int cr = task / (H + 1), n = task - cr * (H + 1);
+            const double2* Xc = X + cr * W;
+            const int64_t s0 = (c0 + cr) * NB;
+            // the column gives min(NB, ns - s0) samples: its bins 0 .. that - 1
+            const int64_t take = r.ns - s0 < NB ? r.ns - s0 : NB;
+            double E = 0.0, O = 0.0;
+            int idx = 0;                                 // (h n) mod NB
+            for (int h = 1; h <= H; ++h) {
+                idx += n;
+                if (idx >= NB) idx -= NB;
+                double c, sn;
+                if (h == H) { c = (n & 1) ? -1.0 : 1.0; sn = 0.0; }
+                else {
+                    const double2 w = tw[idx];
+                    c = 2.0 * w.x;
+                    sn = -(2.0 * w.y);
+                }
+                E = fma(c, Xc[h].x, E);
+                O = fma(sn, Xc[h].y, O);
+            }
+            if (n == 0 || n == H) {
+                if (n < take) put(s0 + n, fabs(E));
+            } else {
+                if (n < take) put(s0 + n, fabs(E + O));
+                if (NB - n < take) put(s0 + NB - n, fabs(E - O));
+            }
+        }
+        __syncthreads();
+    }
+}
 
 // Exclusive prefix sum over the workgroup; `tot` holds kRefineThreads/64 words,
 // rewritten by the next call only after the caller's next two barriers.
@@ -749,139 +874,140 @@ __device__ uint64_t refine_select(const uint64_t (&keys)[kRefinePer], uint32_t& 
     return prefix;
 }
 
+// sigma = 1.4826 * median of the ns sample keys (bit patterns of doubles >= 0)
+// the workgroup holds, kRefinePer per thread (keys past ns are all ones)
+__device__ __forceinline__ double refine_sigma(const uint64_t (&keys)[kRefinePer], int64_t ns,
+                                               uint32_t* hist, uint32_t* shv, uint64_t* red) {
+    const uint32_t half = (uint32_t)(ns / 2);
+    uint32_t rk = half;
+    const uint64_t up = refine_select(keys, rk, hist, shv, red);
+    double med;
+    if ((ns & 1) || rk > 0) {
+        med = __longlong_as_double((long long)up);
+    } else {
+        // exactly `half` keys lie below `up`: the lower central value is the
+        // largest of them
+        uint64_t m = 0;
+#pragma unroll
+        for (int r = 0; r < kRefinePer; ++r) m = keys[r] < up && keys[r] > m ? keys[r] : m;
+        const uint64_t lo = refine_max(m, red);
+        med = 0.5 * (__longlong_as_double((long long)lo) + __longlong_as_double((long long)up));
+    }
+    return 1.4826 * med;
+}
+
+// Column chunks of the listed pairs 0 .. min(count, cap) - 1: eight chunks per
+// pair, each workgroup one chunk at a time; samples of list entry e go to
+// samples[e * kSigmaSamples + s].
+__global__ __launch_bounds__(kRefineColThreads) void k_refine_cols(
+    SpecDev sp, TabDev tb, GridDev g, FoldSet fs, BatchDev b, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ count, uint32_t cap, double* __restrict__ samples) {
+    __shared__ double2 X[kRefineColX];
+    __shared__ double2 tw[kMaxNB];
+    __shared__ HarmDev hd[kMaxNB / 2];
+    __shared__ uint8_t hok[kMaxNB / 2];
+    const uint32_t cnt = *count, n = cnt < cap ? cnt : cap;
+    for (uint32_t item = blockIdx.x; item < n * kRefineChunks; item += gridDim.x) {
+        const uint32_t e = item / kRefineChunks, j = item - e * kRefineChunks;
+        RefinePair r;
+        if (!refine_pair(g, fs, b, list[e], r)) continue;   // uniform over the workgroup
+        const int64_t per = (r.ncol + kRefineChunks - 1) / kRefineChunks;
+        const int64_t c_lo = (int64_t)j * per, c_hi = min(r.ncol, c_lo + per);
+        if (c_lo >= c_hi) continue;
+        refine_setup<kRefineColThreads>(sp, tb, fs, r, tw, hd, hok);
+        double* out = samples + (size_t)e * kSigmaSamples;
+        refine_columns<kRefineColThreads, kRefineColX>(sp, tb, r, c_lo, c_hi, X, tw, hd, hok,
+                                                       [&](int64_t s, double v) { out[s] = v; });
+    }
+}
+
+// The median of each listed pair below cap from its scratch samples.
+__global__ __launch_bounds__(kRefineThreads) void k_refine_select(
+    GridDev g, FoldSet fs, BatchDev b, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ count, uint32_t cap, const double* __restrict__ samples) {
+    __shared__ uint32_t hist[2 * kRefineThreads];
+    __shared__ uint32_t shv[32];
+    __shared__ uint64_t red[kRefineThreads / 64];
+    const uint32_t cnt = *count, n = cnt < cap ? cnt : cap;
+    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+        const uint32_t pair = list[e];
+        RefinePair r;
+        if (!refine_pair(g, fs, b, pair, r)) {
+            if (threadIdx.x == 0) b.sigma[pair] = 0.0;
+            continue;
+        }
+        const double* in = samples + (size_t)e * kSigmaSamples;
+        uint64_t keys[kRefinePer];
+#pragma unroll
+        for (int q = 0; q < kRefinePer; ++q) {
+            const int64_t s = threadIdx.x + (int64_t)kRefineThreads * q;
+            keys[q] = s < r.ns ? (uint64_t)__double_as_longlong(in[s]) : ~(uint64_t)0;
+        }
+        const double sig = refine_sigma(keys, r.ns, hist, shv, red);
+        if (threadIdx.x == 0) b.sigma[pair] = sig;
+        __syncthreads();
+    }
+}
+
+// Listed pairs from cap on (more than the scratch holds), each whole on one
+// workgroup: its columns into LDS, then the median.
 __global__ __launch_bounds__(kRefineThreads) void k_sigma_refine(
-    SpecDev sp, TabDev tb, GridDev g, FoldSet fs, BatchDev b, const uint32_t* __restrict__ flags) {
+    SpecDev sp, TabDev tb, GridDev g, FoldSet fs, BatchDev b, const uint32_t* __restrict__ list,
+    const uint32_t* __restrict__ count, uint32_t cap) {
     __shared__ double v[kSigmaSamples];
     __shared__ double2 X[kRefineX];
     __shared__ double2 tw[kMaxNB];
     __shared__ uint32_t hist[2 * kRefineThreads];
     __shared__ uint32_t shv[32];
     __shared__ uint64_t red[kRefineThreads / 64];
-    __shared__ HarmDev hd[kMaxNB / 2];   // the pair's harmonics l*k, l = 1..Hk
-    __shared__ uint8_t hok[kMaxNB / 2];  // their row guards on the block
-    const int tid = threadIdx.x;
-    const int nf = fs.nf;
-    const int npair = b.nblocks * nf;
-    for (int pair = blockIdx.x; pair < npair; pair += gridDim.x) {
-        if (flags[pair] == 0u) continue;                // uniform over the workgroup
-        const int cl = pair / nf, f = pair - cl * nf;
-        const int64_t cg = batch_block(b, cl);
-        const int64_t first = g.origin + cg * g.Nprof;
-        const int64_t last = min(first + g.Nprof, g.total) - 1;
-        const FoldDev& fd = fs.f[f];
-        const int k = fd.k, Hk = fd.Hk, NB = fd.nbins;
-        const int64_t nv = b.nvalid[cg * nf + f];
-        const int64_t NN = (int64_t)NB * nv;
-        const int64_t cap = fs.cap[f];
-        if (NN == 0) {
-            if (tid == 0) b.sigma[pair] = 0.0;
+    __shared__ HarmDev hd[kMaxNB / 2];
+    __shared__ uint8_t hok[kMaxNB / 2];
+    const uint32_t cnt = *count;
+    for (uint32_t e = cap + blockIdx.x; e < cnt; e += gridDim.x) {
+        const uint32_t pair = list[e];
+        RefinePair r;
+        if (!refine_pair(g, fs, b, pair, r)) {
+            if (threadIdx.x == 0) b.sigma[pair] = 0.0;
             continue;
         }
-        const bool all = NN <= cap;
-        int64_t ncol, ns;
-        if (all) { ncol = nv; ns = NN; }
-        else {
-            ncol = cap / NB;
-            if (ncol < 1) ncol = 1;
-            if (ncol > nv) ncol = nv;
-            ns = ncol * NB < cap ? ncol * NB : cap;
-        }
-        for (int e = tid; e < NB; e += kRefineThreads) tw[e] = fs.twid[f][e];
-        // plans and row guards of the pair's harmonics, once for all its columns
-        for (int l = tid; l < Hk; l += kRefineThreads) {
-            const HarmDev d = tb.harm[(l + 1) * k - 1];
-            hd[l] = d;
-            hok[l] = d.P == 0 || row_guard(d, tb, sp, first, last) ? 1 : 0;
-        }
-        __syncthreads();
-        // rounds of CR columns: every (column, harmonic) stack entry, then every
-        // (column, bin pair) of the FP64 real DFT, spread over the workgroup
-        const int W = Hk + 1;
-        const int CR = kRefineThreads / W;
-        for (int64_t c0 = 0; c0 < ncol; c0 += CR) {
-            const int nc = (int)min((int64_t)CR, ncol - c0);
-            for (int task = tid; task < nc * W; task += kRefineThreads) {
-                const int cr = task / W, l = task - cr * W;
-                const int64_t cc = c0 + cr;
-                const int64_t t = first + (all ? cc : (cc * nv) / ncol);
-                double2 x = make_double2(0.0, 0.0);
-                if (l >= 1 && hok[l - 1]) {
-                    const HarmDev& d = hd[l - 1];
-                    if (tb.m == 16 && !tb.weights_f64) x = interp_with<16>(d, sp, tb, l * k, t, first, last, true, true);
-                    else if (tb.m == 32 && !tb.weights_f64) x = interp_with<32>(d, sp, tb, l * k, t, first, last, true, true);
-                    else x = interp_with<0>(d, sp, tb, l * k, t, first, last, true, true);
-                }
-                X[task] = x;
-            }
-            __syncthreads();
-            const int H = Hk;
-            for (int task = tid; task < nc * (H + 1); task += kRefineThreads) {
-                const int cr = task / (H + 1), n = task - cr * (H + 1);
-                const double2* Xc = X + cr * W;
-                const int64_t s0 = (c0 + cr) * NB;
-                // the column gives min(NB, ns - s0) samples: its bins 0 .. that - 1
-                const int64_t take = ns - s0 < NB ? ns - s0 : NB;
-                double E = 0.0, O = 0.0;
-                int idx = 0;                                 // (h n) mod NB
-                for (int h = 1; h <= H; ++h) {
-                    idx += n;
-                    if (idx >= NB) idx -= NB;
-                    double c, sn;
-                    if (h == H) { c = (n & 1) ? -1.0 : 1.0; sn = 0.0; }
-                    else {
-                        const double2 w = tw[idx];
-                        c = 2.0 * w.x;
-                        sn = -(2.0 * w.y);
-                    }
-                    E = fma(c, Xc[h].x, E);
-                    O = fma(sn, Xc[h].y, O);
-                }
-                if (n == 0 || n == H) {
-                    if (n < take) v[s0 + n] = fabs(E);
-                } else {
-                    if (n < take) v[s0 + n] = fabs(E + O);
-                    if (NB - n < take) v[s0 + NB - n] = fabs(E - O);
-                }
-            }
-            __syncthreads();
-        }
+        refine_setup<kRefineThreads>(sp, tb, fs, r, tw, hd, hok);
+        refine_columns<kRefineThreads, kRefineX>(sp, tb, r, 0, r.ncol, X, tw, hd, hok,
+                                                 [&](int64_t s, double x) { v[s] = x; });
         uint64_t keys[kRefinePer];
 #pragma unroll
-        for (int r = 0; r < kRefinePer; ++r) {
-            const int64_t s = tid + (int64_t)kRefineThreads * r;
-            keys[r] = s < ns ? (uint64_t)__double_as_longlong(v[s]) : ~(uint64_t)0;
+        for (int q = 0; q < kRefinePer; ++q) {
+            const int64_t s = threadIdx.x + (int64_t)kRefineThreads * q;
+            keys[q] = s < r.ns ? (uint64_t)__double_as_longlong(v[s]) : ~(uint64_t)0;
         }
-        const uint32_t half = (uint32_t)(ns / 2);
-        uint32_t rk = half;
-        const uint64_t up = refine_select(keys, rk, hist, shv, red);
-        double med;
-        if ((ns & 1) || rk > 0) {
-            med = __longlong_as_double((long long)up);
-        } else {
-            // exactly `half` keys lie below `up`: the lower central value is the
-            // largest of them
-            uint64_t m = 0;
-#pragma unroll
-            for (int r = 0; r < kRefinePer; ++r) m = keys[r] < up && keys[r] > m ? keys[r] : m;
-            const uint64_t lo = refine_max(m, red);
-            med = 0.5 * (__longlong_as_double((long long)lo) + __longlong_as_double((long long)up));
-        }
-        if (tid == 0) b.sigma[pair] = 1.4826 * med;
+        const double sig = refine_sigma(keys, r.ns, hist, shv, red);
+        if (threadIdx.x == 0) b.sigma[pair] = sig;
         __syncthreads();
     }
 }
 
 hipError_t launch_sigma_refine(hipStream_t st, const SpecDev& sp, const TabDev& tb,
                                const GridDev& g, const FoldSet& fs, const BatchDev& b,
-                               const OutDev& o, uint32_t* flags) {
+                               const OutDev& o, const RefineBufs& rb) {
     const int npair = b.nblocks * fs.nf;
     if (npair <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_reset_u32, dim3((unsigned)((npair + 63) / 64)), dim3(64), 0, st, flags,
-                       npair);
-    hipLaunchKernelGGL(k_refine_mark, dim3(64), dim3(256), 0, st, o, g, b, fs.nf, flags);
-    const int grid = npair < 1024 ? npair : 1024;
-    hipLaunchKernelGGL(k_sigma_refine, dim3((unsigned)grid), dim3(kRefineThreads), 0, st, sp, tb, g,
-                       fs, b, (const uint32_t*)flags);
+    uint32_t* count = rb.flags + npair;
+    hipLaunchKernelGGL(k_reset_u32, dim3((unsigned)((npair + 1 + 63) / 64)), dim3(64), 0, st,
+                       rb.flags, npair + 1);
+    hipLaunchKernelGGL(k_refine_mark, dim3(64), dim3(256), 0, st, o, g, b, fs.nf, rb.flags, rb.list,
+                       count);
+    const uint32_t held = (uint32_t)npair < rb.cap ? (uint32_t)npair : rb.cap;
+    const unsigned gcols = held * kRefineChunks < 4096u ? held * kRefineChunks : 4096u;
+    hipLaunchKernelGGL(k_refine_cols, dim3(gcols), dim3(kRefineColThreads), 0, st, sp, tb, g, fs, b,
+                       (const uint32_t*)rb.list, (const uint32_t*)count, rb.cap, rb.samples);
+    hipLaunchKernelGGL(k_refine_select, dim3(held < 1024u ? held : 1024u), dim3(kRefineThreads), 0,
+                       st, g, fs, b, (const uint32_t*)rb.list, (const uint32_t*)count, rb.cap,
+                       (const double*)rb.samples);
+    if ((uint32_t)npair > rb.cap) {   // more pairs than the scratch holds may be listed
+        const uint32_t rest = (uint32_t)npair - rb.cap;
+        hipLaunchKernelGGL(k_sigma_refine, dim3(rest < 256u ? rest : 256u), dim3(kRefineThreads), 0,
+                           st, sp, tb, g, fs, b, (const uint32_t*)rb.list, (const uint32_t*)count,
+                           rb.cap);
+    }
     return hipGetLastError();
 }
 

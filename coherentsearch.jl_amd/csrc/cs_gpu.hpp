@@ -154,11 +154,22 @@ hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
                          const OutDev& o, int nblocks_grid);
 hipError_t launch_reset_u32(hipStream_t st, uint32_t* p, int n);
 hipError_t launch_copy_u32(hipStream_t st, const uint32_t* src, uint32_t* dst);
+// Scratch of the FP64 sigma refinement: flags (nblocks * nf + 1 words, the
+// last the list count; cleared by the launch itself), the list of flagged
+// pairs (nblocks * nf words) and FP64 samples of the first `cap` listed pairs
+// (cap * kSigmaSamples doubles).
+constexpr int kRefineChunks = 8;      // column chunks (workgroups) per listed pair
+constexpr uint32_t kRefineCap = 2048; // listed pairs the sample scratch holds at most
+struct RefineBufs {
+    uint32_t* flags;
+    uint32_t* list;
+    double* samples;
+    uint32_t cap;
+};
 // FP64 sigma of the batch's (block, fold) pairs that hold gate survivors
-// (flags: scratch of nblocks * nf words, cleared by the launch itself)
 hipError_t launch_sigma_refine(hipStream_t st, const SpecDev& sp, const TabDev& tb,
                                const GridDev& g, const FoldSet& fs, const BatchDev& b,
-                               const OutDev& o, uint32_t* flags);
+                               const OutDev& o, const RefineBufs& rb);
 
 // FP64 bridge path (chunk_metrics / decim / block_metrics).
 hipError_t launch_c2r_f64(hipStream_t st, const double2* ftp, int64_t ldf, int k,

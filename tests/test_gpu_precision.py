@@ -97,6 +97,45 @@ def test_f64_modes_identical_and_equal_to_oracle(cs, O, small_spectrum, nharms, 
     assert srel <= 1e-12
 
 
+def test_f64_refine_past_the_sample_scratch(cs, O, small_spectrum):
+    """Blocks of 16 trials and a threshold of 3 flag about 15,800 (block, fold)
+    pairs, far more than the refinement's sample scratch holds (2048): the
+    first 2048 listed pairs go through the column-chunk and select kernels, the
+    rest through the one-workgroup-per-pair kernel.  Every mode returns the
+    same raw candidates, equal to the oracle's, and every flagged pair's sigma
+    is the oracle's FP64 sigma."""
+    amps, N, dt = small_spectrum
+    T = N * dt
+    nharms, blocksize, thr = 32, 16, 3.0
+    dec = cs.decimation_set(nharms, 4)
+    lo, hi = 1.0 * T, 60.0 * T
+    total = math.floor((hi - lo) / (0.5 / nharms)) + 1
+    nch = -(-total // blocksize)
+    wsig = np.zeros((nch, len(dec)))
+    want = O.search_region(O.make_params(nharms=nharms, m=16, decimations=dec), amps, N, lo,
+                           total, blocksize, threshold=thr, sigmas=wsig)
+    out = {m: _search(cs, amps, N, dt, nharms, dec, blocksize, "f64", m, lo, hi, thr)
+           for m in MODES}
+    for m in MODES[1:]:
+        assert np.array_equal(out[m][0], out["plain"][0]), m
+    got, sig = out["fused"]
+    fidx = {k: i for i, k in enumerate([1] + [k for k in dec if k > 1])}
+    pairs = {(int(c["block"]), fidx[int(c["k"])]) for c in got}
+    _say(f"  {len(got)} candidates, {len(pairs)} (block, fold) pairs with candidates, "
+         f"{nch * len(dec)} pairs")
+    assert len(pairs) > 4096
+    g = {(int(c["t"]), int(c["k"])): c for c in got}
+    w = {(int(c["t"]), int(c["k"])): c for c in want}
+    for key in set(g) ^ set(w):
+        rec = g.get(key, w.get(key))
+        assert abs(float(rec["metric"]) - thr) <= 1e-9 * thr, key
+    worst = max(abs(g[k]["metric"] - w[k]["metric"]) / w[k]["metric"] for k in set(g) & set(w))
+    srel = max(abs(sig[b, f] - wsig[b, f]) / wsig[b, f] for b, f in pairs)
+    _say(f"  worst S/N rel {worst:.2e}, sigma rel {srel:.2e}")
+    assert worst <= 1e-10
+    assert srel <= 1e-12
+
+
 def test_f32_keeps_the_pass_sigma(cs, O, small_spectrum):
     """:f32 reports with the pass's own sigma: the split-BF16 mode's metrics
     move from the :f64 ones by ~1e-5 (not 0), still within 1e-4 of the oracle."""
