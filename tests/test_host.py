@@ -323,3 +323,60 @@ def test_parallel_generation_equals_serial():
     serial = [synth.make_config_spectrum(c) for c in cfgs]
     par = synth.make_config_spectra(cfgs, workers=2)
     assert all(np.array_equal(a, b) for a, b in zip(serial, par))
+
+
+# ---- the time-budget block scheduler (host logic) ---------------------------
+
+class _Blocks:
+    """A stand-in for Engine.search_blocks over nb blocks: one record per
+    selected block (returned in reverse order), each call taking `wait` s."""
+
+    def __init__(self, nb, wait=0.0):
+        self.nb, self.wait, self.calls = nb, wait, []
+
+    def search_blocks(self, r_lo, r_hi, threshold, mask):
+        import time
+        from coherentsearch import _lib
+        assert len(mask) == self.nb
+        sel = np.flatnonzero(mask)
+        self.calls.append(sel)
+        if self.wait:
+            time.sleep(self.wait)
+        out = np.zeros(len(sel), dtype=_lib.CAND_DTYPE)
+        out["t"], out["k"], out["block"], out["metric"] = sel * 100 + 7, 1, sel, 9.0
+        return out[::-1]
+
+
+def test_scheduled_search_rounds_cover_the_order():
+    """Rounds of 1 %, then twice as many blocks each time, in the given order,
+    every block once; the merged records come back sorted by (t, k)."""
+    from coherentsearch.triage import scheduled_search
+    nb = 1000
+    order = np.random.default_rng(1).permutation(nb)
+    eng = _Blocks(nb)
+    raw, done, _ = scheduled_search(eng, 0.0, 1.0, 8.0, order)
+    assert done == nb
+    sizes = [len(c) for c in eng.calls]
+    assert sizes == [10, 20, 40, 80, 160, 320, 370]
+    for call, want in zip(eng.calls, np.split(order, np.cumsum(sizes)[:-1])):
+        assert np.array_equal(call, np.sort(want))
+    assert np.array_equal(raw["block"], np.arange(nb))          # sorted by t = 100 b + 7
+
+
+def test_scheduled_search_budget():
+    """A zero budget runs the first round only; a budget of a few calls stops
+    before the band is done and overshoots it by about one call at most (the
+    bound leaves room for a loaded host)."""
+    import time
+    from coherentsearch.triage import scheduled_search
+    nb = 1000
+    order = np.arange(nb)[::-1]
+    eng = _Blocks(nb)
+    raw, done, _ = scheduled_search(eng, 0.0, 1.0, 8.0, order, budget_s=0.0)
+    assert done == 10 and len(eng.calls) == 1
+    assert np.array_equal(raw["block"], np.arange(990, 1000))
+    eng = _Blocks(nb, wait=0.02)
+    t0 = time.perf_counter()
+    raw, done, secs = scheduled_search(eng, 0.0, 1.0, 8.0, order, budget_s=0.07)
+    assert 10 < done < nb and len(raw) == done
+    assert secs < 0.07 + 0.06 and time.perf_counter() - t0 >= secs
