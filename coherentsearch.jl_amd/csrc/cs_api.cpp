@@ -104,6 +104,7 @@ struct cs_ctx {
     hipEvent_t sweep_ev[2] = {nullptr, nullptr};
     void* pin[2] = {nullptr, nullptr};
     size_t pin_bytes = 0;
+    void* hout = nullptr;   // pinned: a search's counters, then its first kOutRecords candidates
     hipStream_t cst = nullptr;
     int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
     int tune_order = 0;        // CS_TUNE_TILE_ORDER (0: contiguous tile ranges per XCD)
@@ -408,6 +409,7 @@ int cs_ctx_destroy(cs_ctx* c) {
     for (DBuf* b : bufs) release(*b);
     for (void* p : c->pin)
         if (p) (void)hipHostFree(p);
+    if (c->hout) (void)hipHostFree(c->hout);
     for (hipEvent_t e : c->sweep_up)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : c->sweep_ev)
@@ -1228,6 +1230,11 @@ static int finish_stats(cs_ctx* c, const StatsReq& rq, const FoldSet& fs, const 
     return CS_OK;
 }
 
+// Pinned result buffer of cs_search: the two counters (padded to kOutHead
+// bytes), then room for the first kOutRecords candidates.
+constexpr size_t kOutHead = 64;
+constexpr int64_t kOutRecords = 1024;
+
 static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, const uint8_t* mask,
                        int64_t nmask, cs_cand* out, int64_t cap, int64_t* n_out,
                        const StatsReq* sreq = nullptr, const ProbeReq* probe = nullptr,
@@ -1740,18 +1747,33 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         } else if ((s = emit(true))) {
             return s;
         }
+        // the counters and the first candidates come back in one synchronisation
+        // (queued behind the search on its stream, into pinned memory)
+        if (!c->hout && hipHostMalloc(&c->hout, kOutHead + kOutRecords * sizeof(cs_cand_dev),
+                                      hipHostMallocDefault) != hipSuccess) {
+            c->hout = nullptr;
+            return c->fail(CS_ENOMEM, "pinned result buffer allocation failed");
+        }
+        const int64_t first = std::min<int64_t>(cand_cap, kOutRecords);
+        HIPCHK(c, hipMemcpyAsync(c->hout, c->counters.p, 2 * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, c->st));
+        HIPCHK(c, hipMemcpyAsync((char*)c->hout + kOutHead, c->cands.p,
+                                 (size_t)first * sizeof(cs_cand_dev), hipMemcpyDeviceToHost, c->st));
         HIPCHK(c, hipStreamSynchronize(c->st));
         uint32_t counts[2];
-        HIPCHK(c, hipMemcpy(counts, c->counters.p, sizeof(counts), hipMemcpyDeviceToHost));
+        std::memcpy(counts, c->hout, sizeof(counts));
         if (counts[1] > cand_cap && attempt == 0) {
             cand_cap = counts[1] + 1024;
             continue;
         }
         int64_t nc = std::min<int64_t>(counts[1], cand_cap);
         std::vector<cs_cand_dev> hc((size_t)nc);
-        if (nc)
-            HIPCHK(c, hipMemcpy(hc.data(), c->cands.p, (size_t)nc * sizeof(cs_cand_dev),
-                                hipMemcpyDeviceToHost));
+        const int64_t n0 = std::min<int64_t>(nc, first);
+        if (n0)
+            std::memcpy(hc.data(), (const char*)c->hout + kOutHead, (size_t)n0 * sizeof(cs_cand_dev));
+        if (nc > n0)
+            HIPCHK(c, hipMemcpy(hc.data() + n0, (const cs_cand_dev*)c->cands.p + n0,
+                                (size_t)(nc - n0) * sizeof(cs_cand_dev), hipMemcpyDeviceToHost));
         std::sort(hc.begin(), hc.end(), [](const cs_cand_dev& a, const cs_cand_dev& b) {
             return a.t != b.t ? a.t < b.t : a.k < b.k;
         });
