@@ -51,7 +51,6 @@ struct BoolC {
 
 constexpr int kOcc8 = 8;          // occurrences per tile
 constexpr int kThreads = 512;     // 8 waves
-constexpr int kStageFloats = 768;   // per-wave LDS staging of sampled columns
 
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8, v); }
@@ -335,52 +334,16 @@ __device__ __forceinline__ void put_column(__attribute__((address_space(ASN))) f
     }
 }
 
-// One fold of one 16-trial column block: split-BF16 c2r into registers, sigma
-// samples, zero-baseline bound max_w max_p boxsum / sqrt(w) over the fold's
-// bank (all widths <= WM).  Called by the whole wave; `active` gates the
-// outputs of the lane's trial column.
-// What one fold pass hands back: the bound of the lane's trial column, the
-// staging cursor after the fold's sampled columns, and where they go.
-struct FoldOut {
-    float bound;
-    int stg_end;          // staging floats in use after this fold
-    uint64_t dst;         // global address of the staged columns (0: none staged)
-};
-
 // sigma samples (_block_sigma, search.jl:504-538): the whole profile of a
 // sampled column goes to its NB slots; columns start every NBS slots.  The
 // median only sees the multiset, so the order of a column's slots is free.
-// A wave's sampled columns are consecutive slots of one region: they are
-// staged in the wave's LDS buffer and the caller writes them out after its
-// last fold, so no global store is pending across the fold calls.  Columns
-// that are not consecutive (a wave across a block boundary) or do not fit
-// are stored directly.  putl / putg write this lane group's share of the
-// column whose slot block starts at their argument.
-template <int NBS, class PutL, class PutG>
-__device__ __forceinline__ void stage_samples(FoldOut& out, bool active, int scol, int g, int col,
-                                              CS_GLB float* samp, CS_LDS float* stg, int stg_off,
-                                              PutL putl, PutG putg) {
-    const bool smp = active && scol >= 0;
-    const uint64_t bal = __ballot(smp && g == 0);   // one bit per sampled column
-    if (bal == 0) return;
-    // the first sampled lane is wave-uniform: read its values with v_readlane
-    // rather than an LDS permute
-    const int first = __builtin_ctzll(bal);
-    const int c_first = __builtin_amdgcn_readlane(scol, first);
-    const uint64_t sa = (uint64_t)(uintptr_t)samp;
-    const uint64_t s_first =
-        ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(sa >> 32), first) << 32) |
-        (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)sa, first);
-    const int rank = __builtin_popcountll(bal & ((1ull << col) - 1ull));
-    const bool contig = !smp || (scol == c_first + rank && sa == s_first);
-    const int need = __builtin_popcountll(bal) * NBS;
-    if (__ballot(!contig) == 0 && stg_off + need <= kStageFloats) {
-        if (smp) putl(stg + stg_off + rank * NBS);
-        out.stg_end = stg_off + need;
-        out.dst = s_first + (uint64_t)c_first * NBS * sizeof(float);
-    } else if (smp) {
-        putg(samp + (int64_t)scol * NBS);
-    }
+// Each lane group stores its share of the column straight from registers
+// (putg writes it to the column whose slot block starts at its argument):
+// no LDS staging and no copy-out pass at the end of the tile.
+template <class PutG>
+__device__ __forceinline__ void store_samples(bool active, int scol, int nbs, CS_GLB float* samp,
+                                              PutG putg) {
+    if (active && scol >= 0) putg(samp + (int64_t)scol * nbs);
 }
 
 // Words of eight stacked-amplitude rows h0 + e (e < 8) of one LDS plane,
@@ -445,13 +408,12 @@ __device__ __forceinline__ void gather_rows2_k(const CS_LDS uint32_t* re, const 
 }
 
 template <int NB, int WM, int KF = 0>
-__device__ __forceinline__ FoldOut fold_pass_body(const CS_LDS uint32_t* __restrict__ fre,
+__device__ __forceinline__ float fold_pass_body(const CS_LDS uint32_t* __restrict__ fre,
                                                   const CS_LDS uint32_t* __restrict__ fim,
                                                   const CS_LDS u32x4* __restrict__ thi,
                                                   const CS_LDS u32x4* __restrict__ tlo, int k,
                                                   int tcol, uint32_t bankmask, bool active,
-                                                  int scol, CS_GLB float* samp, CS_LDS float* stg,
-                                                  int stg_off) {
+                                                  int scol, CS_GLB float* samp) {
     constexpr int EXT = WM <= 6 ? 5 : WM - 1;
     using G = geo::Geo<NB, EXT>;
     constexpr int H = G::H, MT = G::MT, R = G::R, KS = G::KS, X = G::X;
@@ -490,24 +452,27 @@ __device__ __forceinline__ FoldOut fold_pass_body(const CS_LDS uint32_t* __restr
     }
     // sigma samples: P[n] at slot n (n < H), the Nyquist bin P[H] at H and
     // M[n] at H + n
-    FoldOut out{-INFINITY, stg_off, 0ull};
     {
         const float pH = row_broadcast<(H / R) & 3>(P[H % R]);
-        stage_samples<(NB + 3) / 4 * 4>(
-            out, active, scol, g, col, samp, stg, stg_off,
-            [&](CS_LDS float* d) { put_column<H, R, 3>(d, g, P, M, pH); },
-            [&](CS_GLB float* d) { put_column<H, R, 1>(d, g, P, M, pH); });
+        store_samples(active, scol, (NB + 3) / 4 * 4, samp,
+                      [&](CS_GLB float* d) { put_column<H, R, 1>(d, g, P, M, pH); });
     }
     // boxcar bound
     float m[12];
 #pragma unroll
     for (int a = 0; a < 12; ++a) m[a] = -INFINITY;
     if constexpr (WM <= 6) {
-        float XP[X], XM[X];
-        Fill<G, 0, 0>::run(XP, P, M, g, col);
-        Fill<G, 1, 0>::run(XM, P, M, g, col);
-        WinMax<G, 0, 0>::run(XP, m);
-        WinMax<G, 1, 0>::run(XM, m);
+        // one sequence at a time: its window registers die before the next
+        {
+            float XP[X];
+            Fill<G, 0, 0>::run(XP, P, M, g, col);
+            WinMax<G, 0, 0>::run(XP, m);
+        }
+        {
+            float XM[X];
+            Fill<G, 1, 0>::run(XM, P, M, g, col);
+            WinMax<G, 1, 0>::run(XM, m);
+        }
     } else {
         {
             float XP[X];
@@ -520,8 +485,7 @@ __device__ __forceinline__ FoldOut fold_pass_body(const CS_LDS uint32_t* __restr
             bank_max<G, 1, WM>(XM, m);
         }
     }
-    out.bound = max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
-    return out;
+    return max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
 }
 
 // One sampled column's share of lane group g in the direct layout: registers
@@ -559,13 +523,12 @@ __device__ __forceinline__ void put_run(__attribute__((address_space(ASN))) floa
 // read from the next lane group; sigma samples; the bound
 // max_w max_p boxsum / sqrt(w) over the fold's bank (widths <= WM).
 template <int NB, int WM, int KF = 0>
-__device__ __forceinline__ FoldOut fold_direct_body(const CS_LDS uint32_t* __restrict__ fre,
+__device__ __forceinline__ float fold_direct_body(const CS_LDS uint32_t* __restrict__ fre,
                                                     const CS_LDS uint32_t* __restrict__ fim,
                                                     const CS_LDS u32x4* __restrict__ thi,
                                                     const CS_LDS u32x4* __restrict__ tlo, int k,
                                                     int tcol, uint32_t bankmask, bool active,
-                                                    int scol, CS_GLB float* samp,
-                                                    CS_LDS float* stg, int stg_off) {
+                                                    int scol, CS_GLB float* samp) {
     using D = geo::DGeo<NB, WM>;
     constexpr int H = D::H, RD = D::RD, SH = D::SH, MT = D::MT, NR = D::NR, X = D::X,
                   NF = D::NF, KD = D::KD;
@@ -612,11 +575,8 @@ __device__ __forceinline__ FoldOut fold_direct_body(const CS_LDS uint32_t* __res
         Xs[r] = __shfl(offer, (lane + 16) & 63, 64);
     }
 
-    FoldOut out{-INFINITY, stg_off, 0ull};
-    stage_samples<(NB + 3) / 4 * 4>(
-        out, active, scol, g, col, samp, stg, stg_off,
-        [&](CS_LDS float* d) { put_run<RD, SH, 3>(d, g, Xs); },
-        [&](CS_GLB float* d) { put_run<RD, SH, 1>(d, g, Xs); });
+    store_samples(active, scol, (NB + 3) / 4 * 4, samp,
+                  [&](CS_GLB float* d) { put_run<RD, SH, 1>(d, g, Xs); });
 
     // boxcar bound over the windows at the lane group's RD positions (group 3's
     // last SH repeat group 0's first windows: no mask needed)
@@ -649,31 +609,28 @@ __device__ __forceinline__ FoldOut fold_direct_body(const CS_LDS uint32_t* __res
             for (int r = 0; r < RD; ++r) m[a] = fmaxf(m[a], ps[r + w] - ps[r]);
         }
     }
-    out.bound = max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
-    return out;
+    return max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
 }
 
 // Out-of-line copies of the fold passes, for the kernel that dispatches on
 // the profile length at run time (one call site per length and fold).
 template <int NB, int WM>
-__device__ __noinline__ FoldOut fold_pass(const CS_LDS uint32_t* __restrict__ fre,
+__device__ __noinline__ float fold_pass(const CS_LDS uint32_t* __restrict__ fre,
                                           const CS_LDS uint32_t* __restrict__ fim,
                                           const CS_LDS u32x4* __restrict__ thi,
                                           const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
                                           uint32_t bankmask, bool active, int scol,
-                                          CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
-    return fold_pass_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp, stg,
-                                  stg_off);
+                                          CS_GLB float* samp) {
+    return fold_pass_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
 }
 template <int NB, int WM>
-__device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ fre,
+__device__ __noinline__ float fold_direct(const CS_LDS uint32_t* __restrict__ fre,
                                             const CS_LDS uint32_t* __restrict__ fim,
                                             const CS_LDS u32x4* __restrict__ thi,
                                             const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
                                             uint32_t bankmask, bool active, int scol,
-                                            CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
-    return fold_direct_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                    stg, stg_off);
+                                            CS_GLB float* samp) {
+    return fold_direct_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
 }
 
 // The pass for a fold of NB bins: the direct layout up to 64 bins, the
@@ -681,43 +638,38 @@ __device__ __noinline__ FoldOut fold_direct(const CS_LDS uint32_t* __restrict__ 
 // INL: the pass body inlined into the caller (the kernels specialised on a
 // fold set) rather than called.
 template <int NB, int WM, bool INL, int KF>
-__device__ __forceinline__ FoldOut fold_of(const CS_LDS uint32_t* fre, const CS_LDS uint32_t* fim,
+__device__ __forceinline__ float fold_of(const CS_LDS uint32_t* fre, const CS_LDS uint32_t* fim,
                                            const CS_LDS u32x4* thi, const CS_LDS u32x4* tlo, int k,
                                            int tcol, uint32_t bankmask, bool active, int scol,
-                                           CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+                                           CS_GLB float* samp) {
     if constexpr (geo::c2r_direct(NB)) {
         if constexpr (INL)
             return fold_direct_body<NB, WM, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active,
-                                                scol, samp, stg, stg_off);
+                                                scol, samp);
         else
-            return fold_direct<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                       stg, stg_off);
+            return fold_direct<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
     } else {
         if constexpr (INL)
             return fold_pass_body<NB, WM, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol,
-                                              samp, stg, stg_off);
+                                              samp);
         else
-            return fold_pass<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                     stg, stg_off);
+            return fold_pass<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
     }
 }
 
 template <int NB, bool WIDE, bool INL = false, int KF = 0>
-__device__ __forceinline__ FoldOut run_fold(bool narrow, const CS_LDS uint32_t* fre,
+__device__ __forceinline__ float run_fold(bool narrow, const CS_LDS uint32_t* fre,
                                             const CS_LDS uint32_t* fim, const CS_LDS u32x4* thi,
                                             const CS_LDS u32x4* tlo, int k, int tcol,
                                             uint32_t bankmask, bool active, int scol,
-                                            CS_GLB float* samp, CS_LDS float* stg, int stg_off) {
+                                            CS_GLB float* samp) {
     constexpr int WW = gate_wmax(NB);
     if constexpr (WW <= 6 || !WIDE)
-        return fold_of<NB, 6, INL, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                       stg, stg_off);
+        return fold_of<NB, 6, INL, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
     else if (narrow)
-        return fold_of<NB, 6, INL, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                       stg, stg_off);
+        return fold_of<NB, 6, INL, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
     else
-        return fold_of<NB, WW, INL, 0>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp,
-                                       stg, stg_off);
+        return fold_of<NB, WW, INL, 0>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
 }
 
 // Fold sets with a kernel of their own (profile length of fold f, in order):
@@ -748,6 +700,12 @@ constexpr int set_tiles(int set) {
     for (int f = 0; f < fold_set_size(set); ++f) n += fold_tiles(kFoldSetNB[set][f]);
     return n;
 }
+// first DFT tile of fold f of the set (the tables cs_plan.cpp lays out in fold order)
+constexpr int set_cbase(int set, int f) {
+    int n = 0;
+    for (int i = 0; i < f && i < fold_set_size(set); ++i) n += fold_tiles(kFoldSetNB[set][i]);
+    return n;
+}
 // highest stack row index (in units of the fold factor) a fold pass gathers:
 // 32 per K step of the even/odd layout, 16 per K step of the direct one
 constexpr int fold_hmax(int NB) {
@@ -756,11 +714,11 @@ constexpr int fold_hmax(int NB) {
     return 32 * ((H + 31) / 32);
 }
 // Whether the unclamped gathers of every fold of the set (fold f has factor
-// f + 1) stay below the sample staging area, i.e. read only the two stacked
+// f + 1) stay inside the LDS allocation, i.e. read only the two stacked
 // planes and the DFT tiles behind them, all finite words.
 constexpr bool set_reads_in_tables(int set) {
     const int nh = kFoldSetNB[set][0] / 2;
-    const int stage = 4 * (2 * nh * kFtpLd) + set_tiles(set) * 2 * 64 * 16;
+    const int stage = 4 * (2 * nh * kFtpLd) + set_tiles(set) * 2 * 64 * 16;   // end of the tiles
     for (int f = 0; f < fold_set_size(set); ++f) {
         const int last = 4 * (nh * kFtpLd + (fold_hmax(kFoldSetNB[set][f]) * (f + 1) - 1) * kFtpLd +
                               (kThreads / 64) * 16 - 1);
@@ -791,10 +749,24 @@ __host__ __device__ inline MfmaLds mfma_lds_layout(int nharms, int dft_tiles) {
     MfmaLds L;
     L.ftp = (size_t)2 * nharms * kFtpLd * 4;
     L.dft = (size_t)dft_tiles * 2 * 64 * 16;
-    L.region = (size_t)(kThreads / 64) * kStageFloats * 4;   // per-wave sample staging
+    L.region = 0;   // sampled columns go from registers to HBM
     L.total = L.ftp + L.region + L.dft;
     return L;
 }
+
+// The interpolation operand tables as a by-value kernel argument: their
+// addresses live in the kernel-argument segment, so wherever the compiler
+// re-reads them under scalar-register pressure it does so with scalar loads.
+// Read through the descriptor block they would be vector loads counted with
+// the window prefetch and this wave's stores, whose wait drains all of those.
+struct MfmaOps {
+    const uint4* wA_hi;
+    const uint4* wA_lo;
+    const float2* gA;
+    const int4* hdesc;
+    const uint64_t* rowmask;
+    const int32_t* scol;
+};
 
 // KS: interpolation K steps (1 or 2); WIDE: some fold's bank has widths > 6
 // (kept apart so the narrow kernel does not reserve the wide bounds' registers);
@@ -806,21 +778,23 @@ template <int KS, bool WIDE, bool DUMP, int SET>
 __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev tb, GridDev g,
                                                             const FoldSet* __restrict__ fsd,
                                                             BatchDev b, FusedDev fu,
-                                                            const MfmaDev* __restrict__ mdd) {
+                                                            const MfmaDev* __restrict__ mdd,
+                                                            MfmaOps ops) {
     // fold and table descriptors are read from device memory: indexing a
     // by-value kernel argument with a runtime fold index would copy it to scratch
     const FoldSet& fs = *fsd;
     const MfmaDev& md = *mdd;
     extern __shared__ __align__(16) char smem[];
     const int nh = tb.nharms;
+    // nharms, a compile-time constant in the fold-set kernels
+    const int nhc = SET > 0 ? kFoldSetNB[SET][0] / 2 : nh;
     const int tid = threadIdx.x, lane = tid & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);   // wave-uniform (scalar branches)
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6) & 7;   // wave-uniform (scalar branches)
     uint32_t* fre = (uint32_t*)smem;   // [h][128 trials], (hi | lo << 16)
     uint32_t* fim = fre + nh * kFtpLd;
     const int ntc = md.dft_tiles_max;  // DFT tiles of all folds
     uint4* dhi = (uint4*)(smem + (size_t)2 * nh * kFtpLd * 4);
     uint4* dlo = dhi + (size_t)ntc * 64;
-    const size_t lay_stage = (size_t)2 * nh * kFtpLd * 4 + (size_t)ntc * 2 * 64 * 16;
     for (int i = tid; i < ntc * 64; i += kThreads) {
         dhi[i] = ldg4(md.cA_hi + i);
         dlo[i] = ldg4(md.cA_lo + i);
@@ -865,17 +839,19 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     // stored, so the loads are not ordered behind those stores.  Only the
     // descriptors are waited for here (the window addresses need them); every
     // other load is unconditional, from a clamped index, and its validity is
-    // applied where it is used.  The table bases sit in vector registers: the
-    // loads index them per lane anyway, and a scalar copy would be re-read
-    // from the descriptor block at every use under scalar-register pressure.
+    // applied where it is used.  The table bases are kernel arguments, so
+    // wherever they are re-read under register pressure that is a scalar load
+    // from the argument segment (a reload from scratch would be a vector load,
+    // and its wait would drain the tile's outstanding stores).
     const int32_t* nvb = b.nvalid;
-    const int32_t* scb = md.scol;
-    const uint64_t* rmb = md.rowmask;
-    const int4* hdb = md.hdesc;
+    const int32_t* scb = ops.scol;
+    const uint64_t* rmb = ops.rowmask;
+    const int4* hdb = ops.hdesc;
     const int32_t* slb = b.slot;
     const uint32_t* tlb = b.tiles;
-    asm volatile("" : "+v"(nvb), "+v"(scb), "+v"(rmb), "+v"(hdb), "+v"(slb), "+v"(tlb));
     const int nfv = fs.nf;
+    // every fold's bank mask, lane f holding fold f's
+    const uint32_t bmv = fs.bankmask[lane < nfv ? lane : nfv - 1];
     const uint32_t cg_last = (uint32_t)((g.total - 1 - g.origin) / g.Nprof);
     int64_t st = 0, tbase = 0, occ = 0, t = 0, jj = 0;
     int u = 0;
@@ -891,7 +867,11 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     auto load_desc = [&](int64_t tile) {
         const int uu = (int)((uint32_t)tile % (uint32_t)U);   // tile counts stay below 2^31
         const int s = (lane >> 2) & 7, h = wv + 1 + 8 * s;
-        return glb((const int*)hdb)[((size_t)((h <= nh ? h : nh) - 1) * U + uu) * 4 + (lane & 3)];
+        // a 32-bit element index (the table is far below 2^32 entries), so the
+        // load takes the scalar base plus a per-lane offset
+        const uint32_t idx = ((uint32_t)((h <= nh ? h : nh) - 1) * (uint32_t)U + (uint32_t)uu) * 4u +
+                             (uint32_t)(lane & 3);
+        return glb((const int*)hdb)[idx];
     };
     // tile number at list position k (clamped to the last one: past the
     // range a fetch reloads it), the same address in every lane
@@ -958,42 +938,64 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             wb[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, vo + 16u, 0, 0));
         }
     };
-    if (t_first < t_end) fetch(t_first);
+    // Interpolation operands of one harmonic slot: weight tiles and conj(A)
+    // (L2 resident) run two harmonics ahead in a ring of three.  The first two
+    // first slot of a tile is issued with the tile's fetch, one fold phase
+    // ahead, so the first harmonic does not wait for an L2 round trip.
+    struct Op {
+        uint4 ah, al, q0, q1;
+        int4 d;
+    };
+    Op ring[3];
+    auto issue = [&](int s, Op& x) {
+        // this wave's descriptors of the tile, lane 4s + component, from
+        // the fetch (v_readlane: no scalar load, so no wait that would also
+        // drain this wave's LDS stores)
+        const int4 d = make_int4(__builtin_amdgcn_readlane(dcur, 4 * s),
+                                 __builtin_amdgcn_readlane(dcur, 4 * s + 1),
+                                 __builtin_amdgcn_readlane(dcur, 4 * s + 2),
+                                 __builtin_amdgcn_readlane(dcur, 4 * s + 3));
+        x.d = d;
+        x.ah = ldg4(ops.wA_hi + (size_t)d.z * 64 + lane);
+        x.al = ldg4(ops.wA_lo + (size_t)d.z * 64 + lane);
+        const float2* gp = ops.gA + ((size_t)(d.w & 0x0fffffff) * 2 + c) * 16 + 4 * g4;
+        x.q0 = ldg4(gp);
+        x.q1 = ldg4(gp + 2);
+    };
+    if (t_first < t_end) {
+        fetch(t_first);
+#ifdef CS_RING0
+        issue(0, ring[0]);
+#endif
+    }
 
     for (int64_t tile = t_first; tile < t_end; tile += t_step) {
         // ---------------- interpolation: harmonics wv+1, wv+9, ... of this wave ----------
-        // weight tiles and conj(A) (L2 resident) run two harmonics ahead
-        // the harmonic descriptors (scalar loads) ride in the ring with them
-        struct Op {
-            uint4 ah, al, q0, q1;
-            int4 d;
-        };
-        Op ring[3];
-        auto issue = [&](int s, Op& x) {
-            // this wave's descriptors of the tile, lane 4s + component, from
-            // the fetch (v_readlane: no scalar load, so no wait that would also
-            // drain this wave's LDS stores)
-            const int4 d = make_int4(__builtin_amdgcn_readlane(dcur, 4 * s),
-                                     __builtin_amdgcn_readlane(dcur, 4 * s + 1),
-                                     __builtin_amdgcn_readlane(dcur, 4 * s + 2),
-                                     __builtin_amdgcn_readlane(dcur, 4 * s + 3));
-            x.d = d;
-            x.ah = ldg4(md.wA_hi + (size_t)d.z * 64 + lane);
-            x.al = ldg4(md.wA_lo + (size_t)d.z * 64 + lane);
-            const float2* gp = md.gA + ((size_t)(d.w & 0x0fffffff) * 2 + c) * 16 + 4 * g4;
-            x.q0 = ldg4(gp);
-            x.q1 = ldg4(gp + 2);
-        };
         // two copies of the loop: with the row guard applied, and without it
         // for tiles whose rows all pass (the usual case)
+        // this lane's word in row wv of its plane; re-derived every tile (an
+        // opaque copy) so the compiler keeps one base and adds each slot's
+        // constant row offset instead of holding eight addresses
+        CS_LDS uint32_t* wrow;
+        {
+            uint32_t a = (uint32_t)(uintptr_t)opaque_lds((CS_LDS uint32_t*)fre) +
+                         4u * (uint32_t)((c ? nh * kFtpLd : 0) + wv * kFtpLd + o * 16 + 4 * g4);
+            asm volatile("" : "+v"(a));
+            wrow = (CS_LDS uint32_t*)(uintptr_t)a;
+        }
         auto interp = [&](auto masked) {
+#ifndef CS_RING0
         issue(0, ring[0]);
+#endif
         issue(1, ring[1]);
+#ifdef CS_SCHED0
+        __builtin_amdgcn_sched_barrier(0);
+#endif
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int h = wv + 1 + 8 * s;
             if (s + 2 < 8) issue(s + 2, ring[(s + 2) % 3]);
-            if (h > nh) continue;
+            if (h > nhc) continue;
             const Op& x = ring[s % 3];
             bf16x8 bh, bl;
             {
@@ -1013,7 +1015,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                     bf16x8 ch, cl;
                     unpack8(w, ch, cl);
                     const size_t ti = ((size_t)x.d.z + 1) * 64 + lane;
-                    acc = mfma3(as_bf16x8(ldg4(md.wA_hi + ti)), as_bf16x8(ldg4(md.wA_lo + ti)), ch, cl,
+                    acc = mfma3(as_bf16x8(ldg4(ops.wA_hi + ti)), as_bf16x8(ldg4(ops.wA_lo + ti)), ch, cl,
                                 acc);
                 }
             }
@@ -1033,9 +1035,10 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                     val = ((rmv[v] >> (h - 1)) & 1ull) ? val : 0.0f;
                 ow[v] = split_word(val);
             }
-            uint32_t* dst = (c ? fim : fre) + (h - 1) * kFtpLd + o * 16 + 4 * g4;
-            *(uint2*)dst = make_uint2(ow[0], ow[1]);
-            *(uint2*)(dst + 2) = make_uint2(ow[2], ow[3]);
+            CS_LDS uint32_t* dst = wrow + 8 * s * kFtpLd;   // row h - 1
+            typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+            *(CS_LDS u32x2*)dst = u32x2{ow[0], ow[1]};
+            *(CS_LDS u32x2*)(dst + 2) = u32x2{ow[2], ow[3]};
         }
         };
         // every harmonic row of the tile passes its guard (wave-uniform); the
@@ -1066,12 +1069,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const int nf = fs.nf;
             const CS_LDS uint32_t* lre = opaque_lds((CS_LDS uint32_t*)fre);
             const CS_LDS uint32_t* lim = opaque_lds((CS_LDS uint32_t*)fim);
-            CS_LDS float* lstg = opaque_lds((CS_LDS float*)(smem + lay_stage) + wv * kStageFloats);
-            // where each fold's staged columns sit and go: lane f of four
-            // vector registers holds fold f's {begin, end, destination}
-            uint32_t seg_beg = 0, seg_end = 0, seg_dlo = 0, seg_dhi = 0;
-            float bnd[kMaxFolds];
-            int cursor = 0;
             // the fold descriptors through pointers the compiler cannot follow
             // across tiles: their fields are re-read (scalar cache) where used
             // instead of being held in scalar registers for the whole kernel
@@ -1081,6 +1078,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const int clv = slotv - b.slot0;
             const bool inb = tin && (uint32_t)clv < (uint32_t)b.nblocks;
             const int cl = inb ? clv : 0;
+            // bound column: [local block][trial of the block]
+            const int64_t col_out = (int64_t)cl * np + jj;
             // fold f of NB bins (NBC: the length as a compile-time constant,
             // or 0 to dispatch on the descriptor's length at run time)
             auto fold_step = [&](const int f, auto nbc, auto kfc) {
@@ -1095,21 +1094,24 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 if (act && nv != np) sc = sample_column(jj, nv, NB, fsx->cap[f]);
                 CS_GLB float* gsamp =
                     (CS_GLB float*)(fu.samples + ((int64_t)cl * nf + f) * fu.sld);
-                const CS_LDS u32x4* th = opaque_lds((CS_LDS u32x4*)(dhi + (size_t)mdx->c_base[f] * 64));
-                const CS_LDS u32x4* tl = opaque_lds((CS_LDS u32x4*)(dlo + (size_t)mdx->c_base[f] * 64));
-                const int k = fsx->f[f].k;
-                const uint32_t bm = fsx->bankmask[f];
+                // fold factor and first DFT tile: compile-time constants in the
+                // fold-set kernels (the launcher checked both), else read from
+                // the descriptors; the bank mask from lane f of a register
+                const int cb = SET > 0 ? set_cbase(SET, f) : mdx->c_base[f];
+                const CS_LDS u32x4* th = opaque_lds((CS_LDS u32x4*)(dhi + (size_t)cb * 64));
+                const CS_LDS u32x4* tl = opaque_lds((CS_LDS u32x4*)(dlo + (size_t)cb * 64));
+                const int k = SET > 0 ? f + 1 : fsx->f[f].k;
+                const uint32_t bm = (uint32_t)__builtin_amdgcn_readlane((int)bmv, f);
                 const bool narrow = (bm >> 5) == 0;   // bank within widths 1, 2, 3, 4, 6
-                FoldOut fo{-INFINITY, cursor, 0ull};
+                float bound = -INFINITY;
                 if constexpr (NBC > 0) {
-                    fo = run_fold<NBC, WIDE, true, KF>(narrow, lre, lim, th, tl, k, tcol, bm, act,
-                                                       sc, gsamp, lstg, cursor);
+                    bound = run_fold<NBC, WIDE, true, KF>(narrow, lre, lim, th, tl, k, tcol, bm,
+                                                          act, sc, gsamp);
                 } else {
                     switch (NB) {
-#define X(SZ)                                                                                 \
-    case SZ:                                                                                  \
-        fo = run_fold<SZ, WIDE>(narrow, lre, lim, th, tl, k, tcol, bm, act, sc, gsamp, lstg, \
-                                cursor);                                                      \
+#define X(SZ)                                                                            \
+    case SZ:                                                                             \
+        bound = run_fold<SZ, WIDE>(narrow, lre, lim, th, tl, k, tcol, bm, act, sc, gsamp); \
         break;
                         CS_MFMA_SIZES(X)
 #undef X
@@ -1117,17 +1119,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                             break;
                     }
                 }
-                bnd[f] = act ? fo.bound : -INFINITY;
-                const int end = __builtin_amdgcn_readfirstlane(fo.stg_end);
-                const bool mine = lane == f;
-                seg_beg = mine ? (uint32_t)cursor : seg_beg;
-                seg_end = mine ? (uint32_t)end : seg_end;
-                seg_dlo = mine ? (uint32_t)fo.dst : seg_dlo;
-                seg_dhi = mine ? (uint32_t)(fo.dst >> 32) : seg_dhi;
-                cursor = end;
+                // the fold's bound of the trial, stored as soon as it is known
+                if (inb && g4 == 0) fu.mraw[(int64_t)f * fu.ldm + col_out] = act ? bound : -INFINITY;
             };
-#pragma unroll
-            for (int f = 0; f < kMaxFolds; ++f) bnd[f] = -INFINITY;
             if constexpr (SET == 0) {
 #pragma unroll
                 for (int f = 0; f < kMaxFolds; ++f)
@@ -1141,32 +1135,16 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                               std::integral_constant<int, set_reads_in_tables(SET) ? F + 1 : 0>{});
                 });
             }
-            // next tile's loads go out before this tile's stores
-            // bound column: [local block][trial of the block]
-            const int64_t col_out = (int64_t)cl * np + jj;
-            if (tile + t_step < t_end) fetch(tile + t_step);
+            // the next tile's loads
+            if (tile + t_step < t_end) {
+                fetch(tile + t_step);
+#ifdef CS_RING0
+                issue(0, ring[0]);
+#endif
+            }
             // every wave is past its folds: the next tile's interpolation may
-            // overwrite the stacked amplitudes.  The stores below touch only
-            // this wave's staging area and HBM, so they overlap the other
-            // waves' next interpolation instead of holding them at a barrier.
+            // overwrite the stacked amplitudes
             lds_barrier();
-            // outputs of the tile: bounds, then the staged sigma samples
-            if (inb && g4 == 0) {
-#pragma unroll
-                for (int f = 0; f < kMaxFolds; ++f)
-                    if (f < nf) fu.mraw[(int64_t)f * fu.ldm + col_out] = bnd[f];
-            }
-            for (int f = 0; f < nf; ++f) {
-                const uint64_t dsta =
-                    ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)seg_dhi, f) << 32) |
-                    (uint32_t)__builtin_amdgcn_readlane((int)seg_dlo, f);
-                if (dsta == 0ull) continue;
-                const int beg = __builtin_amdgcn_readlane((int)seg_beg, f);
-                const int nq = (__builtin_amdgcn_readlane((int)seg_end, f) - beg) / 4;
-                CS_GLB f32x4* dst = (CS_GLB f32x4*)(uintptr_t)dsta;
-                const CS_LDS f32x4* src = (const CS_LDS f32x4*)(lstg + beg);
-                for (int q = lane; q < nq; q += 64) dst[q] = src[q];
-            }
         }
     }
 }
@@ -1280,7 +1258,8 @@ static int fold_set_of(const FoldSet& fs, const MfmaDev& md, int nharms) {
         bool same = fs.nf == fold_set_size(set) && nharms == kFoldSetNB[set][0] / 2 &&
                     md.dft_tiles_max == set_tiles(set);
         for (int f = 0; f < fs.nf && same; ++f)
-            same = fs.f[f].nbins == kFoldSetNB[set][f] && fs.f[f].k == f + 1;
+            same = fs.f[f].nbins == kFoldSetNB[set][f] && fs.f[f].k == f + 1 &&
+                   md.c_base[f] == set_cbase(set, f);
         if (same) return set;
     }
     return 0;
@@ -1298,8 +1277,9 @@ hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb
         hipError_t e = hipFuncSetAttribute((const void*)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
+        const MfmaOps ops{md.wA_hi, md.wA_lo, md.gA, md.hdesc, md.rowmask, md.scol};
         hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(kThreads), shm, st, sp, tb, g, fs_dev, b,
-                           fu, md_dev);
+                           fu, md_dev, ops);
         return hipGetLastError();
     };
     if (fu.dump) {
