@@ -939,9 +939,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         }
     };
     // Interpolation operands of one harmonic slot: weight tiles and conj(A)
-    // (L2 resident) run two harmonics ahead in a ring of three.  The first two
-    // first slot of a tile is issued with the tile's fetch, one fold phase
-    // ahead, so the first harmonic does not wait for an L2 round trip.
+    // (L2 resident) run two harmonics ahead in a ring of three.
     struct Op {
         uint4 ah, al, q0, q1;
         int4 d;
@@ -964,9 +962,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     };
     if (t_first < t_end) {
         fetch(t_first);
-#ifdef CS_RING0
-        issue(0, ring[0]);
-#endif
     }
 
     for (int64_t tile = t_first; tile < t_end; tile += t_step) {
@@ -984,13 +979,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             wrow = (CS_LDS uint32_t*)(uintptr_t)a;
         }
         auto interp = [&](auto masked) {
-#ifndef CS_RING0
         issue(0, ring[0]);
-#endif
         issue(1, ring[1]);
-#ifdef CS_SCHED0
-        __builtin_amdgcn_sched_barrier(0);
-#endif
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int h = wv + 1 + 8 * s;
@@ -1138,9 +1128,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             // the next tile's loads
             if (tile + t_step < t_end) {
                 fetch(tile + t_step);
-#ifdef CS_RING0
-                issue(0, ring[0]);
-#endif
             }
             // every wave is past its folds: the next tile's interpolation may
             // overwrite the stacked amplitudes
