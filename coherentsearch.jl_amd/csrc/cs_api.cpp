@@ -86,7 +86,7 @@ struct cs_ctx {
         std::vector<uint32_t> cnt;           // [fold][window][bin]
     } sh;
     DBuf pyr, tfeat, mlp_w, mlp_x, mlp_p;   // block triage: power pyramid, features, MLP buffers
-    DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol;
+    DBuf m_whi, m_wlo, m_gA, m_hdesc, m_chi, m_clo, m_desc, m_rowmask, m_scol, m_nv16;
     std::vector<uint8_t> m_desc_last;   // host copy of what m_desc holds
     // split-BF16 batches: identity list (u32 i at i, m_ident_n entries), and
     // for a block subset its block list, slot table and per-batch tile lists
@@ -403,7 +403,8 @@ int cs_ctx_destroy(cs_ctx* c) {
                     &c->st_metric, &c->st_cnt, &c->st_blk, &c->st_win, &c->st_norm,
                     &c->pyr, &c->tfeat, &c->mlp_w, &c->mlp_x, &c->mlp_p,
                     &c->m_whi, &c->m_wlo, &c->m_gA, &c->m_hdesc, &c->m_chi, &c->m_clo,
-                    &c->m_desc, &c->m_rowmask, &c->m_scol, &c->m_ident, &c->m_sel, &c->m_slot,
+                    &c->m_desc, &c->m_rowmask, &c->m_scol, &c->m_nv16, &c->m_ident, &c->m_sel,
+                    &c->m_slot,
                     &c->m_tiles, &c->spl, &c->rflags, &c->rlist, &c->rsamp, &c->probe_buf, &c->sweep_amps,
                     &c->sweep_cands, &c->sweep_ctr};
     for (DBuf* b : bufs) release(*b);
@@ -1323,14 +1324,14 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
                 mc.c_mt[f] = mt.c_mt[f];
                 mc.c_ks[f] = mt.c_ks[f];
             }
-            {   // sampled columns of a full block, per fold
-                std::vector<int32_t> sc((size_t)nf * Nprof);
+            {   // sampled columns of a full block: eight int16 per block trial
+                std::vector<int16_t> sc((size_t)Nprof * 8, (int16_t)-1);
                 for (int f = 0; f < nf; ++f)
                     for (int64_t j = 0; j < Nprof; ++j)
-                        sc[(size_t)f * Nprof + j] =
-                            mfma_sample_column(j, Nprof, fs.f[f].nbins, fs.cap[f]);
+                        sc[(size_t)j * 8 + f] =
+                            (int16_t)mfma_sample_column(j, Nprof, fs.f[f].nbins, fs.cap[f]);
                 if ((s = upload(c, c->m_scol, sc))) return s;
-                mc.scol = (const int32_t*)c->m_scol.p;
+                mc.scol = (const uint4*)c->m_scol.p;
             }
             c->mfma_rlo = r_lo;
             c->mfma_nbs = key;
@@ -1341,6 +1342,8 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             return c->fail(CS_EINVAL, "more than 2^32 trials in one search");
         if (c->namps + 2 * kSpecPad >= ((int64_t)1 << 28))
             return c->fail(CS_EINVAL, "fused MFMA mode: spectra are limited to 2^28 amplitudes");
+        if (Nprof > 65535 || Nprof < 16)
+            return c->fail(CS_EINVAL, "fused MFMA mode: blocks of 16 to 65535 trials");
         if ((s = ensure(c, c->m_rowmask, (size_t)nchunks * sizeof(uint64_t)))) return s;
         const int64_t spl_ld = c->namps + 2 * kSpecPad;
         if (!c->spl_valid) {
@@ -1361,11 +1364,16 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         // unless tuned
         mfma_nwg = c->tune_wg > 0 ? c->tune_wg : (c->ncu > 0 ? c->ncu : 256);
     }
-    const int64_t Bc =
+    int64_t Bc =
         mfma ? std::min<int64_t>(nchunks, c->tune_batch > 0 ? c->tune_batch : kBatchBlocksMfma)
              : std::max<int64_t>(1, std::min<int64_t>(
                    nchunks, c->tune_batch > 0 ? c->tune_batch
                                               : (fused ? kBatchTrialsFused : kBatchTrials) / Nprof));
+    if (mfma) {   // the split-BF16 kernel's 32-bit store offsets: samples and a fold's bounds
+        const int64_t cap_s = (int64_t)kMfmaNoStore / ((int64_t)nf * kSigmaSamplesPadded * 4);
+        const int64_t cap_m = (int64_t)kMfmaNoStore / ((int64_t)Nprof * 4);
+        Bc = std::max<int64_t>(1, std::min(Bc, std::min(cap_s, cap_m)));
+    }
     const int64_t BT = Bc * Nprof;
     fs.ldp = BT;
 
@@ -1382,6 +1390,12 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
                 nv[cc * nf + f] = (int32_t)nvalid_of(ks[f], rstart, lodr, n, nyq);
         }
         if ((s = upload(c, c->nvalid, nv))) return s;
+        // the same counts as eight uint16 per block for the split-BF16 kernel
+        // (one 16-byte load per trial; blocks of at most 65535 trials there)
+        std::vector<uint16_t> nv16((size_t)nchunks * 8, 0);
+        for (int64_t cc = 0; cc < nchunks; ++cc)
+            for (int f = 0; f < nf; ++f) nv16[cc * 8 + f] = (uint16_t)std::min<int32_t>(nv[cc * nf + f], 65535);
+        if ((s = upload(c, c->m_nv16, nv16))) return s;
         c->nv_rlo = r_lo;
         c->nv_total = total;
         c->nv_N = c->N;
@@ -1570,6 +1584,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
             HIPCHK(c, launch_slot_table(c->st, (const int32_t*)c->m_sel.p, (int64_t)selv.size(),
                                         nchunks, (int32_t*)c->m_slot.p));
         }
+        md.nv16 = (const uint4*)c->m_nv16.p;
         std::vector<uint8_t> desc(sizeof(FoldSet) + sizeof(MfmaDev));
         std::memcpy(desc.data(), &fs, sizeof(FoldSet));
         std::memcpy(desc.data() + sizeof(FoldSet), &md, sizeof(MfmaDev));

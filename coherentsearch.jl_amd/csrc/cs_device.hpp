@@ -152,6 +152,17 @@ __host__ __device__ inline int sample_column(int64_t j, int64_t nv, int nbins, i
     const int64_t c = (j * ncol + nv - 1) / nv;
     return (c < ncol && (c * nv) / ncol == j) ? (int)c : -1;
 }
+// The same in 32-bit arithmetic, exact while nv * cap / nbins < 2^32 (block
+// sizes below 2^19 with cap 8192; the split-BF16 path checks this on the host).
+__host__ __device__ inline int sample_column32(uint32_t j, uint32_t nv, uint32_t nbins,
+                                               uint32_t cap) {
+    if (nbins * nv <= cap) return (int)j;
+    uint32_t ncol = cap / nbins;
+    if (ncol < 1) ncol = 1;
+    if (ncol > nv) ncol = nv;
+    const uint32_t c = (j * ncol + nv - 1) / nv;
+    return (c < ncol && (c * nv) / ncol == j) ? (int)c : -1;
+}
 
 __device__ __forceinline__ void block_of(const GridDev& g, int64_t t, int64_t& cg,
                                          int64_t& first, int64_t& last) {

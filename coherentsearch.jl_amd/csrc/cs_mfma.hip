@@ -181,6 +181,20 @@ __device__ __forceinline__ int4 ld_const4(const int4* p, size_t i) {
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
+// Half-word f of a 16-byte value: unsigned, signed, and replaced by the low
+// half of v (f a compile-time constant where these are used)
+__device__ __forceinline__ uint32_t half_u(const uint4& q, int f) {
+    const uint32_t w = f < 2 ? q.x : f < 4 ? q.y : f < 6 ? q.z : q.w;
+    return (f & 1) ? w >> 16 : w & 0xFFFFu;
+}
+__device__ __forceinline__ int half_s(const uint4& q, int f) {
+    return (int)(int16_t)(uint16_t)half_u(q, f);
+}
+__device__ __forceinline__ void set_half(uint4& q, int f, uint32_t v) {
+    uint32_t& w = f < 2 ? q.x : f < 4 ? q.y : f < 6 ? q.z : q.w;
+    w = (f & 1) ? ((w & 0x0000FFFFu) | (v << 16)) : ((w & 0xFFFF0000u) | (v & 0xFFFFu));
+}
+
 // Row stride (words) of the stacked-amplitude planes.  A c2r operand read
 // touches rows 8k apart in the four lane groups of a column; with 130-word
 // rows those land 16k banks apart, distinct for k = 1, 2, 3, 5, 6.
@@ -243,26 +257,27 @@ struct Fill {
     }
 };
 
-// per-width maxima (widths 1, 2, 3, 4, 6) of the windows at every position of
-// this lane's sequence (each a real boxcar, cs_geo.hpp): running pair sums
-template <class G, int W, int I>
-struct WinMax {
-    static __device__ __forceinline__ void run(const float (&Xs)[G::X], float (&m)[12]) {
-        if constexpr (I < G::R) {
-            const float s1 = Xs[I];
-            const float s2 = Xs[I] + Xs[I + 1];
-            const float s3 = s2 + Xs[I + 2];
-            const float s4 = s2 + (Xs[I + 2] + Xs[I + 3]);
-            const float s6 = s4 + (Xs[I + 4] + Xs[I + 5]);
-            m[0] = fmaxf(m[0], s1);
-            m[1] = fmaxf(m[1], s2);
-            m[2] = fmaxf(m[2], s3);
-            m[3] = fmaxf(m[3], s4);
-            m[4] = fmaxf(m[4], s6);
-            WinMax<G, W, I + 1>::run(Xs, m);
+// per-width maxima (widths 1, 2, 3, 4, 6) of the windows at the R positions
+// from I of a sequence Xs (each a real boxcar, cs_geo.hpp): running pair
+// sums.  BM: the fold's bank as a compile-time mask (bit a: width bank_w(a)),
+// whose absent widths are neither summed nor maximised; 0 when the bank is
+// only known at run time (all five computed).
+template <uint32_t BM, int N, int I, int R>
+__device__ __forceinline__ void win_max(const float (&Xs)[N], float (&m)[12]) {
+    constexpr bool all = BM == 0;
+#pragma unroll
+    for (int i = I; i < I + R; ++i) {
+        const float s2 = Xs[i] + Xs[i + 1];
+        if constexpr (all || (BM & 1u)) m[0] = fmaxf(m[0], Xs[i]);
+        if constexpr (all || (BM & 2u)) m[1] = fmaxf(m[1], s2);
+        if constexpr (all || (BM & 4u)) m[2] = fmaxf(m[2], s2 + Xs[i + 2]);
+        if constexpr (all || (BM & 8u) || (BM & 16u)) {
+            const float s4 = s2 + (Xs[i + 2] + Xs[i + 3]);
+            if constexpr (all || (BM & 8u)) m[3] = fmaxf(m[3], s4);
+            if constexpr (all || (BM & 16u)) m[4] = fmaxf(m[4], s4 + (Xs[i + 4] + Xs[i + 5]));
         }
     }
-};
+}
 
 // per-width maxima over the bank widths <= WM: prefix-sum differences
 template <class G, int W, int WM>
@@ -280,10 +295,19 @@ __device__ __forceinline__ void bank_max(const float (&Xs)[G::X], float (&m)[12]
     }
 }
 
-// max over the fold's bank of (largest window sum of width w) / sqrt(w).  The
-// bank mask is wave-uniform: read into a scalar register, its bits choose the
-// widths by scalar branches, and the usual bank (widths 1, 2, 3, 4, 6 all
-// present) takes one straight max chain with no per-width select.
+// max over the fold's bank of (largest window sum of width w) / sqrt(w).  With
+// a compile-time bank (BM != 0) one straight max chain over its widths; else
+// the mask is wave-uniform: read into a scalar register, its bits choose the
+// widths by scalar branches, and the bank of widths 1, 2, 3, 4, 6 takes one
+// straight chain with no per-width select.
+template <uint32_t BM>
+__device__ __forceinline__ float bank_bound_ct(const float (&m)[12]) {
+    float res = -INFINITY;
+#pragma unroll
+    for (int a = 0; a < 5; ++a)
+        if ((BM >> a) & 1u) res = fmaxf(res, m[a] * (1.0f / sqrtf((float)bank_w(a))));
+    return res;
+}
 template <int NA>
 __device__ __forceinline__ float bank_bound(const float (&m)[12], uint32_t bankmask) {
     const uint32_t bm = __builtin_amdgcn_readfirstlane(bankmask);
@@ -299,51 +323,64 @@ __device__ __forceinline__ float bank_bound(const float (&m)[12], uint32_t bankm
     return res;
 }
 
+// Stores through a buffer resource with a per-lane byte offset.  An offset at
+// or past the resource's size drops the store, so a lane that has nothing to
+// store passes kNoStore (plus the chunk's own small offset, which stays below
+// 2^32) and no branch is needed: the folds stay one straight-line region that
+// the compiler can interleave.  The buffers these resources cover are kept
+// below kNoStore bytes by the host.
+constexpr uint32_t kNoStore = kMfmaNoStore;
+typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
+template <int CH>
+__device__ __forceinline__ void buf_store(const float* v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    if constexpr (CH == 4) {
+        const u32x4 w = {__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2]),
+                         __float_as_uint(v[3])};
+        __builtin_amdgcn_raw_buffer_store_b128(w, r, off, 0, 0);
+    } else if constexpr (CH == 3) {
+        const u32x3 w = {__float_as_uint(v[0]), __float_as_uint(v[1]), __float_as_uint(v[2])};
+        __builtin_amdgcn_raw_buffer_store_b96(w, r, off, 0, 0);
+    } else if constexpr (CH == 2) {
+        typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+        const u32x2 w = {__float_as_uint(v[0]), __float_as_uint(v[1])};
+        __builtin_amdgcn_raw_buffer_store_b64(w, r, off, 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v[0]), r, off, 0, 0);
+    }
+}
+
+// Where a lane's sampled column goes: the batch's sample regions and the
+// byte offset of the column's slot block, kNoStore when the lane's trial is
+// not a sampled column of the fold (sigma samples, _block_sigma,
+// search.jl:504-538: the whole profile of a sampled column goes to its NB
+// slots, columns NBS slots apart; the median only sees the multiset, so the
+// order of a column's slots is free).
+struct ColumnSink {
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t off;
+};
+
 // One sampled column's share of this lane group (runs n = g*R .. g*R + R - 1
-// of P and M) into a column slot block: P[n] at n, M[n] at H + n, P[H] in
-// place of M[0]; whole CH-word chunks.
-template <int H, int R, int ASN>
-__device__ __forceinline__ void put_column(__attribute__((address_space(ASN))) float* base, int g,
-                                           const float (&P)[R], const float (&M)[R], float pH) {
-    typedef float f2 __attribute__((ext_vector_type(2)));
-    typedef __attribute__((address_space(ASN))) f32x4 V4;
-    typedef __attribute__((address_space(ASN))) f2 V2;
+// of P and M) into its slot block: P[n] at n, M[n] at H + n, P[H] in place of
+// M[0]; whole CH-word chunks, the chunks past H dropped.
+template <int H, int R>
+__device__ __forceinline__ void put_column(const ColumnSink& sk, int g, const float (&P)[R],
+                                           const float (&M)[R], float pH) {
     constexpr int CH = (H % 4 == 0) ? 4 : (H % 2 == 0) ? 2 : 1;
 #pragma unroll
     for (int c0 = 0; c0 < R; c0 += CH) {
         const int n0 = g * R + c0;
-        if (n0 + CH <= H) {
-            float pv[CH], mv[CH];
+        const uint32_t off = n0 + CH <= H ? sk.off : kNoStore;
+        float pv[CH], mv[CH];
 #pragma unroll
-            for (int e = 0; e < CH; ++e) {
-                pv[e] = fabsf(P[c0 + e]);
-                mv[e] = fabsf(M[c0 + e]);
-            }
-            if (c0 == 0) mv[0] = n0 == 0 ? fabsf(pH) : mv[0];
-            if constexpr (CH == 4) {
-                *(V4*)(base + n0) = f32x4{pv[0], pv[1], pv[2], pv[3]};
-                *(V4*)(base + H + n0) = f32x4{mv[0], mv[1], mv[2], mv[3]};
-            } else if constexpr (CH == 2) {
-                *(V2*)(base + n0) = f2{pv[0], pv[1]};
-                *(V2*)(base + H + n0) = f2{mv[0], mv[1]};
-            } else {
-                base[n0] = pv[0];
-                base[H + n0] = mv[0];
-            }
+        for (int e = 0; e < CH; ++e) {
+            pv[e] = fabsf(P[c0 + e]);
+            mv[e] = fabsf(M[c0 + e]);
         }
+        if (c0 == 0) mv[0] = n0 == 0 ? fabsf(pH) : mv[0];
+        buf_store<CH>(pv, sk.rsrc, off + 4u * (uint32_t)n0);
+        buf_store<CH>(mv, sk.rsrc, off + 4u * (uint32_t)(H + n0));
     }
-}
-
-// sigma samples (_block_sigma, search.jl:504-538): the whole profile of a
-// sampled column goes to its NB slots; columns start every NBS slots.  The
-// median only sees the multiset, so the order of a column's slots is free.
-// Each lane group stores its share of the column straight from registers
-// (putg writes it to the column whose slot block starts at its argument):
-// no LDS staging and no copy-out pass at the end of the tile.
-template <class PutG>
-__device__ __forceinline__ void store_samples(bool active, int scol, int nbs, CS_GLB float* samp,
-                                              PutG putg) {
-    if (active && scol >= 0) putg(samp + (int64_t)scol * nbs);
 }
 
 // Words of eight stacked-amplitude rows h0 + e (e < 8) of one LDS plane,
@@ -407,13 +444,13 @@ __device__ __forceinline__ void gather_rows2_k(const CS_LDS uint32_t* re, const 
     }
 }
 
-template <int NB, int WM, int KF = 0>
+// BM: the fold's bank as a compile-time mask (0: read from bankmask at run time)
+template <int NB, int WM, int KF = 0, uint32_t BM = 0>
 __device__ __forceinline__ float fold_pass_body(const CS_LDS uint32_t* __restrict__ fre,
-                                                  const CS_LDS uint32_t* __restrict__ fim,
-                                                  const CS_LDS u32x4* __restrict__ thi,
-                                                  const CS_LDS u32x4* __restrict__ tlo, int k,
-                                                  int tcol, uint32_t bankmask, bool active,
-                                                  int scol, CS_GLB float* samp) {
+                                                const CS_LDS uint32_t* __restrict__ fim,
+                                                const CS_LDS u32x4* __restrict__ thi,
+                                                const CS_LDS u32x4* __restrict__ tlo, int k,
+                                                int tcol, uint32_t bankmask, ColumnSink sk) {
     constexpr int EXT = WM <= 6 ? 5 : WM - 1;
     using G = geo::Geo<NB, EXT>;
     constexpr int H = G::H, MT = G::MT, R = G::R, KS = G::KS, X = G::X;
@@ -454,8 +491,7 @@ __device__ __forceinline__ float fold_pass_body(const CS_LDS uint32_t* __restric
     // M[n] at H + n
     {
         const float pH = row_broadcast<(H / R) & 3>(P[H % R]);
-        store_samples(active, scol, (NB + 3) / 4 * 4, samp,
-                      [&](CS_GLB float* d) { put_column<H, R, 1>(d, g, P, M, pH); });
+        put_column<H, R>(sk, g, P, M, pH);
     }
     // boxcar bound
     float m[12];
@@ -466,12 +502,12 @@ __device__ __forceinline__ float fold_pass_body(const CS_LDS uint32_t* __restric
         {
             float XP[X];
             Fill<G, 0, 0>::run(XP, P, M, g, col);
-            WinMax<G, 0, 0>::run(XP, m);
+            win_max<BM, X, 0, R>(XP, m);
         }
         {
             float XM[X];
             Fill<G, 1, 0>::run(XM, P, M, g, col);
-            WinMax<G, 1, 0>::run(XM, m);
+            win_max<BM, X, 0, R>(XM, m);
         }
     } else {
         {
@@ -485,35 +521,27 @@ __device__ __forceinline__ float fold_pass_body(const CS_LDS uint32_t* __restric
             bank_max<G, 1, WM>(XM, m);
         }
     }
-    return max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
+    if constexpr (BM != 0) return max_over_rows(bank_bound_ct<BM>(m));
+    else return max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
 }
 
 // One sampled column's share of lane group g in the direct layout: registers
-// X[r] = bin g*RD + r for the starts the group owns, at slots g*RD + r.
-template <int RD, int SH, int ASN>
-__device__ __forceinline__ void put_run(__attribute__((address_space(ASN))) float* base, int g,
-                                        const float* Xs) {
-    typedef float f2 __attribute__((ext_vector_type(2)));
-    typedef __attribute__((address_space(ASN))) f32x4 V4;
-    typedef __attribute__((address_space(ASN))) f2 V2;
-    constexpr int CH = (RD % 4 == 0) ? 4 : (RD % 2 == 0) ? 2 : 1;
-    constexpr int OWN3 = RD - SH;   // starts of group 3
-    base += g * RD;
+// X[r] = bin g*RD + r at slots g*RD + r, in 4-word chunks (4-byte aligned) and
+// a shorter tail.  Group 3's last SH registers repeat group 0's first bins;
+// they land in the column's padding slots NB .. round_up(NB, 4) - 1, which
+// the median never reads.
+template <int RD>
+__device__ __forceinline__ void put_run(const ColumnSink& sk, int g, const float* Xs) {
+    const uint32_t base = sk.off + 4u * (uint32_t)(g * RD);
 #pragma unroll
-    for (int c0 = 0; c0 < RD; c0 += CH) {
-        float v[CH];
+    for (int c0 = 0; c0 < RD; c0 += 4) {
+        float v[4];
 #pragma unroll
-        for (int e = 0; e < CH; ++e) v[e] = fabsf(Xs[c0 + e]);
-        const bool whole = c0 + CH <= OWN3 || g != 3;
-        if (whole) {
-            if constexpr (CH == 4) *(V4*)(base + c0) = f32x4{v[0], v[1], v[2], v[3]};
-            else if constexpr (CH == 2) *(V2*)(base + c0) = f2{v[0], v[1]};
-            else base[c0] = v[0];
-        } else {
-#pragma unroll
-            for (int e = 0; e < CH; ++e)
-                if (c0 + e < OWN3) base[c0 + e] = v[e];
-        }
+        for (int e = 0; e < 4; ++e) v[e] = c0 + e < RD ? fabsf(Xs[c0 + e]) : 0.0f;
+        if (c0 + 4 <= RD) buf_store<4>(v, sk.rsrc, base + 4u * c0);
+        else if (RD - c0 == 3) buf_store<3>(v, sk.rsrc, base + 4u * c0);
+        else if (RD - c0 == 2) buf_store<2>(v, sk.rsrc, base + 4u * c0);
+        else buf_store<1>(v, sk.rsrc, base + 4u * c0);
     }
 }
 
@@ -522,13 +550,12 @@ __device__ __forceinline__ void put_run(__attribute__((address_space(ASN))) floa
 // bins g*RD .. g*RD + 4*MT - 1 (mod NB), the rest of its window extension
 // read from the next lane group; sigma samples; the bound
 // max_w max_p boxsum / sqrt(w) over the fold's bank (widths <= WM).
-template <int NB, int WM, int KF = 0>
+template <int NB, int WM, int KF = 0, uint32_t BM = 0>
 __device__ __forceinline__ float fold_direct_body(const CS_LDS uint32_t* __restrict__ fre,
-                                                    const CS_LDS uint32_t* __restrict__ fim,
-                                                    const CS_LDS u32x4* __restrict__ thi,
-                                                    const CS_LDS u32x4* __restrict__ tlo, int k,
-                                                    int tcol, uint32_t bankmask, bool active,
-                                                    int scol, CS_GLB float* samp) {
+                                                  const CS_LDS uint32_t* __restrict__ fim,
+                                                  const CS_LDS u32x4* __restrict__ thi,
+                                                  const CS_LDS u32x4* __restrict__ tlo, int k,
+                                                  int tcol, uint32_t bankmask, ColumnSink sk) {
     using D = geo::DGeo<NB, WM>;
     constexpr int H = D::H, RD = D::RD, SH = D::SH, MT = D::MT, NR = D::NR, X = D::X,
                   NF = D::NF, KD = D::KD;
@@ -575,8 +602,7 @@ __device__ __forceinline__ float fold_direct_body(const CS_LDS uint32_t* __restr
         Xs[r] = __shfl(offer, (lane + 16) & 63, 64);
     }
 
-    store_samples(active, scol, (NB + 3) / 4 * 4, samp,
-                  [&](CS_GLB float* d) { put_run<RD, SH, 1>(d, g, Xs); });
+    put_run<RD>(sk, g, Xs);
 
     // boxcar bound over the windows at the lane group's RD positions (group 3's
     // last SH repeat group 0's first windows: no mask needed)
@@ -584,19 +610,7 @@ __device__ __forceinline__ float fold_direct_body(const CS_LDS uint32_t* __restr
 #pragma unroll
     for (int a = 0; a < 12; ++a) m[a] = -INFINITY;
     if constexpr (WM <= 6) {
-#pragma unroll
-        for (int r = 0; r < RD; ++r) {
-            const float s1 = Xs[r];
-            const float s2 = Xs[r] + Xs[r + 1];
-            const float s3 = s2 + Xs[r + 2];
-            const float s4 = s2 + (Xs[r + 2] + Xs[r + 3]);
-            const float s6 = s4 + (Xs[r + 4] + Xs[r + 5]);
-            m[0] = fmaxf(m[0], s1);
-            m[1] = fmaxf(m[1], s2);
-            m[2] = fmaxf(m[2], s3);
-            m[3] = fmaxf(m[3], s4);
-            m[4] = fmaxf(m[4], s6);
-        }
+        win_max<BM, X, 0, RD>(Xs, m);
     } else {
         float ps[X + 1];
         ps[0] = 0.0f;
@@ -609,67 +623,62 @@ __device__ __forceinline__ float fold_direct_body(const CS_LDS uint32_t* __restr
             for (int r = 0; r < RD; ++r) m[a] = fmaxf(m[a], ps[r + w] - ps[r]);
         }
     }
-    return max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
+    if constexpr (BM != 0) return max_over_rows(bank_bound_ct<BM>(m));
+    else return max_over_rows(bank_bound<bank_count(WM <= 6 ? 6 : WM)>(m, bankmask));
 }
 
 // Out-of-line copies of the fold passes, for the kernel that dispatches on
 // the profile length at run time (one call site per length and fold).
 template <int NB, int WM>
 __device__ __noinline__ float fold_pass(const CS_LDS uint32_t* __restrict__ fre,
-                                          const CS_LDS uint32_t* __restrict__ fim,
-                                          const CS_LDS u32x4* __restrict__ thi,
-                                          const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
-                                          uint32_t bankmask, bool active, int scol,
-                                          CS_GLB float* samp) {
-    return fold_pass_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
+                                        const CS_LDS uint32_t* __restrict__ fim,
+                                        const CS_LDS u32x4* __restrict__ thi,
+                                        const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
+                                        uint32_t bankmask, ColumnSink sk) {
+    return fold_pass_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
 }
 template <int NB, int WM>
 __device__ __noinline__ float fold_direct(const CS_LDS uint32_t* __restrict__ fre,
-                                            const CS_LDS uint32_t* __restrict__ fim,
-                                            const CS_LDS u32x4* __restrict__ thi,
-                                            const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
-                                            uint32_t bankmask, bool active, int scol,
-                                            CS_GLB float* samp) {
-    return fold_direct_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
+                                          const CS_LDS uint32_t* __restrict__ fim,
+                                          const CS_LDS u32x4* __restrict__ thi,
+                                          const CS_LDS u32x4* __restrict__ tlo, int k, int tcol,
+                                          uint32_t bankmask, ColumnSink sk) {
+    return fold_direct_body<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
 }
 
 // The pass for a fold of NB bins: the direct layout up to 64 bins, the
 // even/odd layout above; widths up to 6 unless the fold's bank is wider.
 // INL: the pass body inlined into the caller (the kernels specialised on a
 // fold set) rather than called.
-template <int NB, int WM, bool INL, int KF>
+template <int NB, int WM, bool INL, int KF, uint32_t BM>
 __device__ __forceinline__ float fold_of(const CS_LDS uint32_t* fre, const CS_LDS uint32_t* fim,
-                                           const CS_LDS u32x4* thi, const CS_LDS u32x4* tlo, int k,
-                                           int tcol, uint32_t bankmask, bool active, int scol,
-                                           CS_GLB float* samp) {
+                                         const CS_LDS u32x4* thi, const CS_LDS u32x4* tlo, int k,
+                                         int tcol, uint32_t bankmask, const ColumnSink& sk) {
     if constexpr (geo::c2r_direct(NB)) {
         if constexpr (INL)
-            return fold_direct_body<NB, WM, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active,
-                                                scol, samp);
+            return fold_direct_body<NB, WM, KF, BM>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
         else
-            return fold_direct<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
+            return fold_direct<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
     } else {
         if constexpr (INL)
-            return fold_pass_body<NB, WM, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol,
-                                              samp);
+            return fold_pass_body<NB, WM, KF, BM>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
         else
-            return fold_pass<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
+            return fold_pass<NB, WM>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
     }
 }
 
-template <int NB, bool WIDE, bool INL = false, int KF = 0>
+template <int NB, bool WIDE, bool INL = false, int KF = 0, uint32_t BM = 0>
 __device__ __forceinline__ float run_fold(bool narrow, const CS_LDS uint32_t* fre,
-                                            const CS_LDS uint32_t* fim, const CS_LDS u32x4* thi,
-                                            const CS_LDS u32x4* tlo, int k, int tcol,
-                                            uint32_t bankmask, bool active, int scol,
-                                            CS_GLB float* samp) {
+                                          const CS_LDS uint32_t* fim, const CS_LDS u32x4* thi,
+                                          const CS_LDS u32x4* tlo, int k, int tcol,
+                                          uint32_t bankmask, const ColumnSink& sk) {
     constexpr int WW = gate_wmax(NB);
     if constexpr (WW <= 6 || !WIDE)
-        return fold_of<NB, 6, INL, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
+        return fold_of<NB, 6, INL, KF, BM>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
     else if (narrow)
-        return fold_of<NB, 6, INL, KF>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
+        return fold_of<NB, 6, INL, KF, BM>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
     else
-        return fold_of<NB, WW, INL, 0>(fre, fim, thi, tlo, k, tcol, bankmask, active, scol, samp);
+        return fold_of<NB, WW, INL, 0, 0>(fre, fim, thi, tlo, k, tcol, bankmask, sk);
 }
 
 // Fold sets with a kernel of their own (profile length of fold f, in order):
@@ -679,6 +688,11 @@ __device__ __forceinline__ float run_fold(bool narrow, const CS_LDS uint32_t* fr
 constexpr int kFoldSets = 4;
 constexpr int kFoldSetNB[kFoldSets][kMaxFolds] = {
     {0}, {120, 60, 40, 30, 24, 20}, {64, 32, 20, 16}, {16}};
+// and their boxcar banks as width masks (bit a: width 1, 2, 3, 4, 6): the
+// default bank (boxcar_fsp 1.5, boxcar_maxfrac 0.3), width 1 pruned from the
+// folds above the first by the ladder (ladder_boxcar_widths, search.jl:443-459)
+constexpr uint32_t kFoldSetBM[kFoldSets][kMaxFolds] = {
+    {0}, {0x1F, 0x1E, 0x1E, 0x1E, 0x1E, 0x1E}, {0x1F, 0x1E, 0x1E, 0x0E}, {0x0F}};
 constexpr int fold_set_size(int set) {
     int n = 0;
     for (int f = 0; f < kMaxFolds; ++f) n += kFoldSetNB[set][f] > 0 ? 1 : 0;
@@ -765,7 +779,8 @@ struct MfmaOps {
     const float2* gA;
     const int4* hdesc;
     const uint64_t* rowmask;
-    const int32_t* scol;
+    const uint4* scol;   // [block trial]: eight int16 sampled columns (fold f: half-word f)
+    const uint4* nv16;   // [global block]: eight uint16 valid counts
 };
 
 // KS: interpolation K steps (1 or 2); WIDE: some fold's bank has widths > 6
@@ -843,8 +858,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     // wherever they are re-read under register pressure that is a scalar load
     // from the argument segment (a reload from scratch would be a vector load,
     // and its wait would drain the tile's outstanding stores).
-    const int32_t* nvb = b.nvalid;
-    const int32_t* scb = ops.scol;
+    const uint4* nvb = ops.nv16;
+    const uint4* scb = ops.scol;
     const uint64_t* rmb = ops.rowmask;
     const int4* hdb = ops.hdesc;
     const int32_t* slb = b.slot;
@@ -852,13 +867,24 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const int nfv = fs.nf;
     // every fold's bank mask, lane f holding fold f's
     const uint32_t bmv = fs.bankmask[lane < nfv ? lane : nfv - 1];
+    // the batch's sigma sample regions as one buffer (the host keeps it below
+    // kNoStore bytes, so every offset is 32-bit and kNoStore drops a store)
+    const __amdgpu_buffer_rsrc_t srs = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)fu.samples, (short)0, (int)(uint32_t)((int64_t)b.nblocks * nfv * fu.sld * 4),
+        0x00020000);
     const uint32_t cg_last = (uint32_t)((g.total - 1 - g.origin) / g.Nprof);
     int64_t st = 0, tbase = 0, occ = 0, t = 0, jj = 0;
     int u = 0;
     bool tin = false;   // trial inside the search
     int32_t slotv = 0;  // selection position of the trial's block (-1: not selected)
-    int32_t nvf[kMaxFolds], scf[kMaxFolds];
-    uint64_t rmv[4];
+    // the c2r trial's valid count and sampled column of every fold, packed
+    // as uint16 / int16 (fold f in half-word f)
+    uint4 nvq = make_uint4(0u, 0u, 0u, 0u), scq = make_uint4(0u, 0u, 0u, 0u);
+    // the row masks of the interpolation lane's four trials 4 g4 + v: the
+    // blocks of its first and last trial (the four are consecutive, so each
+    // lies in one of the two); trial v is in the last one's when v >= rsplit
+    uint64_t rm0 = 0, rm3 = 0;
+    int rsplit = 4;
     uint4 wa[8], wb[8];
     // the harmonic descriptors of this wave's harmonics (window start x,
     // advance y, weight tile z, conj(A) row w), one value per lane (lane
@@ -908,21 +934,16 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         jj = rel - cgr * np;
         // whether the block is in this batch is decided where it is used
         slotv = glb(slb)[cgc];
-#pragma unroll
-        for (int f = 0; f < kMaxFolds; ++f) {
-            const int ff = f < nfv ? f : nfv - 1;
-            nvf[f] = glb(nvb)[(size_t)cgc * nfv + ff];
-            scf[f] = glb(scb)[(size_t)ff * np + jj];
-        }
+        nvq = ldg4(nvb + cgc);
+        scq = ldg4(scb + (uint32_t)jj);
         {
             const int64_t t0 = tbase + o * S + 4 * g4;
             const uint32_t rel = (uint32_t)(t0 - g.origin);
             const uint32_t cg = rel / np, rem = rel - cg * np;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const uint32_t cgv = cg + (rem + v >= np ? 1u : 0u);
-                rmv[v] = glb(rmb)[cgv < cg_last ? cgv : cg_last];
-            }
+            const uint32_t cg3 = cg + (rem + 3 >= np ? 1u : 0u);
+            rm0 = glb(rmb)[cg < cg_last ? cg : cg_last];
+            rm3 = glb(rmb)[cg3 < cg_last ? cg3 : cg_last];
+            rsplit = rem + 3 >= np ? (int)(np - rem) : 4;
         }
         // every window of this wave's harmonics (L2 / MALL / HBM)
         // (32-bit: tile occurrences stay within the planes' range, whose
@@ -1022,7 +1043,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 const float mine = acc[v], oth = pair_swap(mine);
                 float val = mine * gA[2 * v] + oth * gA[2 * v + 1];   // Im A pre-signed
                 if constexpr (decltype(masked)::value)
-                    val = ((rmv[v] >> (h - 1)) & 1ull) ? val : 0.0f;
+                    val = ((((v >= rsplit) ? rm3 : rm0) >> (h - 1)) & 1ull) ? val : 0.0f;
                 ow[v] = split_word(val);
             }
             CS_LDS uint32_t* dst = wrow + 8 * s * kFtpLd;   // row h - 1
@@ -1034,7 +1055,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         // every harmonic row of the tile passes its guard (wave-uniform); the
         // row masks were loaded with the previous tile's prefetch
         const uint64_t need = nh >= 64 ? ~0ull : (1ull << nh) - 1ull;
-        if (__all(((rmv[0] & rmv[1] & rmv[2] & rmv[3]) & need) == need)) interp(BoolC<false>{});
+        if (__all(((rm0 & rm3) & need) == need)) interp(BoolC<false>{});
         else interp(BoolC<true>{});
         lds_barrier();
         if constexpr (DUMP) {
@@ -1068,22 +1089,47 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const int clv = slotv - b.slot0;
             const bool inb = tin && (uint32_t)clv < (uint32_t)b.nblocks;
             const int cl = inb ? clv : 0;
-            // bound column: [local block][trial of the block]
-            const int64_t col_out = (int64_t)cl * np + jj;
+            // bound column [local block][trial of the block], stored by lane
+            // group 0 (byte offset in a fold's bound row; kNoStore: no store)
+            const uint32_t moff = (inb && g4 == 0) ? 4u * ((uint32_t)cl * np + (uint32_t)jj) : kNoStore;
+            // sampled columns of blocks that are not full (the last block, or a
+            // fold cut at Nyquist) come from sample_column, the host table
+            // holding those of full blocks: one check for all folds, so the
+            // fold passes below are one region with no branch
+            {
+                bool part = false;
+#pragma unroll
+                for (int f = 0; f < kMaxFolds; ++f) {
+                    const uint32_t nv = half_u(nvq, f);
+                    part = part || (f < nf && inb && (uint32_t)jj < nv && nv != np);
+                }
+                if (__builtin_expect(__any(part), 0)) {
+#pragma unroll
+                    for (int f = 0; f < kMaxFolds; ++f) {   // compile-time f: scq stays in registers
+                        const uint32_t nv = half_u(nvq, f);
+                        if (f < nf && inb && (uint32_t)jj < nv && nv != np)
+                            set_half(scq, f,
+                                     (uint32_t)sample_column32((uint32_t)jj, nv, (uint32_t)fsx->f[f].nbins,
+                                                               (uint32_t)fsx->cap[f]));
+                    }
+                }
+            }
             // fold f of NB bins (NBC: the length as a compile-time constant,
             // or 0 to dispatch on the descriptor's length at run time)
-            auto fold_step = [&](const int f, auto nbc, auto kfc) {
+            auto fold_step = [&](const int f, auto nbc, auto kfc, auto bmc) {
                 constexpr int NBC = decltype(nbc)::value;
-                constexpr int KF = decltype(kfc)::value;   // fold factor, 0: run time
+                constexpr int KF = decltype(kfc)::value;        // fold factor, 0: run time
+                constexpr uint32_t BM = decltype(bmc)::value;   // bank mask, 0: run time
                 const int NB = NBC > 0 ? NBC : fsx->f[f].nbins;
-                const int64_t nv = nvf[f];
-                const bool act = inb && jj < nv;
-                // sampled column of this trial (sample_column for blocks that are
-                // not full, else the host table)
-                int sc = scf[f];
-                if (act && nv != np) sc = sample_column(jj, nv, NB, fsx->cap[f]);
-                CS_GLB float* gsamp =
-                    (CS_GLB float*)(fu.samples + ((int64_t)cl * nf + f) * fu.sld);
+                const bool act = inb && (uint32_t)jj < half_u(nvq, f);
+                // this trial's sampled column of the fold, if any: its slot block
+                // in region (cl, f), columns round_up(NB, 4) slots apart
+                const int sc = half_s(scq, f);
+                const ColumnSink sk{
+                    srs, (act && sc >= 0)
+                             ? 4u * ((uint32_t)(cl * nf + f) * (uint32_t)fu.sld +
+                                     (uint32_t)sc * (uint32_t)((NB + 3) & ~3))
+                             : kNoStore};
                 // fold factor and first DFT tile: compile-time constants in the
                 // fold-set kernels (the launcher checked both), else read from
                 // the descriptors; the bank mask from lane f of a register
@@ -1095,13 +1141,13 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 const bool narrow = (bm >> 5) == 0;   // bank within widths 1, 2, 3, 4, 6
                 float bound = -INFINITY;
                 if constexpr (NBC > 0) {
-                    bound = run_fold<NBC, WIDE, true, KF>(narrow, lre, lim, th, tl, k, tcol, bm,
-                                                          act, sc, gsamp);
+                    bound = run_fold<NBC, WIDE, true, KF, BM>(narrow, lre, lim, th, tl, k, tcol, bm,
+                                                              sk);
                 } else {
                     switch (NB) {
-#define X(SZ)                                                                            \
-    case SZ:                                                                             \
-        bound = run_fold<SZ, WIDE>(narrow, lre, lim, th, tl, k, tcol, bm, act, sc, gsamp); \
+#define X(SZ)                                                                  \
+    case SZ:                                                                   \
+        bound = run_fold<SZ, WIDE>(narrow, lre, lim, th, tl, k, tcol, bm, sk); \
         break;
                         CS_MFMA_SIZES(X)
 #undef X
@@ -1110,19 +1156,25 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                     }
                 }
                 // the fold's bound of the trial, stored as soon as it is known
-                if (inb && g4 == 0) fu.mraw[(int64_t)f * fu.ldm + col_out] = act ? bound : -INFINITY;
+                // (row f of the bounds as its own buffer: offsets stay 32-bit)
+                const __amdgpu_buffer_rsrc_t mrs = __builtin_amdgcn_make_buffer_rsrc(
+                    (void*)(fu.mraw + (int64_t)f * fu.ldm), (short)0, (int)(fu.ldm * 4), 0x00020000);
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(act ? bound : -INFINITY), mrs,
+                                                      moff, 0, 0);
             };
             if constexpr (SET == 0) {
 #pragma unroll
                 for (int f = 0; f < kMaxFolds; ++f)
                     if (f < nf)
-                        fold_step(f, std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{});
+                        fold_step(f, std::integral_constant<int, 0>{}, std::integral_constant<int, 0>{},
+                                  std::integral_constant<uint32_t, 0u>{});
             } else {
                 // the launcher checked that the search's folds are exactly this set
                 static_for<fold_set_size(SET)>([&](auto fc) {
                     constexpr int F = decltype(fc)::value;
                     fold_step(F, std::integral_constant<int, kFoldSetNB[SET][F]>{},
-                              std::integral_constant<int, set_reads_in_tables(SET) ? F + 1 : 0>{});
+                              std::integral_constant<int, set_reads_in_tables(SET) ? F + 1 : 0>{},
+                              std::integral_constant<uint32_t, kFoldSetBM[SET][F]>{});
                 });
             }
             // the next tile's loads
@@ -1246,7 +1298,7 @@ static int fold_set_of(const FoldSet& fs, const MfmaDev& md, int nharms) {
                     md.dft_tiles_max == set_tiles(set);
         for (int f = 0; f < fs.nf && same; ++f)
             same = fs.f[f].nbins == kFoldSetNB[set][f] && fs.f[f].k == f + 1 &&
-                   md.c_base[f] == set_cbase(set, f);
+                   md.c_base[f] == set_cbase(set, f) && fs.bankmask[f] == kFoldSetBM[set][f];
         if (same) return set;
     }
     return 0;
@@ -1264,7 +1316,7 @@ hipError_t launch_fused_mfma(hipStream_t st, const SpecDev& sp, const TabDev& tb
         hipError_t e = hipFuncSetAttribute((const void*)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         if (e != hipSuccess) return e;
-        const MfmaOps ops{md.wA_hi, md.wA_lo, md.gA, md.hdesc, md.rowmask, md.scol};
+        const MfmaOps ops{md.wA_hi, md.wA_lo, md.gA, md.hdesc, md.rowmask, md.scol, md.nv16};
         hipLaunchKernelGGL(kern, dim3((unsigned)nwg), dim3(kThreads), shm, st, sp, tb, g, fs_dev, b,
                            fu, md_dev, ops);
         return hipGetLastError();

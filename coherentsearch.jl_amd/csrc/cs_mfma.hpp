@@ -14,6 +14,11 @@ constexpr int kSpecPad = 64;
 // columns of 32 slots at 30 bins (8736), 8960 keeps regions 256-byte aligned.
 constexpr int kSigmaSampleAlign = 4;
 constexpr int kSigmaSamplesPadded = 8960;
+// The split-BF16 kernel stores bounds and sigma samples through buffer
+// resources with 32-bit byte offsets, an offset of kMfmaNoStore or more being a
+// dropped store: a batch's sample regions, and each fold's row of bounds, stay
+// below that many bytes.
+constexpr uint32_t kMfmaNoStore = 0xFFFFF000u;
 
 struct MfmaDev {
     const uint4* wA_hi;       // interpolation weight tiles, [tile][64 lanes] x 8 bf16
@@ -27,7 +32,9 @@ struct MfmaDev {
     const uint4* cA_lo;
     const uint32_t* spl;      // split spectrum: [plane re/im][kSpecPad + bin], (hi | lo << 16)
     const uint64_t* rowmask;  // [global block]: bit h-1 set if harmonic row h is kept
-    const int32_t* scol;      // [fold][block trial]: sampled column of a full block, or -1
+    const uint4* scol;        // [block trial]: sampled column of a full block per fold,
+                              //   eight int16 (fold f in half-word f), -1: none
+    const uint4* nv16;        // [global block]: valid trials per fold, eight uint16
     int64_t spl_ld;           // plane stride (namps + 2 * kSpecPad)
     int64_t S;                // occurrence stride, lcm(16, q) trials
     int64_t ntiles;           // (ceil(total / (8 S))) * U tiles of 128 trials in the search
