@@ -181,6 +181,17 @@ __device__ __forceinline__ int4 ld_const4(const int4* p, size_t i) {
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
+// n / d (and n % d in r) for 32-bit n and 2 <= d with m = floor(2^32 / d):
+// the high product is the quotient or one less (n < 2^32), one correction
+__device__ __forceinline__ uint32_t udiv_m(uint32_t n, uint32_t d, uint32_t m, uint32_t& r) {
+    uint32_t q = __umulhi(n, m);
+    r = n - q * d;
+    const bool c = r >= d;
+    q += c ? 1u : 0u;
+    r -= c ? d : 0u;
+    return q;
+}
+
 // Half-word f of a 16-byte value: unsigned, signed, and replaced by the low
 // half of v (f a compile-time constant where these are used)
 __device__ __forceinline__ uint32_t half_u(const uint4& q, int f) {
@@ -819,23 +830,26 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     // ranges per XCD (workgroups are dealt to XCDs round-robin), so tiles of
     // neighbouring 16-trial groups, whose windows overlap, run together and
     // share that XCD's L2.  `tile` below is a list position.
-    int64_t t_first, t_end, t_step;
+    // (32-bit: a batch has far fewer than 2^31 tiles)
+    int t_first, t_end, t_step;
     {
         const int G = gridDim.x, nx = (G % 8 == 0 && md.tile_order == 0) ? 8 : 1;
         if (nx > 1) {
             const int x = blockIdx.x % nx, l = blockIdx.x / nx;
-            t_first = b.ntile * x / nx + l;
-            t_end = b.ntile * (x + 1) / nx;
+            t_first = (int)(b.ntile * x / nx) + l;
+            t_end = (int)(b.ntile * (x + 1) / nx);
             t_step = G / nx;
         } else {
             t_first = blockIdx.x;
-            t_end = b.ntile;
+            t_end = (int)b.ntile;
             t_step = G;
         }
     }
 
     const int j16 = lane & 15, o = j16 >> 1, c = j16 & 1, g4 = lane >> 4;
-    const int64_t S = md.S;
+    // trial indices stay 32-bit: the host limits a search to total + 8 S < 2^32
+    const uint32_t S = (uint32_t)md.S;
+    const uint32_t gtotal = (uint32_t)g.total, gorigin = (uint32_t)g.origin;
     const int U = md.U;
     const CS_GLB uint32_t* plane = glb(md.spl) + (size_t)c * md.spl_ld + kSpecPad;
     const int64_t bmin = -kSpecPad, bmax = sp.namps + kSpecPad - 32 * KS;
@@ -846,6 +860,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const uint32_t lane_off = 4u * (uint32_t)((int64_t)c * md.spl_ld + kSpecPad + 8 * g4);
     const int bmin32 = (int)bmin, bmax32 = (int)bmax;
     const uint32_t np = (uint32_t)g.Nprof;
+    const uint32_t npm = (uint32_t)(0x100000000ull / np);   // floor(2^32 / np), np >= 16
 
     // Everything a tile reads from global memory before its compute: the
     // harmonic descriptors' window starts, the c2r trial's valid counts and
@@ -873,7 +888,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         (void*)fu.samples, (short)0, (int)(uint32_t)((int64_t)b.nblocks * nfv * fu.sld * 4),
         0x00020000);
     const uint32_t cg_last = (uint32_t)((g.total - 1 - g.origin) / g.Nprof);
-    int64_t st = 0, tbase = 0, occ = 0, t = 0, jj = 0;
+    uint32_t st = 0, tbase = 0, t = 0, jj = 0;
+    int occ = 0;
     int u = 0;
     bool tin = false;   // trial inside the search
     int32_t slotv = 0;  // selection position of the trial's block (-1: not selected)
@@ -890,7 +906,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     // advance y, weight tile z, conj(A) row w), one value per lane (lane
     // 4s + c holds component c of harmonic slot s; slots past nharms repeat
     // the last harmonic), loaded one tile ahead of their use
-    auto load_desc = [&](int64_t tile) {
+    auto load_desc = [&](uint32_t tile) {
         const int uu = (int)((uint32_t)tile % (uint32_t)U);   // tile counts stay below 2^31
         const int s = (lane >> 2) & 7, h = wv + 1 + 8 * s;
         // a 32-bit element index (the table is far below 2^32 entries), so the
@@ -901,7 +917,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     };
     // tile number at list position k (clamped to the last one: past the
     // range a fetch reloads it), the same address in every lane
-    auto tile_at = [&](int64_t k) { return glb(tlb)[k < t_end ? k : t_end - 1]; };
+    auto tile_at = [&](int k) { return glb(tlb)[k < t_end ? k : t_end - 1]; };
     // idc: tile number of the next position to fetch; idn: of the one after
     // (in flight since the previous fetch)
     uint32_t idc = 0, idn = 0;
@@ -912,7 +928,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         dnext = load_desc(idc);
         idn = tile_at(t_first + t_step);
     }
-    auto fetch = [&](int64_t tile) {
+    auto fetch = [&](int tile) {
         const uint32_t tl = idc;
         const uint32_t stu = tl / (uint32_t)U;
         st = stu;
@@ -924,22 +940,31 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         idc = __builtin_amdgcn_readfirstlane(idn);
         dnext = load_desc(idc);
         idn = tile_at(tile + 2 * t_step);
-        tbase = st * kOcc8 * S + 16 * (int64_t)u;
-        occ = (int64_t)kOcc8 * st + o;
-        t = tbase + (int64_t)wv * S + (lane & 15);
-        tin = t < g.total;
-        const uint32_t rel = (uint32_t)(t - g.origin);
-        const uint32_t cgr = rel / np;
+        tbase = st * kOcc8 * S + 16u * (uint32_t)u;
+        occ = (int)(kOcc8 * st) + o;
+        t = tbase + (uint32_t)wv * S + (uint32_t)(lane & 15);
+        tin = t < gtotal;
+        // block and block trial of t: the wave's first trial divided on the
+        // scalar unit, then at most one carry (the 16 trials of a wave span at
+        // most two blocks, blocks holding at least 16 trials)
+        uint32_t cgr;
+        {
+            uint32_t rb;
+            const uint32_t qb = udiv_m(tbase + (uint32_t)wv * S - gorigin, np, npm, rb);
+            const uint32_t rem = rb + (uint32_t)(lane & 15);
+            const bool carry = rem >= np;
+            cgr = qb + (carry ? 1u : 0u);
+            jj = carry ? rem - np : rem;
+        }
         const uint32_t cgc = cgr < cg_last ? cgr : cg_last;
-        jj = rel - cgr * np;
         // whether the block is in this batch is decided where it is used
         slotv = glb(slb)[cgc];
         nvq = ldg4(nvb + cgc);
         scq = ldg4(scb + (uint32_t)jj);
         {
-            const int64_t t0 = tbase + o * S + 4 * g4;
-            const uint32_t rel = (uint32_t)(t0 - g.origin);
-            const uint32_t cg = rel / np, rem = rel - cg * np;
+            uint32_t r0, rem;
+            const uint32_t q0 = udiv_m(tbase - gorigin, np, npm, r0);
+            const uint32_t cg = q0 + udiv_m(r0 + (uint32_t)o * S + 4u * (uint32_t)g4, np, npm, rem);
             const uint32_t cg3 = cg + (rem + 3 >= np ? 1u : 0u);
             rm0 = glb(rmb)[cg < cg_last ? cg : cg_last];
             rm3 = glb(rmb)[cg3 < cg_last ? cg3 : cg_last];
@@ -985,7 +1010,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         fetch(t_first);
     }
 
-    for (int64_t tile = t_first; tile < t_end; tile += t_step) {
+    for (int tile = t_first; tile < t_end; tile += t_step) {
         // ---------------- interpolation: harmonics wv+1, wv+9, ... of this wave ----------
         // two copies of the loop: with the row guard applied, and without it
         // for tiles whose rows all pass (the usual case)
@@ -1038,10 +1063,12 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 // gA[k, g] * complex(ar, ai): the even lane holds re and forms
-                // ar re - ai im, the odd lane holds im and forms ar im + ai re
-                // (each one rounding per product and one for the sum)
+                // ar re - ai im, the odd lane holds im and forms ar im + ai re,
+                // as one product and one fused multiply-add (the split-BF16
+                // amplitudes carry ~1e-5 relative error, against which the
+                // reference's separate rounding of the second product is nothing)
                 const float mine = acc[v], oth = pair_swap(mine);
-                float val = mine * gA[2 * v] + oth * gA[2 * v + 1];   // Im A pre-signed
+                float val = __builtin_fmaf(mine, gA[2 * v], oth * gA[2 * v + 1]);   // Im A pre-signed
                 if constexpr (decltype(masked)::value)
                     val = ((((v >= rsplit) ? rm3 : rm0) >> (h - 1)) & 1ull) ? val : 0.0f;
                 ow[v] = split_word(val);
@@ -1062,8 +1089,9 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             // diagnostic copy of the stacked amplitudes this wave's DFT reads:
             // occurrence wv, trial lane & 15 of the group, each word hi + lo
             const int i = lane & 15;
-            const int64_t rel = tbase + (int64_t)wv * S + i - fu.dump_t0;
-            if (rel >= 0 && rel < fu.dump_n && tbase + (int64_t)wv * S + i < g.total) {
+            const int64_t tw = (int64_t)tbase + (int64_t)wv * S + i;
+            const int64_t rel = tw - fu.dump_t0;
+            if (rel >= 0 && rel < fu.dump_n && tw < g.total) {
                 for (int h = 1 + (lane >> 4); h <= nh; h += 4) {
                     const uint32_t wr = fre[(h - 1) * kFtpLd + wv * 16 + i];
                     const uint32_t wi = fim[(h - 1) * kFtpLd + wv * 16 + i];
