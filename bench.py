@@ -756,7 +756,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="fused", choices=["plain", "fused", "fused_valu"])
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="searches timed in the plain FP32 mode for the ratio (0 = skip)")
