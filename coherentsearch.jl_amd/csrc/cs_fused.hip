@@ -314,7 +314,8 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
                 const int cc = (int)(((uint32_t)sc * M) >> 20);
                 pos = sc + cc * (nbs - nbins);
             }
-            const uint32_t v = __float_as_uint(S[pos]);
+            // the sample's magnitude (the split-BF16 kernel stores signed values)
+            const uint32_t v = __float_as_uint(S[pos]) & 0x7FFFFFFFu;
             keys[r] = s < ns ? v : 0xFFFFFFFFu;
         }
         const uint32_t half = (uint32_t)(ns / 2);

@@ -373,7 +373,9 @@ struct ColumnSink {
 
 // One sampled column's share of this lane group (runs n = g*R .. g*R + R - 1
 // of P and M) into its slot block: P[n] at n, M[n] at H + n, P[H] in place of
-// M[0]; whole CH-word chunks, the chunks past H dropped.
+// M[0]; whole CH-word chunks, the chunks past H dropped.  Signed values: the
+// select takes each sample's magnitude as it reads it, so no lane spends an
+// instruction on the columns that are not stored.
 template <int H, int R>
 __device__ __forceinline__ void put_column(const ColumnSink& sk, int g, const float (&P)[R],
                                            const float (&M)[R], float pH) {
@@ -385,10 +387,10 @@ __device__ __forceinline__ void put_column(const ColumnSink& sk, int g, const fl
         float pv[CH], mv[CH];
 #pragma unroll
         for (int e = 0; e < CH; ++e) {
-            pv[e] = fabsf(P[c0 + e]);
-            mv[e] = fabsf(M[c0 + e]);
+            pv[e] = P[c0 + e];
+            mv[e] = M[c0 + e];
         }
-        if (c0 == 0) mv[0] = n0 == 0 ? fabsf(pH) : mv[0];
+        if (c0 == 0) mv[0] = n0 == 0 ? pH : mv[0];
         buf_store<CH>(pv, sk.rsrc, off + 4u * (uint32_t)n0);
         buf_store<CH>(mv, sk.rsrc, off + 4u * (uint32_t)(H + n0));
     }
@@ -540,7 +542,7 @@ __device__ __forceinline__ float fold_pass_body(const CS_LDS uint32_t* __restric
 // X[r] = bin g*RD + r at slots g*RD + r, in 4-word chunks (4-byte aligned) and
 // a shorter tail.  Group 3's last SH registers repeat group 0's first bins;
 // they land in the column's padding slots NB .. round_up(NB, 4) - 1, which
-// the median never reads.
+// the median never reads.  Signed values, as in put_column.
 template <int RD>
 __device__ __forceinline__ void put_run(const ColumnSink& sk, int g, const float* Xs) {
     const uint32_t base = sk.off + 4u * (uint32_t)(g * RD);
@@ -548,7 +550,7 @@ __device__ __forceinline__ void put_run(const ColumnSink& sk, int g, const float
     for (int c0 = 0; c0 < RD; c0 += 4) {
         float v[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = c0 + e < RD ? fabsf(Xs[c0 + e]) : 0.0f;
+        for (int e = 0; e < 4; ++e) v[e] = c0 + e < RD ? Xs[c0 + e] : 0.0f;
         if (c0 + 4 <= RD) buf_store<4>(v, sk.rsrc, base + 4u * c0);
         else if (RD - c0 == 3) buf_store<3>(v, sk.rsrc, base + 4u * c0);
         else if (RD - c0 == 2) buf_store<2>(v, sk.rsrc, base + 4u * c0);
@@ -571,7 +573,7 @@ __device__ __forceinline__ float fold_direct_body(const CS_LDS uint32_t* __restr
     constexpr int H = D::H, RD = D::RD, SH = D::SH, MT = D::MT, NR = D::NR, X = D::X,
                   NF = D::NF, KD = D::KD;
     static_assert(KD <= 2, "the direct layout covers H <= 32");
-    const int lane = threadIdx.x & 63, col = lane & 15, g = lane >> 4;
+    const int lane = threadIdx.x & 63, g = lane >> 4;
     // stack row h*k at plane row h*k - 1 (rows < 64, strides < 2^24: 24-bit products)
     const int rstride = k * kFtpLd, rbase = tcol - kFtpLd;
 
