@@ -856,10 +856,13 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const CS_GLB uint32_t* plane = glb(md.spl) + (size_t)c * md.spl_ld + kSpecPad;
     const int64_t bmin = -kSpecPad, bmax = sp.namps + kSpecPad - 32 * KS;
     // the pre-split planes as one buffer resource (2 spl_ld words, under 2^31
-    // bytes): a window is a 32-bit byte offset from this lane's plane column
+    // bytes): a window is a 32-bit byte offset from this lane's plane column.
+    // Lane group g4 reads window taps 4 g4 .. 4 g4 + 3 and 16 + 4 g4 ..
+    // 16 + 4 g4 + 3 (interp_tap): the first load takes one contiguous 64-byte
+    // run per occurrence and plane.
     const __amdgpu_buffer_rsrc_t srsc = __builtin_amdgcn_make_buffer_rsrc(
         (void*)md.spl, (short)0, (int)(2 * md.spl_ld * 4), 0x00020000);
-    const uint32_t lane_off = 4u * (uint32_t)((int64_t)c * md.spl_ld + kSpecPad + 8 * g4);
+    const uint32_t lane_off = 4u * (uint32_t)((int64_t)c * md.spl_ld + kSpecPad + 4 * g4);
     const int bmin32 = (int)bmin, bmax32 = (int)bmax;
     const uint32_t np = (uint32_t)g.Nprof;
     const uint32_t npm = (uint32_t)(0x100000000ull / np);   // floor(2^32 / np), np >= 16
@@ -982,29 +985,44 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                      __builtin_amdgcn_readlane(dreg, 4 * s + 1) * occ32;
             b0 = b0 < bmin32 ? bmin32 : (b0 > bmax32 ? bmax32 : b0);
             const uint32_t vo = lane_off + 4u * (uint32_t)b0;
+            // the second run: lane groups past the class's last nonzero weight
+            // load past the buffer (zeros, no memory read)
+            const int run2 = (__builtin_amdgcn_readlane(dreg, 4 * s + 3) >> kDescRun2Shift) & 7;
             wa[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, vo, 0, 0));
-            wb[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, vo + 16u, 0, 0));
+            wb[s] = __builtin_bit_cast(
+                uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, g4 < run2 ? vo + 64u : kNoStore, 0, 0));
         }
     };
-    // Interpolation operands of one harmonic slot: weight tiles and conj(A)
-    // (L2 resident) run two harmonics ahead in a ring of three.
+    // Interpolation operands of one harmonic slot: the weight tiles (L2
+    // resident) run two harmonics ahead in a ring of three; conj(A), eight
+    // distinct 32-byte runs per load (L1 resident), one harmonic ahead in a
+    // ring of two, which keeps eight registers free through the loop.
     struct Op {
-        uint4 ah, al, q0, q1;
+        uint4 ah, al;
         int4 d;
     };
+    struct OpA {
+        uint4 q0, q1;
+    };
     Op ring[3];
+    OpA ringa[2];
+    // this wave's descriptors of the tile, lane 4s + component, from the
+    // fetch (v_readlane: no scalar load, so no wait that would also drain
+    // this wave's LDS stores)
+    auto desc = [&](int s) {
+        return make_int4(__builtin_amdgcn_readlane(dcur, 4 * s), __builtin_amdgcn_readlane(dcur, 4 * s + 1),
+                         __builtin_amdgcn_readlane(dcur, 4 * s + 2),
+                         __builtin_amdgcn_readlane(dcur, 4 * s + 3));
+    };
     auto issue = [&](int s, Op& x) {
-        // this wave's descriptors of the tile, lane 4s + component, from
-        // the fetch (v_readlane: no scalar load, so no wait that would also
-        // drain this wave's LDS stores)
-        const int4 d = make_int4(__builtin_amdgcn_readlane(dcur, 4 * s),
-                                 __builtin_amdgcn_readlane(dcur, 4 * s + 1),
-                                 __builtin_amdgcn_readlane(dcur, 4 * s + 2),
-                                 __builtin_amdgcn_readlane(dcur, 4 * s + 3));
+        const int4 d = desc(s);
         x.d = d;
         x.ah = ldg4(ops.wA_hi + (size_t)d.z * 64 + lane);
         x.al = ldg4(ops.wA_lo + (size_t)d.z * 64 + lane);
-        const float2* gp = ops.gA + ((size_t)(d.w & 0x0fffffff) * 2 + c) * 16 + 4 * g4;
+    };
+    auto issue_a = [&](int s, OpA& x) {
+        const int cls = __builtin_amdgcn_readlane(dcur, 4 * s + 3) & kDescClass;
+        const float2* gp = ops.gA + ((size_t)cls * 2 + c) * 16 + 4 * g4;
         x.q0 = ldg4(gp);
         x.q1 = ldg4(gp + 2);
     };
@@ -1029,12 +1047,15 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         auto interp = [&](auto masked) {
         issue(0, ring[0]);
         issue(1, ring[1]);
+        issue_a(0, ringa[0]);
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int h = wv + 1 + 8 * s;
             if (s + 2 < 8) issue(s + 2, ring[(s + 2) % 3]);
+            if (s + 1 < 8) issue_a(s + 1, ringa[(s + 1) % 2]);
             if (h > nhc) continue;
             const Op& x = ring[s % 3];
+            const OpA& xa = ringa[s % 2];
             bf16x8 bh, bl;
             {
                 const uint32_t w[8] = {wa[s].x, wa[s].y, wa[s].z, wa[s].w,
@@ -1047,8 +1068,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 if ((x.d.w >> 28) == 2) {   // second 32-bin K step
                     int64_t b0 = (int64_t)x.d.x + (int64_t)x.d.y * occ;
                     b0 = b0 < bmin ? bmin : (b0 > bmax ? bmax : b0);
-                    const CS_GLB uint32_t* p = plane + b0 + 32 + 8 * g4;
-                    const uint4 x0 = ldg4_a4((const void*)p), x1 = ldg4_a4((const void*)(p + 4));
+                    const CS_GLB uint32_t* p = plane + b0 + 32 + 4 * g4;
+                    const uint4 x0 = ldg4_a4((const void*)p), x1 = ldg4_a4((const void*)(p + 16));
                     const uint32_t w[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
                     bf16x8 ch, cl;
                     unpack8(w, ch, cl);
@@ -1057,10 +1078,10 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                                 acc);
                 }
             }
-            const float gA[8] = {__builtin_bit_cast(float, x.q0.x), __builtin_bit_cast(float, x.q0.y),
-                                 __builtin_bit_cast(float, x.q0.z), __builtin_bit_cast(float, x.q0.w),
-                                 __builtin_bit_cast(float, x.q1.x), __builtin_bit_cast(float, x.q1.y),
-                                 __builtin_bit_cast(float, x.q1.z), __builtin_bit_cast(float, x.q1.w)};
+            const float gA[8] = {__builtin_bit_cast(float, xa.q0.x), __builtin_bit_cast(float, xa.q0.y),
+                                 __builtin_bit_cast(float, xa.q0.z), __builtin_bit_cast(float, xa.q0.w),
+                                 __builtin_bit_cast(float, xa.q1.x), __builtin_bit_cast(float, xa.q1.y),
+                                 __builtin_bit_cast(float, xa.q1.z), __builtin_bit_cast(float, xa.q1.w)};
             uint32_t ow[4];
 #pragma unroll
             for (int v = 0; v < 4; ++v) {

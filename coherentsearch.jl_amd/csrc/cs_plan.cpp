@@ -230,6 +230,11 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
         for (size_t c = 0; c < starts.size(); ++c) {
             t.clsmap[(size_t)(h - 1) * q + starts[c]] = ncls;
             t.cls_tile.push_back(ntile);
+            // lane groups whose second run (taps 16 + 4g .. 16 + 4g + 3 of the
+            // first K step) reaches a nonzero weight: the trials read taps
+            // 0 .. nj - 1
+            const int nj = m + D[c][15];
+            t.cls_run2.push_back((uint8_t)(ks > 1 ? 4 : std::max(0, std::min(4, (nj - 16 + 3) / 4))));
             // parity 0 (lanes holding Re) carries -Im A, parity 1 Im A
             for (int par = 0; par < 2; ++par)
                 for (int i = 0; i < 16; ++i) {
@@ -241,7 +246,7 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
                 for (int l = 0; l < 64; ++l) {
                     int i = l % 16;
                     for (int e = 0; e < 8; ++e) {
-                        int k = kk * 32 + (l / 16) * 8 + e;
+                        int k = kk * 32 + interp_tap(l / 16, e);
                         int tap = k - D[c][i];
                         float w = (tap >= 0 && tap < m)
                                       ? pl.W32[(size_t)(d.tab_off + P[c][i]) * m + tap] : 0.0f;
@@ -256,6 +261,7 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
             ++ncls;
         }
     }
+    if (ncls > kDescClass) { t.why = "too many interpolation classes"; return t; }
     // residue of the group starting at trial 16u (S is a multiple of q, so this
     // is also the residue of every occurrence 16u + o*S)
     t.clsu.assign((size_t)nh * t.U * 2, -1);
@@ -287,7 +293,7 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
             t.hdesc[e + 0] = (int32_t)b0;
             t.hdesc[e + 1] = (int32_t)(hp * (t.S / q));
             t.hdesc[e + 2] = t.clsu[2 * ((size_t)(h - 1) * t.U + u)];
-            t.hdesc[e + 3] = cls | (t.ks[h - 1] << 28);
+            t.hdesc[e + 3] = cls | ((int32_t)t.cls_run2[cls] << kDescRun2Shift) | (t.ks[h - 1] << 28);
         }
     }
     int ctile = 0;

@@ -14,6 +14,19 @@ constexpr int kMaxWidths = 24;
 constexpr int kMaxFolds = 8;
 constexpr int kQmax = 1 << 16;        // DIRECT_QMAX, directinterp.jl:111
 
+// Split-BF16 interpolation operands: element e (0..7) of lane group g (lanes
+// 16g .. 16g + 15) holds K slot interp_tap(g, e) of a 32-slot step, window
+// taps 4g .. 4g + 3 then 16 + 4g .. 16 + 4g + 3, so the first four words of
+// the four lane groups are one contiguous 64-byte run of the window (the
+// contraction sums over K, so its order is free).
+constexpr int interp_tap(int g, int e) { return e < 4 ? 4 * g + e : 12 + 4 * g + e; }
+// The last word of a harmonic descriptor: class index in bits 0..23; in bits
+// 24..26 the lane groups whose second run (taps 16 + 4g .. 16 + 4g + 3 of
+// the first K step) holds a nonzero weight of some trial of the class (0 to
+// 4: only those load it); K steps << 28.
+constexpr int kDescRun2Shift = 24;
+constexpr int32_t kDescClass = (1 << kDescRun2Shift) - 1;
+
 // One harmonic's recipe (DirectPlan, directinterp.jl:179-204).  Phase index p
 // (0-based) runs over the P residues the cycle visits; row[row_off + res] = p.
 struct HarmDev {
@@ -80,10 +93,12 @@ struct MfmaTables {
     std::vector<float> gA;                // [class][parity][16][2]: (Re A, -Im A | Im A)
     std::vector<int32_t> clsmap;          // [(h-1)*q + residue] -> class, -1 if no group starts there
     std::vector<int32_t> cls_tile;        // [class] -> first tile
+    std::vector<uint8_t> cls_run2;        // [class] -> lane groups of the second window run read
     std::vector<int32_t> ks;              // [h-1] -> K steps (1 or 2)
     std::vector<int32_t> clsu;            // [(h-1)*U + u] -> {first tile, class} of the group at 16u
     std::vector<int32_t> hdesc;           // [(h-1)*U + u] x {first bin of trial 16u, bin advance
-                                          //  per occurrence, first tile, class | K steps << 28}
+                                          //  per occurrence, first tile, class | second-run
+                                          //  lane groups << 24 | K steps << 28}
     std::vector<uint16_t> cA_hi, cA_lo;   // c2r tiles of every fold, rows permuted
     int32_t c_base[8] = {0}, c_mt[8] = {0}, c_ks[8] = {0};
 };

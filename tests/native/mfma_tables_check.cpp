@@ -67,6 +67,10 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
         return;
     }
     srand(seed);
+    int nrun[5] = {0, 0, 0, 0, 0};
+    for (uint8_t v : t.cls_run2) ++nrun[v];
+    printf("nharms %d m %d hidr %g: %zu classes; second-run lane groups 0..4: %d %d %d %d %d\n", nharms, m,
+           hidr, t.cls_run2.size(), nrun[0], nrun[1], nrun[2], nrun[3], nrun[4]);
     // 1 + 2: descriptors and weight tiles
     std::vector<double> xr(4096);
     for (int h = 1; h <= nharms; ++h) {
@@ -74,7 +78,10 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
         for (int u = 0; u < t.U; ++u) {
             const int32_t* e = &t.hdesc[4 * ((size_t)(h - 1) * t.U + u)];
             const int64_t b0u = e[0], adv = e[1];
-            const int tile = e[2], cls = e[3] & 0x0fffffff, ks = e[3] >> 28;
+            const int tile = e[2], cls = e[3] & kDescClass, ks = e[3] >> 28;
+            const int run2 = (e[3] >> kDescRun2Shift) & 7;
+            CHECK(run2 == t.cls_run2[cls] && run2 <= 4 && (ks == 1 || run2 == 4),
+                  "h=%d u=%d: second-run lane groups %d (class %d, ks %d)\n", h, u, run2, cls, ks);
             CHECK(ks == t.ks[h - 1], "h=%d ks %d vs %d\n", h, ks, t.ks[h - 1]);
             for (int trial = 0; trial < 6; ++trial) {
                 const int64_t st = rand() % 4000, o = rand() % 8;
@@ -95,6 +102,10 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
                 const int64_t off = bi - g0;
                 CHECK(off >= 0 && off + m <= 32 * ks, "h=%d u=%d i=%d offset %lld\n", h, u, i,
                       (long long)off);
+                // every tap a trial reads lies in a run the kernel loads: the
+                // first (taps 0..15) or a second-run lane group below run2
+                CHECK(ks > 1 || off + m <= 16 + 4 * run2, "h=%d u=%d i=%d reads tap %lld, run2 %d\n", h,
+                      u, i, (long long)(off + m - 1), run2);
                 double want = 0.0, scale = 0.0;
                 for (int tap = 0; tap < m; ++tap) {
                     const double w = pl.W32[(size_t)(d.tab_off + p) * m + tap];
@@ -107,7 +118,7 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
                         for (int el = 0; el < 8; ++el) {
                             const size_t idx = (((size_t)(tile + kk) * 64) + (kg * 16 + i)) * 8 + el;
                             const double w = (double)bf2f(t.wA_hi[idx]) + (double)bf2f(t.wA_lo[idx]);
-                            got += w * xr[kk * 32 + kg * 8 + el];
+                            got += w * xr[kk * 32 + interp_tap(kg, el)];
                         }
                 CHECK(std::fabs(got - want) <= 1e-5 * scale + 1e-12,
                       "h=%d u=%d i=%d tile row %g vs %g\n", h, u, i, got, want);
