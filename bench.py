@@ -152,15 +152,17 @@ def timed_search(eng, r_lo, r_hi, threshold, steps, warmup, label):
     for _ in range(warmup):
         raw = eng.search_raw(r_lo, r_hi, threshold)
     stage = {}
+    lines = []
     t0 = time.perf_counter()
     for i in range(steps):
         raw = eng.search_raw(r_lo, r_hi, threshold)
         tm = eng.timings()
         for k, v in tm.items():
             stage[k] = stage.get(k, 0.0) + v
-        log(f"[bench] {label} step {i + 1}/{steps}: {len(raw)} raw candidates, "
-            f"device {tm['total']:.2f} ms")
+        lines.append((len(raw), tm['total']))
     wall = time.perf_counter() - t0
+    for i, (nr, dev) in enumerate(lines):   # logged after the timed loop
+        log(f"[bench] {label} step {i + 1}/{steps}: {nr} raw candidates, device {dev:.2f} ms")
     for k in stage:
         stage[k] /= steps
     return raw, wall, stage

@@ -56,6 +56,7 @@ class Engine:
         self._spec_key = None
         self.N = None
         self.T = None
+        self._cbuf = None
 
     def close(self):
         if getattr(self, "_h", None):
@@ -149,10 +150,18 @@ class Engine:
                   "cs_set_tuning_cache")
 
     # -- entry points ------------------------------------------------------------
+    def _out(self, cap: int):
+        """The engine's candidate output buffer, grown to `cap` records (kept
+        between calls: a fresh zero-filled 2 MB array per search would cost
+        more host time than a small search takes on the device)."""
+        if self._cbuf is None or len(self._cbuf) < cap:
+            self._cbuf = np.empty(cap, dtype=_lib.CAND_DTYPE)
+        return self._cbuf.ctypes.data_as(C.POINTER(_lib.CsCand))
+
     def search_raw(self, r_lo: float, r_hi: float, threshold: float) -> np.ndarray:
         cap = 1 << 16
         while True:
-            out = (_lib.CsCand * cap)()
+            out = self._out(cap)
             n = C.c_int64()
             st = _lib.lib().cs_search(self._h, float(r_lo), float(r_hi), float(threshold), out,
                                       cap, C.byref(n))
@@ -160,19 +169,19 @@ class Engine:
                 cap = n.value
                 continue
             self._chk(st, "cs_search")
-            return np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
+            return self._cbuf[:n.value].copy()
 
     def _cands(self, call, what):
         cap = 1 << 16
         while True:
-            out = (_lib.CsCand * cap)()
+            out = self._out(cap)
             n = C.c_int64()
             st = call(out, cap, n)
             if st == _lib.CS_ENOSPC and n.value > cap:
                 cap = n.value
                 continue
             self._chk(st, what)
-            return np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
+            return self._cbuf[:n.value].copy()
 
     def search_stats(self, r_lo: float, r_hi: float, threshold: float, ms) -> np.ndarray:
         """cs_search_stats: raw candidates as search_raw, with every trial's
@@ -229,7 +238,7 @@ class Engine:
         counts = np.zeros(len(files), dtype=np.int64)
         cap = 1 << 16
         while True:
-            out = (_lib.CsCand * cap)()
+            out = self._out(cap)
             n = C.c_int64()
             st = _lib.lib().cs_search_files(self._h, len(files), recs, float(threshold), out, cap,
                                             counts.ctypes.data_as(C.POINTER(C.c_int64)),
@@ -239,7 +248,7 @@ class Engine:
                 cap = n.value
                 continue
             self._chk(st, "cs_search_files")
-            allc = np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
+            allc = self._cbuf[:n.value].copy()
             offs = np.concatenate([[0], np.cumsum(counts)])
             return [allc[offs[i]:offs[i + 1]] for i in range(len(files))]
 
@@ -291,7 +300,7 @@ class Engine:
         m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
         cap = 1 << 16
         while True:
-            out = (_lib.CsCand * cap)()
+            out = self._out(cap)
             n = C.c_int64()
             st = _lib.lib().cs_search_blocks(self._h, float(r_lo), float(r_hi), float(threshold),
                                              m.ctypes.data_as(C.POINTER(C.c_uint8)), len(m), out,
@@ -300,7 +309,7 @@ class Engine:
                 cap = n.value
                 continue
             self._chk(st, "cs_search_blocks")
-            return np.frombuffer(out, dtype=_lib.CAND_DTYPE, count=n.value).copy()
+            return self._cbuf[:n.value].copy()
 
     def timings(self) -> dict:
         a = np.zeros(8)
