@@ -772,12 +772,16 @@ __host__ __device__ constexpr bool mfma_size(int nb) {
            nb == 64 || nb == 120 || nb == 128;
 }
 
+// LDS of the split-BF16 kernel: the stacked amplitudes, the DFT tiles, then
+// (when the plan still fits in 160 KB) each wave's 2 KB landing area for the
+// next tile's first conj(A) operands (32 bytes per lane)
+constexpr size_t kWpfBytes = (size_t)(kThreads / 64) * 2 * 64 * 16;
 __host__ __device__ inline MfmaLds mfma_lds_layout(int nharms, int dft_tiles) {
     MfmaLds L;
     L.ftp = (size_t)2 * nharms * kFtpLd * 4;
     L.dft = (size_t)dft_tiles * 2 * 64 * 16;
-    L.region = 0;   // sampled columns go from registers to HBM
-    L.total = L.ftp + L.region + L.dft;
+    L.region = L.ftp + L.dft + kWpfBytes <= (size_t)160 * 1024 ? kWpfBytes : 0;
+    L.total = L.ftp + L.dft + L.region;
     return L;
 }
 
@@ -823,6 +827,10 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
     const int ntc = md.dft_tiles_max;  // DFT tiles of all folds
     uint4* dhi = (uint4*)(smem + (size_t)2 * nh * kFtpLd * 4);
     uint4* dlo = dhi + (size_t)ntc * 64;
+    // this wave's landing area for the next tile's first conj(A) operands
+    // (q0 at 0, q1 at 64; 16 bytes per lane), used when the plan made room
+    uint4* pfw = dlo + (size_t)ntc * 64 + (size_t)wv * 128;
+    const bool wpf = md.wpf != 0;
     for (int i = tid; i < ntc * 64; i += kThreads) {
         dhi[i] = ldg4(md.cA_hi + i);
         dlo[i] = ldg4(md.cA_lo + i);
@@ -933,6 +941,39 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         dnext = load_desc(idc);
         idn = tile_at(t_first + t_step);
     }
+    // Interpolation operands of one harmonic slot: the weight tiles (L2
+    // resident) run two harmonics ahead in a ring of three; conj(A), eight
+    // distinct 32-byte runs per load (L1 resident), one harmonic ahead in a
+    // ring of two, which keeps eight registers free through the loop.
+    struct Op {
+        uint4 ah, al;
+        int4 d;
+    };
+    struct OpA {
+        uint4 q0, q1;
+    };
+    Op ring[3];
+    OpA ringa[2];
+    // this wave's descriptors of the tile, lane 4s + component, from the
+    // fetch (v_readlane: no scalar load, so no wait that would also drain
+    // this wave's LDS stores)
+    auto desc = [&](int s) {
+        return make_int4(__builtin_amdgcn_readlane(dcur, 4 * s), __builtin_amdgcn_readlane(dcur, 4 * s + 1),
+                         __builtin_amdgcn_readlane(dcur, 4 * s + 2),
+                         __builtin_amdgcn_readlane(dcur, 4 * s + 3));
+    };
+    auto issue = [&](int s, Op& x) {
+        const int4 d = desc(s);
+        x.d = d;
+        x.ah = ldg4(ops.wA_hi + (size_t)d.z * 64 + lane);
+        x.al = ldg4(ops.wA_lo + (size_t)d.z * 64 + lane);
+    };
+    auto issue_a = [&](int s, OpA& x) {
+        const int cls = __builtin_amdgcn_readlane(dcur, 4 * s + 3) & kDescClass;
+        const float2* gp = ops.gA + ((size_t)cls * 2 + c) * 16 + 4 * g4;
+        x.q0 = ldg4(gp);
+        x.q1 = ldg4(gp + 2);
+    };
     auto fetch = [&](int tile) {
         const uint32_t tl = idc;
         const uint32_t stu = tl / (uint32_t)U;
@@ -940,6 +981,15 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         u = (int)(tl - stu * (uint32_t)U);
         const int dreg = dnext;
         dcur = dreg;
+        // the tile's first conj(A) operands straight into LDS (no registers
+        // held through the folds): its interpolation's first complex product
+        // does not wait on them
+        if (wpf) {
+            const int cls0 = __builtin_amdgcn_readlane(dreg, 3) & kDescClass;
+            const float2* gp = ops.gA + ((size_t)cls0 * 2 + c) * 16 + 4 * g4;
+            __builtin_amdgcn_global_load_lds((const CS_GLB void*)gp, (CS_LDS void*)pfw, 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const CS_GLB void*)(gp + 2), (CS_LDS void*)(pfw + 64), 16, 0, 0);
+        }
         // unconditional: a load under a branch leaves the merged wait state
         // pending at once
         idc = __builtin_amdgcn_readfirstlane(idn);
@@ -993,39 +1043,6 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
                 uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, g4 < run2 ? vo + 64u : kNoStore, 0, 0));
         }
     };
-    // Interpolation operands of one harmonic slot: the weight tiles (L2
-    // resident) run two harmonics ahead in a ring of three; conj(A), eight
-    // distinct 32-byte runs per load (L1 resident), one harmonic ahead in a
-    // ring of two, which keeps eight registers free through the loop.
-    struct Op {
-        uint4 ah, al;
-        int4 d;
-    };
-    struct OpA {
-        uint4 q0, q1;
-    };
-    Op ring[3];
-    OpA ringa[2];
-    // this wave's descriptors of the tile, lane 4s + component, from the
-    // fetch (v_readlane: no scalar load, so no wait that would also drain
-    // this wave's LDS stores)
-    auto desc = [&](int s) {
-        return make_int4(__builtin_amdgcn_readlane(dcur, 4 * s), __builtin_amdgcn_readlane(dcur, 4 * s + 1),
-                         __builtin_amdgcn_readlane(dcur, 4 * s + 2),
-                         __builtin_amdgcn_readlane(dcur, 4 * s + 3));
-    };
-    auto issue = [&](int s, Op& x) {
-        const int4 d = desc(s);
-        x.d = d;
-        x.ah = ldg4(ops.wA_hi + (size_t)d.z * 64 + lane);
-        x.al = ldg4(ops.wA_lo + (size_t)d.z * 64 + lane);
-    };
-    auto issue_a = [&](int s, OpA& x) {
-        const int cls = __builtin_amdgcn_readlane(dcur, 4 * s + 3) & kDescClass;
-        const float2* gp = ops.gA + ((size_t)cls * 2 + c) * 16 + 4 * g4;
-        x.q0 = ldg4(gp);
-        x.q1 = ldg4(gp + 2);
-    };
     if (t_first < t_end) {
         fetch(t_first);
     }
@@ -1047,7 +1064,12 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
         auto interp = [&](auto masked) {
         issue(0, ring[0]);
         issue(1, ring[1]);
-        issue_a(0, ringa[0]);
+        if (wpf) {
+            ringa[0].q0 = pfw[lane];
+            ringa[0].q1 = pfw[64 + lane];
+        } else {
+            issue_a(0, ringa[0]);
+        }
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int h = wv + 1 + 8 * s;
@@ -1288,7 +1310,9 @@ MfmaLds mfma_lds_plan(int nharms, const FoldSet& fs, MfmaDev& md) {
         tot += md.c_mt[f] * md.c_ks[f] * (geo::c2r_direct(fs.f[f].nbins) ? 1 : 2);
     md.dft_tiles_max = tot;
     md.nbp = 0;
-    return mfma_lds_layout(nharms, tot);
+    const MfmaLds L = mfma_lds_layout(nharms, tot);
+    md.wpf = L.region > 0 ? 1 : 0;
+    return L;
 }
 
 hipError_t launch_split_spectrum(hipStream_t st, const SpecDev& sp, uint32_t* spl) {

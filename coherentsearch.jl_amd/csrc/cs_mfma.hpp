@@ -48,6 +48,7 @@ struct MfmaDev {
     int32_t dft_tiles_max;    // c2r DFT tiles of all folds (hi or lo)
     int32_t tile_order;       // 0: contiguous tile-list ranges per XCD; 1: round-robin
     int32_t generic;          // 1: the run-time-dispatch kernel even for a known fold set
+    int32_t wpf;              // 1: the next tile's first conj(A) operands are prefetched into LDS
 };
 
 struct MfmaLds {
