@@ -19,6 +19,7 @@ CS_OK, CS_EINVAL, CS_EDIM, CS_ENOSPC, CS_EDEVICE, CS_ENOMEM = range(6)
 CS_MODE_PLAIN, CS_MODE_FUSED, CS_MODE_FUSED_VALU = 0, 1, 2
 CS_TUNE_BATCH_BLOCKS, CS_TUNE_WORKGROUPS, CS_TUNE_GRAPHS, CS_TUNE_TILE_ORDER = 1, 2, 3, 4
 CS_TUNE_GENERIC = 5
+CS_TUNE_NO_PREFETCH = 6
 CS_TRIAGE_FEATURES, CS_TRIAGE_PARAMS = 32, 2177
 
 

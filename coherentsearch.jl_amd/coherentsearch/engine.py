@@ -97,12 +97,16 @@ class Engine:
 
     def set_tuning(self, batch_blocks: Optional[int] = None, workgroups: Optional[int] = None,
                    graphs: Optional[bool] = None, tile_order: Optional[int] = None,
-                   generic: Optional[bool] = None):
+                   generic: Optional[bool] = None, no_prefetch: Optional[bool] = None):
         """Launch shape of the search (0 leaves the default, None leaves the
-        knob as it is), graph replay of the search's launches, and (generic)
-        the split-BF16 kernel that dispatches on profile lengths at run time
-        instead of one specialised on the fold set; results do not depend on
-        them."""
+        knob as it is), graph replay of the search's launches, (generic) the
+        split-BF16 kernel that dispatches on profile lengths at run time
+        instead of one specialised on the fold set, and (no_prefetch) that
+        kernel without its LDS prefetch of each tile's first conj(A)
+        operands; results do not depend on them."""
+        if no_prefetch is not None:
+            self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_NO_PREFETCH,
+                                               int(bool(no_prefetch))), "cs_set_tuning")
         if generic is not None:
             self._chk(_lib.lib().cs_set_tuning(self._h, _lib.CS_TUNE_GENERIC, int(bool(generic))),
                       "cs_set_tuning")
@@ -125,7 +129,8 @@ class Engine:
         for name, knob in (("batch_blocks", _lib.CS_TUNE_BATCH_BLOCKS),
                            ("workgroups", _lib.CS_TUNE_WORKGROUPS),
                            ("graphs", _lib.CS_TUNE_GRAPHS), ("tile_order", _lib.CS_TUNE_TILE_ORDER),
-                           ("generic", _lib.CS_TUNE_GENERIC)):
+                           ("generic", _lib.CS_TUNE_GENERIC),
+                           ("no_prefetch", _lib.CS_TUNE_NO_PREFETCH)):
             v = C.c_int64()
             self._chk(_lib.lib().cs_get_tuning(self._h, knob, C.byref(v)), "cs_get_tuning")
             out[name] = int(v.value)

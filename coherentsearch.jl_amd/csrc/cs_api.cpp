@@ -109,6 +109,7 @@ struct cs_ctx {
     int tune_wg = 0;           // CS_TUNE_WORKGROUPS (0: one per CU)
     int tune_order = 0;        // CS_TUNE_TILE_ORDER (0: contiguous tile ranges per XCD)
     int tune_generic = 0;      // CS_TUNE_GENERIC (1: no fold-set kernel)
+    int tune_noprefetch = 0;   // CS_TUNE_NO_PREFETCH (1: no LDS prefetch of conj(A))
     bool use_graph = false;    // CS_TUNE_GRAPHS: replay a captured launch sequence
     std::string tune_cache;    // cs_set_tuning_cache: cs_autotune's cache for cs_search_files
     hipGraph_t g_graph = nullptr;
@@ -450,6 +451,9 @@ int cs_set_tuning(cs_ctx* c, int32_t knob, int64_t value) {
     } else if (knob == CS_TUNE_GENERIC) {
         if (value > 1) return c->fail(CS_EINVAL, "CS_TUNE_GENERIC takes 0 or 1");
         c->tune_generic = (int)value;
+    } else if (knob == CS_TUNE_NO_PREFETCH) {
+        if (value > 1) return c->fail(CS_EINVAL, "CS_TUNE_NO_PREFETCH takes 0 or 1");
+        c->tune_noprefetch = (int)value;
     } else {
         return c->fail(CS_EINVAL, "unknown tuning knob");
     }
@@ -465,6 +469,7 @@ int cs_get_tuning(const cs_ctx* c, int32_t knob, int64_t* value) {
     else if (knob == CS_TUNE_GRAPHS) *value = c->use_graph ? 1 : 0;
     else if (knob == CS_TUNE_TILE_ORDER) *value = c->tune_order;
     else if (knob == CS_TUNE_GENERIC) *value = c->tune_generic;
+    else if (knob == CS_TUNE_NO_PREFETCH) *value = c->tune_noprefetch;
     else return CS_EINVAL;
     return CS_OK;
 }
@@ -1358,6 +1363,7 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
         md.tile_order = c->tune_order;
         md.generic = c->tune_generic;
         const MfmaLds lay = mfma_lds_plan(nh, fs, md);
+        if (c->tune_noprefetch) md.wpf = 0;
         if (lay.total > 160 * 1024)
             return c->fail(CS_EINVAL, "fused MFMA mode: LDS plan exceeds 160 KB");
         // one persistent workgroup per compute unit (dealt over the 8 XCDs),

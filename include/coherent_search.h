@@ -47,6 +47,10 @@ extern "C" {
                                    fold on its profile length at run time even where a kernel
                                    specialised on the search's fold set exists (same results);
                                    default 0 */
+#define CS_TUNE_NO_PREFETCH 6   /* split-BF16 mode: 1 loads each tile's first conj(A) operands
+                                   at its interpolation instead of into LDS during the previous
+                                   tile (same results; the kernel does so by itself when its
+                                   LDS plan has no room); default 0 */
 
 /* SearchParams (src/search.jl:131-144).  `decimations` is pointer + length
  * (the reference's Vector{Int}).  precision is the reference's profile-stage

@@ -307,6 +307,37 @@ def test_fold_set_kernel_equals_generic(cs, small_spectrum, nharms, maxdecim, bl
     eng.close()
 
 
+@pytest.mark.parametrize("generic", [False, True])
+def test_lds_prefetch_off_equals_on(cs, small_spectrum, generic):
+    """The split-BF16 kernel's LDS prefetch of each tile's first conj(A)
+    operands (direct-to-LDS loads during the previous tile) changes no result:
+    raw candidates, every pass sigma (:f32) and the probe's stacked amplitudes
+    and bounds equal those of the same kernel loading them at the
+    interpolation (CS_TUNE_NO_PREFETCH), for the fold-set and the generic
+    kernel."""
+    _say(f"LDS prefetch off vs on, generic={generic}")
+    amps, N, dt = small_spectrum
+    T_ = N * dt
+    p = cs.SearchParams(nharms=60, m=16, decimations=cs.decimation_set(60, 6), precision="f32")
+    eng = cs.Engine(p, 2048)
+    eng.set_mode("fused")
+    eng.set_spectrum(_ft(cs, amps, N, dt))
+    eng.set_tuning(generic=generic)
+    r_lo, r_hi = 1.0 * T_, 60.0 * T_
+    out = {}
+    for off in (False, True):
+        eng.set_tuning(no_prefetch=off)
+        assert eng.tuning()["no_prefetch"] == int(off)
+        raw = eng.search_raw(r_lo, r_hi, 6.0)
+        out[off] = (raw, eng.last_sigmas(), eng.probe_split_bf16(r_lo, r_hi, 1000, 256))
+    assert len(out[False][0]) > 0
+    assert np.array_equal(out[False][0], out[True][0])
+    assert np.array_equal(out[False][1], out[True][1])
+    for a, b in zip(out[False][2], out[True][2]):
+        assert np.array_equal(a, b)
+    eng.close()
+
+
 def test_search_files_with_tuning_cache(cs, tmp_path):
     """A sweep whose length classes take their launch shapes from cs_autotune's
     cache (cs_set_tuning_cache) returns exactly the raw candidates of the sweep
