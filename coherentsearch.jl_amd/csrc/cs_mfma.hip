@@ -341,6 +341,9 @@ __device__ __forceinline__ float bank_bound(const float (&m)[12], uint32_t bankm
 // the compiler can interleave.  The buffers these resources cover are kept
 // below kNoStore bytes by the host.
 constexpr uint32_t kNoStore = kMfmaNoStore;
+// the same offset for a load: past every buffer the kernel reads through a
+// resource, so the load returns zeros and reads no memory
+constexpr uint32_t kNoLoad = kMfmaNoStore;
 typedef unsigned int u32x3 __attribute__((ext_vector_type(3)));
 template <int CH>
 __device__ __forceinline__ void buf_store(const float* v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
@@ -1040,7 +1043,7 @@ __global__ __launch_bounds__(kThreads, 1) void k_fused_mfma(SpecDev sp, TabDev t
             const int run2 = (__builtin_amdgcn_readlane(dreg, 4 * s + 3) >> kDescRun2Shift) & 7;
             wa[s] = __builtin_bit_cast(uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, vo, 0, 0));
             wb[s] = __builtin_bit_cast(
-                uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, g4 < run2 ? vo + 64u : kNoStore, 0, 0));
+                uint4, __builtin_amdgcn_raw_buffer_load_b128(srsc, g4 < run2 ? vo + 64u : kNoLoad, 0, 0));
         }
     };
     if (t_first < t_end) {
