@@ -1605,7 +1605,11 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     // every batch's kernels; stage events only when not captured into a graph.
     auto emit = [&](bool ev) -> int {
         if (mfma) HIPCHK(c, launch_rowmask(c->st, sp, tb, g, nchunks, (uint64_t*)c->m_rowmask.p));
-        HIPCHK(c, launch_reset_u32(c->st, surv_count, sweep ? 1 : 2));
+        // the counters, and the first batch's refinement flags and count,
+        // in one launch
+        const int rpair0 = (refine && nbatch > 0) ? (int)(batches[0].second * nf) + 1 : 0;
+        HIPCHK(c, launch_reset2_u32(c->st, surv_count, sweep ? 1 : 2,
+                                    rpair0 ? rbufs.flags : nullptr, rpair0));
         for (int64_t bi = 0; bi < nbatch; ++bi) {
             hipEvent_t* E = &c->ev[bi * nev];
             auto rec = [&](int q) { return ev ? hipEventRecord(E[q], c->st) : hipSuccess; };
@@ -1667,7 +1671,9 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
                 else HIPCHK(c, launch_fused(c->st, sp, tb, g, fs, b, fu));
                 HIPCHK(c, rec(1));
                 HIPCHK(c, rec(2));
-                HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
+                // (the first batch's count is zero since the reset above:
+                // nothing between pushes survivors)
+                if (bi > 0) HIPCHK(c, launch_reset_u32(c->st, surv_count, 1));
                 HIPCHK(c, launch_select(c->st, fs, g, b, fu, o));
                 HIPCHK(c, rec(3));
                 HIPCHK(c, rec(4));
@@ -1695,8 +1701,11 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
                 HIPCHK(c, launch_gate_f32(c->st, fs, g, b, o));
                 HIPCHK(c, rec(4));
             }
-            if (refine)
-                HIPCHK(c, launch_sigma_refine(c->st, sp, tb, g, fs, b, o, rbufs));
+            if (refine) {
+                RefineBufs rb = rbufs;
+                rb.flags_clear = bi == 0;
+                HIPCHK(c, launch_sigma_refine(c->st, sp, tb, g, fs, b, o, rb));
+            }
             HIPCHK(c, launch_rescan(c->st, sp, tb, g, fs, b, o, 8192));
             HIPCHK(c, rec(5));
         }

@@ -991,8 +991,9 @@ hipError_t launch_sigma_refine(hipStream_t st, const SpecDev& sp, const TabDev& 
     const int npair = b.nblocks * fs.nf;
     if (npair <= 0) return hipSuccess;
     uint32_t* count = rb.flags + npair;
-    hipLaunchKernelGGL(k_reset_u32, dim3((unsigned)((npair + 1 + 63) / 64)), dim3(64), 0, st,
-                       rb.flags, npair + 1);
+    if (!rb.flags_clear)
+        hipLaunchKernelGGL(k_reset_u32, dim3((unsigned)((npair + 1 + 63) / 64)), dim3(64), 0, st,
+                           rb.flags, npair + 1);
     hipLaunchKernelGGL(k_refine_mark, dim3(64), dim3(256), 0, st, o, g, b, fs.nf, rb.flags, rb.list,
                        count);
     const uint32_t held = (uint32_t)npair < rb.cap ? (uint32_t)npair : rb.cap;
@@ -1016,6 +1017,12 @@ __global__ void k_reset_u32(uint32_t* p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+__global__ void k_reset2_u32(uint32_t* p, int n, uint32_t* q, int m) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+    else if (q && i - n < m) q[i - n] = 0u;
+}
+
 __global__ void k_copy_u32(const uint32_t* src, uint32_t* dst) {
     if (threadIdx.x == 0) *dst = *src;
 }
@@ -1027,6 +1034,12 @@ hipError_t launch_copy_u32(hipStream_t st, const uint32_t* src, uint32_t* dst) {
 
 hipError_t launch_reset_u32(hipStream_t st, uint32_t* p, int n) {
     hipLaunchKernelGGL(k_reset_u32, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, p, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_reset2_u32(hipStream_t st, uint32_t* p, int n, uint32_t* q, int m) {
+    const int tot = n + (q ? m : 0);
+    hipLaunchKernelGGL(k_reset2_u32, dim3((unsigned)((tot + 63) / 64)), dim3(64), 0, st, p, n, q, m);
     return hipGetLastError();
 }
 

@@ -153,6 +153,8 @@ hipError_t launch_rescan(hipStream_t st, const SpecDev& sp, const TabDev& tb,
                          const GridDev& g, const FoldSet& fs, const BatchDev& b,
                          const OutDev& o, int nblocks_grid);
 hipError_t launch_reset_u32(hipStream_t st, uint32_t* p, int n);
+// Zero two u32 ranges with one launch (q may be null)
+hipError_t launch_reset2_u32(hipStream_t st, uint32_t* p, int n, uint32_t* q, int m);
 hipError_t launch_copy_u32(hipStream_t st, const uint32_t* src, uint32_t* dst);
 // Scratch of the FP64 sigma refinement: flags (nblocks * nf + 1 words, the
 // last the list count; cleared by the launch itself), the list of flagged
@@ -165,6 +167,7 @@ struct RefineBufs {
     uint32_t* list;
     double* samples;
     uint32_t cap;
+    bool flags_clear;   // the batch's flags and count are already zero (no reset launch)
 };
 // FP64 sigma of the batch's (block, fold) pairs that hold gate survivors
 hipError_t launch_sigma_refine(hipStream_t st, const SpecDev& sp, const TabDev& tb,
