@@ -1293,16 +1293,18 @@ static int search_impl(cs_ctx* c, double r_lo, double r_hi, double threshold, co
     MfmaDev md{};
     int mfma_nwg = 0;
     if (mfma) {
-        std::vector<int> nbs, wms, key;
+        std::vector<int> nbs, wms, fks, key;
         for (int f = 0; f < nf; ++f) {
             nbs.push_back(fs.f[f].nbins);
             wms.push_back(fold_wm(fs.f[f].nbins, fs.bankmask[f]));
+            fks.push_back(fs.f[f].k);
         }
         key = nbs;
         key.insert(key.end(), wms.begin(), wms.end());
+        key.insert(key.end(), fks.begin(), fks.end());
         if (!(c->mfma_valid && c->mfma_rlo == r_lo && c->mfma_nbs == key)) {
             c->mfma_valid = false;
-            MfmaTables mt = build_mfma_tables(build_plans(nh, c->m, c->hidr, r_lo), nbs, wms);
+            MfmaTables mt = build_mfma_tables(build_plans(nh, c->m, c->hidr, r_lo), nbs, wms, fks);
             if (!mt.ok) return c->fail(CS_EINVAL, "fused MFMA mode: " + mt.why);
             if ((s = upload(c, c->m_whi, mt.wA_hi))) return s;
             if ((s = upload(c, c->m_wlo, mt.wA_lo))) return s;

@@ -108,6 +108,20 @@ CS_HD constexpr KSlot dslot(int kidx, int hp) {
     return kidx < hp ? KSlot{0, kidx + 1} : KSlot{1, kidx - hp + 1};
 }
 
+// K-slot order of a direct-layout tile for fold factor k.  Operand e of lane
+// group g takes slot dperm(g, e, s, kd) of its 32-slot chunk: runs of
+// d = 2^s consecutive harmonics, the lane groups that read one plane
+// starting d harmonics apart.  kd = 1: groups 0-1 read Re (slots 0-15),
+// groups 2-3 Im (16-31); kd = 2: all four read the chunk's plane.  With
+// d*k = 8 (mod 16) the two lane groups of a half-wave gather stacked rows d*k
+// apart, 16 (mod 32) banks at a row stride of 2 (mod 32) words, so their
+// 16-word runs never share a bank.
+CS_HD constexpr int dshift(int k) { return (k & 1) ? 3 : (k & 2) ? 2 : (k & 4) ? 1 : 0; }
+CS_HD constexpr int dperm(int g, int e, int s, int kd) {
+    return kd == 1 ? 16 * (g >> 1) + ((g & 1) << s) + (e & ((1 << s) - 1)) + ((e >> s) << (s + 1))
+                   : (g << s) + (e & ((1 << s) - 1)) + ((e >> s) << (s + 2));
+}
+
 struct Src {
     int lane;   // lane group 0..3 of the column that holds the value
     int kind;   // 0 = P register, 1 = M register

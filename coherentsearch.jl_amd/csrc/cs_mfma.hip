@@ -206,9 +206,11 @@ __device__ __forceinline__ void set_half(uint4& q, int f, uint32_t v) {
     w = (f & 1) ? ((w & 0x0000FFFFu) | (v << 16)) : ((w & 0xFFFF0000u) | (v & 0xFFFFu));
 }
 
-// Row stride (words) of the stacked-amplitude planes.  A c2r operand read
-// touches rows 8k apart in the four lane groups of a column; with 130-word
-// rows those land 16k banks apart, distinct for k = 1, 2, 3, 5, 6.
+// Row stride (words) of the stacked-amplitude planes.  An even/odd fold's c2r
+// operand read touches rows 8k apart in the four lane groups of a column;
+// with 130-word rows the two groups of a half-wave land 16k (mod 32) banks
+// apart, disjoint for odd k.  The direct folds space the groups' rows by the
+// fold factor (geo::dperm), disjoint for every k.
 constexpr int kFtpLd = 130;
 
 // ---------------------------------------------------------------------------
@@ -399,25 +401,11 @@ __device__ __forceinline__ void put_column(const ColumnSink& sk, int g, const fl
     }
 }
 
-// Words of eight stacked-amplitude rows h0 + e (e < 8) of one LDS plane,
-// stride rstride words from rbase; rows past H read row H (its c2r
-// coefficients beyond H are exactly zero, so any finite row serves).  The
-// byte addresses advance by one add and are clamped by one min each.
-template <int H>
-__device__ __forceinline__ void gather_rows(const CS_LDS uint32_t* pl, int h0, int rstride,
-                                            int rbase, uint32_t* w) {
-    const uint32_t b = (uint32_t)(uintptr_t)pl;
-    const uint32_t st = 4u * (uint32_t)rstride;
-    uint32_t a = b + 4u * (uint32_t)((int)__umul24((unsigned)h0, (unsigned)rstride) + rbase);
-    const uint32_t cap = b + 4u * (uint32_t)((int)__umul24((unsigned)H, (unsigned)rstride) + rbase);
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        w[e] = *(const CS_LDS uint32_t*)(uintptr_t)(a < cap ? a : cap);
-        a += st;
-    }
-}
-
-// The same rows of both planes (im = re + dplane bytes), one address chain.
+// Words of eight stacked-amplitude rows h0 + e (e < 8) of both LDS planes
+// (im = re + dplane bytes), stride rstride words from rbase, one address
+// chain; rows past H read row H (its c2r coefficients beyond H are exactly
+// zero, so any finite row serves).  The byte addresses advance by one add and
+// are clamped by one min each.
 template <int H>
 __device__ __forceinline__ void gather_rows2(const CS_LDS uint32_t* re, const CS_LDS uint32_t* im,
                                              int h0, int rstride, int rbase, uint32_t* wr,
@@ -443,13 +431,6 @@ __device__ __forceinline__ void gather_rows2(const CS_LDS uint32_t* re, const CS
 // stacked planes and the DFT tiles.  The row offsets become immediate LDS
 // offsets.  Stack row h*KF sits at plane row h*KF - 1.
 template <int KF>
-__device__ __forceinline__ void gather_rows_k(const CS_LDS uint32_t* pl, int h0, int tcol,
-                                              uint32_t* w) {
-    const CS_LDS uint32_t* p = pl + ((int)__umul24((unsigned)h0, KF) - 1) * kFtpLd + tcol;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) w[e] = p[e * KF * kFtpLd];
-}
-template <int KF>
 __device__ __forceinline__ void gather_rows2_k(const CS_LDS uint32_t* re, const CS_LDS uint32_t* im,
                                                int h0, int tcol, uint32_t* wr, uint32_t* wi) {
     const int off = ((int)__umul24((unsigned)h0, KF) - 1) * kFtpLd + tcol;
@@ -458,6 +439,34 @@ __device__ __forceinline__ void gather_rows2_k(const CS_LDS uint32_t* re, const 
         wr[e] = re[off + e * KF * kFtpLd];
         wi[e] = im[off + e * KF * kFtpLd];
     }
+}
+
+// Direct-layout gathers in the tiles' K-slot order (geo::dperm): operand e
+// reads row h0 + (e mod d) + ng*d*(e div d), d = 2^s, for the ng lane groups
+// that share a plane (2 for one K chunk of Re | Im, 4 for a chunk per plane).
+// Run-time fold factor: rows past H read row H, as in gather_rows2.
+template <int H, int NG>
+__device__ __forceinline__ void gather_drows(const CS_LDS uint32_t* pl, int h0, int s,
+                                             int rstride, int rbase, uint32_t* w) {
+    const uint32_t b = (uint32_t)(uintptr_t)pl;
+    const uint32_t st = 4u * (uint32_t)rstride;
+    const uint32_t a = b + 4u * (uint32_t)((int)__umul24((unsigned)h0, (unsigned)rstride) + rbase);
+    const uint32_t cap = b + 4u * (uint32_t)((int)__umul24((unsigned)H, (unsigned)rstride) + rbase);
+    const int dm = (1 << s) - 1;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const uint32_t x = a + st * (uint32_t)((e & dm) + NG * ((e >> s) << s));
+        w[e] = *(const CS_LDS uint32_t*)(uintptr_t)(x < cap ? x : cap);
+    }
+}
+// Compile-time fold factor KF (immediate offsets, no clamp, as gather_rows2_k)
+template <int KF, int NG>
+__device__ __forceinline__ void gather_drows_k(const CS_LDS uint32_t* pl, int h0, int tcol,
+                                               uint32_t* w) {
+    constexpr int S = geo::dshift(KF), D = 1 << S;
+    const CS_LDS uint32_t* p = pl + ((int)__umul24((unsigned)h0, KF) - 1) * kFtpLd + tcol;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) w[e] = p[(e % D + NG * D * (e / D)) * KF * kFtpLd];
 }
 
 // BM: the fold's bank as a compile-time mask (0: read from bankmask at run time)
@@ -580,18 +589,27 @@ __device__ __forceinline__ float fold_direct_body(const CS_LDS uint32_t* __restr
     // stack row h*k at plane row h*k - 1 (rows < 64, strides < 2^24: 24-bit products)
     const int rstride = k * kFtpLd, rbase = tcol - kFtpLd;
 
-    // B operands: K slot kk*32 + 8g + e holds Re h (first half) or Im h
-    // (second half); slots past the last harmonic read row H (zero coefficients)
+    // B operands: K slot kk*32 + geo::dperm(g, e) holds Re h (first half) or
+    // Im h (second half); slots past the last harmonic read row H (zero
+    // coefficients)
     bf16x8 bh[KD], bl[KD];
+    const int s = KF > 0 ? geo::dshift(KF) : geo::dshift(k);
     if constexpr (KD == 1) {
         uint32_t w[8];
-        if constexpr (KF > 0) gather_rows_k<KF>(g < 2 ? fre : fim, 8 * (g & 1) + 1, tcol, w);
-        else gather_rows<H>(g < 2 ? fre : fim, 8 * (g & 1) + 1, rstride, rbase, w);
+        const int h0 = ((g & 1) << s) + 1;
+        if constexpr (KF > 0) gather_drows_k<KF, 2>(g < 2 ? fre : fim, h0, tcol, w);
+        else gather_drows<H, 2>(g < 2 ? fre : fim, h0, s, rstride, rbase, w);
         unpack8(w, bh[0], bl[0]);
     } else {
         uint32_t wr[8], wi[8];
-        if constexpr (KF > 0) gather_rows2_k<KF>(fre, fim, 8 * g + 1, tcol, wr, wi);
-        else gather_rows2<H>(fre, fim, 8 * g + 1, rstride, rbase, wr, wi);
+        const int h0 = (g << s) + 1;
+        if constexpr (KF > 0) {
+            gather_drows_k<KF, 4>(fre, h0, tcol, wr);
+            gather_drows_k<KF, 4>(fim, h0, tcol, wi);
+        } else {
+            gather_drows<H, 4>(fre, h0, s, rstride, rbase, wr);
+            gather_drows<H, 4>(fim, h0, s, rstride, rbase, wi);
+        }
         unpack8(wr, bh[0], bl[0]);
         unpack8(wi, bh[1], bl[1]);
     }

@@ -187,7 +187,7 @@ static void split_bf16(float x, uint16_t& hi, uint16_t& lo) {
 }
 
 MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins,
-                             const std::vector<int>& fold_wm) {
+                             const std::vector<int>& fold_wm, const std::vector<int>& fold_k) {
     MfmaTables t;
     if (!pl.rational) { t.why = "trial step is not a small rational"; return t; }
     const int m = pl.m, nh = pl.nharms;
@@ -301,7 +301,9 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
         const int NB = fold_nbins[f], H = NB / 2;
         if (geo::c2r_direct(NB)) {
             // direct tiles: row i = 4g + v of row tile a computes bin
-            // (g*RD + 4a + v) mod NB from K slots (Re h | Im h)
+            // (g*RD + 4a + v) mod NB from K slots (Re h | Im h), lane group g's
+            // operand e in slot geo::dperm of the fold factor's spacing
+            const int sh = geo::dshift(f < fold_k.size() ? fold_k[f] : 1);
             const int wm = f < fold_wm.size() ? fold_wm[f] : 6;
             const int RD = (NB + 3) / 4, SH = 4 * RD - NB, ext = wm - 1;
             const int mt = std::max((RD + 3) / 4, (ext + SH + 3) / 4);
@@ -315,7 +317,8 @@ MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins
                         const int i = l % 16;
                         const int n = ((i / 4) * RD + 4 * a + i % 4) % NB;
                         for (int e = 0; e < 8; ++e) {
-                            const geo::KSlot ks = geo::dslot(kk * 32 + (l / 16) * 8 + e, hp);
+                            const geo::KSlot ks =
+                                geo::dslot(kk * 32 + geo::dperm(l / 16, e, sh, kd), hp);
                             const int64_t ee = ((int64_t)ks.h * n) % NB;
                             const double ang = 2.0 * (double)ee / (double)NB;
                             double v = 0.0;

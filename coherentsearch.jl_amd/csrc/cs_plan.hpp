@@ -109,8 +109,9 @@ struct FoldDev {
     int32_t widths[kMaxWidths];
 };
 
-// fold_wm[f]: widest boxcar the fold's pass evaluates (geo fold_wm)
+// fold_wm[f]: widest boxcar the fold's pass evaluates (geo fold_wm);
+// fold_k[f]: the fold factor (orders the direct tiles' K slots)
 MfmaTables build_mfma_tables(const Plans& pl, const std::vector<int>& fold_nbins,
-                             const std::vector<int>& fold_wm);
+                             const std::vector<int>& fold_wm, const std::vector<int>& fold_k);
 
 }  // namespace cs

@@ -51,7 +51,8 @@ static int64_t first_bin(const Plans& pl, const HarmDev& d, int64_t t, int* phas
 }
 
 static void check_config(int nharms, int m, double hidr, double r_lo, const std::vector<int>& nbins,
-                         unsigned seed, const std::vector<int>& wms = {}) {
+                         unsigned seed, const std::vector<int>& wms = {},
+                         const std::vector<int>& fks = {}) {
     Plans pl = build_plans(nharms, m, hidr, r_lo);
     if (!pl.rational) {
         printf("config nharms=%d: not rational\n", nharms);
@@ -60,7 +61,9 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
     }
     std::vector<int> wm = wms;
     if (wm.empty()) wm.assign(nbins.size(), 6);
-    MfmaTables t = build_mfma_tables(pl, nbins, wm);
+    std::vector<int> fk = fks;   // fold factors 1, 2, ... unless given
+    for (size_t f = fk.size(); f < nbins.size(); ++f) fk.push_back((int)f + 1);
+    MfmaTables t = build_mfma_tables(pl, nbins, wm, fk);
     if (!t.ok) {
         printf("tables failed: %s\n", t.why.c_str());
         ++bad;
@@ -153,23 +156,42 @@ static void check_config(int nharms, int m, double hidr, double r_lo, const std:
         double scale = 0.0;
         for (int h = 1; h <= H; ++h) scale += std::fabs(Xr[h]) + std::fabs(Xi[h]);
         if (NB <= 64) {
-            // direct layout: K slot kk*32 + 8(l/16) + e is Re h (first HP slots)
-            // or Im h (next HP), the kernel reading row min(h, H); lane group g,
-            // register r = 4a + v holds bin (g*RD + r) mod NB
+            // direct layout: lane group g's operand e of K chunk kk is the
+            // stacked row the kernel's gather reads, min(h, H) of plane Re or Im:
+            // one chunk (H <= 16): groups 0-1 Re, 2-3 Im, h = 1 + d(g&1) +
+            // (e mod d) + 2d(e div d); two chunks: chunk 0 Re, 1 Im,
+            // h = 1 + dg + (e mod d) + 4d(e div d); d = 8 for odd fold factors
+            // k, else 8 / (largest power of two dividing k), at least 1.
+            // Lane group g, register r = 4a + v holds bin (g*RD + r) mod NB.
             const int RD = (NB + 3) / 4, SH = 4 * RD - NB, ext = wm[f] - 1;
             const int mt = std::max((RD + 3) / 4, (ext + SH + 3) / 4);
             const int kd = H <= 16 ? 1 : H <= 32 ? 2 : 4, hp = 16 * kd;
             CHECK(t.c_mt[f] == mt && t.c_ks[f] == kd, "NB=%d direct tile counts %d %d\n", NB,
                   t.c_mt[f], t.c_ks[f]);
+            (void)hp;
+            const int k = fk[f];
+            int d = 8;
+            for (int kk2 = k; d > 1 && kk2 % 2 == 0; kk2 /= 2) d /= 2;
+            // the two lane groups of a half-wave (0-1, 2-3) gather rows whose
+            // 16-word runs at the 130-word stride fall in disjoint halves of
+            // the 32 banks
+            for (int el = 0; el < 8; ++el) {
+                const int r0 = (1 + (el % d) + (kd == 1 ? 2 : 4) * d * (el / d)) * k - 1;
+                const int r1 = r0 + d * k;
+                CHECK(((r1 - r0) * 130) % 32 == 16, "NB=%d k=%d: rows %d, %d share banks\n", NB,
+                      k, r0, r1);
+            }
             std::vector<double> acc(16 * mt, 0.0);
             for (int a = 0; a < mt; ++a)
                 for (int kk = 0; kk < kd; ++kk) {
                     const size_t tile = (size_t)t.c_base[f] + a * kd + kk;
                     for (int l = 0; l < 64; ++l)
                         for (int el = 0; el < 8; ++el) {
-                            const int i = l % 16, kidx = kk * 32 + (l / 16) * 8 + el;
-                            const bool im = kidx >= hp;
-                            const int h = (im ? kidx - hp : kidx) + 1;
+                            const int i = l % 16, g = l / 16;
+                            const bool im = kd == 1 ? g >= 2 : kk == 1;
+                            const int h = kd == 1
+                                              ? 1 + d * (g & 1) + (el % d) + 2 * d * (el / d)
+                                              : 1 + d * g + (el % d) + 4 * d * (el / d);
                             const size_t idx = (tile * 64 + l) * 8 + el;
                             const double c = (double)bf2f(t.cA_hi[idx]) + (double)bf2f(t.cA_lo[idx]);
                             const bool live = im ? h < H : h <= H;
@@ -247,7 +269,7 @@ int main() {
     check_config(32, 32, 0.5, 4000.0, {64, 32, 20, 16}, 14);
     // wide banks (more direct row tiles): nharms 60, folds 120, 60, 40, 30
     check_config(60, 32, 0.5, 0.1 * T2, {120, 60, 40, 30}, 15, {28, 13, 9, 9});
-    check_config(32, 16, 0.5, 0.1 * 16777216 * 64e-6, {64, 32, 16}, 16, {19, 9, 6});
+    check_config(32, 16, 0.5, 0.1 * 16777216 * 64e-6, {64, 32, 16}, 16, {19, 9, 6}, {1, 2, 4});
     check_config(12, 16, 0.5, 1000.0, {24}, 17);
     // other trial-grid periods: hidr 0.25 (q = 240, S = 240), hidr 0.3
     // (q = 200, S = 400), hidr 1/3 at nharms 30 (q = 90, S = 720), and the
