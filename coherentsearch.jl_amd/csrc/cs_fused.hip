@@ -155,10 +155,13 @@ __global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g,
 
 // ---------------------------------------------------------------------------
 // Exact selection of the median |x| of one (block, fold)'s samples: radix
-// select on the FP32 bit patterns (12, 10, 10 bits; the last pass is skipped
-// when the second leaves a single key) with plain LDS histogram atomics, then
-// for an even count the lower central value from the final rank and a max.  The 12-bit leading digit (sign, exponent, three mantissa bits)
-// spreads a wave's keys over enough bins that same-address atomics stay few.
+// select on the FP32 bit patterns of the magnitudes (bits 30..19, 18..9,
+// 8..0; the last pass is skipped when the second leaves a single key) with
+// plain LDS histogram atomics, then for an even count the lower central value
+// from the final rank and a max.  The sign bit of every sample key is zero,
+// so the first digit starts below it: exponent and four mantissa bits spread
+// a wave's keys over enough bins that same-address atomics stay few.  Padding
+// keys (0xFFFFFFFF) fail the first pass's prefix test and are never counted.
 // ---------------------------------------------------------------------------
 constexpr int kST = 256;
 constexpr int kSPer = 32;
@@ -248,19 +251,19 @@ __device__ __forceinline__ uint32_t radix_pass(const uint32_t (&keys)[kSPer], ui
 __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t& kth, uint32_t* hist,
                                uint32_t* sh) {
     uint32_t cnt;
-    const uint32_t d0 = radix_pass<20, 12>(keys, 0u, 0u, kth, cnt, hist, sh);
-    const uint32_t p1 = d0 << 20;
-    const uint32_t d1 = radix_pass<10, 10>(keys, p1, 0xFFF00000u, kth, cnt, hist, sh);
-    const uint32_t p2 = p1 | (d1 << 10);
+    const uint32_t d0 = radix_pass<19, 12>(keys, 0u, 0x80000000u, kth, cnt, hist, sh);
+    const uint32_t p1 = d0 << 19;
+    const uint32_t d1 = radix_pass<9, 10>(keys, p1, 0xFFF80000u, kth, cnt, hist, sh);
+    const uint32_t p2 = p1 | (d1 << 9);
     if (cnt == 1) {
-        // one key has the selected 22-bit prefix: it is the answer (rank 0
+        // one key has the selected 23-bit prefix: it is the answer (rank 0
         // among its equals); the maximum over the keys with that prefix (the
         // others count as 0) finds it without the third pass.  cnt is
         // block-uniform, so is this branch.
         uint32_t m = 0;
 #pragma unroll
         for (int r = 0; r < kSPer; ++r)
-            m = (keys[r] & 0xFFFFFC00u) == p2 && keys[r] > m ? keys[r] : m;
+            m = (keys[r] & 0xFFFFFE00u) == p2 && keys[r] > m ? keys[r] : m;
         m = wave_scan(m, MaxU32{});
         if ((threadIdx.x & 63) == 63) sh[32 + (threadIdx.x >> 6)] = m;
         __syncthreads();
@@ -270,7 +273,7 @@ __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t& kth, uin
         kth = 0;
         return up;
     }
-    const uint32_t d2 = radix_pass<0, 10>(keys, p2, 0xFFFFFC00u, kth, cnt, hist, sh);
+    const uint32_t d2 = radix_pass<0, 9>(keys, p2, 0xFFFFFE00u, kth, cnt, hist, sh);
     return p2 | d2;
 }
 
