@@ -278,6 +278,8 @@ int main() {
     check_config(60, 16, 0.3, 523.0, {120, 60, 40, 30, 24, 20}, 19);
     check_config(30, 16, 1.0 / 3.0, 777.0, {60, 30, 20}, 20);
     check_config(64, 16, 0.5, 911.0, {128, 64}, 21);
+    // fold factor 8 (K slots one harmonic apart per lane group)
+    check_config(64, 16, 0.5, 911.0, {128, 64, 16}, 22, {}, {1, 2, 8});
     printf(bad ? "FAIL %d\n" : "OK\n", bad);
     return bad ? 1 : 0;
 }

@@ -136,8 +136,10 @@ def test_register_geometry_host_check():
 
 
 def test_mfma_operand_tables_host_check():
-    """Window starts, weight tiles, conj(A) and the permuted even/odd c2r tiles
-    of the split-BF16 kernel against values recomputed from the plans
+    """Window starts, weight tiles, conj(A), the permuted even/odd c2r tiles and
+    the direct c2r tiles in their per-fold-factor K-slot order (against the
+    gather's row formula, with the two lane groups of each half-wave in
+    disjoint LDS banks) of the split-BF16 kernel, recomputed from the plans
     (tests/native/mfma_tables_check.cpp)."""
     out = _native(os.path.join(NATIVE, "mfma_tables_check.cpp"),
                   [os.path.join(CSRC, "cs_plan.cpp")])
