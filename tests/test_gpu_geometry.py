@@ -142,3 +142,30 @@ def test_harmonic_bins_near_1e8_vs_oracle(big_spectrum, mode):
     _say(f"  {len(got)} vs {len(want)} candidates, common {ncommon}, worst rel {worst:.2e}, "
          f"best r {rbest:.3f} S/N {float(best['metric']):.1f}")
     eng.close()
+
+
+def test_fold_factor_8_vs_oracle(small_spectrum):
+    """Folds 1, 2 and 8 at nharms 64 (128, 64 and 16 bins) in the split-BF16
+    mode: no fold-set kernel matches, so the run-time kernel gathers the 16-bin
+    fold's stacked rows at the K-slot spacing of fold factor 8 (one harmonic
+    between lane groups) and the 64-bin fold's at that of factor 2."""
+    import coherentsearch as cs
+    import oracle as O
+    amps, N, dt = small_spectrum
+    T = N * dt
+    nharms, dec, blocksize = 64, [1, 2, 8], 2048
+    _say(f"fold factor 8: nharms={nharms} decims={dec} blocksize={blocksize}")
+    p = cs.SearchParams(nharms=nharms, m=16, decimations=dec)
+    eng = cs.Engine(p, blocksize)
+    eng.set_mode("fused")
+    eng.set_spectrum(cs.FFTFile.from_arrays(amps, N, dt))
+    r_lo, r_hi = 3.0 * T, 30.0 * T
+    total = math.floor((r_hi - r_lo) / (0.5 / nharms)) + 1
+    got = eng.search_raw(r_lo, r_hi, 6.5)
+    want = O.search_region(O.make_params(nharms=nharms, m=16, decimations=dec),
+                           amps, N, r_lo, total, blocksize, threshold=6.5)
+    assert len(want) > 0
+    assert {int(r["k"]) for r in want} >= {8}, "no fold-8 candidate to compare"
+    worst, ncommon = _compare(got, want, 6.5)
+    _say(f"  {len(got)} vs {len(want)} candidates, common {ncommon}, worst rel {worst:.2e}")
+    eng.close()
