@@ -166,6 +166,7 @@ __global__ __launch_bounds__(256) void k_fused(SpecDev sp, TabDev tb, GridDev g,
 constexpr int kST = 256;
 constexpr int kSPer = 32;
 constexpr int kHist = 4096;
+constexpr int kHist2 = 1024;   // the second pass's histogram
 static_assert(kST * kSPer == 8192, "the select's division by nbins is exact below 8192 samples");
 
 // Inclusive scan over the 64 lanes of a wave (whole wave active) with DPP
@@ -206,16 +207,25 @@ __device__ __forceinline__ uint32_t blk_excl_scan(uint32_t v, uint32_t* tot) {
 // One radix pass: the histogram of digit (key >> SH) & (2^BITS - 1) over the
 // keys whose bits above SH + BITS equal `prefix` (under `mask`); returns the
 // digit that holds rank `kth` and leaves in `kth` the rank inside it.
-template <int SH, int BITS>
+// ZERO: clear `hist` first (one barrier); otherwise an earlier pass cleared
+// it.  The NZ words at `znext` (the next pass's histogram, whose readers are
+// past this call's first barrier) are cleared alongside the counting, so the
+// next pass needs no clearing barrier of its own.
+template <int SH, int BITS, bool ZERO, int NZ>
 __device__ __forceinline__ uint32_t radix_pass(const uint32_t (&keys)[kSPer], uint32_t prefix,
                                                uint32_t mask, uint32_t& kth, uint32_t& cnt,
-                                               uint32_t* hist, uint32_t* sh) {
+                                               uint32_t* hist, uint32_t* sh, uint32_t* znext) {
     constexpr int NBIN = 1 << BITS, PQ = NBIN / kST;
     static_assert(PQ >= 1 && NBIN <= kHist, "digit width");
+    static_assert(NZ % kST == 0, "whole rows of the next histogram");
     constexpr uint32_t dmask = (uint32_t)NBIN - 1u;
+    if constexpr (ZERO) {
 #pragma unroll
-    for (int q = 0; q < PQ; ++q) hist[threadIdx.x + kST * q] = 0;
-    __syncthreads();
+        for (int q = 0; q < PQ; ++q) hist[threadIdx.x + kST * q] = 0;
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < NZ / kST; ++q) znext[threadIdx.x + kST * q] = 0;
 #pragma unroll
     for (int r = 0; r < kSPer; ++r) {
         const uint32_t key = keys[r];
@@ -247,13 +257,18 @@ __device__ __forceinline__ uint32_t radix_pass(const uint32_t (&keys)[kSPer], ui
 }
 
 // The key of rank kth (0-based); kth is left as its rank among the keys equal
-// to it (the last digit's bin holds exactly those keys).
+// to it (the last digit's bin holds exactly those keys).  The second pass
+// counts into hist2 (cleared during the first); the third into hist again,
+// cleared during the second: the first pass's reads of hist come before its
+// scan barrier.
 __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t& kth, uint32_t* hist,
-                               uint32_t* sh) {
+                               uint32_t* hist2, uint32_t* sh) {
     uint32_t cnt;
-    const uint32_t d0 = radix_pass<19, 12>(keys, 0u, 0x80000000u, kth, cnt, hist, sh);
+    const uint32_t d0 =
+        radix_pass<19, 12, true, kHist2>(keys, 0u, 0x80000000u, kth, cnt, hist, sh, hist2);
     const uint32_t p1 = d0 << 19;
-    const uint32_t d1 = radix_pass<9, 10>(keys, p1, 0xFFF80000u, kth, cnt, hist, sh);
+    const uint32_t d1 =
+        radix_pass<9, 10, false, 512>(keys, p1, 0xFFF80000u, kth, cnt, hist2, sh, hist);
     const uint32_t p2 = p1 | (d1 << 9);
     if (cnt == 1) {
         // one key has the selected 23-bit prefix: it is the answer (rank 0
@@ -273,13 +288,14 @@ __device__ uint32_t select_kth(const uint32_t (&keys)[kSPer], uint32_t& kth, uin
         kth = 0;
         return up;
     }
-    const uint32_t d2 = radix_pass<0, 9>(keys, p2, 0xFFFFFE00u, kth, cnt, hist, sh);
+    const uint32_t d2 = radix_pass<0, 9, false, 0>(keys, p2, 0xFFFFFE00u, kth, cnt, hist, sh, hist);
     return p2 | d2;
 }
 
 __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev b, FusedDev fu,
                                                 OutDev o) {
     __shared__ uint32_t hist[kHist];
+    __shared__ uint32_t hist2[kHist2];
     __shared__ uint32_t sh[64];
     const int cl = blockIdx.x, f = blockIdx.y, nf = fs.nf;
     const int64_t cg = batch_block(b, cl);
@@ -323,7 +339,7 @@ __global__ __launch_bounds__(kST) void k_select(FoldSet fs, GridDev g, BatchDev 
         }
         const uint32_t half = (uint32_t)(ns / 2);
         uint32_t rk = half;   // becomes the rank of the selected key among its equals
-        const uint32_t up = select_kth(keys, rk, hist, sh);
+        const uint32_t up = select_kth(keys, rk, hist, hist2, sh);
         double med;
         if ((ns & 1) || rk > 0) {
             // odd count; or the key of rank half - 1 equals `up` as well
